@@ -1,0 +1,186 @@
+"""Benchmark: progressive path-tracing passes on one MI355X.
+
+A step is one progressive pass (``PathTracer::encode``) of ``spp`` path
+samples per pixel over the workload image, with the scene already resident
+on the GPU. ``value`` is path samples per second over the timed steps.
+
+Prints one JSON line. ``roofline`` prices the integrator kernel's algorithmic
+work (event counts of the CPU oracle on a band of the same workload times
+the FLOP cost table below) against its HIP-event time; ``cpu_baseline`` times
+the FP32 oracle on the host cores on a bounded band of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+from stargaze_amd import scenes  # noqa: E402
+from stargaze_amd.pathtracer import Options, PathTracer  # noqa: E402
+
+METRIC = "converged Mpath-samples/s at 4K Halo @ fixed RMSE; adjoint:forward time ratio"
+
+# FP32 operations per event of the work model (transcendentals count 4).
+# Derived from the operation counts of the corresponding WGSL functions.
+FLOP_COST = {
+    "paths": 30,          # camera_direction, seeding, accumulation share
+    "segments": 10,       # closest-hit / ring loop bookkeeping per segment
+    "sphere": 40,         # relative_center + cross-product discriminant
+    "sphere_hit": 60,     # roots, distance selection, body-local normal
+    "ring": 45,           # plane hit via perpendicular decomposition
+    "light_sample": 90,   # cone width, two draws, basis_direction
+    "visibility": 5,      # shadow-ray bookkeeping (its tests are counted)
+    "ring_optics": 45,    # banded optical depth, four sines
+    "atmo_cell": 45,      # density, extinction, segment integral
+    "density_step": 14,   # one light-path density sample
+    "vnoise": 195,        # eight hash31 lattice values + trilinear mix
+    "box": 30,
+    "rock": 150,
+    "vegetation": 400,
+    "terrain_node": 25,
+    "triangle": 35,
+    "vertex": 70,         # surface shading: sampling frame, throughput
+    "rng": 0,             # integer work
+}
+PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32, dense
+PEAK_HBM_GBS = 8000.0
+
+WORKLOADS = {
+    # name: (builder, width, height, spp per pass, max bounces)
+    "ringed_giant_1080p": (scenes.ringed_giant, 1920, 1080, 64, 6),
+    "airless_triple_256": (scenes.airless_triple, 256, 256, 16, 4),
+}
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def work_model(builder, width, height, spp, bounces, rows=4):
+    """Event counts per path sample from the oracle on a central band."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    t = oracle.OracleTracer((width, height), Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
+    y0 = height // 2 - rows // 2
+    ev = oracle.count_events(t, builder(width, height), (y0, y0 + rows))
+    n = width * rows * spp
+    per = {k: v / n for k, v in ev.items()}
+    flops = sum(per[k] * FLOP_COST[k] for k in FLOP_COST)
+    return per, flops
+
+
+def cpu_baseline(builder, width, height, spp, bounces, target_s=12.0):
+    """FP32 oracle on all host cores over a band of rows sized to ~target_s."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle
+
+    frame = builder(width, height)
+    opts = Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0)
+    rows = 8
+    while True:
+        t = oracle.OracleTracer((width, height), opts)
+        y0 = max(0, height // 2 - rows // 2)
+        t0 = time.perf_counter()
+        t.encode(frame, rows=(y0, min(height, y0 + rows)))
+        dt = time.perf_counter() - t0
+        if dt >= target_s * 0.5 or rows >= height:
+            break
+        rows = min(height, max(rows * 2, int(rows * target_s / max(dt, 1e-3))))
+    samples = width * (min(height, y0 + rows) - y0) * spp
+    cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
+    return {
+        "value": samples / dt / 1e6,
+        "unit": "Mpath-samples/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"FP32 oracle (C restatement of the WGSL), {rows} central rows x {width} px x {spp} spp "
+        f"of the same workload, {dt:.1f} s",
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="ringed_giant_1080p", choices=sorted(WORKLOADS))
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
+        return 2
+    builder, width, height, spp, bounces = WORKLOADS[args.workload]
+    frame = builder(width, height)
+    tracer = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
+    log(f"workload {args.workload}: {width}x{height}, {spp} spp/step, {bounces} bounces")
+    for i in range(args.warmup):
+        tracer.encode(frame)
+    tracer.synchronize()
+    log("warmup done")
+    tracer.set_profiling(True)
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        tracer.encode(frame)
+        log(f"step {i + 1}/{args.steps}")
+    tracer.synchronize()
+    elapsed = time.perf_counter() - t0
+    kernel_ms, launches = tracer.kernel_time()
+    img = tracer.read_accumulation()
+    assert np.all(np.isfinite(img)), "non-finite radiance"
+    samples_per_step = width * height * spp
+    value = samples_per_step * args.steps / elapsed / 1e6
+    per_event, flops = work_model(builder, width, height, spp, bounces)
+    kernel_s = kernel_ms / 1e3 / max(launches, 1)
+    achieved = flops * samples_per_step / kernel_s / 1e12
+    traffic_algorithmic = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
+    line = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "Mpath-samples/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": args.workload,
+            "width": width,
+            "height": height,
+            "spp_per_step": spp,
+            "max_bounces": bounces,
+        },
+        "roofline": {
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": None,
+            "kernel": "k_trace",
+            "kernel_ms": round(kernel_s * 1e3, 3),
+            "flop_per_path_sample": round(flops, 1),
+            "algorithmic_bytes_per_launch": traffic_algorithmic,
+        },
+    }
+    if not args.no_cpu_baseline:
+        log("cpu baseline")
+        line["cpu_baseline"] = cpu_baseline(builder, width, height, spp, bounces)
+    print(json.dumps(line), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

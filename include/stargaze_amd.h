@@ -1,0 +1,205 @@
+/*
+ * stargaze_amd — C ABI of the MI355X path tracer.
+ *
+ * Drop-in for the GPU work that stargaze's `PathTracer::encode` issues
+ * (src/pathtracer.rs:789-1009): the caller keeps building the same frame
+ * (`Frame{globals, bodies, ground, scene_time}`, src/renderer.rs:50-56) and
+ * hands it over through these entry points instead of wgpu buffers and
+ * compute dispatches. Layouts below are byte-identical to the reference's
+ * `Globals` (176 B), `Sphere`/WGSL `Body` (160 B), `Primitive` (48 B),
+ * `RayStar` (32 B), `Settings` (240 B) and `CameraAir` (16 B).
+ *
+ * Conventions:
+ *  - every function returns 0 on success and a negative SG_E* code on error
+ *    (the reference panics or returns anyhow::Result instead);
+ *  - host inputs are copied during the call; the library owns device memory;
+ *  - work is queued on one HIP stream per context; read-back calls synchronize;
+ *  - one context per host thread.
+ */
+#ifndef STARGAZE_AMD_H
+#define STARGAZE_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* src/pathtracer.rs:21-22, src/ground.rs:8-9, src/graphics.rs:11 */
+#define SG_BODY_CAPACITY 256u
+#define SG_MAX_BATCH_SAMPLES 64u
+#define SG_GROUND_CAPACITY 256u
+#define SG_SKY_TEXELS 128u
+#define SG_MAX_SAMPLES (1u << 24)
+/* Finest terrain grid this build traverses (the reference stops at 2048). */
+#define SG_MAX_TERRAIN_CELLS 8192u
+
+#define SG_OK 0
+#define SG_E_INVALID (-1)   /* bad argument, size or layout */
+#define SG_E_CAPACITY (-2)  /* too many bodies / primitives / samples */
+#define SG_E_DEVICE (-3)    /* a HIP call failed; see sg_last_error */
+#define SG_E_STATE (-4)     /* call made in the wrong state */
+
+/* src/renderer.rs:13-30 `Globals`; WGSL src/shaders/pathtrace.wgsl:5-18 */
+typedef struct sg_globals {
+    float cam_right[4];
+    float cam_up[4];
+    float cam_forward[4]; /* w = tan(vertical FOV / 2) */
+    float viewport[4];    /* trace width, height, exposure, display aspect */
+    float atmo_center[4]; /* xyz = host centre (AU), w = host radius */
+    float atmo_rayleigh[4]; /* rgb = Rayleigh (1/AU), w = Mie */
+    float atmo_params[4]; /* x = Mie g, y = scale height, z = top altitude, w = host index + 1 */
+    float ground_east[4]; /* w = observer height (m) */
+    float ground_up[4];   /* w = host radius (m) */
+    float ground_north[4]; /* w = host index + 1 (0 disables the landscape) */
+    uint32_t ground_counts[4]; /* props, terrain on, theme | log2(cells) << 8, prop top bits */
+} sg_globals;
+
+/* src/renderer.rs:32-48 `Sphere`; WGSL `Body` src/shaders/pathtrace.wgsl:26-37 */
+typedef struct sg_sphere {
+    float center[3];
+    float radius;
+    float color[3];
+    float emissive;
+    float center_low[4]; /* xyz = residual centre, w = procedural albedo strength */
+    float ring_plane[4]; /* xyz = pole, w = inner radius / body radius */
+    float ring_params[4]; /* outer radius / body radius, optical depth, banded, unused */
+    float ring_color[4];
+    float axis_x[4];     /* w = surface style */
+    float axis_y[4];     /* w = pattern seed */
+    float surface_color[4];
+    float surface_params[4];
+} sg_sphere;
+
+/* src/ground.rs:164-170 `Primitive`; WGSL src/shaders/ground.wgsl:3-7 */
+typedef struct sg_primitive {
+    float center[4]; /* w: 0 box, 1 ellipsoid, 2 rock, 2.25 tumbled, 3 conifer, 4 grass, 5 log, <0 group */
+    float extent[4];
+    float albedo[4];
+} sg_primitive;
+
+/* src/stars.rs:143-148 `RayStar`; WGSL `CatalogueStar` src/shaders/pathtrace.wgsl:38-41 */
+typedef struct sg_ray_star {
+    float direction[4]; /* xyz = unit direction, w = sin(angular radius)^2 */
+    float radiance[4];
+} sg_ray_star;
+
+/* src/pathtracer.rs:44-52 `Settings`; WGSL src/shaders/pathtrace.wgsl:19-25 */
+typedef struct sg_settings {
+    sg_globals globals;
+    uint32_t counts[4];   /* bodies, accumulated samples, samples this dispatch, max surface vertices */
+    uint32_t sampling[4]; /* noise sequence */
+    float background[4];  /* Milky Way brightness, map width, map height, point-source gain */
+    uint32_t lights[4];   /* first three emissive indices, total count */
+} sg_settings;
+
+/* WGSL `CameraAir` src/shaders/pathtrace.wgsl:54 */
+typedef struct sg_camera_air {
+    float radiance[3];
+    uint32_t rng;
+} sg_camera_air;
+
+/* src/pathtracer.rs:76-112 `Options` (display-only fields kept for parity). */
+typedef struct sg_options {
+    uint32_t noise_seed;
+    uint32_t samples_per_frame;
+    uint32_t max_bounces;
+    uint32_t sample_limit;
+    float exposure;
+    int32_t auto_exposure;
+    float ev_bias;
+    float auto_key;
+    float auto_percentile;
+    float milky_way;
+} sg_options;
+
+/* src/renderer.rs:50-56 `Frame` (labels are UI-only and not taken). */
+typedef struct sg_frame {
+    const sg_globals *globals;
+    const sg_sphere *bodies;
+    uint32_t body_count;
+    const sg_primitive *ground;
+    uint32_t ground_count;
+    double scene_time;
+} sg_frame;
+
+/* Probe rays: the reference pins geometry with probe shaders that call the
+ * production helpers (src/ring_tests.rs:67-101, src/ground_tests.rs:694-726).
+ * These entry points run the same device functions the integrator uses. */
+typedef struct sg_probe_ray {
+    float offset[3];
+    uint32_t anchor; /* 0xffffffff = camera origin */
+    float direction[3];
+    uint32_t target; /* body index, light index, or unused */
+} sg_probe_ray;
+
+typedef struct sg_probe_result {
+    float distance;
+    uint32_t index;
+    float normal[3];
+    float extra[3];
+} sg_probe_result;
+
+enum sg_probe_kind {
+    SG_PROBE_SPHERE = 0,     /* sphere_hit(ray, target)              pathtrace.wgsl:130 */
+    SG_PROBE_CLOSEST = 1,    /* closest_hit(ray)                     pathtrace.wgsl:173 */
+    SG_PROBE_RING = 2,       /* ring_hit + ring_transmission         rings.wgsl:8,83 */
+    SG_PROBE_VISIBILITY = 3, /* light_visibility(ray, target)        rings.wgsl:90 */
+    SG_PROBE_TERRAIN = 4,    /* terrain_query, metre frame           terrain.wgsl:152 */
+    SG_PROBE_LANDSCAPE = 5,  /* landscape_hit(ray)                   ground.wgsl:123 */
+    SG_PROBE_CONE = 6        /* cone_width + sample_light draw       pathtrace.wgsl:197,214 */
+};
+
+typedef struct sg_ctx sg_ctx;
+
+/* PathTracer::new (src/pathtracer.rs:292). `catalogue`/`sky_cells` are the
+ * output of stars::ray_catalogue (src/stars.rs:152-205); either may be empty. */
+int sg_create(sg_ctx **out, uint32_t width, uint32_t height,
+              const sg_ray_star *catalogue, uint32_t catalogue_count,
+              const uint32_t *sky_cells, uint32_t sky_cell_count,
+              const sg_options *options);
+/* Drop (src/pathtracer.rs PathTracer has no explicit destroy). */
+int sg_destroy(sg_ctx *ctx);
+/* Milky Way RGB10 map (src/sky.rs:103-117; bound at binding 6). */
+int sg_set_milky_way(sg_ctx *ctx, const uint32_t *texels, uint32_t width, uint32_t height);
+/* PathTracer::resize (src/pathtracer.rs:673) */
+int sg_resize(sg_ctx *ctx, uint32_t width, uint32_t height);
+/* PathTracer::samples / size / needs_redraw (src/pathtracer.rs:742-752) */
+uint32_t sg_samples(const sg_ctx *ctx);
+int sg_size(const sg_ctx *ctx, uint32_t *width, uint32_t *height);
+int sg_needs_redraw(const sg_ctx *ctx);
+/* PathTracer::prepare_frame (src/pathtracer.rs:756): 1 = fresh history. */
+int sg_prepare_frame(sg_ctx *ctx, const sg_frame *frame);
+/* PathTracer::set_quality (src/pathtracer.rs:761) */
+int sg_set_quality(sg_ctx *ctx, uint32_t samples, uint32_t bounces, uint32_t limit);
+/* PathTracer::set_exposure_controls (src/pathtracer.rs:777) */
+int sg_set_exposure_controls(sg_ctx *ctx, int32_t auto_exposure, float bias);
+/* Terrain::data for the ground's theme (src/terrain.rs:47-52, uploaded at
+ * src/pathtracer.rs:914-932): heights, derivatives (cells <= 1024), min/max. */
+int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint32_t cells);
+/* PathTracer::encode (src/pathtracer.rs:789): queue one progressive pass. */
+int sg_encode(sg_ctx *ctx, const sg_frame *frame);
+/* Read-back of `accumulation` / `backdrop_accumulation` (vec4 f32 per pixel,
+ * src/pathtracer_tests.rs:160-182). Synchronizes the context stream. */
+int sg_read_accumulation(sg_ctx *ctx, float *rgba, uint64_t pixel_count);
+int sg_read_backdrop(sg_ctx *ctx, float *rgba, uint64_t pixel_count);
+/* Per-pixel running mean of squared radiance (rgb, sample count in w); only
+ * kept after sg_enable_moments(ctx, 1). The reference keeps means only; the
+ * second moment is what tile-wise Monte Carlo error bars need. */
+int sg_enable_moments(sg_ctx *ctx, int32_t enable);
+int sg_read_moments(sg_ctx *ctx, float *rgba, uint64_t pixel_count);
+int sg_synchronize(sg_ctx *ctx);
+/* Device time of the integrator kernels since the last reset (HIP events on
+ * the context stream), for measurement. */
+int sg_set_profiling(sg_ctx *ctx, int32_t enable);
+int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
+const char *sg_last_error(const sg_ctx *ctx);
+/* Run production device helpers on a batch of probe rays (see sg_probe_kind). */
+int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind,
+             const sg_probe_ray *rays, uint32_t count, sg_probe_result *results);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,110 @@
+"""ctypes binding of ``libstargaze_amd.so`` (the HIP library).
+
+The library is built in-tree by ``stargaze_amd.build`` and loaded from the
+package directory. There is no fallback: if the library is missing or fails
+to load, every entry point raises.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+LIBRARY = Path(__file__).resolve().parent / "libstargaze_amd.so"
+
+SG_OK = 0
+ERRORS = {-1: "invalid argument", -2: "capacity exceeded", -3: "device error", -4: "invalid state"}
+
+# Every symbol include/stargaze_amd.h declares.
+EXPORTS = (
+    "sg_create",
+    "sg_destroy",
+    "sg_set_milky_way",
+    "sg_resize",
+    "sg_samples",
+    "sg_size",
+    "sg_needs_redraw",
+    "sg_prepare_frame",
+    "sg_set_quality",
+    "sg_set_exposure_controls",
+    "sg_upload_terrain",
+    "sg_encode",
+    "sg_read_accumulation",
+    "sg_read_backdrop",
+    "sg_enable_moments",
+    "sg_read_moments",
+    "sg_synchronize",
+    "sg_set_profiling",
+    "sg_kernel_time",
+    "sg_last_error",
+    "sg_probe",
+)
+
+
+class SgFrame(ctypes.Structure):
+    """``sg_frame`` (include/stargaze_amd.h)."""
+
+    _fields_ = [
+        ("globals", ctypes.c_void_p),
+        ("bodies", ctypes.c_void_p),
+        ("body_count", ctypes.c_uint32),
+        ("ground", ctypes.c_void_p),
+        ("ground_count", ctypes.c_uint32),
+        ("scene_time", ctypes.c_double),
+    ]
+
+
+class NativeError(RuntimeError):
+    pass
+
+
+_lib: ctypes.CDLL | None = None
+
+
+def load() -> ctypes.CDLL:
+    """Load the HIP library, failing loudly when it is absent."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIBRARY.exists():
+        raise NativeError(
+            f"{LIBRARY} is missing: build it with `python -m stargaze_amd.build` "
+            "(hipcc --offload-arch=gfx950); there is no CPU fallback"
+        )
+    lib = ctypes.CDLL(os.fspath(LIBRARY))
+    P, U32, I32, U64 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int32, ctypes.c_uint64
+    sig = {
+        "sg_create": ([ctypes.POINTER(P), U32, U32, P, U32, P, U32, P], I32),
+        "sg_destroy": ([P], I32),
+        "sg_set_milky_way": ([P, P, U32, U32], I32),
+        "sg_resize": ([P, U32, U32], I32),
+        "sg_samples": ([P], U32),
+        "sg_size": ([P, ctypes.POINTER(U32), ctypes.POINTER(U32)], I32),
+        "sg_needs_redraw": ([P], I32),
+        "sg_prepare_frame": ([P, ctypes.POINTER(SgFrame)], I32),
+        "sg_set_quality": ([P, U32, U32, U32], I32),
+        "sg_set_exposure_controls": ([P, I32, ctypes.c_float], I32),
+        "sg_upload_terrain": ([P, P, U64, U32], I32),
+        "sg_encode": ([P, ctypes.POINTER(SgFrame)], I32),
+        "sg_read_accumulation": ([P, P, U64], I32),
+        "sg_read_backdrop": ([P, P, U64], I32),
+        "sg_enable_moments": ([P, I32], I32),
+        "sg_read_moments": ([P, P, U64], I32),
+        "sg_synchronize": ([P], I32),
+        "sg_set_profiling": ([P, I32], I32),
+        "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
+        "sg_last_error": ([P], ctypes.c_char_p),
+        "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
+    }
+    for name in EXPORTS:
+        fn = getattr(lib, name)
+        fn.argtypes, fn.restype = sig[name]
+    _lib = lib
+    return lib
+
+
+def check(lib: ctypes.CDLL, ctx, rc: int, what: str) -> int:
+    if rc < 0:
+        detail = lib.sg_last_error(ctx).decode() if ctx else ""
+        raise NativeError(f"{what} failed: {ERRORS.get(rc, rc)}: {detail}")
+    return rc

@@ -1,0 +1,434 @@
+// Device-side restatement of stargaze's path-tracing helpers for gfx950.
+//
+// Each function cites the WGSL it stands in for (paths relative to the
+// reference repository). Differences that are deliberate:
+//  * Body centres are rebased in FP64 from their f32 high/low parts, and ray
+//    origins are kept in FP64 (`Ray::offset`), so the camera-relative
+//    rebasing of src/shaders/pathtrace.wgsl:100-113 loses nothing.
+//  * Sphere quadratics for a ray that starts within four radii of the centre
+//    (the observer's host planet, bounce rays leaving a body) run in FP64;
+//    distant bodies keep the reference's f32 cross-product form.
+//  * The metre-frame host-sphere quadratic of src/shaders/ground.wgsl:138-158
+//    and the 5 mm outgoing offset of :649-662 run in FP64.
+// Everything else is FP32, as in the reference.
+#pragma once
+#include "sg_math.h"
+#include "../../include/stargaze_amd.h"
+
+#define SG_MISS 0xffffffffu
+#define SG_LOCAL_BASE 0x80000000u   // src/shaders/ground.wgsl:14
+#define SG_TERRAIN_ID 0x80100000u   // src/shaders/terrain.wgsl:23
+#define SG_TERRAIN_STACK 40         // 12 internal levels (8192 cells) need 37
+
+namespace sg {
+
+constexpr float PI = 3.141592653589793f;
+constexpr float TAU = 6.283185307179586f;
+constexpr float METRES_PER_AU = 149597870700.0f;   // src/shaders/ground.wgsl:15
+constexpr double METRES_PER_AU_D = 149597870700.0;
+constexpr float STAR_DOT_PIXELS = 3.0f;            // src/shaders/pathtrace.wgsl:65
+constexpr uint32_t ATMO_VIEW_STEPS = 8u;           // src/shaders/pathtrace.wgsl:67
+constexpr uint32_t ATMO_SUN_STEPS = 4u;            // src/shaders/pathtrace.wgsl:68
+constexpr float POINT_DISC_RADIUS = 0.00136f;      // src/shaders/pathtrace.wgsl:1300
+constexpr float RING_INDIRECT_CAP = 0.003f;        // src/shaders/pathtrace.wgsl:1336
+
+// WGSL `Body` (src/shaders/pathtrace.wgsl:26-37), byte-identical to sg_sphere.
+struct Body {
+    float4 center, material, low, ring_plane, ring_params, ring_color, axis_x, axis_y,
+        surface_color, surface_params;
+};
+static_assert(sizeof(Body) == 160, "Body layout");
+static_assert(sizeof(sg_sphere) == 160, "Sphere layout");
+static_assert(sizeof(sg_globals) == 176, "Globals layout");
+static_assert(sizeof(sg_settings) == 240, "Settings layout");
+static_assert(sizeof(sg_primitive) == 48, "Primitive layout");
+static_assert(sizeof(sg_ray_star) == 32, "RayStar layout");
+static_assert(sizeof(sg_camera_air) == 16, "CameraAir layout");
+
+// Terrain buffer geometry derived from ground_counts.z (src/shaders/terrain.wgsl:6-20).
+struct TerrainLayout {
+    uint32_t cells, width, compact, stride;
+    uint64_t vertices, nodes, bounds, cache, vertex_cache;
+};
+
+SG_FN TerrainLayout terrain_layout(uint32_t ground_counts_z) {
+    TerrainLayout t;
+    uint32_t log2 = ground_counts_z >> 8;
+    t.cells = log2 != 0 ? (1u << log2) : 512u;
+    t.width = t.cells + 1;
+    t.compact = t.cells > 1024u;
+    t.stride = t.compact ? 2u : 6u;
+    t.vertices = (uint64_t)t.width * t.width;
+    uint64_t leaves = t.cells / 2u;
+    t.nodes = (4u * leaves * leaves - 1u) / 3u;
+    t.bounds = (t.compact ? 1u : 3u) * t.vertices;
+    t.cache = t.bounds + 2u * t.nodes;
+    t.vertex_cache = t.cache + 6u * t.nodes;
+    return t;
+}
+
+// Everything a kernel reads about the scene. Passed by value as a kernel
+// argument, so per-scene constants come through scalar loads.
+struct Scene {
+    sg_settings s;
+    const Body *bodies;
+    const sg_primitive *objects;
+    const float4 *sky_light;     // 128 texels (src/shaders/ground.wgsl:10)
+    const float *terrain;
+    const sg_ray_star *catalogue;
+    const uint32_t *sky_cells;   // 256*128+1 offsets, then indices
+    const uint32_t *milky_way;
+    TerrainLayout t;
+    uint32_t catalogue_count;
+};
+
+struct Ray {
+    d3 offset;        // relative to bodies[anchor] (or the camera when anchor == MISS)
+    f3 direction;
+    uint32_t anchor;
+};
+struct Hit {
+    float distance;
+    uint32_t index;
+    f3 normal;
+};
+struct LightSample {
+    f3 direction;
+    float pdf;
+};
+struct RingHit {
+    float distance;
+    uint32_t index;
+    f3 point;
+};
+
+SG_FN Hit miss_hit() { return Hit{1.0e30f, SG_MISS, f3{0.f, 0.f, 0.f}}; }
+
+// --- RNG (src/shaders/pathtrace.wgsl:87-98) ------------------------------
+SG_FN uint32_t hash_u32(uint32_t x) {
+    x = (x ^ (x >> 16)) * 0x7feb352du;
+    x = (x ^ (x >> 15)) * 0x846ca68bu;
+    return x ^ (x >> 16);
+}
+SG_FN float rnd(uint32_t &state) {
+    state = state * 747796405u + 2891336453u;
+    uint32_t word = ((state >> ((state >> 28) + 4u)) ^ state) * 277803737u;
+    return (float)(((word >> 22) ^ word) >> 8) * (1.0f / 16777216.0f);
+}
+
+// --- bodies and rebasing -----------------------------------------------------
+SG_FN uint32_t body_count(const Scene &sc) { return sc.s.counts[0]; }
+SG_FN const Body &body(const Scene &sc, uint32_t i) { return sc.bodies[i]; }
+SG_FN d3 center_d(const Body &b) {
+    return d3{(double)b.center.x + (double)b.low.x, (double)b.center.y + (double)b.low.y,
+              (double)b.center.z + (double)b.low.z};
+}
+// src/shaders/pathtrace.wgsl:100-113, in FP64.
+SG_FN d3 relative_center(const Scene &sc, const Ray &ray, uint32_t index) {
+    if (ray.anchor == index) return -ray.offset;
+    d3 c = center_d(body(sc, index));
+    if (ray.anchor == SG_MISS) return c - ray.offset;
+    return (c - center_d(body(sc, ray.anchor))) - ray.offset;
+}
+
+// src/shaders/pathtrace.wgsl:134-171: cross-product discriminant, one
+// cancellation-free root and the product of roots for the other, normal kept
+// body-local.
+template <class T>
+SG_FN Hit sphere_solve(V3<T> oc, V3<T> d, T radius, uint32_t index) {
+    T a = dot(d, d);
+    T b = dot(oc, d);
+    V3<T> pc = cross(oc, d);
+    T h = a * radius * radius - dot(pc, pc);
+    if (h < T(0)) return miss_hit();
+    T root = sqrtT(h);
+    T q = b + (b >= T(0) ? root : -root);
+    if (q == T(0)) return miss_hit();
+    T c = dot(oc, oc) - radius * radius;
+    T nearv = b < T(0) ? q / a : c / q;
+    T farv = b < T(0) ? c / q : q / a;
+    T distance = nearv;
+    T delta = -root / a;
+    if (distance <= T(0)) {
+        distance = farv;
+        delta = root / a;
+    }
+    if (distance <= T(0)) return miss_hit();
+    V3<T> perpendicular = cross(d, pc) / a;
+    V3<T> n = -perpendicular + d * delta;
+    n = n * (T(1) / sqrtT(dot(n, n)));
+    return Hit{(float)distance, index, f3{(float)n.x, (float)n.y, (float)n.z}};
+}
+
+SG_FN Hit sphere_hit_radius(const Scene &sc, const Ray &ray, uint32_t index, float radius) {
+    d3 oc = relative_center(sc, ray, index);
+    double r = radius;
+    if (dot(oc, oc) < 16.0 * r * r) return sphere_solve<double>(oc, tod(ray.direction), r, index);
+    return sphere_solve<float>(tof(oc), ray.direction, radius, index);
+}
+SG_FN Hit sphere_hit(const Scene &sc, const Ray &ray, uint32_t index) {
+    return sphere_hit_radius(sc, ray, index, body(sc, index).center.w);
+}
+
+// src/shaders/pathtrace.wgsl:187-194
+SG_FN f3 basis_direction(f3 axis, float x, float y, float z) {
+    f3 helper = fabsf(axis.z) > 0.9f ? f3{0.f, 1.f, 0.f} : f3{0.f, 0.f, 1.f};
+    f3 tangent = normalize(cross(helper, axis));
+    return normalize(tangent * x + cross(axis, tangent) * y + axis * z);
+}
+
+// src/shaders/pathtrace.wgsl:197-202
+SG_FN float cone_width(const Scene &sc, const Ray &ray, uint32_t index) {
+    f3 oc = tof(relative_center(sc, ray, index));
+    float r = body(sc, index).center.w;
+    float sin2 = clampf(r * r / dot(oc, oc), 1.0e-20f, 1.0f);
+    return sin2 / (1.0f + sqrtf(1.0f - sin2));
+}
+// src/shaders/pathtrace.wgsl:205-211
+SG_FN uint32_t light_count(const Scene &sc) {
+    return sc.s.lights[3] <= 3u ? sc.s.lights[3] : sc.s.counts[0];
+}
+SG_FN uint32_t light_index(const Scene &sc, uint32_t slot) {
+    return sc.s.lights[3] <= 3u ? sc.s.lights[slot] : slot;
+}
+// src/shaders/pathtrace.wgsl:214-234
+SG_FN LightSample sample_light_stratum(const Scene &sc, const Ray &ray, uint32_t index,
+                                       uint32_t stratum, uint32_t count, uint32_t &rng) {
+    f3 axis = normalize(tof(relative_center(sc, ray, index)));
+    float width = cone_width(sc, ray, index);
+    float radial = rnd(rng);
+    float azimuth = rnd(rng);
+    if (count == 4u) {
+        radial = ((float)(stratum & 1u) + radial) * 0.5f;
+        azimuth = ((float)(stratum >> 1) + azimuth) * 0.5f;
+    }
+    float one_minus_cos = radial * width;
+    float sin_theta = sqrtf(one_minus_cos * (2.0f - one_minus_cos));
+    float phi = TAU * azimuth;
+    float sp, cp;
+    sincosf(phi, &sp, &cp);
+    return LightSample{basis_direction(axis, sin_theta * cp, sin_theta * sp, 1.0f - one_minus_cos),
+                       1.0f / (TAU * width)};
+}
+SG_FN LightSample sample_light(const Scene &sc, const Ray &ray, uint32_t index, uint32_t &rng) {
+    return sample_light_stratum(sc, ray, index, 0u, 1u, rng);
+}
+// src/shaders/pathtrace.wgsl:239-263. `accumulated` is counts.y of the
+// dispatch the sample belongs to.
+SG_FN uint32_t eclipse_light_samples(const Scene &sc, const Ray &ray, uint32_t index,
+                                     uint32_t accumulated) {
+    if (accumulated != 0u) return 1u;
+    f3 source = tof(relative_center(sc, ray, index));
+    float distance = length(source);
+    f3 axis = source / distance;
+    float source_chord = sqrtf(2.0f * cone_width(sc, ray, index));
+    for (uint32_t b = 0; b < body_count(sc); ++b) {
+        if (b == index) continue;
+        f3 offset = tof(relative_center(sc, ray, b));
+        float d2 = dot(offset, offset);
+        float radius = body(sc, b).center.w;
+        if (d2 <= radius * radius) continue;
+        float d = sqrtf(d2);
+        if (d - radius >= distance + body(sc, index).center.w) continue;
+        float sin2 = clampf(radius * radius / d2, 0.0f, 1.0f);
+        float limb_chord = sqrtf(2.0f * sin2 / (1.0f + sqrtf(1.0f - sin2)));
+        float separation = length(axis - offset / d);
+        if (separation < source_chord + limb_chord && separation + source_chord > limb_chord) return 4u;
+    }
+    return 1u;
+}
+// src/shaders/pathtrace.wgsl:264-267
+SG_FN float power_weight(float a, float b) {
+    float ratio = b / fmaxf(a, 1.0e-30f);
+    return 1.0f / (1.0f + ratio * ratio);
+}
+
+// --- background (src/shaders/pathtrace.wgsl:115-128, 269-334) -------------
+SG_FN f3 world_direction(const Scene &sc, f3 d) {
+    const sg_globals &g = sc.s.globals;
+    f3 r = {g.cam_right[0], g.cam_right[1], g.cam_right[2]};
+    f3 u = {g.cam_up[0], g.cam_up[1], g.cam_up[2]};
+    f3 f = {g.cam_forward[0], g.cam_forward[1], g.cam_forward[2]};
+    return normalize(r * d.x + u * d.y - f * d.z);
+}
+SG_FN bool inside_star_dot(const Scene &sc, f3 view_direction, f3 star_direction) {
+    const sg_globals &g = sc.s.globals;
+    f3 r = {g.cam_right[0], g.cam_right[1], g.cam_right[2]};
+    f3 u = {g.cam_up[0], g.cam_up[1], g.cam_up[2]};
+    f3 f = {g.cam_forward[0], g.cam_forward[1], g.cam_forward[2]};
+    f3 sv = {dot(star_direction, r), dot(star_direction, u), -dot(star_direction, f)};
+    if (sv.z >= 0.0f) return false;
+    float dx = view_direction.x / -view_direction.z - sv.x / -sv.z;
+    float dy = view_direction.y / -view_direction.z - sv.y / -sv.z;
+    float pixel_scale = g.viewport[1] / (2.0f * g.cam_forward[3]);
+    dx *= pixel_scale;
+    dy *= pixel_scale;
+    return dx * dx + dy * dy <= STAR_DOT_PIXELS * STAR_DOT_PIXELS;
+}
+SG_FN f3 environment(const Scene &sc, f3 view_direction, bool primary) {
+    f3 light = f3s(0.0f);
+    if (!primary || sc.catalogue_count == 0u) return light;
+    f3 direction = world_direction(sc, view_direction);
+    float phi = atan2f(direction.y, direction.x) + PI;
+    float theta = acosf(clampf(direction.z, -1.0f, 1.0f));
+    uint32_t x = min((uint32_t)(phi * (256.0f / TAU)), 255u);
+    uint32_t y = min((uint32_t)(theta * (128.0f / PI)), 127u);
+    uint32_t cell = y * 256u + x;
+    for (uint32_t j = sc.sky_cells[cell]; j < sc.sky_cells[cell + 1u]; ++j) {
+        const sg_ray_star &star = sc.catalogue[sc.sky_cells[32769u + j]];
+        f3 sd = {star.direction[0], star.direction[1], star.direction[2]};
+        f3 perpendicular = cross(direction, sd);
+        if (dot(direction, sd) > 0.0f && dot(perpendicular, perpendicular) <= star.direction[3]) {
+            if (inside_star_dot(sc, view_direction, sd))
+                light += f3{star.radiance[0], star.radiance[1], star.radiance[2]};
+        }
+    }
+    return light;
+}
+SG_FN f3 milky_way_texel(const Scene &sc, int px, int py) {
+    int width = (int)sc.s.background[1];
+    int height = (int)sc.s.background[2];
+    int x = (px + width) % width;
+    int y = min(max(py, 0), height - 1);
+    uint32_t packed = sc.milky_way[(uint32_t)(y * width + x)];
+    return f3{(float)(packed & 1023u), (float)((packed >> 10) & 1023u), (float)((packed >> 20) & 1023u)} /
+           1023.0f;
+}
+SG_FN f3 milky_way(const Scene &sc, f3 view_direction) {
+    if (sc.s.background[0] == 0.0f || sc.milky_way == nullptr) return f3s(0.0f);
+    f3 direction = world_direction(sc, view_direction);
+    f3 pole = normalize(f3{0.35f, 0.82f, 0.45f});
+    float a = -1.0f / (1.0f + pole.z);
+    f3 right = {pole.x * pole.y * a, 1.0f + pole.y * pole.y * a, -pole.y};
+    f3 forward = cross(pole, right);
+    float longitude = atan2f(dot(direction, forward), dot(direction, right));
+    float u = 0.5f - longitude / TAU;
+    float v = acosf(clampf(dot(direction, pole), -1.0f, 1.0f)) / PI;
+    float px = u * sc.s.background[1] - 0.5f;
+    float py = v * sc.s.background[2] - 0.5f;
+    int lx = (int)floorf(px), ly = (int)floorf(py);
+    float wx = fractf(px), wy = fractf(py);
+    f3 lum = mix3(mix3(milky_way_texel(sc, lx, ly), milky_way_texel(sc, lx + 1, ly), wx),
+                  mix3(milky_way_texel(sc, lx, ly + 1), milky_way_texel(sc, lx + 1, ly + 1), wx), wy);
+    return lum * sc.s.background[0];
+}
+
+// --- procedural albedo (src/shaders/pathtrace.wgsl:338-470, 897-925) ------
+SG_FN float hash31(f3 p) {
+    f3 q = {fractf(p.x * 0.1031f), fractf(p.y * 0.1030f), fractf(p.z * 0.0973f)};
+    float s = dot(q, f3{q.y + 33.33f, q.x + 33.33f, q.z + 33.33f});
+    q = q + f3s(s);
+    return fractf((q.x + q.y) * q.z);
+}
+SG_FN float vnoise(f3 p) {
+    f3 i = {floorf(p.x), floorf(p.y), floorf(p.z)};
+    f3 f = p - i;
+    f3 u = f * f * (f3s(3.0f) - f * 2.0f);
+    float a = mixf(hash31(i), hash31(i + f3{1, 0, 0}), u.x);
+    float b = mixf(hash31(i + f3{0, 1, 0}), hash31(i + f3{1, 1, 0}), u.x);
+    float c = mixf(hash31(i + f3{0, 0, 1}), hash31(i + f3{1, 0, 1}), u.x);
+    float d = mixf(hash31(i + f3{0, 1, 1}), hash31(i + f3{1, 1, 1}), u.x);
+    return mixf(mixf(a, b, u.y), mixf(c, d, u.y), u.z);
+}
+SG_FN float fbm(f3 p, uint32_t octaves) {
+    // Column-major mat3x3 of the WGSL: columns (0,1.6,1.2), (-1.6,.72,-.96), (-1.2,-.96,1.28).
+    float amplitude = 0.5f, sum = 0.0f, norm = 0.0f;
+    for (uint32_t o = 0; o < octaves; ++o) {
+        sum += amplitude * vnoise(p);
+        norm += amplitude;
+        f3 r = {0.00f * p.x - 1.60f * p.y - 1.20f * p.z,
+                1.60f * p.x + 0.72f * p.y - 0.96f * p.z,
+                1.20f * p.x - 0.96f * p.y + 1.28f * p.z};
+        p = r + f3{17.3f, 5.1f, 11.7f};
+        amplitude *= 0.5f;
+    }
+    return sum / norm;
+}
+SG_FN f3 body_local(const Body &b, f3 n) {
+    f3 x = xyz(b.axis_x), y = xyz(b.axis_y);
+    return f3{dot(n, x), dot(n, y), dot(n, cross(x, y))};
+}
+SG_FN f3 seed_offset(float seed) {
+    return f3{hash31(f3{seed, 1.0f, 7.0f}), hash31(f3{3.0f, seed, 5.0f}), hash31(f3{9.0f, 2.0f, seed})} *
+           512.0f;
+}
+// src/shaders/pathtrace.wgsl:394-470
+SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
+    f3 off = seed_offset(b.axis_y.w);
+    f3 zone = xyz(b.material), belt = xyz(b.surface_color);
+    float turbulence = b.surface_color.w;
+    float bands = b.surface_params.x;
+    uint32_t storms = (uint32_t)(b.surface_params.y + 0.5f);
+    f3 q = local;
+    float masks[3] = {0, 0, 0}, rims[3] = {0, 0, 0};
+    for (uint32_t k = 0; k < storms && k < 3u; ++k) {
+        float fk = (float)k;
+        float lat = (hash31(off + f3{7.1f * fk, 1.3f, 2.9f}) - 0.5f) * 1.1f;
+        float lon = hash31(off + f3{3.7f, 5.3f * fk, 8.1f}) * TAU;
+        float size = (k == 0u ? 0.11f : 0.05f) * (0.8f + 0.4f * hash31(off + f3{fk, 9.0f, 4.0f}));
+        f3 centre = {cosf(lat) * cosf(lon), cosf(lat) * sinf(lon), sinf(lat)};
+        if (dot(q, centre) <= 0.0f) continue;
+        f3 east = {-sinf(lon), cosf(lon), 0.0f};
+        f3 north = cross(centre, east);
+        float dx = dot(q - centre, east) / (2.0f * size);
+        float dy = dot(q - centre, north) / size;
+        float e = sqrtf(dx * dx + dy * dy);
+        float influence = 1.0f - smoothstepf(0.0f, 2.4f, e);
+        float angle = (lat < 0.0f ? 3.2f : -3.2f) * influence * influence;
+        float c = cosf(angle), s = sinf(angle);
+        float rx = c * dx - s * dy;
+        float ry = s * dx + c * dy;
+        q = normalize(q + east * ((rx - dx) * 2.0f * size) + north * ((ry - dy) * size));
+        float ragged = e + 0.25f * (fbm(q * 18.0f + off, 3) - 0.5f);
+        masks[k] = 1.0f - smoothstepf(0.7f, 1.0f, ragged);
+        rims[k] = smoothstepf(0.75f, 1.0f, ragged) * (1.0f - smoothstepf(1.0f, 1.35f, ragged));
+    }
+    f3 p = f3{q.x * 2.2f, q.y * 2.2f, q.z * 9.0f} + off;
+    float warp1 = fbm(p, 5) - 0.5f;
+    float warp2 = fbm(p * 2.1f + f3{5.2f, 1.3f, 7.7f} + f3s(4.0f * warp1), 5) - 0.5f;
+    float lat = q.z + 0.07f * sinf(3.0f * PI * q.z + off.z) + 0.04f * sinf(5.0f * PI * q.z + off.y);
+    float shear = 0.35f + 0.65f * fabsf(cosf(PI * bands * lat + off.x));
+    float lon = atan2f(q.y, q.x);
+    float ripple = sinf(lon * (9.0f + floorf(bands)) + 6.0f * warp1 + off.y) * 0.5f +
+                   sinf(lon * 23.0f + 9.0f * warp2 + off.z) * 0.25f;
+    float mu = lat + turbulence * shear * (0.16f * warp1 + 0.10f * warp2 + 0.025f * ripple);
+    float wave = 0.5f + 0.5f * sinf(PI * bands * mu + off.x);
+    float irregular = fbm(f3{mu * bands * 1.7f, off.y, off.z}, 3);
+    float profile = 0.5f * wave + 0.5f * irregular;
+    float tone = fbm(f3{mu * bands * 0.9f, off.z, off.x}, 2);
+    float fine = fbm(f3{q.x * 7.0f, q.y * 7.0f, q.z * 70.0f} + off + f3s(3.0f * warp2), 4);
+    float eddies = fbm(p * 5.0f + f3{warp1, warp2, warp1} * 6.0f, 4);
+    f3 color = mix3(belt, zone, smoothstepf(0.3f, 0.75f, profile));
+    color = mix3(color, color * (f3s(0.55f) + color * 0.5f), smoothstepf(0.35f, 0.75f, tone) * 0.8f);
+    color = color * ((0.78f + 0.44f * fine) * (0.9f + 0.2f * eddies));
+    float polar = smoothstepf(0.55f, 0.92f, fabsf(local.z));
+    color = mix3(color, mix3(zone, belt, 0.55f) * (0.8f + 0.2f * fine), polar * 0.8f);
+    for (uint32_t k = 0; k < storms && k < 3u; ++k) {
+        f3 tint = k == 0u ? belt * belt * 1.5f : zone * 1.15f;
+        float core = 1.0f - 0.25f * (1.0f - smoothstepf(0.0f, 0.6f, masks[k] - rims[k]));
+        color = mix3(color, tint * (0.7f + 0.6f * fine) * core, masks[k] * 0.92f);
+        color = mix3(color, zone * 1.1f, rims[k] * 0.45f);
+    }
+    return color;
+}
+// src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1 (gas giant).
+// Styles 2-6 are rejected by the host library before any launch.
+SG_FN f3 surface_albedo(const Body &b, f3 normal) {
+    f3 local = body_local(b, normal);
+    uint32_t style = (uint32_t)(b.axis_x.w + 0.5f);
+    if (style > 0u) return clamp3(gas_giant_albedo(b, local), 0.0f, 1.0f);
+    float mottle = 1.0f;
+    if (b.low.w > 0.0f) {
+        f3 p = local * 2.4f + seed_offset(b.axis_y.w);
+        float amplitude = 0.5f, noise = 0.0f;
+        for (uint32_t o = 0; o < 5u; ++o) {
+            noise += amplitude * vnoise(p);
+            p = p * 2.07f;
+            amplitude *= 0.5f;
+        }
+        mottle = mixf(1.0f, 0.68f + 0.64f * noise, b.low.w);
+    }
+    return clamp3(xyz(b.material) * mottle, 0.0f, 1.0f);
+}
+
+}  // namespace sg

@@ -1,0 +1,603 @@
+// Rings, landscape props, terrain quadtree and closest-hit / visibility
+// queries (src/shaders/rings.wgsl, ground.wgsl:1-251,597-666, terrain.wgsl,
+// forest.wgsl:56-136 and pathtrace.wgsl:173-185).
+#pragma once
+#include "sg_device.h"
+
+namespace sg {
+
+// --- local surface frame (src/shaders/ground.wgsl:17-61) -------------------
+SG_FN d3 camera_origin(const Scene &sc, const Ray &ray) {
+    if (ray.anchor == SG_MISS) return ray.offset;
+    return center_d(body(sc, ray.anchor)) + ray.offset;
+}
+SG_FN f3 g_east(const Scene &sc) { const float *v = sc.s.globals.ground_east; return f3{v[0], v[1], v[2]}; }
+SG_FN f3 g_up(const Scene &sc) { const float *v = sc.s.globals.ground_up; return f3{v[0], v[1], v[2]}; }
+SG_FN f3 g_north(const Scene &sc) { const float *v = sc.s.globals.ground_north; return f3{v[0], v[1], v[2]}; }
+SG_FN f3 to_ground(const Scene &sc, f3 v) { return f3{dot(v, g_east(sc)), dot(v, g_up(sc)), dot(v, g_north(sc))}; }
+SG_FN d3 to_ground_d(const Scene &sc, d3 v) {
+    return d3{dot(v, tod(g_east(sc))), dot(v, tod(g_up(sc))), dot(v, tod(g_north(sc)))};
+}
+SG_FN f3 from_ground(const Scene &sc, f3 v) { return g_east(sc) * v.x + g_up(sc) * v.y + g_north(sc) * v.z; }
+SG_FN d3 from_ground_d(const Scene &sc, d3 v) {
+    return tod(g_east(sc)) * v.x + tod(g_up(sc)) * v.y + tod(g_north(sc)) * v.z;
+}
+// Metre-frame origin, FP64 (src/shaders/ground.wgsl:29-32).
+SG_FN d3 ground_origin(const Scene &sc, const Ray &ray) {
+    d3 o = to_ground_d(sc, camera_origin(sc, ray) * METRES_PER_AU_D);
+    o.y += (double)sc.s.globals.ground_east[3];
+    return o;
+}
+SG_FN f3 rotate_north(f3 p, float angle) {
+    float c = cosf(angle), s = sinf(angle);
+    return f3{c * p.x - s * p.y, s * p.x + c * p.y, p.z};
+}
+SG_FN f3 rock_angles(float packed) {
+    uint32_t bits = (uint32_t)packed;
+    return f3{(float)(bits & 255u), (float)((bits >> 8) & 255u), (float)((bits >> 16) & 255u)} * (TAU / 256.0f);
+}
+SG_FN f3 rotate_east(f3 p, float angle) {
+    float c = cosf(angle), s = sinf(angle);
+    return f3{p.x, c * p.y - s * p.z, s * p.y + c * p.z};
+}
+SG_FN f3 rotate_up(f3 p, float angle) {
+    float c = cosf(angle), s = sinf(angle);
+    return f3{c * p.x + s * p.z, p.y, -s * p.x + c * p.z};
+}
+SG_FN f3 rock_to_local(f3 p, f3 a) { return rotate_east(rotate_up(rotate_north(p, -a.z), -a.y), -a.x); }
+SG_FN f3 rock_from_local(f3 p, f3 a) { return rotate_north(rotate_up(rotate_east(p, a.x), a.y), a.z); }
+
+SG_FN float comp(f3 v, int i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+
+// src/shaders/ground.wgsl:64-78
+SG_FN float2 box_interval(f3 p, f3 d, f3 extent) {
+    float nearv = -1.0e30f, farv = 1.0e30f;
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) {
+        float pa = comp(p, axis), da = comp(d, axis), ea = comp(extent, axis);
+        if (fabsf(da) < 1.0e-12f) {
+            if (fabsf(pa) > ea) return make_float2(1.0e30f, -1.0e30f);
+        } else {
+            float a = (-ea - pa) / da;
+            float b = (ea - pa) / da;
+            nearv = fmaxf(nearv, fminf(a, b));
+            farv = fminf(farv, fmaxf(a, b));
+        }
+    }
+    return make_float2(nearv, farv);
+}
+
+// src/shaders/ground.wgsl:82-121: box with 8 clipped corners.
+SG_FN float4 rock_hit(f3 p, f3 d, f3 e) {
+    f3 origin = p / e, direction = d / e;
+    float nearv = -1.0e30f, farv = 1.0e30f;
+    f3 near_normal = f3s(0.f), far_normal = f3s(0.f);
+    const float4 none = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+#pragma unroll
+    for (uint32_t face = 0; face < 14u; ++face) {
+        f3 n = f3s(0.f);
+        float limit = 1.0f;
+        if (face < 6u) {
+            float sgn = (face & 1u) == 1u ? 1.0f : -1.0f;
+            if (face / 2u == 0u) n.x = sgn; else if (face / 2u == 1u) n.y = sgn; else n.z = sgn;
+        } else {
+            uint32_t bits = face - 6u;
+            n = f3{(bits & 1u) ? 1.f : -1.f, (bits & 2u) ? 1.f : -1.f, (bits & 4u) ? 1.f : -1.f};
+            limit = 1.65f + 0.08f * (float)(bits % 3u);
+        }
+        float denominator = dot(n, direction);
+        float gap = limit - dot(n, origin);
+        if (fabsf(denominator) < 1.0e-12f) {
+            if (gap < 0.0f) return none;
+        } else {
+            float distance = gap / denominator;
+            if (denominator < 0.0f && distance > nearv) { nearv = distance; near_normal = n / e; }
+            if (denominator > 0.0f && distance < farv) { farv = distance; far_normal = n / e; }
+        }
+        if (nearv > farv) return none;
+    }
+    if (farv <= 0.001f) return none;
+    f3 n = nearv > 0.001f ? normalize(near_normal) : normalize(far_normal);
+    return make_float4(n.x, n.y, n.z, nearv > 0.001f ? nearv : farv);
+}
+
+// --- Möller–Trumbore (src/shaders/terrain.wgsl:65-81) ----------------------
+SG_FN f3 terrain_triangle(f3 o, f3 d, f3 a, f3 b, f3 c) {
+    f3 ab = b - a, ac = c - a;
+    f3 p = cross(d, ac);
+    float determinant = dot(ab, p);
+    if (fabsf(determinant) < 1.0e-12f) return f3{0.f, 0.f, 1.0e30f};
+    float inverse = 1.0f / determinant;
+    f3 relative = o - a;
+    float u = dot(relative, p) * inverse;
+    f3 q = cross(relative, ab);
+    float v = dot(d, q) * inverse;
+    float distance = dot(ac, q) * inverse;
+    if (u < -1.0e-6f || v < -1.0e-6f || u + v > 1.000001f || distance <= 0.001f) return f3{0.f, 0.f, 1.0e30f};
+    return f3{u, v, distance};
+}
+
+// --- vegetation (src/shaders/forest.wgsl:56-136) -----------------------------
+SG_FN float4 trunk_hit(f3 p, f3 d, f3 e) {
+    f3 o = p / e, v = d / e;
+    float4 result = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+    float a = v.x * v.x + v.z * v.z;
+    float b = o.x * v.x + o.z * v.z;
+    float c = o.x * o.x + o.z * o.z - 1.0f;
+    float disc = b * b - a * c;
+    if (a > 1.0e-15f && disc >= 0.0f) {
+        float sq = sqrtf(disc);
+        for (uint32_t side = 0; side < 2u; ++side) {
+            float t = (-b + (side == 1u ? sq : -sq)) / a;
+            f3 q = o + v * t;
+            if (t > 0.001f && t < result.w && fabsf(q.y) <= 1.0f) {
+                f3 n = normalize(f3{q.x, 0.f, q.z} / e);
+                result = make_float4(n.x, n.y, n.z, t);
+            }
+        }
+    }
+    if (fabsf(v.y) > 1.0e-15f) {
+        for (uint32_t side = 0; side < 2u; ++side) {
+            float sign_y = side == 1u ? 1.0f : -1.0f;
+            float t = (sign_y - o.y) / v.y;
+            f3 q = o + v * t;
+            if (t > 0.001f && t < result.w && q.x * q.x + q.z * q.z <= 1.0f) result = make_float4(0.f, sign_y, 0.f, t);
+        }
+    }
+    return result;
+}
+SG_FN float4 conifer_hit(f3 p, f3 d, f3 e) {
+    float4 result = trunk_hit(p + f3{0.f, e.y * 0.08f, 0.f}, d, f3{e.x * 0.065f, e.y * 0.92f, e.z * 0.065f});
+    f3 o = p / e, v = d / e;
+    for (uint32_t tier = 0; tier < 6u; ++tier) {
+        float t = (float)tier;
+        float base = -0.6f + t * 0.27f;
+        float height = fminf(0.66f - t * 0.045f, 1.0f - base);
+        float radius = 0.94f - t * 0.148f;
+        float ox = sinf(t * 4.1f) * 0.022f, oz = cosf(t * 3.7f) * 0.022f;
+        f3 q = o - f3{ox, base + height, oz};
+        float k = radius / height;
+        float a = v.x * v.x + v.z * v.z - k * k * v.y * v.y;
+        float b = q.x * v.x + q.z * v.z - k * k * q.y * v.y;
+        float c = q.x * q.x + q.z * q.z - k * k * q.y * q.y;
+        float disc = b * b - a * c;
+        if (fabsf(a) < 1.0e-12f || disc < 0.0f) continue;
+        float sq = sqrtf(disc);
+        for (uint32_t side = 0; side < 2u; ++side) {
+            float distance = (-b + (side == 1u ? sq : -sq)) / a;
+            f3 h = q + v * distance;
+            if (distance <= 0.001f || distance >= result.w || h.y > 0.0f || h.y < -height) continue;
+            float angle = atan2f(h.z, h.x);
+            float skirt = -h.y / height;
+            float branches = 0.5f + 0.5f * sinf(angle * (9.0f + t) + t * 3.7f);
+            float needle = vnoise((p + d * distance) * f3{9.0f, 17.0f, 9.0f} + f3s(t * 7.0f));
+            if (skirt > 0.76f + 0.19f * branches || (skirt > 0.4f && needle < 0.26f)) continue;
+            f3 n = normalize(f3{h.x, -k * k * h.y, h.z} / e);
+            result = make_float4(n.x, n.y, n.z, distance);
+        }
+    }
+    return result;
+}
+SG_FN float4 grass_hit(f3 p, f3 d, f3 e) {
+    float4 result = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+    for (uint32_t blade = 0; blade < 9u; ++blade) {
+        float i = (float)blade;
+        float angle = i * 2.39996f;
+        f3 side = {cosf(angle), 0.0f, sinf(angle)};
+        f3 base = side * (e.x * 0.15f) - f3{0.f, e.y, 0.f};
+        f3 tip = side * (e.x * (0.6f + 0.3f * sinf(i * 3.0f))) + f3{0.f, e.y * (0.5f + 0.5f * cosf(i * 1.3f)), 0.f};
+        f3 width = f3{-side.z, 0.0f, side.x} * 0.035f;
+        f3 hit = terrain_triangle(p, d, base - width, base + width, tip);
+        if (hit.z < result.w) {
+            f3 normal = normalize(cross(width, tip - base));
+            if (dot(normal, d) > 0.0f) normal = -normal;
+            result = make_float4(normal.x, normal.y, normal.z, hit.z);
+        }
+    }
+    return result;
+}
+
+// --- terrain quadtree (src/shaders/terrain.wgsl:6-234) -----------------------
+SG_FN float terrain_scale(const Scene &sc) { return clampf(sc.s.globals.ground_up[3] / 200000.0f, 0.0001f, 1.0f); }
+SG_FN float terrain_coordinate(const Scene &sc, uint32_t i) {
+    float u = 2.0f * (float)i / (float)sc.t.cells - 1.0f;
+    return u * fabsf(u) * 16000.0f * terrain_scale(sc);
+}
+SG_FN float terrain_curvature(const Scene &sc, float px, float pz) {
+    float r = sc.s.globals.ground_up[3];
+    float d2 = px * px + pz * pz;
+    return -d2 / (r + sqrtf(fmaxf(r * r - d2, 0.0f)));
+}
+SG_FN f3 terrain_vertex(const Scene &sc, uint32_t x, uint32_t z) {
+    uint64_t i = sc.t.vertex_cache + ((uint64_t)z * sc.t.width + x) * sc.t.stride;
+    if (sc.t.compact) return f3{terrain_coordinate(sc, x), sc.terrain[i], terrain_coordinate(sc, z)};
+    return f3{sc.terrain[i], sc.terrain[i + 1], sc.terrain[i + 2]};
+}
+SG_FN f3 terrain_normal(const Scene &sc, uint32_t x, uint32_t z) {
+    uint64_t i = sc.t.vertex_cache + ((uint64_t)z * sc.t.width + x) * sc.t.stride;
+    if (sc.t.compact) {
+        uint32_t packed = (uint32_t)sc.terrain[i + 1];
+        float nx = (float)(packed & 4095u) / 4095.0f * 2.0f - 1.0f;
+        float nz = (float)(packed >> 12) / 4095.0f * 2.0f - 1.0f;
+        return normalize(f3{nx, sqrtf(fmaxf(1.0f - (nx * nx + nz * nz), 1.0e-6f)), nz});
+    }
+    return f3{sc.terrain[i + 3], sc.terrain[i + 4], sc.terrain[i + 5]};
+}
+// Slab test against a cached node AABB (src/shaders/terrain.wgsl:118-130).
+SG_FN float2 terrain_interval(const Scene &sc, f3 o, f3 inv, uint32_t node) {
+    const float *c = sc.terrain + sc.t.cache + (uint64_t)node * 6u;
+    f3 lo = {c[0], c[1], c[2]}, hi = {c[3], c[4], c[5]};
+    bool px = fabsf(inv.x) >= 1.0e29f, py = fabsf(inv.y) >= 1.0e29f, pz = fabsf(inv.z) >= 1.0e29f;
+    if ((px && (o.x < lo.x || o.x > hi.x)) || (py && (o.y < lo.y || o.y > hi.y)) ||
+        (pz && (o.z < lo.z || o.z > hi.z)))
+        return make_float2(1.0e30f, -1.0e30f);
+    f3 a = (lo - o) * inv, b = (hi - o) * inv;
+    float ex = px ? -1.0e30f : fminf(a.x, b.x), ey = py ? -1.0e30f : fminf(a.y, b.y),
+          ez = pz ? -1.0e30f : fminf(a.z, b.z);
+    float xx = px ? 1.0e30f : fmaxf(a.x, b.x), xy = py ? 1.0e30f : fmaxf(a.y, b.y),
+          xz = pz ? 1.0e30f : fmaxf(a.z, b.z);
+    return make_float2(fmaxf(ex, fmaxf(ey, ez)), fminf(xx, fminf(xy, xz)));
+}
+// Morton de-interleave (src/shaders/terrain.wgsl:134-140).
+SG_FN uint32_t terrain_compact_bits(uint32_t value) {
+    uint32_t x = value & 0x55555555u;
+    x = (x | (x >> 1)) & 0x33333333u;
+    x = (x | (x >> 2)) & 0x0f0f0f0fu;
+    x = (x | (x >> 4)) & 0x00ff00ffu;
+    return (x | (x >> 8)) & 0x0000ffffu;
+}
+SG_FN uint32_t first_leading_bit(uint32_t v) { return 31u - (uint32_t)__builtin_clz(v); }
+
+// Near-to-far stack traversal (src/shaders/terrain.wgsl:152-234). Theme 0
+// shading: the smooth barycentric normal (terrain_shading is identity there).
+SG_FN float4 terrain_query(const Scene &sc, f3 o, f3 d, float initial_limit, bool occlusion) {
+    float4 result = make_float4(0.f, 0.f, 0.f, 1.0e30f);
+    float limit = initial_limit;
+    f3 inv = {(d.x >= 0.f ? 1.f : -1.f) / fmaxf(fabsf(d.x), 1.0e-30f),
+              (d.y >= 0.f ? 1.f : -1.f) / fmaxf(fabsf(d.y), 1.0e-30f),
+              (d.z >= 0.f ? 1.f : -1.f) / fmaxf(fabsf(d.z), 1.0e-30f)};
+    float2 root = terrain_interval(sc, o, inv, 0u);
+    if (root.x > root.y || root.y <= 0.001f || root.x >= limit) return result;
+    uint32_t stack_node[SG_TERRAIN_STACK];
+    float stack_t[SG_TERRAIN_STACK];
+    stack_node[0] = 0u;
+    stack_t[0] = fmaxf(root.x, 0.0f);
+    uint32_t count = 1;
+    const uint32_t cells = sc.t.cells;
+    while (count != 0u) {
+        count -= 1u;
+        uint32_t pending = stack_node[count];
+        if (stack_t[count] >= limit) continue;
+        uint32_t level = first_leading_bit(3u * pending + 1u) / 2u;
+        uint32_t size = cells >> level;
+        uint32_t morton = pending - ((1u << (2u * level)) - 1u) / 3u;
+        uint32_t nx = terrain_compact_bits(morton) * size;
+        uint32_t nz = terrain_compact_bits(morton >> 1) * size;
+        if (size == 2u) {
+            for (uint32_t z = nz; z < nz + 2u; ++z) {
+                for (uint32_t x = nx; x < nx + 2u; ++x) {
+                    f3 a = terrain_vertex(sc, x, z);
+                    f3 b = terrain_vertex(sc, x + 1u, z);
+                    f3 c = terrain_vertex(sc, x, z + 1u);
+                    f3 e = terrain_vertex(sc, x + 1u, z + 1u);
+                    f3 first = terrain_triangle(o, d, a, c, b);
+                    f3 second = terrain_triangle(o, d, e, b, c);
+                    if (first.z < limit) {
+                        if (occlusion) return make_float4(0.f, 0.f, 0.f, first.z);
+                        f3 n = normalize(terrain_normal(sc, x, z) * (1.0f - first.x - first.y) +
+                                         terrain_normal(sc, x, z + 1u) * first.x +
+                                         terrain_normal(sc, x + 1u, z) * first.y);
+                        result = make_float4(n.x, n.y, n.z, first.z);
+                        limit = first.z;
+                    }
+                    if (second.z < limit) {
+                        if (occlusion) return make_float4(0.f, 0.f, 0.f, second.z);
+                        f3 n = normalize(terrain_normal(sc, x + 1u, z + 1u) * (1.0f - second.x - second.y) +
+                                         terrain_normal(sc, x + 1u, z) * second.x +
+                                         terrain_normal(sc, x, z + 1u) * second.y);
+                        result = make_float4(n.x, n.y, n.z, second.z);
+                        limit = second.z;
+                    }
+                }
+            }
+        } else {
+            uint32_t children[4];
+            float distances[4];
+            uint32_t half = size / 2u;
+#pragma unroll
+            for (uint32_t child = 0; child < 4u; ++child) {
+                children[child] = pending * 4u + child + 1u;
+                float2 bounds = terrain_interval(sc, o, inv, children[child]);
+                distances[child] = (bounds.x <= bounds.y && bounds.y > 0.001f && bounds.x < limit)
+                                       ? fmaxf(bounds.x, 0.0f)
+                                       : 1.0e30f;
+            }
+            (void)half;
+            // Insertion sort, descending, so the nearest child is pushed last.
+#pragma unroll
+            for (uint32_t i = 1; i < 4u; ++i) {
+#pragma unroll
+                for (uint32_t j = i; j > 0u; --j) {
+                    if (distances[j - 1u] >= distances[j]) break;
+                    float dt = distances[j - 1u];
+                    uint32_t ct = children[j - 1u];
+                    distances[j - 1u] = distances[j];
+                    children[j - 1u] = children[j];
+                    distances[j] = dt;
+                    children[j] = ct;
+                }
+            }
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i) {
+                if (distances[i] < limit && count < SG_TERRAIN_STACK) {
+                    stack_node[count] = children[i];
+                    stack_t[count] = distances[i];
+                    count += 1u;
+                }
+            }
+        }
+    }
+    return result;
+}
+
+// Highest landscape point (src/shaders/ground.wgsl:244-251).
+SG_FN float landscape_top(const Scene &sc) {
+    float top = __uint_as_float(sc.s.globals.ground_counts[3]);
+    if (sc.s.globals.ground_counts[1] != 0u) top = fmaxf(top, sc.terrain[sc.t.cache + 4u]);
+    return top;
+}
+
+// src/shaders/ground.wgsl:123-242. The host-sphere quadratic runs in FP64.
+SG_FN Hit landscape_query(const Scene &sc, const Ray &ray, float limit, bool occlusion) {
+    Hit closest{limit, SG_MISS, f3s(0.f)};
+    const sg_globals &g = sc.s.globals;
+    if (g.ground_north[3] == 0.0f) return closest;
+    d3 od = ground_origin(sc, ray);
+    f3 d = to_ground(sc, ray.direction);
+    float radius = g.ground_up[3];
+    double rd = radius;
+    if (dot(od, od) > 4.0 * rd * rd) return sphere_hit(sc, ray, (uint32_t)g.ground_north[3] - 1u);
+    f3 o = tof(od);
+    if (d.y >= 0.0f && o.y > landscape_top(sc)) return closest;
+    {
+        d3 dd = tod(d);
+        double a = dot(dd, dd);
+        double b = -rd * dd.y - dot(od, dd);
+        double c = od.x * od.x + od.z * od.z + od.y * (2.0 * rd + od.y);
+        double h = b * b - a * c;
+        if (h >= 0.0) {
+            double root = sqrt(h);
+            double q = b + (b >= 0.0 ? root : -root);
+            if (q != 0.0) {
+                double t0 = fmin(c / q, q / a), t1 = fmax(c / q, q / a);
+                double distance = t0 > 0.001 ? t0 : t1;
+                if (distance > 0.001 && (float)(distance / METRES_PER_AU_D) < closest.distance) {
+                    if (occlusion) return Hit{(float)(distance / METRES_PER_AU_D), SG_LOCAL_BASE, f3s(0.f)};
+                    d3 p = od + dd * distance;
+                    d3 n = normalize(d3{p.x / rd, 1.0 + p.y / rd, p.z / rd});
+                    closest = Hit{(float)(distance / METRES_PER_AU_D), SG_LOCAL_BASE, from_ground(sc, tof(n))};
+                }
+            }
+        }
+    }
+    if (g.ground_counts[1] != 0u) {
+        float tl = fminf(closest.distance, 1.0e18f) * METRES_PER_AU;
+        float4 terrain = terrain_query(sc, o, d, tl, occlusion);
+        if (terrain.w < tl && terrain.w < 1.0e29f) {
+            closest = Hit{terrain.w / METRES_PER_AU, SG_TERRAIN_ID,
+                          from_ground(sc, f3{terrain.x, terrain.y, terrain.z})};
+            if (occlusion) return closest;
+        }
+    }
+    for (uint32_t i = 0; i < g.ground_counts[0]; ++i) {
+        const sg_primitive &object = sc.objects[i];
+        f3 oc = {object.center[0], object.center[1], object.center[2]};
+        f3 e = {object.extent[0], object.extent[1], object.extent[2]};
+        float kind = object.center[3];
+        if (kind < 0.0f) {
+            float2 bounds = box_interval(o - oc, d, e);
+            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance)
+                i += (uint32_t)(-kind);
+            continue;
+        }
+        f3 p = o - oc, v = d;
+        f3 angles = f3s(0.f);
+        if (kind == 2.25f) {
+            angles = rock_angles(object.extent[3]);
+            p = rock_to_local(p, angles);
+            v = rock_to_local(v, angles);
+        } else {
+            p = rotate_north(p, -object.extent[3]);
+            v = rotate_north(v, -object.extent[3]);
+        }
+        float distance = 1.0e30f;
+        f3 n = f3s(0.f);
+        if (kind > 2.5f) {
+            float2 bounds = box_interval(p, v, e);
+            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) continue;
+            float4 veg = kind < 3.5f ? conifer_hit(p, v, e) : (kind < 4.5f ? grass_hit(p, v, e) : trunk_hit(p, v, e));
+            distance = veg.w;
+            n = f3{veg.x, veg.y, veg.z};
+        } else if (kind > 1.5f) {
+            float2 bounds = box_interval(p, v, e);
+            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) continue;
+            float4 rock = rock_hit(p, v, e);
+            distance = rock.w;
+            n = f3{rock.x, rock.y, rock.z};
+        } else if (kind > 0.5f) {
+            f3 sp = p / e, sd = v / e;
+            float aa = dot(sd, sd), bb = dot(sp, sd), cc = dot(sp, sp) - 1.0f;
+            float hh = bb * bb - aa * cc;
+            if (hh >= 0.0f) {
+                float root = sqrtf(hh);
+                distance = (-bb - root) / aa;
+                if (distance <= 0.001f) distance = (-bb + root) / aa;
+                n = normalize((p + v * distance) / (e * e));
+            }
+        } else {
+            float2 interval = box_interval(p, v, e);
+            if (interval.x <= interval.y && interval.y > 0.001f) {
+                distance = interval.x > 0.001f ? interval.x : interval.y;
+                f3 point = (p + v * distance) / e;
+                f3 ap = fabs3(point);
+                if (ap.x >= ap.y && ap.x >= ap.z) n.x = signf(point.x);
+                else if (ap.y >= ap.z) n.y = signf(point.y);
+                else n.z = signf(point.z);
+            }
+        }
+        if (distance > 0.001f && distance < 1.0e29f && distance / METRES_PER_AU < closest.distance) {
+            if (occlusion) return Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, f3s(0.f)};
+            n = kind == 2.25f ? rock_from_local(n, angles) : rotate_north(n, object.extent[3]);
+            closest = Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, from_ground(sc, n)};
+        }
+    }
+    return closest;
+}
+SG_FN Hit landscape_hit(const Scene &sc, const Ray &ray) { return landscape_query(sc, ray, 1.0e30f, false); }
+
+// src/shaders/pathtrace.wgsl:173-185
+SG_FN Hit closest_hit(const Scene &sc, const Ray &ray) {
+    Hit closest = miss_hit();
+    const float host = sc.s.globals.ground_north[3];
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (host == (float)(i + 1u)) continue;
+        Hit hit = sphere_hit(sc, ray, i);
+        if (hit.distance < closest.distance) closest = hit;
+    }
+    if (host != 0.0f) {
+        Hit local = landscape_hit(sc, ray);
+        if (local.distance < closest.distance) closest = local;
+    }
+    return closest;
+}
+
+// --- rings (src/shaders/rings.wgsl) -----------------------------------------
+SG_FN RingHit ring_miss(float limit) { return RingHit{limit, SG_MISS, f3s(0.f)}; }
+SG_FN RingHit ring_hit(const Scene &sc, const Ray &ray, uint32_t index) {
+    const Body &b = body(sc, index);
+    if (b.ring_params.x <= b.ring_plane.w || b.ring_params.y <= 0.0f) return ring_miss(1.0e30f);
+    f3 n = xyz(b.ring_plane);
+    float denom = dot(ray.direction, n);
+    if (fabsf(denom) < 1.0e-7f) return ring_miss(1.0e30f);
+    f3 oc = tof(relative_center(sc, ray, index));
+    float a = dot(ray.direction, ray.direction);
+    f3 perpendicular = cross(ray.direction, cross(ray.direction, oc)) / a;
+    float along = -dot(perpendicular, n) / denom;
+    float distance = dot(oc, ray.direction) / a + along;
+    if (distance <= 0.0f) return ring_miss(1.0e30f);
+    f3 point = perpendicular + ray.direction * along;
+    float radius = length(point) / b.center.w;
+    if (radius < b.ring_plane.w || radius > b.ring_params.x) return ring_miss(1.0e30f);
+    return RingHit{distance, index, point};
+}
+SG_FN RingHit closest_ring(const Scene &sc, const Ray &ray, float limit) {
+    RingHit closest = ring_miss(limit);
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        RingHit hit = ring_hit(sc, ray, i);
+        if (hit.index != SG_MISS && hit.distance < closest.distance) closest = hit;
+    }
+    return closest;
+}
+// Analytic banded profile (src/shaders/rings.wgsl:50-81).
+SG_FN float4 ring_optics(const Scene &sc, uint32_t index, f3 point) {
+    const Body &b = body(sc, index);
+    float radius = length(point) / b.center.w;
+    float x = (radius - b.ring_plane.w) / (b.ring_params.x - b.ring_plane.w);
+    float tau = b.ring_params.y;
+    f3 color = xyz(b.ring_color);
+    if (b.ring_params.z > 0.5f) {
+        float broad = 0.5f + 0.5f * sinf(43.0f * x + 1.8f * sinf(17.0f * x));
+        float fine = 0.5f + 0.5f * sinf(1900.0f * x + 2.0f * sinf(311.0f * x));
+        float micro = 0.5f + 0.5f * sinf(6100.0f * x);
+        float density = 0.12f;
+        f3 tint = {0.70f, 0.68f, 0.65f};
+        if (x >= 0.2792f && x < 0.6910f) {
+            density = 1.25f + 1.1f * broad;
+            tint = f3s(1.0f);
+        } else if (x >= 0.6910f && x < 0.7649f) {
+            density = 0.025f;
+            tint = f3s(0.75f);
+        } else if (x >= 0.7649f) {
+            density = 0.48f + 0.22f * broad;
+            tint = f3{0.92f, 0.94f, 0.98f};
+        }
+        density *= 0.65f + 0.45f * fine + 0.12f * micro;
+        if (fabsf(x - 0.9487f) < 0.00262f || fabsf(x - 0.99565f) < 0.00034f) density = 0.0f;
+        tau *= density;
+        color = color * tint * (0.94f + 0.06f * broad);
+    }
+    return make_float4(color.x, color.y, color.z, tau);
+}
+SG_FN float ring_transmission(const Scene &sc, const Ray &ray, const RingHit &hit) {
+    float mu = fmaxf(fabsf(dot(ray.direction, xyz(body(sc, hit.index).ring_plane))), 1.0e-7f);
+    return expf(-ring_optics(sc, hit.index, hit.point).w / mu);
+}
+SG_FN float one_minus_exp(float x) { return x < 1.0e-3f ? x * (1.0f - 0.5f * x) : 1.0f - expf(-x); }
+// Finite-slab single scattering (src/shaders/rings.wgsl:151-166).
+SG_FN f3 ring_bsdf(float4 optics, float outgoing, float incoming) {
+    float mo = fmaxf(fabsf(outgoing), 1.0e-7f), mi = fmaxf(fabsf(incoming), 1.0e-7f);
+    float a = optics.w / mo, b = optics.w / mi;
+    f3 c = {optics.x, optics.y, optics.z};
+    if (outgoing * incoming < 0.0f) {
+        float delta = fabsf(a - b);
+        float integral = delta < 1.0e-3f ? 1.0f - 0.5f * delta : one_minus_exp(delta) / fmaxf(delta, 1.0e-20f);
+        float factor = optics.w / (mo * mi) * expf(-fminf(a, b)) * integral;
+        return c * (factor / (4.0f * PI));
+    }
+    float factor = one_minus_exp(a + b) / (mo + mi);
+    return c * (factor / (4.0f * PI));
+}
+SG_FN Ray ring_origin(const Scene &sc, const RingHit &hit, f3 direction) {
+    const Body &b = body(sc, hit.index);
+    f3 n = xyz(b.ring_plane);
+    float sgn = dot(direction, n) >= 0.0f ? 1.0f : -1.0f;
+    f3 point = hit.point - n * dot(hit.point, n);
+    f3 o = point + n * (sgn * b.center.w * 2.0e-6f);
+    return Ray{tod(o), direction, hit.index};
+}
+
+// Shadow rays (src/shaders/rings.wgsl:90-115).
+SG_FN float light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
+    Hit surface = sphere_hit(sc, ray, light);
+    if (surface.index == SG_MISS) return 0.0f;
+    const float host = sc.s.globals.ground_north[3];
+    if (host == (float)(light + 1u) && closest_hit(sc, ray).index != light) return 0.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (i == light || host == (float)(i + 1u)) continue;
+        Hit hit = sphere_hit(sc, ray, i);
+        if (hit.index != SG_MISS && (hit.distance < surface.distance || (hit.distance == surface.distance && i < light)))
+            return 0.0f;
+    }
+    if (host != 0.0f) {
+        Hit local = landscape_query(sc, ray, surface.distance, true);
+        if (local.index != SG_MISS && local.distance < surface.distance) return 0.0f;
+    }
+    float transmission = 1.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        RingHit hit = ring_hit(sc, ray, i);
+        if (hit.index != SG_MISS && hit.distance < surface.distance) transmission *= ring_transmission(sc, ray, hit);
+    }
+    return transmission;
+}
+// src/shaders/rings.wgsl:121-141
+SG_FN float air_light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
+    const sg_globals &g = sc.s.globals;
+    if (g.ground_north[3] == 0.0f || (float)ground_origin(sc, ray).y <= landscape_top(sc))
+        return light_visibility(sc, ray, light);
+    Hit surface = sphere_hit(sc, ray, light);
+    if (surface.index == SG_MISS) return 0.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (i == light) continue;
+        Hit hit = sphere_hit(sc, ray, i);
+        if (hit.index != SG_MISS && (hit.distance < surface.distance || (hit.distance == surface.distance && i < light)))
+            return 0.0f;
+    }
+    float transmission = 1.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        RingHit hit = ring_hit(sc, ray, i);
+        if (hit.index != SG_MISS && hit.distance < surface.distance) transmission *= ring_transmission(sc, ray, hit);
+    }
+    return transmission;
+}
+
+}  // namespace sg

@@ -1,0 +1,646 @@
+// Atmosphere, sky cache, landscape materials and the path integrator
+// (src/shaders/pathtrace.wgsl:927-1626, forest.wgsl:1-54, ground.wgsl:256-666).
+#pragma once
+#include "sg_geometry.h"
+
+namespace sg {
+
+// --- ground materials, theme 0 (src/shaders/ground.wgsl:256-281, 597-666) ----
+SG_FN float footprint_of(const Scene &sc, float distance) {
+    return distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
+}
+SG_FN f3 forest_floor(const Scene &sc, f3 p, float distance, f3 normal) {
+    float broad = vnoise(p * 0.045f + f3{7.0f, 0.0f, 13.0f});
+    float middle = vnoise(p * 0.7f);
+    float grain = vnoise(p * 18.0f);
+    float detail = 1.0f - smoothstepf(0.006f, 0.055f, footprint_of(sc, distance));
+    f3 moss = mix3(f3{0.028f, 0.054f, 0.014f}, f3{0.105f, 0.14f, 0.039f}, middle);
+    f3 earth = mix3(f3{0.055f, 0.029f, 0.013f}, f3{0.17f, 0.115f, 0.057f}, middle);
+    f3 color = mix3(earth, moss, smoothstepf(0.35f, 0.63f, broad));
+    float needles = powf(fmaxf(0.0f, sinf(p.x * 92.0f + p.z * 137.0f + middle * 7.0f)), 18.0f) *
+                    smoothstepf(0.50f, 0.68f, grain);
+    color = mix3(color, f3{0.22f, 0.145f, 0.067f}, needles * detail * 0.5f);
+    color = color * (1.0f + (grain - 0.5f) * 0.38f * detail);
+    float slope = 1.0f - fmaxf(normal.y, 0.0f);
+    f3 stone = mix3(f3{0.095f, 0.105f, 0.10f}, f3{0.24f, 0.23f, 0.20f}, middle);
+    color = mix3(color, stone, smoothstepf(0.10f, 0.43f, slope + (broad - 0.5f) * 0.17f));
+    float center = 2.0f + 0.30f * p.z + 2.0f * sinf(p.z * 0.055f);
+    float edge = fabsf(p.x - center) + (middle - 0.5f) * 0.55f;
+    float path = (1.0f - smoothstepf(0.8f, 2.1f, edge)) * (1.0f - smoothstepf(45.0f, 95.0f, p.z)) *
+                 smoothstepf(-10.0f, 0.0f, p.z);
+    f3 gravel = mix3(f3{0.12f, 0.095f, 0.062f}, f3{0.26f, 0.23f, 0.17f}, grain * detail + 0.5f * (1.0f - detail));
+    return mix3(color, gravel, path * 0.85f);
+}
+SG_FN d3 landscape_point(const Scene &sc, const Ray &ray, const Hit &hit) {
+    return ground_origin(sc, ray) + tod(to_ground(sc, ray.direction)) * ((double)hit.distance * METRES_PER_AU_D);
+}
+SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
+    f3 p = tof(landscape_point(sc, ray, hit));
+    float distance = hit.distance * METRES_PER_AU;
+    f3 n = to_ground(sc, hit.normal);
+    if (hit.index == SG_LOCAL_BASE || hit.index == SG_TERRAIN_ID) return forest_floor(sc, p, distance, n);
+    const sg_primitive &object = sc.objects[hit.index - SG_LOCAL_BASE - 1u];
+    f3 tint = {object.albedo[0], object.albedo[1], object.albedo[2]};
+    float kind = object.albedo[3];
+    if (kind == 0.0f) return tint;
+    float middle = vnoise(p * 1.7f);
+    float fine = mixf(vnoise(p * 12.0f), 0.5f, smoothstepf(18.0f, 110.0f, distance));
+    if (kind > 4.5f) return tint * (0.65f + 0.6f * middle);
+    if (kind > 2.5f) {
+        f3 q = rotate_north(p - f3{object.center[0], object.center[1], object.center[2]}, -object.extent[3]) /
+               f3{object.extent[0], object.extent[1], object.extent[2]};
+        if (kind > 3.5f || (q.y < -0.35f && sqrtf(q.x * q.x + q.z * q.z) < 0.09f)) {
+            float bark = vnoise(p * f3{25.0f, 1.5f, 25.0f});
+            return f3{0.13f, 0.073f, 0.032f} * (0.5f + 0.85f * bark);
+        }
+        return tint * (0.7f + 0.55f * middle + 0.2f * fine);
+    }
+    float strata = sinf(p.y * 8.0f + 2.5f * middle);
+    float lichen = smoothstepf(0.52f, 0.76f, middle) * fmaxf(n.y, 0.0f);
+    f3 stone = tint * (0.7f + 0.45f * fine + 0.07f * strata);
+    return mix3(stone, f3{0.11f, 0.145f, 0.037f}, lichen * 0.65f);
+}
+SG_FN f3 landscape_shading_normal(const Scene &sc, const Ray &ray, const Hit &hit) {
+    if (hit.index != SG_LOCAL_BASE && hit.index != SG_TERRAIN_ID) return hit.normal;
+    f3 p = tof(landscape_point(sc, ray, hit));
+    float distance = hit.distance * METRES_PER_AU;
+    float strength = 0.11f * (1.0f - smoothstepf(8.0f, 65.0f, distance));
+    float dx = vnoise((p + f3{0.025f, 0.f, 0.f}) * 6.0f) - vnoise((p - f3{0.025f, 0.f, 0.f}) * 6.0f);
+    float dz = vnoise((p + f3{0.f, 0.f, 0.025f}) * 6.0f) - vnoise((p - f3{0.f, 0.f, 0.025f}) * 6.0f);
+    return normalize(hit.normal + from_ground(sc, f3{-dx, 0.0f, -dz}) * strength);
+}
+// 5 mm outgoing offset with reprojection onto the host sphere, FP64.
+SG_FN Ray landscape_outgoing(const Scene &sc, const Ray &ray, const Hit &hit) {
+    d3 p = landscape_point(sc, ray, hit);
+    if (hit.index == SG_LOCAL_BASE) {
+        double radius = sc.s.globals.ground_up[3];
+        double d2 = p.x * p.x + p.z * p.z;
+        if (p.y > -radius && d2 < radius * radius) p.y = -d2 / (radius + sqrt(radius * radius - d2));
+    }
+    p = p + tod(to_ground(sc, hit.normal)) * 0.005;
+    p.y -= (double)sc.s.globals.ground_east[3];
+    return Ray{from_ground_d(sc, p) / METRES_PER_AU_D, hit.normal, SG_MISS};
+}
+
+// --- host atmosphere (src/shaders/pathtrace.wgsl:927-1234) ------------------
+struct Scatter {
+    f3 transmittance, inscatter;
+};
+SG_FN f3 atmo_center(const Scene &sc) { const float *v = sc.s.globals.atmo_center; return f3{v[0], v[1], v[2]}; }
+SG_FN float host_radius(const Scene &sc) { return sc.s.globals.atmo_center[3]; }
+SG_FN float atmosphere_top(const Scene &sc) { return sc.s.globals.atmo_center[3] + sc.s.globals.atmo_params[2]; }
+SG_FN float atmo_scale_height(const Scene &sc) { return sc.s.globals.atmo_params[1]; }
+SG_FN f3 beta_rayleigh(const Scene &sc) { const float *v = sc.s.globals.atmo_rayleigh; return f3{v[0], v[1], v[2]}; }
+SG_FN float beta_mie(const Scene &sc) { return sc.s.globals.atmo_rayleigh[3]; }
+
+SG_FN float shell_exit(const Scene &sc, f3 origin, f3 dir) {
+    f3 oc = origin - atmo_center(sc);
+    float b = dot(oc, dir);
+    float top = atmosphere_top(sc);
+    float c = dot(oc, oc) - top * top;
+    float disc = b * b - c;
+    if (disc < 0.0f) return 0.0f;
+    return -b + sqrtf(disc);
+}
+SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps) {
+    float distance = shell_exit(sc, origin, dir);
+    if (distance <= 0.0f) return 0.0f;
+    float dt = distance / (float)steps;
+    float inv_h = 1.0f / atmo_scale_height(sc);
+    float r = host_radius(sc);
+    f3 c = atmo_center(sc);
+    float sum = 0.0f;
+    for (uint32_t i = 0; i < steps; ++i) {
+        f3 p = origin + dir * (((float)i + 0.5f) * dt);
+        float h = fmaxf(length(p - c) - r, 0.0f);
+        sum += expf(-h * inv_h);
+    }
+    return sum * dt;
+}
+// FP64 rebasing of an anchored origin to the host centre.
+SG_FN d3 host_offset(const Scene &sc, const Ray &origin) {
+    d3 c = tod(atmo_center(sc));
+    if (origin.anchor == SG_MISS) return origin.offset - c;
+    return (center_d(body(sc, origin.anchor)) - c) + origin.offset;
+}
+SG_FN bool inside_air(const Scene &sc, const Ray &origin) {
+    if (sc.s.globals.atmo_params[3] < 0.5f) return false;
+    f3 oc = tof(host_offset(sc, origin));
+    float top = atmosphere_top(sc);
+    return dot(oc, oc) < top * top;
+}
+SG_FN f3 air_transmittance(const Scene &sc, const Ray &origin, f3 dir) {
+    if (!inside_air(sc, origin)) return f3s(1.0f);
+    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
+    f3 p = atmo_center(sc) + tof(host_offset(sc, origin));
+    return exp3(-beta_ext * density_integral(sc, p, dir, ATMO_SUN_STEPS));
+}
+SG_FN float rayleigh_phase(float c) { return 3.0f / (16.0f * PI) * (1.0f + c * c); }
+SG_FN float mie_phase(float c, float g) {
+    float g2 = g * g;
+    return (1.0f - g2) / (4.0f * PI * powf(1.0f + g2 - 2.0f * g * c, 1.5f));
+}
+
+struct ReflectorSample {
+    uint32_t index;
+    float inverse_probability;
+};
+SG_FN Ray camera_ray(f3 origin, f3 dir) { return Ray{tod(origin), dir, SG_MISS}; }
+
+// src/shaders/pathtrace.wgsl:1021-1043
+SG_FN ReflectorSample sample_reflector(const Scene &sc, f3 origin, uint32_t &rng) {
+    Ray ray = camera_ray(origin, f3{0.f, 0.f, -1.f});
+    uint32_t host = (uint32_t)sc.s.globals.atmo_params[3] - 1u;
+    uint32_t selected = SG_MISS;
+    float selected_weight = 0.0f, total = 0.0f;
+    for (uint32_t b = 0; b < body_count(sc); ++b) {
+        const Body &bd = body(sc, b);
+        if (b == host || bd.material.w > 0.5f) continue;
+        float weight = cone_width(sc, ray, b) * maxc(xyz(bd.material));
+        if (weight <= 0.0f) continue;
+        total += weight;
+        if (rnd(rng) * total < weight) {
+            selected = b;
+            selected_weight = weight;
+        }
+    }
+    return ReflectorSample{selected, total / fmaxf(selected_weight, 1.0e-30f)};
+}
+// src/shaders/pathtrace.wgsl:1048-1071
+SG_FN f3 reflected_radiance(const Scene &sc, const Ray &ray, uint32_t index, uint32_t &rng) {
+    Hit hit = sphere_hit(sc, ray, index);
+    if (hit.index == SG_MISS) return f3s(0.0f);
+    const Body &b = body(sc, index);
+    f3 albedo = surface_albedo(b, hit.normal);
+    Ray outgoing{tod(hit.normal * (b.center.w * (1.0f + 2.0e-6f))), hit.normal, index};
+    f3 radiance = f3s(0.0f);
+    for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+        uint32_t s = light_index(sc, slot);
+        if (body(sc, s).material.w < 0.5f) continue;
+        LightSample light = sample_light(sc, outgoing, s, rng);
+        float cosine = fmaxf(dot(hit.normal, light.direction), 0.0f);
+        if (cosine > 0.0f) {
+            outgoing.direction = light.direction;
+            radiance += albedo * xyz(body(sc, s).material) *
+                        (cosine * light_visibility(sc, outgoing, s) / (PI * light.pdf));
+        }
+    }
+    return radiance;
+}
+// src/shaders/pathtrace.wgsl:1076-1087
+SG_FN float brightest_risen_light(const Scene &sc, f3 origin) {
+    Ray ray = camera_ray(origin, f3{0.f, 0.f, -1.f});
+    float brightest = 0.0f;
+    for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+        uint32_t b = light_index(sc, slot);
+        if (body(sc, b).material.w < 0.5f) continue;
+        if (dot(normalize(tof(relative_center(sc, ray, b))), g_up(sc)) <= 0.0f) continue;
+        brightest = fmaxf(brightest, maxc(xyz(body(sc, b).material)) * TAU * cone_width(sc, ray, b));
+    }
+    return brightest;
+}
+
+// Single scattering along a camera ray (src/shaders/pathtrace.wgsl:1092-1213).
+SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t &rng, uint32_t accumulated) {
+    Scatter result{f3s(1.0f), f3s(0.0f)};
+    if (sc.s.globals.atmo_params[3] < 0.5f) return result;
+    float exit = shell_exit(sc, origin, dir);
+    float t_max = fminf(limit, exit);
+    if (t_max <= 0.0f) return result;
+    f3 beta_r = beta_rayleigh(sc);
+    float beta_m = beta_mie(sc);
+    f3 beta_ext = beta_r + f3s(beta_m);
+    float g = sc.s.globals.atmo_params[0];
+    float r = host_radius(sc);
+    f3 center = atmo_center(sc);
+    float H = atmo_scale_height(sc);
+    uint32_t steps = min(max((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    float dt = t_max / (float)steps;
+    uint32_t jitter_rng = hash_u32(rng ^ 0x3c6ef372u);
+    float jitter_draw = rnd(jitter_rng);
+    float jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jitter_draw : 0.5f;
+
+    uint32_t reflected_rng = hash_u32(rng ^ 0xa511e9b3u);
+    ReflectorSample reflector = sample_reflector(sc, origin, reflected_rng);
+    f3 reflected = f3s(0.0f), reflected_direction = dir, reflected_phase = f3s(0.0f);
+    if (reflector.index != SG_MISS) {
+        Ray reflected_ray = camera_ray(origin, dir);
+        LightSample light = sample_light(sc, reflected_ray, reflector.index, reflected_rng);
+        reflected_ray.direction = light.direction;
+        reflected = reflected_radiance(sc, reflected_ray, reflector.index, reflected_rng) *
+                    (reflector.inverse_probability / light.pdf);
+        reflected_direction = light.direction;
+        float cos_theta = dot(dir, light.direction);
+        reflected_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
+    }
+    if (reflected.x > 0.0f || reflected.y > 0.0f || reflected.z > 0.0f) {
+        float sun = brightest_risen_light(sc, origin);
+        if (sun > 0.0f) {
+            float ratio = maxc(reflected) / sun;
+            float minimum = rsqrtf(clampf((float)accumulated + 1.0f, 1.0f, 4096.0f));
+            float keep = clampf(ratio * 1.0e3f, minimum, 1.0f);
+            reflected = rnd(reflected_rng) < keep ? reflected / keep : f3s(0.0f);
+        }
+    }
+    bool planetshine = reflected.x > 0.0f || reflected.y > 0.0f || reflected.z > 0.0f;
+
+    f3 t_view = f3s(1.0f), inscatter = f3s(0.0f);
+    for (uint32_t i = 0; i < steps; ++i) {
+        f3 p = origin + dir * (((float)i + jitter) * dt);
+        float altitude = fmaxf(length(p - center) - r, 0.0f);
+        float density = expf(-altitude / H);
+        float column = density * dt;
+        f3 tau = beta_ext * column;
+        f3 step_t = exp3(-tau);
+        f3 segment;
+        segment.x = tau.x < 1.0e-3f ? column * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) / fmaxf(beta_ext.x, 1.0e-20f);
+        segment.y = tau.y < 1.0e-3f ? column * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) / fmaxf(beta_ext.y, 1.0e-20f);
+        segment.z = tau.z < 1.0e-3f ? column * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) / fmaxf(beta_ext.z, 1.0e-20f);
+        for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+            uint32_t b = light_index(sc, slot);
+            if (body(sc, b).material.w < 0.5f) continue;
+            Ray light_ray = camera_ray(p, dir);
+            uint32_t light_samples = eclipse_light_samples(sc, light_ray, b, accumulated);
+            for (uint32_t ls = 0; ls < light_samples; ++ls) {
+                LightSample light = sample_light_stratum(sc, light_ray, b, ls, light_samples, rng);
+                light_ray.direction = light.direction;
+                float visibility = air_light_visibility(sc, light_ray, b);
+                if (visibility == 0.0f) continue;
+                f3 e_sun = xyz(body(sc, b).material) * (visibility / ((float)light_samples * light.pdf));
+                f3 sun_t = exp3(-beta_ext * density_integral(sc, p, light.direction, ATMO_SUN_STEPS));
+                float cos_theta = dot(dir, light.direction);
+                f3 beta_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
+                inscatter += t_view * segment * beta_phase * e_sun * sun_t;
+            }
+        }
+        if (planetshine) {
+            float visibility = air_light_visibility(sc, camera_ray(p, reflected_direction), reflector.index);
+            if (visibility > 0.0f) {
+                f3 light_t = exp3(-beta_ext * density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS));
+                inscatter += t_view * segment * reflected_phase * reflected * (light_t * visibility);
+            }
+        }
+        t_view *= step_t;
+    }
+    result.transmittance = t_view;
+    result.inscatter = inscatter;
+    return result;
+}
+// src/shaders/pathtrace.wgsl:1217-1234
+SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t seed) {
+    float exit = shell_exit(sc, origin, dir);
+    float t_max = fminf(limit, exit);
+    if (t_max <= 0.0f) return f3s(1.0f);
+    float H = atmo_scale_height(sc);
+    uint32_t steps = min(max((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    float dt = t_max / (float)steps;
+    uint32_t jitter_rng = hash_u32(seed ^ 0x3c6ef372u);
+    float jitter_draw = rnd(jitter_rng);
+    float jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jitter_draw : 0.5f;
+    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
+    f3 center = atmo_center(sc);
+    float r = host_radius(sc);
+    f3 transmittance = f3s(1.0f);
+    for (uint32_t i = 0; i < steps; ++i) {
+        f3 p = origin + dir * (((float)i + jitter) * dt);
+        float altitude = fmaxf(length(p - center) - r, 0.0f);
+        float column = expf(-altitude / H) * dt;
+        transmittance *= exp3(-beta_ext * column);
+    }
+    return transmittance;
+}
+
+// --- surface sky cache (src/shaders/forest.wgsl:3-54) ------------------------
+SG_FN f3 surface_sky_texel(const Scene &sc, int x, int y) {
+    float4 t = sc.sky_light[(uint32_t)min(max(y, 0), 7) * 16u + (uint32_t)((x + 16) % 16)];
+    return f3{t.x, t.y, t.z};
+}
+SG_FN f3 surface_sky_lookup(const Scene &sc, f3 dir) {
+    f3 p = to_ground(sc, dir);
+    if (p.y <= 0.0f) return f3s(0.0f);
+    float u = fmodf(atan2f(p.z, p.x) / TAU + 1.0f, 1.0f) * 16.0f - 0.5f;
+    float v = p.y * 8.0f - 0.5f;
+    int cx = (int)floorf(u), cy = (int)floorf(v);
+    float fx = fractf(u), fy = fractf(v);
+    return mix3(mix3(surface_sky_texel(sc, cx, cy), surface_sky_texel(sc, cx + 1, cy), fx),
+                mix3(surface_sky_texel(sc, cx, cy + 1), surface_sky_texel(sc, cx + 1, cy + 1), fx), fy);
+}
+SG_FN bool sky_blocked(const Scene &sc, const Ray &ray) {
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (sc.s.globals.ground_north[3] == (float)(i + 1u)) continue;
+        if (sphere_hit(sc, ray, i).index != SG_MISS) return true;
+    }
+    return landscape_query(sc, ray, 1.0e30f, true).index != SG_MISS;
+}
+SG_FN f3 surface_sky(const Scene &sc, const Ray &origin, f3 normal, uint32_t &rng) {
+    float u = rnd(rng);
+    float phi = TAU * rnd(rng);
+    f3 dir = basis_direction(normal, sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), sqrtf(1.0f - u));
+    f3 light = surface_sky_lookup(sc, dir);
+    if (maxc(light) <= 0.0f) return f3s(0.0f);
+    Ray ray{origin.offset, dir, origin.anchor};
+    if (sky_blocked(sc, ray)) return f3s(0.0f);
+    float transmission = 1.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        RingHit hit = ring_hit(sc, ray, i);
+        if (hit.index != SG_MISS) transmission *= ring_transmission(sc, ray, hit);
+    }
+    return light * transmission;
+}
+// One texel of the 16x8 sky cache (src/shaders/forest.wgsl:3-17).
+SG_FN f3 sky_cache_texel(const Scene &sc, uint32_t x, uint32_t y) {
+    f3 light = f3s(0.0f);
+    if (sc.s.globals.atmo_params[3] > 0.5f && sc.s.globals.ground_counts[1] != 0u) {
+        uint32_t index = y * 16u + x;
+        float mu = ((float)y + 0.5f) / 8.0f;
+        float phi = TAU * ((float)x + 0.5f) / 16.0f;
+        float r = sqrtf(1.0f - mu * mu);
+        f3 dir = from_ground(sc, f3{r * cosf(phi), mu, r * sinf(phi)});
+        uint32_t rng = hash_u32(index ^ 0x91ce452du);
+        light = atmosphere(sc, f3s(0.0f), dir, 1.0e30f, rng, sc.s.counts[1]).inscatter;
+    }
+    return light;
+}
+
+// --- ringshine / point discs (src/shaders/pathtrace.wgsl:1237-1345) ---------
+SG_FN float ring_area(const Scene &sc, uint32_t index) {
+    const Body &b = body(sc, index);
+    float r = b.center.w;
+    return PI * r * r * (b.ring_params.x * b.ring_params.x - b.ring_plane.w * b.ring_plane.w);
+}
+SG_FN f3 ringshine(const Scene &sc, uint32_t index, f3 origin, f3 normal, bool last_vertex, uint32_t &rng) {
+    const Body &b = body(sc, index);
+    if (b.ring_params.x <= b.ring_plane.w || b.ring_params.y <= 0.0f) return f3s(0.0f);
+    f3 n = xyz(b.ring_plane);
+    float inner = b.ring_plane.w * b.ring_plane.w;
+    float outer = b.ring_params.x * b.ring_params.x;
+    float radius = sqrtf(mixf(inner, outer, rnd(rng))) * b.center.w;
+    float phi = TAU * rnd(rng);
+    f3 point = basis_direction(n, cosf(phi), sinf(phi), 0.0f) * radius;
+    f3 offset = point - origin;
+    float distance2 = dot(offset, offset);
+    float distance = sqrtf(distance2);
+    f3 direction = offset / distance;
+    float cos_surface = dot(normal, direction);
+    float cos_ring = fabsf(dot(direction, n));
+    if (cos_surface <= 0.0f || cos_ring < 1.0e-6f) return f3s(0.0f);
+    Hit blocker = sphere_hit(sc, Ray{tod(origin), direction, index}, index);
+    if (blocker.index != SG_MISS && blocker.distance < distance) return f3s(0.0f);
+    float4 optics = ring_optics(sc, index, point);
+    if (optics.w == 0.0f) return f3s(0.0f);
+    float mu_out = dot(-direction, n);
+    f3 shine = f3s(0.0f);
+    for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+        uint32_t i = light_index(sc, slot);
+        if (body(sc, i).material.w < 0.5f) continue;
+        LightSample light = sample_light(sc, Ray{tod(point), n, index}, i, rng);
+        float mu_in = dot(light.direction, n);
+        float visibility = light_visibility(sc, ring_origin(sc, RingHit{0.0f, index, point}, light.direction), i);
+        shine += ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) * (fabsf(mu_in) * visibility / light.pdf);
+    }
+    float pdf = distance2 / (ring_area(sc, index) * cos_ring);
+    float weight = last_vertex ? 1.0f : power_weight(pdf, cos_surface / PI);
+    return shine * (cos_surface * weight / (PI * pdf));
+}
+SG_FN float pixel_angle(const Scene &sc) {
+    return 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
+}
+SG_FN Hit point_disc_hit(const Scene &sc, const Ray &ray, const Hit &closest, float &scale, uint32_t &rng) {
+    Hit result = closest;
+    float total = 0.0f;
+    float pixel = pixel_angle(sc);
+    float angle = fminf(fmaxf(POINT_DISC_RADIUS, 0.7f * pixel), STAR_DOT_PIXELS * pixel);
+    uint32_t host = (uint32_t)sc.s.globals.atmo_params[3] - 1u;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (i == host || sc.s.globals.ground_north[3] == (float)(i + 1u)) continue;
+        float radius = body(sc, i).center.w;
+        float drawn = angle * length(tof(relative_center(sc, ray, i)));
+        if (radius <= 0.0f || drawn <= radius) continue;
+        Hit hit = sphere_hit_radius(sc, ray, i, drawn);
+        if (hit.distance < closest.distance) {
+            float ratio = (radius / drawn) * (radius / drawn);
+            total += ratio;
+            if (rnd(rng) * total < ratio) result = hit;
+        }
+    }
+    if (total > 0.0f) scale = total;
+    return result;
+}
+SG_FN float point_gain(const Scene &sc, float extent, float distance) {
+    float pixels = 2.0f * extent / fmaxf(distance, 1.0e-30f) / pixel_angle(sc);
+    return mixf(sc.s.background[3], 1.0f, smoothstepf(1.0f, 3.0f, pixels));
+}
+
+// --- the integrator (src/shaders/pathtrace.wgsl:1347-1617) -----------------
+// One camera path. The camera-air pass of the reference
+// (integrate_camera_air, :1630-1643) is folded in: it traced the same camera
+// ray with the same RNG state, so its in-scatter and continuation RNG are
+// computed here at the first vertex instead of being read from a buffer.
+struct PathResult {
+    float4 value;  // rgb radiance, backdrop luminance
+    f3 backdrop;
+};
+SG_FN PathResult trace_path(const Scene &sc, Ray ray, uint32_t &rng, uint32_t accumulated) {
+    const sg_settings &s = sc.s;
+    const float host_ground = s.globals.ground_north[3];
+    const bool air_on = s.globals.atmo_params[3] >= 0.5f;
+    const uint32_t host = (uint32_t)s.globals.atmo_params[3] - 1u;
+    f3 throughput = f3s(1.0f), radiance = f3s(0.0f), backdrop = f3s(0.0f);
+    float previous_pdf = 0.0f;
+    Ray previous_vertex = ray;
+    uint32_t depth = 0, null_crossings = 0;
+    bool camera_segment = true, in_air = false, sky_on_escape = false, point_gained = false;
+    float ring_cap = 1.0e30f;
+    f3 ring_base = f3s(0.0f);
+    uint32_t shine_body = SG_MISS;
+    const uint32_t max_vertices = s.counts[3];
+    for (;;) {
+        Hit hit = closest_hit(sc, ray);
+        float disc_scale = 1.0f;
+        if (depth == 0u && null_crossings == 0u && s.background[3] > 0.0f)
+            hit = point_disc_hit(sc, ray, hit, disc_scale, rng);
+        RingHit ring = closest_ring(sc, ray, hit.distance);
+        if (in_air) {
+            in_air = false;
+            if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host))
+                throughput *= air_transmittance(sc, ray, ray.direction);
+        }
+        if (camera_segment) {
+            camera_segment = false;
+            float limit = 1.0e30f;
+            if (hit.index != SG_MISS) limit = hit.distance;
+            if (ring.index != SG_MISS) limit = fminf(limit, ring.distance);
+            if (air_on) {
+                f3 origin = tof(ray.offset);
+                uint32_t air_rng = rng;
+                Scatter air = atmosphere(sc, origin, ray.direction, fminf(hit.distance, ring.distance), air_rng,
+                                         accumulated);
+                f3 transmittance = camera_air_transmittance(sc, origin, ray.direction, limit, rng);
+                radiance += throughput * air.inscatter;
+                throughput *= transmittance;
+                rng = air_rng;
+            }
+        }
+        if (ring.index != SG_MISS) {
+            const Body &planet = body(sc, ring.index);
+            if (depth == 0u && !point_gained && s.background[3] > 0.0f) {
+                point_gained = true;
+                throughput *= point_gain(sc, planet.center.w * planet.ring_params.x, ring.distance);
+            }
+            f3 n = xyz(planet.ring_plane);
+            float4 optics = ring_optics(sc, ring.index, ring.point);
+            float mu_out = dot(-ray.direction, n);
+            float transmission = ring_transmission(sc, ray, ring);
+            float scatter_probability = one_minus_exp(optics.w / fmaxf(fabsf(mu_out), 1.0e-7f));
+            float bsdf_pdf = scatter_probability / (4.0f * PI);
+            bool last_vertex = depth + 1u == max_vertices;
+            float ring_mis = 1.0f;
+            if (shine_body == ring.index && null_crossings == 0u) {
+                float cos_ring = fmaxf(fabsf(dot(ray.direction, n)), 1.0e-6f);
+                float shine_pdf = ring.distance * ring.distance / (ring_area(sc, ring.index) * cos_ring);
+                ring_mis = power_weight(previous_pdf, shine_pdf);
+            }
+            shine_body = SG_MISS;
+            float unshadowed = 0.0f, direct = 0.0f;
+            for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+                uint32_t i = light_index(sc, slot);
+                if (body(sc, i).material.w < 0.5f || optics.w == 0.0f) continue;
+                LightSample light = sample_light(sc, Ray{tod(ring.point), n, ring.index}, i, rng);
+                float mu_in = dot(light.direction, n);
+                float visibility = light_visibility(sc, ring_origin(sc, ring, light.direction), i);
+                float weight = last_vertex ? 1.0f : power_weight(light.pdf, bsdf_pdf);
+                f3 lit = throughput * ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) *
+                         (fabsf(mu_in) / light.pdf);
+                radiance += lit * (visibility * weight * ring_mis);
+                unshadowed += luminance(lit);
+                direct += luminance(lit) * visibility;
+            }
+            if (rnd(rng) < transmission) {
+                ray = ring_origin(sc, ring, ray.direction);
+                null_crossings += 1u;
+                if (null_crossings > s.counts[0]) break;
+                continue;
+            }
+            if (last_vertex || scatter_probability == 0.0f) break;
+            if (ring_cap > 1.0e29f) {
+                ring_cap = fmaxf(direct, RING_INDIRECT_CAP * unshadowed);
+                ring_base = radiance;
+            }
+            float z = 1.0f - 2.0f * rnd(rng);
+            float phi = TAU * rnd(rng);
+            float radial = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+            f3 direction = basis_direction(n, radial * cosf(phi), radial * sinf(phi), z);
+            throughput *= ring_bsdf(optics, mu_out, z) * (fabsf(z) / bsdf_pdf);
+            previous_pdf = bsdf_pdf;
+            sky_on_escape = false;
+            ray = ring_origin(sc, ring, direction);
+            previous_vertex = ray;
+            depth += 1u;
+            null_crossings = 0u;
+            if (depth >= 3u) {
+                float survival = clampf(maxc(throughput), 0.05f, 0.95f);
+                if (rnd(rng) >= survival) break;
+                throughput /= survival;
+            }
+            continue;
+        }
+        if (hit.index == SG_MISS) {
+            radiance += throughput * environment(sc, ray.direction, depth == 0u);
+            if (sky_on_escape) radiance += throughput * surface_sky_lookup(sc, ray.direction);
+            if (depth == 0u) {
+                backdrop = milky_way(sc, ray.direction) * throughput;
+                radiance += backdrop;
+            }
+            break;
+        }
+        const bool local = hit.index >= SG_LOCAL_BASE;
+        throughput *= disc_scale;
+        const Body *hb = local ? nullptr : &sc.bodies[hit.index];
+        if (!local && hb->material.w > 0.5f) {
+            float weight = 1.0f;
+            if (depth > 0u) {
+                float light_pdf = 1.0f / (TAU * cone_width(sc, previous_vertex, hit.index));
+                weight = power_weight(previous_pdf, light_pdf);
+            }
+            radiance += throughput * xyz(hb->material) * weight;
+            break;
+        }
+        if (!local && depth == 0u && !point_gained && s.background[3] > 0.0f) {
+            point_gained = true;
+            float extent = hb->center.w * fmaxf(1.0f, hb->ring_params.x);
+            throughput *= point_gain(sc, extent, hit.distance);
+        }
+        f3 normal = local ? landscape_shading_normal(sc, ray, hit) : hit.normal;
+        Ray outgoing = local ? landscape_outgoing(sc, ray, hit)
+                             : Ray{tod(normal * (hb->center.w * (1.0f + 2.0e-6f))), normal, hit.index};
+        f3 direct_light = f3s(0.0f);
+        bool last_vertex = depth + 1u == max_vertices;
+        for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+            uint32_t i = light_index(sc, slot);
+            if (body(sc, i).material.w < 0.5f) continue;
+            LightSample sample = sample_light(sc, outgoing, i, rng);
+            float cosine = fmaxf(dot(normal, sample.direction), 0.0f);
+            if (cosine <= 0.0f) continue;
+            outgoing.direction = sample.direction;
+            float visibility = light_visibility(sc, outgoing, i);
+            if (visibility > 0.0f) {
+                float bsdf_pdf = cosine / PI;
+                float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
+                direct_light += xyz(body(sc, i).material) * air_transmittance(sc, outgoing, sample.direction) *
+                                (bsdf_pdf * weight * visibility / sample.pdf);
+            }
+        }
+        f3 albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
+        radiance += throughput * albedo * direct_light;
+        shine_body = SG_MISS;
+        if (!local) {
+            radiance += throughput * albedo * ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, rng);
+            shine_body = hit.index;
+        }
+        bool sky_lit = local && s.globals.ground_counts[1] != 0u && s.globals.atmo_params[3] > 0.5f;
+        if (sky_lit && last_vertex) radiance += throughput * albedo * surface_sky(sc, outgoing, normal, rng);
+        if (last_vertex) break;
+        float u = rnd(rng);
+        float phi = TAU * rnd(rng);
+        float cosine = sqrtf(1.0f - u);
+        outgoing.direction = basis_direction(normal, sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), cosine);
+        previous_pdf = cosine / PI;
+        throughput *= albedo;
+        if (depth >= 2u) {
+            float survival = clampf(maxc(throughput), 0.05f, 0.95f);
+            if (rnd(rng) >= survival) break;
+            throughput /= survival;
+        }
+        ray = outgoing;
+        previous_vertex = outgoing;
+        in_air = inside_air(sc, outgoing);
+        sky_on_escape = sky_lit;
+        depth += 1u;
+        null_crossings = 0u;
+    }
+    (void)host_ground;
+    if (ring_cap < 1.0e29f) {
+        f3 indirect = radiance - ring_base;
+        float lum = luminance(indirect);
+        if (lum > ring_cap) radiance = ring_base + indirect * (ring_cap / lum);
+    }
+    PathResult out;
+    out.value = make_float4(radiance.x, radiance.y, radiance.z, luminance(backdrop));
+    out.backdrop = backdrop;
+    return out;
+}
+
+// src/shaders/pathtrace.wgsl:1619-1626
+SG_FN f3 camera_direction(const Scene &sc, uint32_t x, uint32_t y, uint32_t &rng) {
+    float jx = rnd(rng);
+    float jy = rnd(rng);
+    float u = ((float)x + jx) / sc.s.globals.viewport[0];
+    float v = ((float)y + jy) / sc.s.globals.viewport[1];
+    float ndc_x = 2.0f * u - 1.0f, ndc_y = 1.0f - 2.0f * v;
+    float tangent = sc.s.globals.cam_forward[3];
+    return normalize(f3{ndc_x * tangent * sc.s.globals.viewport[3], ndc_y * tangent, -1.0f});
+}
+// Path seed for (pixel, sample) (src/shaders/pathtrace.wgsl:1635,1656).
+SG_FN uint32_t path_seed(uint32_t pixel, uint32_t sequence) { return hash_u32(pixel ^ hash_u32(sequence + 0x9e3779b9u)); }
+
+}  // namespace sg

@@ -1,0 +1,784 @@
+// HIP kernels and the C ABI of the MI355X path tracer.
+//
+// Launch structure (per progressive pass, one HIP stream per context):
+//   k_terrain_cache   once per terrain upload / host radius change
+//                     (cache_terrain_geometry, src/shaders/terrain.wgsl:242-285)
+//   k_sky_cache       on a fresh accumulation (cache_surface_sky, forest.wgsl:3-17)
+//   k_trace           every pass: W*H*count path samples, one lane per
+//                     (pixel, sample); lanes of one pixel share a wavefront,
+//                     and the per-pixel mean and running-average blend of
+//                     src/shaders/pathtrace.wgsl:1645-1675 are formed in LDS in
+//                     the reference's summation order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "sg_integrator.h"
+
+using namespace sg;
+
+namespace {
+
+constexpr uint32_t kTraceBlock = 256;
+
+struct TraceArgs {
+    float4 *accumulation;
+    float4 *backdrop;
+    float4 *moments;       // null unless second moments are kept
+    uint32_t width, height;
+    uint32_t count;        // samples this pass (counts.z)
+    uint32_t accumulated;  // samples already in the image (counts.y)
+    uint32_t sequence;     // noise sequence (sampling.x)
+    uint32_t has_air;      // per-sample dispatch semantics
+    uint32_t pixels_per_block;
+};
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
+    __shared__ float4 s_value[kTraceBlock];
+    __shared__ float4 s_backdrop[kTraceBlock];
+    const uint32_t t = threadIdx.x;
+    const uint32_t lp = t / ta.count;
+    const uint32_t s = t - lp * ta.count;
+    const uint32_t pixel = blockIdx.x * ta.pixels_per_block + lp;
+    const uint32_t pixels = ta.width * ta.height;
+    const bool active = lp < ta.pixels_per_block && pixel < pixels;
+    float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 backdrop = f3s(0.f);
+    if (active) {
+        const uint32_t x = pixel % ta.width, y = pixel / ta.width;
+        const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
+        uint32_t rng = path_seed(pixel, ta.sequence + s);
+        f3 direction = camera_direction(sc, x, y, rng);
+        PathResult r = trace_path(sc, Ray{d3{0.0, 0.0, 0.0}, direction, SG_MISS}, rng, accumulated);
+        value = r.value;
+        backdrop = r.backdrop;
+    }
+    s_value[t] = value;
+    s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
+    __syncthreads();
+    if (!active || s != 0u) return;
+    const uint32_t base = lp * ta.count;
+    float4 mean, bmean;
+    float4 m2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!ta.has_air) {
+        // One dispatch of counts.z samples (src/shaders/pathtrace.wgsl:1653-1674).
+        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), bsum = sum, sq = sum;
+        for (uint32_t k = 0; k < ta.count; ++k) {
+            float4 v = s_value[base + k], b = s_backdrop[base + k];
+            sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+            bsum.x += b.x; bsum.y += b.y; bsum.z += b.z;
+            sq.x += v.x * v.x; sq.y += v.y * v.y; sq.z += v.z * v.z;
+        }
+        const float n = (float)ta.count;
+        mean = make_float4(sum.x / n, sum.y / n, sum.z / n, sum.w / n);
+        bmean = make_float4(bsum.x / n, bsum.y / n, bsum.z / n, 0.f);
+        m2 = make_float4(sq.x / n, sq.y / n, sq.z / n, 0.f);
+        const uint32_t new_count = ta.accumulated + ta.count;
+        if (ta.accumulated > 0u) {
+            const float w = n / (float)new_count;
+            float4 old = ta.accumulation[pixel];
+            mean = make_float4(old.x + (mean.x - old.x) * w, old.y + (mean.y - old.y) * w,
+                               old.z + (mean.z - old.z) * w, old.w + (mean.w - old.w) * w);
+            float4 ob = ta.backdrop[pixel];
+            bmean = make_float4(ob.x + (bmean.x - ob.x) * w, ob.y + (bmean.y - ob.y) * w,
+                                ob.z + (bmean.z - ob.z) * w, 0.f);
+            if (ta.moments) {
+                float4 om = ta.moments[pixel];
+                m2 = make_float4(om.x + (m2.x - om.x) * w, om.y + (m2.y - om.y) * w, om.z + (m2.z - om.z) * w, 0.f);
+            }
+        }
+        m2.w = (float)new_count;
+    } else {
+        // One dispatch per sample with counts.z = 1 (src/pathtracer.rs:861-882,971-976).
+        if (ta.accumulated > 0u) {
+            mean = ta.accumulation[pixel];
+            bmean = ta.backdrop[pixel];
+            if (ta.moments) m2 = ta.moments[pixel];
+        }
+        for (uint32_t k = 0; k < ta.count; ++k) {
+            float4 v = s_value[base + k], b = s_backdrop[base + k];
+            float4 v2 = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0.f);
+            const uint32_t old_count = ta.accumulated + k;
+            if (old_count > 0u) {
+                const float w = 1.0f / (float)(old_count + 1u);
+                mean = make_float4(mean.x + (v.x - mean.x) * w, mean.y + (v.y - mean.y) * w,
+                                   mean.z + (v.z - mean.z) * w, mean.w + (v.w - mean.w) * w);
+                bmean = make_float4(bmean.x + (b.x - bmean.x) * w, bmean.y + (b.y - bmean.y) * w,
+                                    bmean.z + (b.z - bmean.z) * w, 0.f);
+                m2 = make_float4(m2.x + (v2.x - m2.x) * w, m2.y + (v2.y - m2.y) * w, m2.z + (v2.z - m2.z) * w, 0.f);
+            } else {
+                mean = v;
+                bmean = make_float4(b.x, b.y, b.z, 0.f);
+                m2 = v2;
+            }
+        }
+        bmean.w = 0.f;
+        m2.w = (float)(ta.accumulated + ta.count);
+    }
+    ta.accumulation[pixel] = mean;
+    ta.backdrop[pixel] = bmean;
+    if (ta.moments) ta.moments[pixel] = m2;
+}
+
+// cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
+__global__ void k_terrain_cache(Scene sc, float *terrain) {
+    const uint64_t index = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const TerrainLayout &t = sc.t;
+    if (index >= t.vertices) return;
+    const float scale = terrain_scale(sc);
+    if (index < t.nodes) {
+        uint32_t node = (uint32_t)index;
+        uint32_t level = first_leading_bit(3u * node + 1u) / 2u;
+        uint32_t size = t.cells >> level;
+        uint32_t morton = node - ((1u << (2u * level)) - 1u) / 3u;
+        uint32_t nx = terrain_compact_bits(morton) * size, nz = terrain_compact_bits(morton >> 1) * size;
+        float lox = terrain_coordinate(sc, nx), loz = terrain_coordinate(sc, nz);
+        float hix = terrain_coordinate(sc, nx + size), hiz = terrain_coordinate(sc, nz + size);
+        float nearx = clampf(0.0f, lox, hix), nearz = clampf(0.0f, loz, hiz);
+        float farx = fmaxf(fabsf(lox), fabsf(hix)), farz = fmaxf(fabsf(loz), fabsf(hiz));
+        float lower = terrain[t.bounds + (uint64_t)node * 2u] * scale + terrain_curvature(sc, farx, farz) - 0.02f;
+        float upper = terrain[t.bounds + (uint64_t)node * 2u + 1u] * scale + terrain_curvature(sc, nearx, nearz) + 0.02f;
+        float *c = terrain + t.cache + (uint64_t)node * 6u;
+        c[0] = lox; c[1] = lower; c[2] = loz;
+        c[3] = hix; c[4] = upper; c[5] = hiz;
+    }
+    const uint32_t x = (uint32_t)(index % t.width), z = (uint32_t)(index / t.width);
+    const float px = terrain_coordinate(sc, x), pz = terrain_coordinate(sc, z);
+    const float vy = terrain[index] * scale + terrain_curvature(sc, px, pz);
+    const float r = sc.s.globals.ground_up[3];
+    const float inv = 1.0f / sqrtf(fmaxf(r * r - (px * px + pz * pz), 1.0f));
+    const float cgx = px * inv, cgz = pz * inv;
+    f3 normal;
+    if (t.compact) {
+        uint32_t lx = x > 0u ? x - 1u : 0u, lz = z > 0u ? z - 1u : 0u;
+        uint32_t hx = min(x + 1u, t.cells), hz = min(z + 1u, t.cells);
+        float dx = (terrain[(uint64_t)z * t.width + hx] - terrain[(uint64_t)z * t.width + lx]) /
+                   ((terrain_coordinate(sc, hx) - terrain_coordinate(sc, lx)) / scale);
+        float dz = (terrain[(uint64_t)hz * t.width + x] - terrain[(uint64_t)lz * t.width + x]) /
+                   ((terrain_coordinate(sc, hz) - terrain_coordinate(sc, lz)) / scale);
+        normal = normalize(f3{-dx + cgx, 1.0f, -dz + cgz});
+    } else {
+        normal = normalize(f3{-terrain[t.vertices + index] + cgx, 1.0f, -terrain[2u * t.vertices + index] + cgz});
+    }
+    float *v = terrain + t.vertex_cache + index * t.stride;
+    if (t.compact) {
+        uint32_t qx = (uint32_t)rintf(clampf(normal.x * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
+        uint32_t qz = (uint32_t)rintf(clampf(normal.z * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
+        v[0] = vy;
+        v[1] = (float)(qx | (qz << 12));
+        return;
+    }
+    v[0] = px; v[1] = vy; v[2] = pz;
+    v[3] = normal.x; v[4] = normal.y; v[5] = normal.z;
+}
+
+// cache_surface_sky (src/shaders/forest.wgsl:3-17): 16x8 texels.
+__global__ void k_sky_cache(Scene sc, float4 *sky) {
+    const uint32_t index = threadIdx.x;
+    if (index >= 128u) return;
+    f3 light = sky_cache_texel(sc, index % 16u, index / 16u);
+    sky[index] = make_float4(light.x, light.y, light.z, 0.f);
+}
+
+// Probe kernel: production helpers on caller-supplied rays.
+__global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint32_t count, sg_probe_result *out) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    const sg_probe_ray pr = rays[i];
+    Ray ray{d3{pr.offset[0], pr.offset[1], pr.offset[2]}, f3{pr.direction[0], pr.direction[1], pr.direction[2]},
+            pr.anchor};
+    sg_probe_result r;
+    memset(&r, 0, sizeof(r));
+    r.index = SG_MISS;
+    r.distance = 1.0e30f;
+    switch (kind) {
+    case SG_PROBE_SPHERE:
+    case SG_PROBE_CLOSEST:
+    case SG_PROBE_LANDSCAPE: {
+        Hit h = kind == SG_PROBE_SPHERE ? sphere_hit(sc, ray, pr.target)
+                                        : (kind == SG_PROBE_CLOSEST ? closest_hit(sc, ray) : landscape_hit(sc, ray));
+        r.distance = h.distance;
+        r.index = h.index;
+        r.normal[0] = h.normal.x; r.normal[1] = h.normal.y; r.normal[2] = h.normal.z;
+        break;
+    }
+    case SG_PROBE_RING: {
+        RingHit h = ring_hit(sc, ray, pr.target);
+        r.distance = h.distance;
+        r.index = h.index;
+        r.extra[0] = h.point.x; r.extra[1] = h.point.y; r.extra[2] = h.point.z;
+        if (h.index != SG_MISS) {
+            r.normal[0] = ring_transmission(sc, ray, h);
+            r.normal[1] = ring_optics(sc, h.index, h.point).w;
+        }
+        break;
+    }
+    case SG_PROBE_VISIBILITY:
+        r.distance = light_visibility(sc, ray, pr.target);
+        r.index = 0;
+        break;
+    case SG_PROBE_TERRAIN: {
+        f3 o = tof(ray.offset);
+        float4 h = terrain_query(sc, o, ray.direction, 1.0e30f, pr.target != 0u);
+        r.distance = h.w;
+        r.index = h.w < 1.0e29f ? SG_TERRAIN_ID : SG_MISS;
+        r.normal[0] = h.x; r.normal[1] = h.y; r.normal[2] = h.z;
+        break;
+    }
+    case SG_PROBE_CONE: {
+        r.distance = cone_width(sc, ray, pr.anchor == SG_MISS ? 0u : pr.anchor);
+        break;
+    }
+    default:
+        break;
+    }
+    out[i] = r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Host library: the PathTracer state machine of src/pathtracer.rs:164-1009.
+// ---------------------------------------------------------------------------
+struct sg_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint32_t width = 0, height = 0;
+    sg_options options{};
+    // History (src/pathtracer.rs:164-188)
+    std::vector<uint8_t> key;
+    uint32_t samples = 0, sequence = 0;
+    uint32_t exposure_idle_frames = 0;
+    bool exposure_was_enabled = false;
+    // Device buffers
+    float4 *accumulation = nullptr, *backdrop = nullptr, *moments = nullptr;
+    bool moments_on = false;
+    Body *bodies = nullptr;
+    sg_primitive *objects = nullptr;
+    uint32_t object_capacity = 0;
+    float4 *sky_light = nullptr;
+    float *terrain = nullptr;
+    uint64_t terrain_floats = 0;
+    uint32_t terrain_cells = 0;
+    bool terrain_cached = false;
+    uint32_t terrain_radius_bits = 0;
+    sg_ray_star *catalogue = nullptr;
+    uint32_t catalogue_count = 0;
+    uint32_t *sky_cells = nullptr;
+    uint32_t *milky_way = nullptr;
+    uint32_t milky_w = 0, milky_h = 0;
+    sg_probe_ray *probe_rays = nullptr;
+    sg_probe_result *probe_results = nullptr;
+    uint32_t probe_capacity = 0;
+    // Measurement
+    bool profiling = false;
+    hipEvent_t ev_start = nullptr, ev_stop = nullptr;
+    double kernel_ms = 0.0;
+    uint64_t launches = 0;
+    std::string error;
+};
+
+namespace {
+
+int fail(sg_ctx *ctx, int code, const std::string &message) {
+    if (ctx) ctx->error = message;
+    return code;
+}
+#define SG_HIP(ctx, call)                                                                        \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return fail((ctx), SG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+uint32_t float_bits(float f) {
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    return u;
+}
+
+int alloc_image(sg_ctx *ctx) {
+    const size_t bytes = std::max<size_t>((size_t)ctx->width * ctx->height * 16u, 16u);
+    if (ctx->accumulation) (void)hipFree(ctx->accumulation);
+    if (ctx->backdrop) (void)hipFree(ctx->backdrop);
+    if (ctx->moments) (void)hipFree(ctx->moments);
+    ctx->accumulation = ctx->backdrop = ctx->moments = nullptr;
+    SG_HIP(ctx, hipMalloc(&ctx->accumulation, bytes));
+    SG_HIP(ctx, hipMalloc(&ctx->backdrop, bytes));
+    SG_HIP(ctx, hipMemsetAsync(ctx->accumulation, 0, bytes, ctx->stream));
+    SG_HIP(ctx, hipMemsetAsync(ctx->backdrop, 0, bytes, ctx->stream));
+    if (ctx->moments_on) {
+        SG_HIP(ctx, hipMalloc(&ctx->moments, bytes));
+        SG_HIP(ctx, hipMemsetAsync(ctx->moments, 0, bytes, ctx->stream));
+    }
+    return SG_OK;
+}
+
+Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
+    Scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.s = settings;
+    sc.bodies = ctx->bodies;
+    sc.objects = ctx->objects;
+    sc.sky_light = ctx->sky_light;
+    sc.terrain = ctx->terrain;
+    sc.catalogue = ctx->catalogue;
+    sc.sky_cells = ctx->sky_cells;
+    sc.catalogue_count = ctx->catalogue_count;
+    sc.milky_way = ctx->milky_way;
+    sc.t = terrain_layout(settings.globals.ground_counts[2]);
+    return sc;
+}
+
+// History::update (src/pathtracer.rs:172-188).
+void history_update(sg_ctx *ctx, const sg_frame *frame) {
+    std::vector<uint8_t> key;
+    const uint8_t *g = reinterpret_cast<const uint8_t *>(frame->globals);
+    key.insert(key.end(), g, g + sizeof(sg_globals));
+    const uint8_t *b = reinterpret_cast<const uint8_t *>(frame->bodies);
+    if (frame->body_count) key.insert(key.end(), b, b + sizeof(sg_sphere) * frame->body_count);
+    const uint8_t *p = reinterpret_cast<const uint8_t *>(frame->ground);
+    if (frame->ground_count) key.insert(key.end(), p, p + sizeof(sg_primitive) * frame->ground_count);
+    const uint8_t *t = reinterpret_cast<const uint8_t *>(&frame->scene_time);
+    key.insert(key.end(), t, t + 8);
+    const uint8_t *mb = reinterpret_cast<const uint8_t *>(&ctx->options.max_bounces);
+    key.insert(key.end(), mb, mb + 4);
+    const uint8_t *mw = reinterpret_cast<const uint8_t *>(&ctx->options.milky_way);
+    key.insert(key.end(), mw, mw + 4);
+    if (key != ctx->key) {
+        ctx->key.swap(key);
+        ctx->samples = 0;
+    }
+}
+
+int validate_frame(sg_ctx *ctx, const sg_frame *frame) {
+    if (!frame || !frame->globals) return fail(ctx, SG_E_INVALID, "frame and frame->globals are required");
+    if (frame->body_count > SG_BODY_CAPACITY) return fail(ctx, SG_E_CAPACITY, "too many scene bodies");
+    if (frame->body_count && !frame->bodies) return fail(ctx, SG_E_INVALID, "bodies pointer is null");
+    if (frame->ground_count && !frame->ground) return fail(ctx, SG_E_INVALID, "ground pointer is null");
+    if (frame->globals->ground_counts[0] != frame->ground_count)
+        return fail(ctx, SG_E_INVALID, "globals.ground_counts[0] must equal the primitive count");
+    for (uint32_t i = 0; i < frame->body_count; ++i) {
+        const float style = frame->bodies[i].axis_x[3];
+        if (frame->bodies[i].emissive < 0.5f && (uint32_t)(style + 0.5f) > 1u)
+            return fail(ctx, SG_E_INVALID, "surface styles 2-6 (cratered, rocky, icy, volcanic, terran) are not built");
+    }
+    const sg_globals *g = frame->globals;
+    if (g->ground_north[3] != 0.0f) {
+        if ((g->ground_counts[2] & 255u) != 0u)
+            return fail(ctx, SG_E_INVALID, "only ground theme 0 (forest) is built");
+        uint32_t host = (uint32_t)g->ground_north[3] - 1u;
+        if (host >= frame->body_count) return fail(ctx, SG_E_INVALID, "ground host index out of range");
+        if (g->ground_counts[1] != 0u && !ctx->terrain)
+            return fail(ctx, SG_E_STATE, "terrain enabled but no terrain uploaded (sg_upload_terrain)");
+    }
+    if (g->atmo_params[3] >= 0.5f && (uint32_t)g->atmo_params[3] - 1u >= frame->body_count)
+        return fail(ctx, SG_E_INVALID, "atmosphere host index out of range");
+    return SG_OK;
+}
+
+int ensure_objects(sg_ctx *ctx, uint32_t count) {
+    if (count <= ctx->object_capacity && ctx->objects) return SG_OK;
+    uint32_t cap = std::max<uint32_t>(SG_GROUND_CAPACITY, count);
+    if (ctx->objects) SG_HIP(ctx, hipFree(ctx->objects));
+    ctx->objects = nullptr;
+    SG_HIP(ctx, hipMalloc(&ctx->objects, (size_t)cap * sizeof(sg_primitive)));
+    ctx->object_capacity = cap;
+    return SG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *catalogue, uint32_t catalogue_count,
+              const uint32_t *sky_cells, uint32_t sky_cell_count, const sg_options *options) {
+    if (!out) return SG_E_INVALID;
+    *out = nullptr;
+    if (width == 0 || height == 0) return SG_E_INVALID;
+    sg_ctx *ctx = new sg_ctx();
+    ctx->width = width;
+    ctx->height = height;
+    if (options) {
+        ctx->options = *options;
+    } else {
+        // Options::default (src/pathtracer.rs:97-112)
+        ctx->options = sg_options{0u, 1u, 8u, SG_MAX_SAMPLES, 1.0f, 0, 0.0f, 0.25f, 0.9f, 4.0e-4f};
+    }
+    auto bail = [&](int code) {
+        sg_destroy(ctx);
+        return code;
+    };
+    if (hipGetDevice(&ctx->device) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess)
+        return bail(SG_E_DEVICE);
+    if (alloc_image(ctx) != SG_OK) return bail(SG_E_DEVICE);
+    if (hipMalloc(&ctx->bodies, SG_BODY_CAPACITY * sizeof(Body)) != hipSuccess) return bail(SG_E_DEVICE);
+    if (ensure_objects(ctx, SG_GROUND_CAPACITY) != SG_OK) return bail(SG_E_DEVICE);
+    if (hipMalloc(&ctx->sky_light, SG_SKY_TEXELS * sizeof(float4)) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipMemsetAsync(ctx->sky_light, 0, SG_SKY_TEXELS * sizeof(float4), ctx->stream) != hipSuccess)
+        return bail(SG_E_DEVICE);
+    if (catalogue_count > 0) {
+        const uint32_t cells = 256u * 128u + 1u;
+        if (!catalogue || !sky_cells || sky_cell_count < cells) return bail(SG_E_INVALID);
+        // Offsets must be monotone, end at the index-list length, and every
+        // index must name a catalogue star, so the miss shader stays in bounds.
+        for (uint32_t i = 0; i + 1 < cells; ++i)
+            if (sky_cells[i] > sky_cells[i + 1]) return bail(SG_E_INVALID);
+        if ((uint64_t)sky_cells[cells - 1] + cells != sky_cell_count) return bail(SG_E_INVALID);
+        for (uint32_t i = cells; i < sky_cell_count; ++i)
+            if (sky_cells[i] >= catalogue_count) return bail(SG_E_INVALID);
+        if (hipMalloc(&ctx->catalogue, catalogue_count * sizeof(sg_ray_star)) != hipSuccess ||
+            hipMalloc(&ctx->sky_cells, sky_cell_count * sizeof(uint32_t)) != hipSuccess)
+            return bail(SG_E_DEVICE);
+        if (hipMemcpy(ctx->catalogue, catalogue, catalogue_count * sizeof(sg_ray_star), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(ctx->sky_cells, sky_cells, sky_cell_count * sizeof(uint32_t), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return bail(SG_E_DEVICE);
+        ctx->catalogue_count = catalogue_count;
+    }
+    *out = ctx;
+    return SG_OK;
+}
+
+int sg_destroy(sg_ctx *ctx) {
+    if (!ctx) return SG_OK;
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
+                       ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
+                       ctx->probe_results};
+    for (void *b : buffers)
+        if (b) (void)hipFree(b);
+    if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
+    if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return SG_OK;
+}
+
+int sg_set_milky_way(sg_ctx *ctx, const uint32_t *texels, uint32_t width, uint32_t height) {
+    if (!ctx || !texels || width == 0 || height == 0) return SG_E_INVALID;
+    if (ctx->milky_way) SG_HIP(ctx, hipFree(ctx->milky_way));
+    ctx->milky_way = nullptr;
+    SG_HIP(ctx, hipMalloc(&ctx->milky_way, (size_t)width * height * 4u));
+    SG_HIP(ctx, hipMemcpy(ctx->milky_way, texels, (size_t)width * height * 4u, hipMemcpyHostToDevice));
+    ctx->milky_w = width;
+    ctx->milky_h = height;
+    return SG_OK;
+}
+
+int sg_resize(sg_ctx *ctx, uint32_t width, uint32_t height) {
+    if (!ctx || width == 0 || height == 0) return SG_E_INVALID;
+    if (width == ctx->width && height == ctx->height) return SG_OK;
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->width = width;
+    ctx->height = height;
+    int rc = alloc_image(ctx);
+    if (rc != SG_OK) return rc;
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->exposure_idle_frames = 0;
+    return SG_OK;
+}
+
+uint32_t sg_samples(const sg_ctx *ctx) { return ctx ? ctx->samples : 0u; }
+
+int sg_size(const sg_ctx *ctx, uint32_t *width, uint32_t *height) {
+    if (!ctx || !width || !height) return SG_E_INVALID;
+    *width = ctx->width;
+    *height = ctx->height;
+    return SG_OK;
+}
+
+int sg_needs_redraw(const sg_ctx *ctx) {
+    if (!ctx) return SG_E_INVALID;
+    return (ctx->samples < ctx->options.sample_limit || (ctx->options.auto_exposure && ctx->exposure_idle_frames < 60))
+               ? 1
+               : 0;
+}
+
+int sg_prepare_frame(sg_ctx *ctx, const sg_frame *frame) {
+    if (!ctx || !frame || !frame->globals) return SG_E_INVALID;
+    history_update(ctx, frame);
+    return ctx->samples == 0 ? 1 : 0;
+}
+
+int sg_set_quality(sg_ctx *ctx, uint32_t samples, uint32_t bounces, uint32_t limit) {
+    if (!ctx) return SG_E_INVALID;
+    bounces = std::min<uint32_t>(std::max<uint32_t>(bounces, 1u), 64u);
+    if (ctx->options.max_bounces != bounces) {
+        ctx->key.clear();
+        ctx->samples = 0;
+        ctx->exposure_idle_frames = 0;
+    }
+    ctx->options.samples_per_frame = std::min<uint32_t>(std::max<uint32_t>(samples, 1u), 64u);
+    ctx->options.max_bounces = bounces;
+    ctx->options.sample_limit = std::min<uint32_t>(std::max<uint32_t>(limit, 1u), SG_MAX_SAMPLES);
+    return SG_OK;
+}
+
+int sg_set_exposure_controls(sg_ctx *ctx, int32_t auto_exposure, float bias) {
+    if (!ctx) return SG_E_INVALID;
+    ctx->options.auto_exposure = auto_exposure ? 1 : 0;
+    ctx->options.ev_bias = bias;
+    ctx->exposure_idle_frames = 0;
+    return SG_OK;
+}
+
+int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint32_t cells) {
+    if (!ctx || !data) return SG_E_INVALID;
+    if (cells < 4u || cells > SG_MAX_TERRAIN_CELLS || (cells & (cells - 1u)) != 0u)
+        return fail(ctx, SG_E_INVALID, "terrain cells must be a power of two in [4, 8192]");
+    uint32_t log2 = 0;
+    while ((1u << log2) < cells) ++log2;
+    TerrainLayout t = terrain_layout(log2 << 8);
+    if (float_count != t.cache) return fail(ctx, SG_E_INVALID, "terrain data size does not match its cell count");
+    const uint64_t total = t.vertex_cache + (uint64_t)t.stride * t.vertices;
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (ctx->terrain && ctx->terrain_floats != total) {
+        SG_HIP(ctx, hipFree(ctx->terrain));
+        ctx->terrain = nullptr;
+    }
+    if (!ctx->terrain) SG_HIP(ctx, hipMalloc(&ctx->terrain, total * sizeof(float)));
+    SG_HIP(ctx, hipMemcpy(ctx->terrain, data, float_count * sizeof(float), hipMemcpyHostToDevice));
+    ctx->terrain_floats = total;
+    ctx->terrain_cells = cells;
+    ctx->terrain_cached = false;
+    return SG_OK;
+}
+
+int sg_enable_moments(sg_ctx *ctx, int32_t enable) {
+    if (!ctx) return SG_E_INVALID;
+    bool on = enable != 0;
+    if (on == ctx->moments_on) return SG_OK;
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->moments_on = on;
+    ctx->key.clear();
+    ctx->samples = 0;
+    return alloc_image(ctx);
+}
+
+// PathTracer::encode (src/pathtracer.rs:789-1009).
+int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
+    if (!ctx) return SG_E_INVALID;
+    int rc = validate_frame(ctx, frame);
+    if (rc != SG_OK) return rc;
+    if (frame->ground_count > 0) {
+        rc = ensure_objects(ctx, frame->ground_count);
+        if (rc != SG_OK) return rc;
+    }
+    history_update(ctx, frame);
+    const sg_options &o = ctx->options;
+    uint32_t remaining = o.sample_limit > ctx->samples ? o.sample_limit - ctx->samples : 0u;
+    uint32_t count = std::min(std::min(o.samples_per_frame, SG_MAX_BATCH_SAMPLES), remaining);
+    sg_settings st;
+    std::memset(&st, 0, sizeof(st));
+    st.globals = *frame->globals;
+    sg_globals &g = st.globals;
+    g.viewport[3] = g.viewport[0] / std::max(g.viewport[1], 1.0f);
+    g.viewport[0] = (float)ctx->width;
+    g.viewport[1] = (float)ctx->height;
+    g.viewport[2] = o.exposure;
+    float prop_top = 0.0f;
+    for (uint32_t i = 0; i < frame->ground_count; ++i) {
+        const sg_primitive &p = frame->ground[i];
+        float e = std::sqrt(p.extent[0] * p.extent[0] + p.extent[1] * p.extent[1] + p.extent[2] * p.extent[2]);
+        prop_top = std::max(prop_top, p.center[1] + e);
+    }
+    g.ground_counts[3] = float_bits(prop_top);
+    uint32_t cells = ctx->terrain_cells ? ctx->terrain_cells : 512u;
+    uint32_t log2 = 0;
+    while ((1u << log2) < cells) ++log2;
+    g.ground_counts[2] |= log2 << 8;
+    uint32_t lights[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < frame->body_count; ++i) {
+        if (frame->bodies[i].emissive >= 0.5f) {
+            if (lights[3] < 3u) lights[lights[3]] = i;
+            lights[3] += 1u;
+        }
+    }
+    std::memcpy(st.lights, lights, sizeof(lights));
+    st.sampling[0] = o.noise_seed + ctx->sequence;
+    st.counts[0] = frame->body_count;
+    st.counts[1] = ctx->samples;
+    st.counts[2] = count;
+    st.counts[3] = o.max_bounces;
+    st.background[0] = o.milky_way;
+    st.background[1] = ctx->milky_way ? (float)ctx->milky_w : 1024.0f;
+    st.background[2] = ctx->milky_way ? (float)ctx->milky_h : 512.0f;
+    st.background[3] = 100.0f;  // stars::POINT_SOURCE_GAIN (src/stars.rs:130)
+    const bool has_air = frame->globals->atmo_params[3] >= 0.5f;
+    if (count > 0) {
+        if (frame->body_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        if (frame->ground_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        Scene sc = make_scene(ctx, st);
+        const bool terrain_on = g.ground_counts[1] != 0u && g.ground_north[3] != 0.0f;
+        const uint32_t radius_bits = float_bits(g.ground_up[3]);
+        if (terrain_on && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {
+            const uint64_t n = sc.t.vertices;
+            hipLaunchKernelGGL(k_terrain_cache, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, sc,
+                               ctx->terrain);
+            SG_HIP(ctx, hipGetLastError());
+            ctx->terrain_cached = true;
+            ctx->terrain_radius_bits = radius_bits;
+        }
+        if (ctx->samples == 0 && has_air && terrain_on) {
+            hipLaunchKernelGGL(k_sky_cache, dim3(1), dim3(128), 0, ctx->stream, sc, ctx->sky_light);
+            SG_HIP(ctx, hipGetLastError());
+        }
+        TraceArgs ta;
+        ta.accumulation = ctx->accumulation;
+        ta.backdrop = ctx->backdrop;
+        ta.moments = ctx->moments_on ? ctx->moments : nullptr;
+        ta.width = ctx->width;
+        ta.height = ctx->height;
+        ta.count = count;
+        ta.accumulated = ctx->samples;
+        ta.sequence = st.sampling[0];
+        ta.has_air = has_air ? 1u : 0u;
+        ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
+        const uint32_t block = ta.pixels_per_block * count;
+        const uint64_t pixels = (uint64_t)ctx->width * ctx->height;
+        const uint32_t grid = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);
+        if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+        hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, sc, ta);
+        SG_HIP(ctx, hipGetLastError());
+        if (ctx->profiling) {
+            SG_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+            SG_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+            float ms = 0.f;
+            SG_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+            ctx->kernel_ms += ms;
+            ctx->launches += 1;
+        }
+    }
+    ctx->samples += count;
+    ctx->sequence += count;
+    ctx->exposure_idle_frames = count > 0 ? 0u : std::min<uint32_t>(ctx->exposure_idle_frames + 1u, 0xffffffffu);
+    ctx->exposure_was_enabled = o.auto_exposure != 0;
+    return SG_OK;
+}
+
+static int read_image(sg_ctx *ctx, const float4 *src, float *dst, uint64_t pixel_count) {
+    if (!ctx || !dst) return SG_E_INVALID;
+    if (!src) return fail(ctx, SG_E_STATE, "buffer not allocated");
+    if (pixel_count != (uint64_t)ctx->width * ctx->height)
+        return fail(ctx, SG_E_INVALID, "pixel count does not match the image size");
+    SG_HIP(ctx, hipMemcpyAsync(dst, src, pixel_count * 16u, hipMemcpyDeviceToHost, ctx->stream));
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+int sg_read_accumulation(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->accumulation : nullptr, rgba, n); }
+int sg_read_backdrop(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->backdrop : nullptr, rgba, n); }
+int sg_read_moments(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->moments : nullptr, rgba, n); }
+
+int sg_synchronize(sg_ctx *ctx) {
+    if (!ctx) return SG_E_INVALID;
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+int sg_set_profiling(sg_ctx *ctx, int32_t enable) {
+    if (!ctx) return SG_E_INVALID;
+    ctx->profiling = enable != 0;
+    ctx->kernel_ms = 0.0;
+    ctx->launches = 0;
+    return SG_OK;
+}
+
+int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
+    if (!ctx || !milliseconds || !launches) return SG_E_INVALID;
+    *milliseconds = ctx->kernel_ms;
+    *launches = ctx->launches;
+    return SG_OK;
+}
+
+const char *sg_last_error(const sg_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
+             sg_probe_result *results) {
+    if (!ctx || !frame || !frame->globals || (count && (!rays || !results))) return SG_E_INVALID;
+    if (kind > SG_PROBE_CONE) return fail(ctx, SG_E_INVALID, "unknown probe kind");
+    if (frame->body_count > SG_BODY_CAPACITY) return fail(ctx, SG_E_CAPACITY, "too many scene bodies");
+    if (count == 0) return SG_OK;
+    for (uint32_t i = 0; i < count; ++i) {
+        uint32_t a = rays[i].anchor;
+        if (a != SG_MISS && a >= frame->body_count) return fail(ctx, SG_E_INVALID, "probe anchor out of range");
+        if ((kind == SG_PROBE_SPHERE || kind == SG_PROBE_RING || kind == SG_PROBE_VISIBILITY) &&
+            rays[i].target >= frame->body_count)
+            return fail(ctx, SG_E_INVALID, "probe target out of range");
+    }
+    if ((kind == SG_PROBE_TERRAIN || frame->globals->ground_counts[1] != 0u) && !ctx->terrain)
+        return fail(ctx, SG_E_STATE, "terrain probe without uploaded terrain");
+    if (frame->ground_count) {
+        int rc = ensure_objects(ctx, frame->ground_count);
+        if (rc != SG_OK) return rc;
+    }
+    if (count > ctx->probe_capacity) {
+        if (ctx->probe_rays) SG_HIP(ctx, hipFree(ctx->probe_rays));
+        if (ctx->probe_results) SG_HIP(ctx, hipFree(ctx->probe_results));
+        ctx->probe_rays = nullptr;
+        ctx->probe_results = nullptr;
+        SG_HIP(ctx, hipMalloc(&ctx->probe_rays, (size_t)count * sizeof(sg_probe_ray)));
+        SG_HIP(ctx, hipMalloc(&ctx->probe_results, (size_t)count * sizeof(sg_probe_result)));
+        ctx->probe_capacity = count;
+    }
+    sg_settings st;
+    std::memset(&st, 0, sizeof(st));
+    st.globals = *frame->globals;
+    float prop_top = 0.0f;
+    for (uint32_t i = 0; i < frame->ground_count; ++i) {
+        const sg_primitive &p = frame->ground[i];
+        float e = std::sqrt(p.extent[0] * p.extent[0] + p.extent[1] * p.extent[1] + p.extent[2] * p.extent[2]);
+        prop_top = std::max(prop_top, p.center[1] + e);
+    }
+    st.globals.ground_counts[3] = float_bits(prop_top);
+    uint32_t cells = ctx->terrain_cells ? ctx->terrain_cells : 512u;
+    uint32_t log2 = 0;
+    while ((1u << log2) < cells) ++log2;
+    st.globals.ground_counts[2] = (st.globals.ground_counts[2] & 255u) | (log2 << 8);
+    st.counts[0] = frame->body_count;
+    uint32_t lights[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < frame->body_count; ++i)
+        if (frame->bodies[i].emissive >= 0.5f) {
+            if (lights[3] < 3u) lights[lights[3]] = i;
+            lights[3] += 1u;
+        }
+    std::memcpy(st.lights, lights, sizeof(lights));
+    if (frame->body_count)
+        SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    if (frame->ground_count)
+        SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
+                                   hipMemcpyHostToDevice, ctx->stream));
+    Scene sc = make_scene(ctx, st);
+    const uint32_t radius_bits = float_bits(st.globals.ground_up[3]);
+    if (ctx->terrain && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {
+        hipLaunchKernelGGL(k_terrain_cache, dim3((unsigned)((sc.t.vertices + 255) / 256)), dim3(256), 0, ctx->stream,
+                           sc, ctx->terrain);
+        SG_HIP(ctx, hipGetLastError());
+        ctx->terrain_cached = true;
+        ctx->terrain_radius_bits = radius_bits;
+    }
+    SG_HIP(ctx, hipMemcpyAsync(ctx->probe_rays, rays, (size_t)count * sizeof(sg_probe_ray), hipMemcpyHostToDevice,
+                               ctx->stream));
+    hipLaunchKernelGGL(k_probe, dim3((count + 63) / 64), dim3(64), 0, ctx->stream, sc, kind, ctx->probe_rays, count,
+                       ctx->probe_results);
+    SG_HIP(ctx, hipGetLastError());
+    SG_HIP(ctx, hipMemcpyAsync(results, ctx->probe_results, (size_t)count * sizeof(sg_probe_result),
+                               hipMemcpyDeviceToHost, ctx->stream));
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+}  // extern "C"

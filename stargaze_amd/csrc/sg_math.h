@@ -1,0 +1,57 @@
+// Small vector algebra shared by the HIP kernels (device) and the host
+// library (host). Plain structs keep the generated code free of any
+// vector-type overload ambiguity.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SG_FN __host__ __device__ __forceinline__
+
+template <class T> struct V3 { T x, y, z; };
+typedef V3<float> f3;
+typedef V3<double> d3;
+
+template <class T> SG_FN V3<T> v3(T x, T y, T z) { return V3<T>{x, y, z}; }
+SG_FN f3 f3s(float s) { return f3{s, s, s}; }
+template <class T> SG_FN V3<T> operator+(V3<T> a, V3<T> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <class T> SG_FN V3<T> operator-(V3<T> a, V3<T> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <class T> SG_FN V3<T> operator-(V3<T> a) { return {-a.x, -a.y, -a.z}; }
+template <class T> SG_FN V3<T> operator*(V3<T> a, V3<T> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <class T> SG_FN V3<T> operator/(V3<T> a, V3<T> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <class T> SG_FN V3<T> operator*(V3<T> a, T s) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> SG_FN V3<T> operator*(T s, V3<T> a) { return {a.x * s, a.y * s, a.z * s}; }
+template <class T> SG_FN V3<T> operator/(V3<T> a, T s) { return {a.x / s, a.y / s, a.z / s}; }
+template <class T> SG_FN V3<T> &operator+=(V3<T> &a, V3<T> b) { a = a + b; return a; }
+template <class T> SG_FN V3<T> &operator*=(V3<T> &a, V3<T> b) { a = a * b; return a; }
+template <class T> SG_FN V3<T> &operator*=(V3<T> &a, T s) { a = a * s; return a; }
+template <class T> SG_FN V3<T> &operator/=(V3<T> &a, T s) { a = a / s; return a; }
+template <class T> SG_FN T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <class T> SG_FN V3<T> cross(V3<T> a, V3<T> b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+SG_FN float sqrtT(float x) { return sqrtf(x); }
+SG_FN double sqrtT(double x) { return sqrt(x); }
+SG_FN float length(f3 a) { return sqrtf(dot(a, a)); }
+SG_FN double length(d3 a) { return sqrt(dot(a, a)); }
+SG_FN f3 normalize(f3 a) { return a * (1.0f / sqrtf(dot(a, a))); }
+SG_FN d3 normalize(d3 a) { return a * (1.0 / sqrt(dot(a, a))); }
+SG_FN f3 tof(d3 a) { return {(float)a.x, (float)a.y, (float)a.z}; }
+SG_FN d3 tod(f3 a) { return {(double)a.x, (double)a.y, (double)a.z}; }
+SG_FN f3 xyz(float4 a) { return {a.x, a.y, a.z}; }
+SG_FN f3 fabs3(f3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+SG_FN f3 fmin3(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
+SG_FN f3 fmax3(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
+SG_FN f3 exp3(f3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
+SG_FN float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
+SG_FN float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+SG_FN f3 clamp3(f3 a, float lo, float hi) { return {clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)}; }
+// WGSL mix/smoothstep/fract semantics.
+SG_FN float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+SG_FN f3 mix3(f3 a, f3 b, float t) { return a * (1.0f - t) + b * t; }
+SG_FN float smoothstepf(float e0, float e1, float x) {
+    float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+    return t * t * (3.0f - 2.0f * t);
+}
+SG_FN float fractf(float x) { return x - floorf(x); }
+SG_FN float luminance(f3 c) { return dot(c, f3{0.2126f, 0.7152f, 0.0722f}); }
+SG_FN float signf(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }

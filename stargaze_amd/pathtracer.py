@@ -1,0 +1,190 @@
+"""Host-side mirror of the reference's ``PathTracer`` (src/pathtracer.rs:190-1015).
+
+Same method names, argument meaning and error behaviour (errors raise where
+the reference panics). All work runs in the HIP library through the C ABI;
+this module only packs arrays and calls it.
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _native
+from .layouts import GLOBALS, OPTIONS, PRIMITIVE, PROBE_RAY, PROBE_RESULT, RAY_STAR, SPHERE, MAX_SAMPLES
+
+
+@dataclass
+class Options:
+    """``Options`` (src/pathtracer.rs:76-112), with the reference defaults."""
+
+    noise_seed: int = 0
+    samples_per_frame: int = 1
+    max_bounces: int = 8
+    sample_limit: int = MAX_SAMPLES
+    exposure: float = 1.0
+    auto_exposure: bool = False
+    ev_bias: float = 0.0
+    auto_key: float = 0.25
+    auto_percentile: float = 0.9
+    milky_way: float = 4.0e-4
+
+    def pack(self) -> np.ndarray:
+        o = np.zeros((), OPTIONS)
+        for name in OPTIONS.names:
+            o[name] = getattr(self, name)
+        return o
+
+
+@dataclass
+class Frame:
+    """``Frame`` (src/renderer.rs:50-56) without UI labels."""
+
+    globals: np.ndarray  # GLOBALS record
+    bodies: np.ndarray = field(default_factory=lambda: np.zeros(0, SPHERE))
+    ground: np.ndarray = field(default_factory=lambda: np.zeros(0, PRIMITIVE))
+    scene_time: float = 0.0
+
+    def native(self) -> tuple[_native.SgFrame, tuple]:
+        g = np.ascontiguousarray(np.asarray(self.globals, GLOBALS).reshape(()))
+        b = np.ascontiguousarray(np.asarray(self.bodies, SPHERE).reshape(-1))
+        p = np.ascontiguousarray(np.asarray(self.ground, PRIMITIVE).reshape(-1))
+        f = _native.SgFrame(
+            g.ctypes.data,
+            b.ctypes.data if b.size else None,
+            b.size,
+            p.ctypes.data if p.size else None,
+            p.size,
+            float(self.scene_time),
+        )
+        return f, (g, b, p)  # keep the arrays alive for the call
+
+
+class PathTracer:
+    """One progressive accumulation on the current HIP device."""
+
+    def __init__(
+        self,
+        dimensions: tuple[int, int],
+        catalogue: np.ndarray | None = None,
+        sky_cells: np.ndarray | None = None,
+        options: Options | None = None,
+    ):
+        self._lib = _native.load()
+        self.options = options or Options()
+        ctx = ctypes.c_void_p()
+        cat = np.ascontiguousarray(catalogue if catalogue is not None else np.zeros(0, RAY_STAR), RAY_STAR)
+        cells = np.ascontiguousarray(sky_cells if sky_cells is not None else np.zeros(0, np.uint32), np.uint32)
+        opt = self.options.pack()
+        rc = self._lib.sg_create(
+            ctypes.byref(ctx),
+            int(dimensions[0]),
+            int(dimensions[1]),
+            cat.ctypes.data if cat.size else None,
+            cat.size,
+            cells.ctypes.data if cells.size else None,
+            cells.size,
+            opt.ctypes.data,
+        )
+        _native.check(self._lib, None, rc, "sg_create")
+        self._ctx = ctx
+
+    # -- lifetime ---------------------------------------------------------
+    def close(self) -> None:
+        if getattr(self, "_ctx", None):
+            self._lib.sg_destroy(self._ctx)
+            self._ctx = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> int:
+        return _native.check(self._lib, self._ctx, rc, what)
+
+    # -- reference API ----------------------------------------------------
+    def resize(self, dimensions: tuple[int, int]) -> None:
+        self._check(self._lib.sg_resize(self._ctx, int(dimensions[0]), int(dimensions[1])), "resize")
+
+    def samples(self) -> int:
+        return int(self._lib.sg_samples(self._ctx))
+
+    def size(self) -> tuple[int, int]:
+        w, h = ctypes.c_uint32(), ctypes.c_uint32()
+        self._check(self._lib.sg_size(self._ctx, ctypes.byref(w), ctypes.byref(h)), "size")
+        return int(w.value), int(h.value)
+
+    def needs_redraw(self) -> bool:
+        return bool(self._check(self._lib.sg_needs_redraw(self._ctx), "needs_redraw"))
+
+    def prepare_frame(self, frame: Frame) -> bool:
+        f, keep = frame.native()
+        return bool(self._check(self._lib.sg_prepare_frame(self._ctx, ctypes.byref(f)), "prepare_frame"))
+
+    def set_quality(self, samples: int, bounces: int, limit: int) -> None:
+        self._check(self._lib.sg_set_quality(self._ctx, samples, bounces, limit), "set_quality")
+        self.options.samples_per_frame = max(1, min(64, samples))
+        self.options.max_bounces = max(1, min(64, bounces))
+        self.options.sample_limit = max(1, min(MAX_SAMPLES, limit))
+
+    def set_exposure_controls(self, auto: bool, bias: float) -> None:
+        self._check(self._lib.sg_set_exposure_controls(self._ctx, int(auto), float(bias)), "set_exposure_controls")
+
+    def encode(self, frame: Frame) -> None:
+        f, keep = frame.native()
+        self._check(self._lib.sg_encode(self._ctx, ctypes.byref(f)), "encode")
+
+    # -- data -------------------------------------------------------------
+    def upload_terrain(self, data: np.ndarray, cells: int) -> None:
+        d = np.ascontiguousarray(data, np.float32)
+        self._check(self._lib.sg_upload_terrain(self._ctx, d.ctypes.data, d.size, int(cells)), "upload_terrain")
+
+    def set_milky_way(self, texels: np.ndarray) -> None:
+        t = np.ascontiguousarray(texels, np.uint32)
+        self._check(self._lib.sg_set_milky_way(self._ctx, t.ctypes.data, t.shape[1], t.shape[0]), "set_milky_way")
+
+    def enable_moments(self, enable: bool = True) -> None:
+        self._check(self._lib.sg_enable_moments(self._ctx, int(enable)), "enable_moments")
+
+    def _read(self, fn, what: str) -> np.ndarray:
+        w, h = self.size()
+        out = np.empty((h, w, 4), np.float32)
+        self._check(fn(self._ctx, out.ctypes.data, w * h), what)
+        return out
+
+    def read_accumulation(self) -> np.ndarray:
+        """Linear HDR mean per pixel, (H, W, 4) f32 (rgb, backdrop luminance)."""
+        return self._read(self._lib.sg_read_accumulation, "read_accumulation")
+
+    def read_backdrop(self) -> np.ndarray:
+        return self._read(self._lib.sg_read_backdrop, "read_backdrop")
+
+    def read_moments(self) -> np.ndarray:
+        return self._read(self._lib.sg_read_moments, "read_moments")
+
+    def synchronize(self) -> None:
+        self._check(self._lib.sg_synchronize(self._ctx), "synchronize")
+
+    def set_profiling(self, enable: bool) -> None:
+        self._check(self._lib.sg_set_profiling(self._ctx, int(enable)), "set_profiling")
+
+    def kernel_time(self) -> tuple[float, int]:
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        self._check(self._lib.sg_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "kernel_time")
+        return float(ms.value), int(n.value)
+
+    def probe(self, frame: Frame, kind: int, rays: np.ndarray) -> np.ndarray:
+        """Run production device helpers on ``rays`` (PROBE_RAY records)."""
+        r = np.ascontiguousarray(rays, PROBE_RAY).reshape(-1)
+        out = np.zeros(r.size, PROBE_RESULT)
+        f, keep = frame.native()
+        self._check(
+            self._lib.sg_probe(self._ctx, ctypes.byref(f), int(kind), r.ctypes.data, r.size, out.ctypes.data), "probe"
+        )
+        return out
+
+
+PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE = range(7)

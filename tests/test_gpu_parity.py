@@ -1,0 +1,182 @@
+"""HIP path against the oracle on MI355X: per-primitive and per-image parity."""
+import math
+
+import numpy as np
+import pytest
+
+import cases
+import oracle
+import parity
+from stargaze_amd import pathtracer as ptm
+from stargaze_amd import scenes
+from stargaze_amd.layouts import MISS, PROBE_RAY
+from stargaze_amd.pathtracer import Options, PathTracer
+
+pytestmark = pytest.mark.gpu
+
+
+def make_gpu(dimensions, options, catalogue=None, sky_cells=None):
+    return PathTracer(dimensions, catalogue=catalogue, sky_cells=sky_cells, options=options)
+
+
+@pytest.mark.parametrize("case", cases.ALL, ids=lambda c: c.__name__)
+def test_reference_case_on_gpu(case):
+    print(f"case {case.__name__}", flush=True)
+    case(make_gpu)
+
+
+def _rays_at_bodies(frame, rng, n_per_target, anchors):
+    """Rays from each anchor (camera or a body surface) aimed inside each
+    other body's disc (within 0.9 of its angular radius)."""
+    b = frame.bodies
+    centres = b["center"].astype(np.float64) + b["center_low"][:, :3].astype(np.float64)
+    radii = b["radius"].astype(np.float64)
+    rays = []
+    for anchor in anchors:
+        for target in range(len(b)):
+            if anchor == target:
+                continue
+            for _ in range(n_per_target):
+                if anchor == MISS:
+                    origin = np.zeros(3)
+                    offset = np.zeros(3)
+                else:
+                    nrm = rng.normal(size=3)
+                    nrm /= np.linalg.norm(nrm)
+                    offset = nrm * radii[anchor] * (1.0 + 2.0e-6)
+                    origin = centres[anchor] + offset
+                to = centres[target] - origin
+                dist = np.linalg.norm(to)
+                axis = to / dist
+                sin_max = min(radii[target] / dist, 1.0)
+                t1 = np.cross(axis, [0.0, 0.0, 1.0] if abs(axis[2]) < 0.9 else [0.0, 1.0, 0.0])
+                t1 /= np.linalg.norm(t1)
+                t2 = np.cross(axis, t1)
+                r = 0.9 * sin_max * math.sqrt(rng.uniform())
+                phi = rng.uniform(0, 2 * math.pi)
+                d = axis + r * (math.cos(phi) * t1 + math.sin(phi) * t2)
+                d /= np.linalg.norm(d)
+                ray = np.zeros((), PROBE_RAY)
+                ray["offset"] = offset
+                ray["anchor"] = anchor
+                ray["direction"] = d
+                ray["target"] = target
+                rays.append(ray)
+    return np.array(rays, PROBE_RAY)
+
+
+@pytest.mark.parametrize("scene", ["airless_triple", "ringed_giant"])
+def test_sphere_hits_match_fp64_oracle(scene):
+    """Per-primitive hit distances within 1e-6 relative of the FP64 oracle."""
+    frame = getattr(scenes, scene)(64, 64)
+    rng = np.random.default_rng(7)
+    anchors = [MISS] + [i for i in range(len(frame.bodies)) if frame.bodies["emissive"][i] < 0.5]
+    rays = _rays_at_bodies(frame, rng, 64, anchors)
+    gpu = PathTracer((4, 4), options=Options(milky_way=0.0)).probe(frame, ptm.PROBE_SPHERE, rays)
+    ref = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64").probe(frame, ptm.PROBE_SPHERE, rays)
+    assert np.all(ref["index"] != MISS), "probe rays must hit their targets"
+    assert np.array_equal(gpu["index"], ref["index"].astype(np.uint32))
+    rel = np.abs(gpu["distance"].astype(np.float64) - ref["distance"]) / ref["distance"]
+    print(f"{scene}: {rays.size} rays, max rel {rel.max():.3e}")
+    assert rel.max() <= 1e-6, rel.max()
+    dot = np.einsum("ij,ij->i", gpu["normal"].astype(np.float64), ref["normal"])
+    assert np.all(dot > 1 - 1e-6)
+
+
+def test_ring_hits_match_fp64_oracle():
+    frame = scenes.ringed_giant(64, 64)
+    rng = np.random.default_rng(3)
+    giant = 1
+    b = frame.bodies[giant]
+    centre = b["center"].astype(np.float64) + b["center_low"][:3].astype(np.float64)
+    pole = b["ring_plane"][:3].astype(np.float64)
+    e1 = np.cross(pole, [1.0, 0.0, 0.0])
+    e1 /= np.linalg.norm(e1)
+    e2 = np.cross(pole, e1)
+    rays = np.zeros(512, PROBE_RAY)
+    for i in range(rays.size):
+        rr = b["radius"] * rng.uniform(b["ring_plane"][3] + 0.01, b["ring_params"][0] - 0.01)
+        phi = rng.uniform(0, 2 * math.pi)
+        target = centre + rr * (math.cos(phi) * e1 + math.sin(phi) * e2)
+        d = target / np.linalg.norm(target)
+        rays[i]["anchor"] = MISS
+        rays[i]["direction"] = d
+        rays[i]["target"] = giant
+    gpu = PathTracer((4, 4), options=Options(milky_way=0.0)).probe(frame, ptm.PROBE_RING, rays)
+    ref = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64").probe(frame, ptm.PROBE_RING, rays)
+    hit = ref["index"] != MISS
+    assert hit.mean() > 0.95
+    assert np.array_equal(gpu["index"][hit], ref["index"][hit].astype(np.uint32))
+    rel = np.abs(gpu["distance"][hit] - ref["distance"][hit]) / ref["distance"][hit]
+    print(f"ring: max rel distance {rel.max():.3e}")
+    assert rel.max() <= 1e-6
+    # Transmission through the banded profile: same optical depth to 1e-4
+    # (the f32 profile's sin(6100 x) terms amplify radius rounding).
+    t_rel = np.abs(gpu["normal"][hit, 0] - ref["normal"][hit, 0]) / np.maximum(ref["normal"][hit, 0], 1e-12)
+    assert np.median(t_rel) < 1e-4
+
+
+def _render_pair(frame, size, spp, bounces, batches):
+    opts = Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0)
+    gpu = PathTracer(size, options=opts)
+    gpu.enable_moments(True)
+    orc = oracle.OracleTracer(size, Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0), "f32")
+    orc.enable_moments(True)
+    for _ in range(batches):
+        gpu.encode(frame)
+        orc.encode(frame)
+    return (gpu.read_accumulation(), gpu.read_moments()), (orc.read_accumulation(), orc.read_moments())
+
+
+@pytest.mark.parametrize(
+    "scene,size,spp,bounces,batches",
+    [
+        ("airless_triple", (64, 64), 16, 4, 2),
+        ("ringed_giant", (96, 54), 16, 6, 2),
+    ],
+)
+def test_images_match_oracle_per_tile(scene, size, spp, bounces, batches):
+    frame = getattr(scenes, scene)(*size)
+    (ga, gm), (oa, om) = _render_pair(frame, size, spp, bounces, batches)
+    assert np.all(np.isfinite(ga)) and np.all(ga[..., :3] >= 0)
+    agree = parity.pixel_agreement(ga, oa)
+    z = parity.tile_z(ga, gm, oa, om)
+    over, worst = parity.assert_tiles(z)
+    print(f"{scene}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
+    assert agree >= 0.9, agree
+
+
+def test_batch_size_invariance_with_air():
+    """src/motion_noise_tests.rs:326-363: with air on, 1 spp x N and a batch
+    of N give bitwise-identical accumulations."""
+    f = cases.air_frame(cases.earthlike())
+    f.bodies = np.concatenate([f.bodies, scenes.bodies(scenes.body([1.0, 0.0, -1.0], 0.005, [100000.0] * 3, True))])
+    f.globals["viewport"] = [16, 9, 1, 0]
+    f.globals["cam_forward"][3] = 0.5
+    single = PathTracer((16, 9), options=Options(max_bounces=1, sample_limit=19, milky_way=0.0))
+    batched = PathTracer((16, 9), options=Options(max_bounces=1, sample_limit=19, samples_per_frame=64, milky_way=0.0))
+    a = cases.sample(single, f, 19)
+    b = cases.sample(batched, f, 2)
+    assert single.samples() == 19 and batched.samples() == 19
+    assert np.array_equal(a, b)
+
+
+def test_seed_determinism_and_reset():
+    """src/pathtracer_tests.rs:199-237: same seed, same image; other seed or a
+    reset gives a different noise sequence."""
+    f = scenes.frame()
+    f.globals["viewport"] = [32, 32, 1, 1]
+    f.globals["cam_forward"][3] = 0.5
+    f.bodies = scenes.bodies(scenes.body([0, 0, -3], 1.0, [1.0] * 3, True), scenes.body([0, 2, -3], 1.0, [0.5] * 3, False))
+
+    def make(seed):
+        return PathTracer((32, 32), options=Options(noise_seed=seed, milky_way=0.0))
+
+    t = make(0)
+    first = cases.sample(t, f, 1)
+    f.scene_time = 1.0
+    reset = cases.sample(t, f, 1)
+    assert t.samples() == 1 and not np.array_equal(first, reset)
+    f.scene_time = 0.0
+    assert np.array_equal(first, cases.sample(make(0), f, 1))
+    assert not np.array_equal(first, cases.sample(make(12345), f, 1))

@@ -194,6 +194,14 @@ int sg_synchronize(sg_ctx *ctx);
  * the context stream), for measurement. */
 int sg_set_profiling(sg_ctx *ctx, int32_t enable);
 int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
+/* Integrator selection (no reference counterpart: the reference always runs
+ * its megakernel). SG_INTEGRATOR_WAVEFRONT (default) runs extend / camera-air
+ * / shade kernels over compacted queues of at most `chunk_paths` live paths;
+ * SG_INTEGRATOR_FUSED runs one lane per path through the whole loop. Both
+ * give bit-identical images. `chunk_paths` 0 keeps the current value. */
+#define SG_INTEGRATOR_FUSED 1u
+#define SG_INTEGRATOR_WAVEFRONT 2u
+int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths);
 const char *sg_last_error(const sg_ctx *ctx);
 /* Run production device helpers on a batch of probe rays (see sg_probe_kind). */
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind,

@@ -36,6 +36,7 @@ EXPORTS = (
     "sg_synchronize",
     "sg_set_profiling",
     "sg_kernel_time",
+    "sg_set_integrator",
     "sg_last_error",
     "sg_probe",
 )
@@ -93,6 +94,7 @@ def load() -> ctypes.CDLL:
         "sg_synchronize": ([P], I32),
         "sg_set_profiling": ([P, I32], I32),
         "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
+        "sg_set_integrator": ([P, U32, U32], I32),
         "sg_last_error": ([P], ctypes.c_char_p),
         "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
     }

@@ -432,202 +432,259 @@ SG_FN float point_gain(const Scene &sc, float extent, float distance) {
 }
 
 // --- the integrator (src/shaders/pathtrace.wgsl:1347-1617) -----------------
-// One camera path. The camera-air pass of the reference
-// (integrate_camera_air, :1630-1643) is folded in: it traced the same camera
-// ray with the same RNG state, so its in-scatter and continuation RNG are
-// computed here at the first vertex instead of being read from a buffer.
+// trace() is split at its loop boundaries into steps that carry the whole
+// path in PathState:
+//   path_begin       initial state for one camera sample
+//   path_extend      closest hit, depth-0 point discs (consumes draws), closest ring
+//   path_camera_air  first segment only: the camera-air pass folded in
+//                    (integrate_camera_air, :1630-1643, traced the same camera
+//                    ray with the same RNG state, so its in-scatter and its
+//                    continuation RNG are computed here instead of read back)
+//   path_shade       the rest of one loop iteration; false when the path ends
+//   path_finish      ring indirect cap and the returned vec4
+// The fused kernel loops the steps per lane; the wavefront kernels run each
+// step over a compacted queue. Both consume draws in the reference's order.
+enum : uint32_t {
+    PATH_CAMERA_SEGMENT = 1u << 0,
+    PATH_IN_AIR = 1u << 1,
+    PATH_SKY_ON_ESCAPE = 1u << 2,
+    PATH_POINT_GAINED = 1u << 3,
+};
+struct PathState {
+    Ray ray;
+    Ray previous_vertex;
+    f3 throughput, radiance, ring_base, backdrop;
+    float previous_pdf, ring_cap;
+    uint32_t rng, depth, null_crossings, shine_body, flags, accumulated;
+};
+struct HitRecord {
+    Hit hit;
+    float disc_scale;
+    RingHit ring;
+};
 struct PathResult {
     float4 value;  // rgb radiance, backdrop luminance
     f3 backdrop;
 };
-SG_FN PathResult trace_path(const Scene &sc, Ray ray, uint32_t &rng, uint32_t accumulated) {
+
+SG_FN void path_begin(PathState &st, f3 direction, uint32_t rng, uint32_t accumulated) {
+    st.ray = Ray{d3{0.0, 0.0, 0.0}, direction, SG_MISS};
+    st.previous_vertex = st.ray;
+    st.throughput = f3s(1.0f);
+    st.radiance = f3s(0.0f);
+    st.ring_base = f3s(0.0f);
+    st.backdrop = f3s(0.0f);
+    st.previous_pdf = 0.0f;
+    st.ring_cap = 1.0e30f;
+    st.rng = rng;
+    st.depth = 0u;
+    st.null_crossings = 0u;
+    st.shine_body = SG_MISS;
+    st.flags = PATH_CAMERA_SEGMENT;
+    st.accumulated = accumulated;
+}
+
+SG_FN HitRecord path_extend(const Scene &sc, PathState &st) {
+    HitRecord h;
+    h.hit = closest_hit(sc, st.ray);
+    h.disc_scale = 1.0f;
+    if (st.depth == 0u && st.null_crossings == 0u && sc.s.background[3] > 0.0f)
+        h.hit = point_disc_hit(sc, st.ray, h.hit, h.disc_scale, st.rng);
+    h.ring = closest_ring(sc, st.ray, h.hit.distance);
+    return h;
+}
+
+SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h) {
+    if (!(st.flags & PATH_CAMERA_SEGMENT)) return;
+    st.flags &= ~PATH_CAMERA_SEGMENT;
+    if (sc.s.globals.atmo_params[3] < 0.5f) return;
+    float limit = 1.0e30f;
+    if (h.hit.index != SG_MISS) limit = h.hit.distance;
+    if (h.ring.index != SG_MISS) limit = fminf(limit, h.ring.distance);
+    f3 origin = tof(st.ray.offset);
+    uint32_t air_rng = st.rng;
+    Scatter air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
+                             st.accumulated);
+    f3 transmittance = camera_air_transmittance(sc, origin, st.ray.direction, limit, st.rng);
+    st.radiance += st.throughput * air.inscatter;
+    st.throughput *= transmittance;
+    st.rng = air_rng;
+}
+
+SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
     const sg_settings &s = sc.s;
-    const float host_ground = s.globals.ground_north[3];
-    const bool air_on = s.globals.atmo_params[3] >= 0.5f;
     const uint32_t host = (uint32_t)s.globals.atmo_params[3] - 1u;
-    f3 throughput = f3s(1.0f), radiance = f3s(0.0f), backdrop = f3s(0.0f);
-    float previous_pdf = 0.0f;
-    Ray previous_vertex = ray;
-    uint32_t depth = 0, null_crossings = 0;
-    bool camera_segment = true, in_air = false, sky_on_escape = false, point_gained = false;
-    float ring_cap = 1.0e30f;
-    f3 ring_base = f3s(0.0f);
-    uint32_t shine_body = SG_MISS;
     const uint32_t max_vertices = s.counts[3];
-    for (;;) {
-        Hit hit = closest_hit(sc, ray);
-        float disc_scale = 1.0f;
-        if (depth == 0u && null_crossings == 0u && s.background[3] > 0.0f)
-            hit = point_disc_hit(sc, ray, hit, disc_scale, rng);
-        RingHit ring = closest_ring(sc, ray, hit.distance);
-        if (in_air) {
-            in_air = false;
-            if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host))
-                throughput *= air_transmittance(sc, ray, ray.direction);
+    const Hit &hit = h.hit;
+    const RingHit &ring = h.ring;
+    Ray &ray = st.ray;
+    if (st.flags & PATH_IN_AIR) {
+        st.flags &= ~PATH_IN_AIR;
+        if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host))
+            st.throughput *= air_transmittance(sc, ray, ray.direction);
+    }
+    if (ring.index != SG_MISS) {
+        const Body &planet = body(sc, ring.index);
+        if (st.depth == 0u && !(st.flags & PATH_POINT_GAINED) && s.background[3] > 0.0f) {
+            st.flags |= PATH_POINT_GAINED;
+            st.throughput *= point_gain(sc, planet.center.w * planet.ring_params.x, ring.distance);
         }
-        if (camera_segment) {
-            camera_segment = false;
-            float limit = 1.0e30f;
-            if (hit.index != SG_MISS) limit = hit.distance;
-            if (ring.index != SG_MISS) limit = fminf(limit, ring.distance);
-            if (air_on) {
-                f3 origin = tof(ray.offset);
-                uint32_t air_rng = rng;
-                Scatter air = atmosphere(sc, origin, ray.direction, fminf(hit.distance, ring.distance), air_rng,
-                                         accumulated);
-                f3 transmittance = camera_air_transmittance(sc, origin, ray.direction, limit, rng);
-                radiance += throughput * air.inscatter;
-                throughput *= transmittance;
-                rng = air_rng;
-            }
+        f3 n = xyz(planet.ring_plane);
+        float4 optics = ring_optics(sc, ring.index, ring.point);
+        float mu_out = dot(-ray.direction, n);
+        float transmission = ring_transmission(sc, ray, ring);
+        float scatter_probability = one_minus_exp(optics.w / fmaxf(fabsf(mu_out), 1.0e-7f));
+        float bsdf_pdf = scatter_probability / (4.0f * PI);
+        bool last_vertex = st.depth + 1u == max_vertices;
+        float ring_mis = 1.0f;
+        if (st.shine_body == ring.index && st.null_crossings == 0u) {
+            float cos_ring = fmaxf(fabsf(dot(ray.direction, n)), 1.0e-6f);
+            float shine_pdf = ring.distance * ring.distance / (ring_area(sc, ring.index) * cos_ring);
+            ring_mis = power_weight(st.previous_pdf, shine_pdf);
         }
-        if (ring.index != SG_MISS) {
-            const Body &planet = body(sc, ring.index);
-            if (depth == 0u && !point_gained && s.background[3] > 0.0f) {
-                point_gained = true;
-                throughput *= point_gain(sc, planet.center.w * planet.ring_params.x, ring.distance);
-            }
-            f3 n = xyz(planet.ring_plane);
-            float4 optics = ring_optics(sc, ring.index, ring.point);
-            float mu_out = dot(-ray.direction, n);
-            float transmission = ring_transmission(sc, ray, ring);
-            float scatter_probability = one_minus_exp(optics.w / fmaxf(fabsf(mu_out), 1.0e-7f));
-            float bsdf_pdf = scatter_probability / (4.0f * PI);
-            bool last_vertex = depth + 1u == max_vertices;
-            float ring_mis = 1.0f;
-            if (shine_body == ring.index && null_crossings == 0u) {
-                float cos_ring = fmaxf(fabsf(dot(ray.direction, n)), 1.0e-6f);
-                float shine_pdf = ring.distance * ring.distance / (ring_area(sc, ring.index) * cos_ring);
-                ring_mis = power_weight(previous_pdf, shine_pdf);
-            }
-            shine_body = SG_MISS;
-            float unshadowed = 0.0f, direct = 0.0f;
-            for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
-                uint32_t i = light_index(sc, slot);
-                if (body(sc, i).material.w < 0.5f || optics.w == 0.0f) continue;
-                LightSample light = sample_light(sc, Ray{tod(ring.point), n, ring.index}, i, rng);
-                float mu_in = dot(light.direction, n);
-                float visibility = light_visibility(sc, ring_origin(sc, ring, light.direction), i);
-                float weight = last_vertex ? 1.0f : power_weight(light.pdf, bsdf_pdf);
-                f3 lit = throughput * ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) *
-                         (fabsf(mu_in) / light.pdf);
-                radiance += lit * (visibility * weight * ring_mis);
-                unshadowed += luminance(lit);
-                direct += luminance(lit) * visibility;
-            }
-            if (rnd(rng) < transmission) {
-                ray = ring_origin(sc, ring, ray.direction);
-                null_crossings += 1u;
-                if (null_crossings > s.counts[0]) break;
-                continue;
-            }
-            if (last_vertex || scatter_probability == 0.0f) break;
-            if (ring_cap > 1.0e29f) {
-                ring_cap = fmaxf(direct, RING_INDIRECT_CAP * unshadowed);
-                ring_base = radiance;
-            }
-            float z = 1.0f - 2.0f * rnd(rng);
-            float phi = TAU * rnd(rng);
-            float radial = sqrtf(fmaxf(0.0f, 1.0f - z * z));
-            f3 direction = basis_direction(n, radial * cosf(phi), radial * sinf(phi), z);
-            throughput *= ring_bsdf(optics, mu_out, z) * (fabsf(z) / bsdf_pdf);
-            previous_pdf = bsdf_pdf;
-            sky_on_escape = false;
-            ray = ring_origin(sc, ring, direction);
-            previous_vertex = ray;
-            depth += 1u;
-            null_crossings = 0u;
-            if (depth >= 3u) {
-                float survival = clampf(maxc(throughput), 0.05f, 0.95f);
-                if (rnd(rng) >= survival) break;
-                throughput /= survival;
-            }
-            continue;
-        }
-        if (hit.index == SG_MISS) {
-            radiance += throughput * environment(sc, ray.direction, depth == 0u);
-            if (sky_on_escape) radiance += throughput * surface_sky_lookup(sc, ray.direction);
-            if (depth == 0u) {
-                backdrop = milky_way(sc, ray.direction) * throughput;
-                radiance += backdrop;
-            }
-            break;
-        }
-        const bool local = hit.index >= SG_LOCAL_BASE;
-        throughput *= disc_scale;
-        const Body *hb = local ? nullptr : &sc.bodies[hit.index];
-        if (!local && hb->material.w > 0.5f) {
-            float weight = 1.0f;
-            if (depth > 0u) {
-                float light_pdf = 1.0f / (TAU * cone_width(sc, previous_vertex, hit.index));
-                weight = power_weight(previous_pdf, light_pdf);
-            }
-            radiance += throughput * xyz(hb->material) * weight;
-            break;
-        }
-        if (!local && depth == 0u && !point_gained && s.background[3] > 0.0f) {
-            point_gained = true;
-            float extent = hb->center.w * fmaxf(1.0f, hb->ring_params.x);
-            throughput *= point_gain(sc, extent, hit.distance);
-        }
-        f3 normal = local ? landscape_shading_normal(sc, ray, hit) : hit.normal;
-        Ray outgoing = local ? landscape_outgoing(sc, ray, hit)
-                             : Ray{tod(normal * (hb->center.w * (1.0f + 2.0e-6f))), normal, hit.index};
-        f3 direct_light = f3s(0.0f);
-        bool last_vertex = depth + 1u == max_vertices;
+        st.shine_body = SG_MISS;
+        float unshadowed = 0.0f, direct = 0.0f;
         for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
             uint32_t i = light_index(sc, slot);
-            if (body(sc, i).material.w < 0.5f) continue;
-            LightSample sample = sample_light(sc, outgoing, i, rng);
-            float cosine = fmaxf(dot(normal, sample.direction), 0.0f);
-            if (cosine <= 0.0f) continue;
-            outgoing.direction = sample.direction;
-            float visibility = light_visibility(sc, outgoing, i);
-            if (visibility > 0.0f) {
-                float bsdf_pdf = cosine / PI;
-                float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
-                direct_light += xyz(body(sc, i).material) * air_transmittance(sc, outgoing, sample.direction) *
-                                (bsdf_pdf * weight * visibility / sample.pdf);
-            }
+            if (body(sc, i).material.w < 0.5f || optics.w == 0.0f) continue;
+            LightSample light = sample_light(sc, Ray{tod(ring.point), n, ring.index}, i, st.rng);
+            float mu_in = dot(light.direction, n);
+            float visibility = light_visibility(sc, ring_origin(sc, ring, light.direction), i);
+            float weight = last_vertex ? 1.0f : power_weight(light.pdf, bsdf_pdf);
+            f3 lit = st.throughput * ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) *
+                     (fabsf(mu_in) / light.pdf);
+            st.radiance += lit * (visibility * weight * ring_mis);
+            unshadowed += luminance(lit);
+            direct += luminance(lit) * visibility;
         }
-        f3 albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
-        radiance += throughput * albedo * direct_light;
-        shine_body = SG_MISS;
-        if (!local) {
-            radiance += throughput * albedo * ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, rng);
-            shine_body = hit.index;
+        if (rnd(st.rng) < transmission) {
+            ray = ring_origin(sc, ring, ray.direction);
+            st.null_crossings += 1u;
+            return st.null_crossings <= s.counts[0];
         }
-        bool sky_lit = local && s.globals.ground_counts[1] != 0u && s.globals.atmo_params[3] > 0.5f;
-        if (sky_lit && last_vertex) radiance += throughput * albedo * surface_sky(sc, outgoing, normal, rng);
-        if (last_vertex) break;
-        float u = rnd(rng);
-        float phi = TAU * rnd(rng);
-        float cosine = sqrtf(1.0f - u);
-        outgoing.direction = basis_direction(normal, sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), cosine);
-        previous_pdf = cosine / PI;
-        throughput *= albedo;
-        if (depth >= 2u) {
-            float survival = clampf(maxc(throughput), 0.05f, 0.95f);
-            if (rnd(rng) >= survival) break;
-            throughput /= survival;
+        if (last_vertex || scatter_probability == 0.0f) return false;
+        if (st.ring_cap > 1.0e29f) {
+            st.ring_cap = fmaxf(direct, RING_INDIRECT_CAP * unshadowed);
+            st.ring_base = st.radiance;
         }
-        ray = outgoing;
-        previous_vertex = outgoing;
-        in_air = inside_air(sc, outgoing);
-        sky_on_escape = sky_lit;
-        depth += 1u;
-        null_crossings = 0u;
+        float z = 1.0f - 2.0f * rnd(st.rng);
+        float phi = TAU * rnd(st.rng);
+        float radial = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+        f3 direction = basis_direction(n, radial * cosf(phi), radial * sinf(phi), z);
+        st.throughput *= ring_bsdf(optics, mu_out, z) * (fabsf(z) / bsdf_pdf);
+        st.previous_pdf = bsdf_pdf;
+        st.flags &= ~PATH_SKY_ON_ESCAPE;
+        ray = ring_origin(sc, ring, direction);
+        st.previous_vertex = ray;
+        st.depth += 1u;
+        st.null_crossings = 0u;
+        if (st.depth >= 3u) {
+            float survival = clampf(maxc(st.throughput), 0.05f, 0.95f);
+            if (rnd(st.rng) >= survival) return false;
+            st.throughput /= survival;
+        }
+        return true;
     }
-    (void)host_ground;
-    if (ring_cap < 1.0e29f) {
-        f3 indirect = radiance - ring_base;
+    if (hit.index == SG_MISS) {
+        st.radiance += st.throughput * environment(sc, ray.direction, st.depth == 0u);
+        if (st.flags & PATH_SKY_ON_ESCAPE) st.radiance += st.throughput * surface_sky_lookup(sc, ray.direction);
+        if (st.depth == 0u) {
+            st.backdrop = milky_way(sc, ray.direction) * st.throughput;
+            st.radiance += st.backdrop;
+        }
+        return false;
+    }
+    const bool local = hit.index >= SG_LOCAL_BASE;
+    st.throughput *= h.disc_scale;
+    const Body *hb = local ? nullptr : &sc.bodies[hit.index];
+    if (!local && hb->material.w > 0.5f) {
+        float weight = 1.0f;
+        if (st.depth > 0u) {
+            float light_pdf = 1.0f / (TAU * cone_width(sc, st.previous_vertex, hit.index));
+            weight = power_weight(st.previous_pdf, light_pdf);
+        }
+        st.radiance += st.throughput * xyz(hb->material) * weight;
+        return false;
+    }
+    if (!local && st.depth == 0u && !(st.flags & PATH_POINT_GAINED) && s.background[3] > 0.0f) {
+        st.flags |= PATH_POINT_GAINED;
+        float extent = hb->center.w * fmaxf(1.0f, hb->ring_params.x);
+        st.throughput *= point_gain(sc, extent, hit.distance);
+    }
+    f3 normal = local ? landscape_shading_normal(sc, ray, hit) : hit.normal;
+    Ray outgoing = local ? landscape_outgoing(sc, ray, hit)
+                         : Ray{tod(normal * (hb->center.w * (1.0f + 2.0e-6f))), normal, hit.index};
+    f3 direct_light = f3s(0.0f);
+    bool last_vertex = st.depth + 1u == max_vertices;
+    for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
+        uint32_t i = light_index(sc, slot);
+        if (body(sc, i).material.w < 0.5f) continue;
+        LightSample sample = sample_light(sc, outgoing, i, st.rng);
+        float cosine = fmaxf(dot(normal, sample.direction), 0.0f);
+        if (cosine <= 0.0f) continue;
+        outgoing.direction = sample.direction;
+        float visibility = light_visibility(sc, outgoing, i);
+        if (visibility > 0.0f) {
+            float bsdf_pdf = cosine / PI;
+            float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
+            direct_light += xyz(body(sc, i).material) * air_transmittance(sc, outgoing, sample.direction) *
+                            (bsdf_pdf * weight * visibility / sample.pdf);
+        }
+    }
+    f3 albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
+    st.radiance += st.throughput * albedo * direct_light;
+    st.shine_body = SG_MISS;
+    if (!local) {
+        st.radiance += st.throughput * albedo * ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, st.rng);
+        st.shine_body = hit.index;
+    }
+    bool sky_lit = local && s.globals.ground_counts[1] != 0u && s.globals.atmo_params[3] > 0.5f;
+    if (sky_lit && last_vertex) st.radiance += st.throughput * albedo * surface_sky(sc, outgoing, normal, st.rng);
+    if (last_vertex) return false;
+    float u = rnd(st.rng);
+    float phi = TAU * rnd(st.rng);
+    float cosine = sqrtf(1.0f - u);
+    outgoing.direction = basis_direction(normal, sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), cosine);
+    st.previous_pdf = cosine / PI;
+    st.throughput *= albedo;
+    if (st.depth >= 2u) {
+        float survival = clampf(maxc(st.throughput), 0.05f, 0.95f);
+        if (rnd(st.rng) >= survival) return false;
+        st.throughput /= survival;
+    }
+    ray = outgoing;
+    st.previous_vertex = outgoing;
+    if (inside_air(sc, outgoing)) st.flags |= PATH_IN_AIR;
+    if (sky_lit) st.flags |= PATH_SKY_ON_ESCAPE;
+    else st.flags &= ~PATH_SKY_ON_ESCAPE;
+    st.depth += 1u;
+    st.null_crossings = 0u;
+    return true;
+}
+
+SG_FN PathResult path_finish(const PathState &st) {
+    f3 radiance = st.radiance;
+    if (st.ring_cap < 1.0e29f) {
+        f3 indirect = radiance - st.ring_base;
         float lum = luminance(indirect);
-        if (lum > ring_cap) radiance = ring_base + indirect * (ring_cap / lum);
+        if (lum > st.ring_cap) radiance = st.ring_base + indirect * (st.ring_cap / lum);
     }
     PathResult out;
-    out.value = make_float4(radiance.x, radiance.y, radiance.z, luminance(backdrop));
-    out.backdrop = backdrop;
+    out.value = make_float4(radiance.x, radiance.y, radiance.z, luminance(st.backdrop));
+    out.backdrop = st.backdrop;
     return out;
+}
+
+// Whole path in one lane.
+SG_FN PathResult trace_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated) {
+    PathState st;
+    path_begin(st, direction, rng, accumulated);
+    for (;;) {
+        HitRecord h = path_extend(sc, st);
+        path_camera_air(sc, st, h);
+        if (!path_shade(sc, st, h)) break;
+    }
+    return path_finish(st);
 }
 
 // src/shaders/pathtrace.wgsl:1619-1626

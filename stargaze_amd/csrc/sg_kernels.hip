@@ -53,7 +53,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
         const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
         uint32_t rng = path_seed(pixel, ta.sequence + s);
         f3 direction = camera_direction(sc, x, y, rng);
-        PathResult r = trace_path(sc, Ray{d3{0.0, 0.0, 0.0}, direction, SG_MISS}, rng, accumulated);
+        PathResult r = trace_path(sc, direction, rng, accumulated);
         value = r.value;
         backdrop = r.backdrop;
     }
@@ -122,6 +122,243 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
     ta.accumulation[pixel] = mean;
     ta.backdrop[pixel] = bmean;
     if (ta.moments) ta.moments[pixel] = m2;
+}
+
+// ---------------------------------------------------------------------------
+// Wavefront integrator: the loop of trace() run as one kernel per step over
+// a queue of live paths (structure-of-arrays state, wave64 ballot
+// compaction), then an ordered per-pixel accumulation.
+// ---------------------------------------------------------------------------
+struct WaveArgs {
+    // Path state, indexed by path id within the chunk.
+    double4 *ray_o;    // offset.xyz, anchor bits in .w
+    float4 *ray_d;     // direction.xyz, previous_pdf
+    double4 *prev_o;   // previous vertex offset.xyz, anchor bits
+    float4 *thr;       // throughput.xyz, ring_cap
+    float4 *rad;       // radiance.xyz, shine_body bits
+    float4 *rbase;     // ring_base.xyz, flags | depth << 8 | null_crossings << 16
+    uint32_t *rng;
+    // Hit records from path_extend.
+    float4 *h0;        // hit.distance, hit.index bits, disc_scale, ring.distance
+    float4 *h1;        // hit.normal.xyz, ring.index bits
+    float4 *h2;        // ring.point.xyz
+    // Per-path results.
+    float4 *value;
+    float4 *backdrop;
+    uint32_t width, height;
+    uint32_t count, accumulated, sequence, has_air;
+    uint32_t pixel_begin, paths;
+};
+
+SG_FN double bits_d(uint32_t u) { return (double)u; }
+SG_FN uint32_t d_bits(double d) { return (uint32_t)d; }
+
+__device__ __forceinline__ uint32_t path_accumulated(const WaveArgs &wa, uint32_t id) {
+    return wa.has_air ? wa.accumulated + id % wa.count : wa.accumulated;
+}
+__device__ __forceinline__ void load_ray(const double4 &o, const float4 &d, Ray &r) {
+    r.offset = d3{o.x, o.y, o.z};
+    r.anchor = d_bits(o.w);
+    r.direction = f3{d.x, d.y, d.z};
+}
+__device__ __forceinline__ void load_state(const WaveArgs &wa, uint32_t id, PathState &st) {
+    float4 d = wa.ray_d[id];
+    load_ray(wa.ray_o[id], d, st.ray);
+    st.previous_pdf = d.w;
+    double4 po = wa.prev_o[id];
+    st.previous_vertex.offset = d3{po.x, po.y, po.z};
+    st.previous_vertex.anchor = d_bits(po.w);
+    st.previous_vertex.direction = st.ray.direction;
+    float4 t = wa.thr[id];
+    st.throughput = f3{t.x, t.y, t.z};
+    st.ring_cap = t.w;
+    float4 r = wa.rad[id];
+    st.radiance = f3{r.x, r.y, r.z};
+    st.shine_body = __float_as_uint(r.w);
+    float4 b = wa.rbase[id];
+    st.ring_base = f3{b.x, b.y, b.z};
+    uint32_t packed = __float_as_uint(b.w);
+    st.flags = packed & 255u;
+    st.depth = (packed >> 8) & 255u;
+    st.null_crossings = packed >> 16;
+    st.rng = wa.rng[id];
+    st.backdrop = f3s(0.0f);
+    st.accumulated = path_accumulated(wa, id);
+}
+__device__ __forceinline__ void store_state(const WaveArgs &wa, uint32_t id, const PathState &st) {
+    wa.ray_o[id] = make_double4(st.ray.offset.x, st.ray.offset.y, st.ray.offset.z, bits_d(st.ray.anchor));
+    wa.ray_d[id] = make_float4(st.ray.direction.x, st.ray.direction.y, st.ray.direction.z, st.previous_pdf);
+    wa.prev_o[id] = make_double4(st.previous_vertex.offset.x, st.previous_vertex.offset.y,
+                                 st.previous_vertex.offset.z, bits_d(st.previous_vertex.anchor));
+    wa.thr[id] = make_float4(st.throughput.x, st.throughput.y, st.throughput.z, st.ring_cap);
+    wa.rad[id] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, __uint_as_float(st.shine_body));
+    uint32_t packed = (st.flags & 255u) | ((st.depth & 255u) << 8) | (st.null_crossings << 16);
+    wa.rbase[id] = make_float4(st.ring_base.x, st.ring_base.y, st.ring_base.z, __uint_as_float(packed));
+    wa.rng[id] = st.rng;
+}
+__device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id, HitRecord &h) {
+    float4 a = wa.h0[id], b = wa.h1[id], c = wa.h2[id];
+    h.hit.distance = a.x;
+    h.hit.index = __float_as_uint(a.y);
+    h.disc_scale = a.z;
+    h.ring.distance = a.w;
+    h.hit.normal = f3{b.x, b.y, b.z};
+    h.ring.index = __float_as_uint(b.w);
+    h.ring.point = f3{c.x, c.y, c.z};
+}
+__device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const PathState &st) {
+    PathResult r = path_finish(st);
+    wa.value[id] = r.value;
+    wa.backdrop[id] = make_float4(r.backdrop.x, r.backdrop.y, r.backdrop.z, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_generate(Scene sc, WaveArgs wa, uint32_t *queue) {
+    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
+    if (id >= wa.paths) return;
+    const uint32_t pixel = wa.pixel_begin + id / wa.count;
+    const uint32_t s = id % wa.count;
+    const uint32_t x = pixel % wa.width, y = pixel / wa.width;
+    uint32_t rng = path_seed(pixel, wa.sequence + s);
+    f3 direction = camera_direction(sc, x, y, rng);
+    PathState st;
+    path_begin(st, direction, rng, path_accumulated(wa, id));
+    store_state(wa, id, st);
+    queue[id] = id;
+}
+
+__global__ __launch_bounds__(256) void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = queue[i];
+    PathState st;
+    load_ray(wa.ray_o[id], wa.ray_d[id], st.ray);
+    uint32_t packed = __float_as_uint(wa.rbase[id].w);
+    st.depth = (packed >> 8) & 255u;
+    st.null_crossings = packed >> 16;
+    st.rng = wa.rng[id];
+    HitRecord h = path_extend(sc, st);
+    wa.rng[id] = st.rng;
+    wa.h0[id] = make_float4(h.hit.distance, __uint_as_float(h.hit.index), h.disc_scale, h.ring.distance);
+    wa.h1[id] = make_float4(h.hit.normal.x, h.hit.normal.y, h.hit.normal.z, __uint_as_float(h.ring.index));
+    wa.h2[id] = make_float4(h.ring.point.x, h.ring.point.y, h.ring.point.z, 0.f);
+}
+
+__global__ __launch_bounds__(256) void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = queue[i];
+    PathState st;
+    float4 d = wa.ray_d[id];
+    load_ray(wa.ray_o[id], d, st.ray);
+    float4 t = wa.thr[id], r = wa.rad[id], b = wa.rbase[id];
+    st.throughput = f3{t.x, t.y, t.z};
+    st.radiance = f3{r.x, r.y, r.z};
+    st.flags = __float_as_uint(b.w) & 255u;
+    st.rng = wa.rng[id];
+    st.accumulated = path_accumulated(wa, id);
+    HitRecord h;
+    load_hit(wa, id, h);
+    path_camera_air(sc, st, h);
+    wa.thr[id] = make_float4(st.throughput.x, st.throughput.y, st.throughput.z, t.w);
+    wa.rad[id] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, r.w);
+    wa.rbase[id] = make_float4(b.x, b.y, b.z, __uint_as_float((__float_as_uint(b.w) & ~255u) | st.flags));
+    wa.rng[id] = st.rng;
+}
+
+__global__ __launch_bounds__(256) void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
+                                               uint32_t *next, uint32_t *next_count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    bool alive = false;
+    uint32_t id = 0;
+    if (i < n) {
+        id = queue[i];
+        PathState st;
+        load_state(wa, id, st);
+        HitRecord h;
+        load_hit(wa, id, h);
+        // The camera segment was handled by k_camera_air (air on) or has
+        // nothing to add (air off).
+        st.flags &= ~PATH_CAMERA_SEGMENT;
+        alive = path_shade(sc, st, h);
+        if (alive) store_state(wa, id, st);
+        else finish(wa, id, st);
+    }
+    // Wave64 compaction: one atomic per wavefront, lane offsets from the ballot.
+    const uint64_t mask = __ballot(alive);
+    if (mask == 0) return;
+    const uint32_t lane = __lane_id();
+    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+    uint32_t base = 0;
+    if (lane == leader) base = atomicAdd(next_count, (uint32_t)__popcll(mask));
+    base = __shfl(base, (int)leader);
+    if (alive) next[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = id;
+}
+
+// Ordered per-pixel blend of the chunk's path results
+// (src/shaders/pathtrace.wgsl:1653-1674; per-sample dispatches when air is on).
+__global__ __launch_bounds__(256) void k_accumulate(WaveArgs wa, float4 *accumulation, float4 *backdrop,
+                                                    float4 *moments, uint32_t pixels) {
+    const uint32_t local = blockIdx.x * blockDim.x + threadIdx.x;
+    if (local >= pixels) return;
+    const uint32_t pixel = wa.pixel_begin + local;
+    const uint32_t base = local * wa.count;
+    float4 mean, bmean;
+    float4 m2 = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (!wa.has_air) {
+        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), bsum = sum, sq = sum;
+        for (uint32_t k = 0; k < wa.count; ++k) {
+            float4 v = wa.value[base + k], b = wa.backdrop[base + k];
+            sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
+            bsum.x += b.x; bsum.y += b.y; bsum.z += b.z;
+            sq.x += v.x * v.x; sq.y += v.y * v.y; sq.z += v.z * v.z;
+        }
+        const float nf = (float)wa.count;
+        mean = make_float4(sum.x / nf, sum.y / nf, sum.z / nf, sum.w / nf);
+        bmean = make_float4(bsum.x / nf, bsum.y / nf, bsum.z / nf, 0.f);
+        m2 = make_float4(sq.x / nf, sq.y / nf, sq.z / nf, 0.f);
+        const uint32_t new_count = wa.accumulated + wa.count;
+        if (wa.accumulated > 0u) {
+            const float w = nf / (float)new_count;
+            float4 old = accumulation[pixel];
+            mean = make_float4(old.x + (mean.x - old.x) * w, old.y + (mean.y - old.y) * w,
+                               old.z + (mean.z - old.z) * w, old.w + (mean.w - old.w) * w);
+            float4 ob = backdrop[pixel];
+            bmean = make_float4(ob.x + (bmean.x - ob.x) * w, ob.y + (bmean.y - ob.y) * w, ob.z + (bmean.z - ob.z) * w, 0.f);
+            if (moments) {
+                float4 om = moments[pixel];
+                m2 = make_float4(om.x + (m2.x - om.x) * w, om.y + (m2.y - om.y) * w, om.z + (m2.z - om.z) * w, 0.f);
+            }
+        }
+        m2.w = (float)new_count;
+    } else {
+        if (wa.accumulated > 0u) {
+            mean = accumulation[pixel];
+            bmean = backdrop[pixel];
+            if (moments) m2 = moments[pixel];
+        }
+        for (uint32_t k = 0; k < wa.count; ++k) {
+            float4 v = wa.value[base + k], b = wa.backdrop[base + k];
+            float4 v2 = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0.f);
+            const uint32_t old_count = wa.accumulated + k;
+            if (old_count > 0u) {
+                const float w = 1.0f / (float)(old_count + 1u);
+                mean = make_float4(mean.x + (v.x - mean.x) * w, mean.y + (v.y - mean.y) * w,
+                                   mean.z + (v.z - mean.z) * w, mean.w + (v.w - mean.w) * w);
+                bmean = make_float4(bmean.x + (b.x - bmean.x) * w, bmean.y + (b.y - bmean.y) * w,
+                                    bmean.z + (b.z - bmean.z) * w, 0.f);
+                m2 = make_float4(m2.x + (v2.x - m2.x) * w, m2.y + (v2.y - m2.y) * w, m2.z + (v2.z - m2.z) * w, 0.f);
+            } else {
+                mean = v;
+                bmean = make_float4(b.x, b.y, b.z, 0.f);
+                m2 = v2;
+            }
+        }
+        bmean.w = 0.f;
+        m2.w = (float)(wa.accumulated + wa.count);
+    }
+    accumulation[pixel] = mean;
+    backdrop[pixel] = bmean;
+    if (moments) moments[pixel] = m2;
 }
 
 // cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
@@ -274,6 +511,15 @@ struct sg_ctx {
     sg_probe_ray *probe_rays = nullptr;
     sg_probe_result *probe_results = nullptr;
     uint32_t probe_capacity = 0;
+    // Integrator selection and wavefront buffers
+    uint32_t integrator = SG_INTEGRATOR_WAVEFRONT;
+    uint32_t chunk_paths = 1u << 23;
+    uint32_t wave_capacity = 0;
+    void *wave_block = nullptr;
+    WaveArgs wave{};
+    uint32_t *queues[2] = {nullptr, nullptr};
+    uint32_t *counter = nullptr;
+    uint32_t *counter_host = nullptr;
     // Measurement
     bool profiling = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -391,6 +637,94 @@ int ensure_objects(sg_ctx *ctx, uint32_t count) {
     return SG_OK;
 }
 
+int ensure_wave(sg_ctx *ctx, uint32_t paths) {
+    if (paths <= ctx->wave_capacity && ctx->wave_block) return SG_OK;
+    if (ctx->wave_block) SG_HIP(ctx, hipFree(ctx->wave_block));
+    if (ctx->queues[0]) SG_HIP(ctx, hipFree(ctx->queues[0]));
+    ctx->wave_block = nullptr;
+    ctx->queues[0] = nullptr;
+    // ray_o, prev_o: 32 B; ray_d, thr, rad, rbase, h0, h1, h2, value, backdrop: 16 B; rng: 4 B.
+    const size_t n = paths;
+    const size_t bytes = n * (32 * 2 + 16 * 9 + 4) + 256 * 12;
+    SG_HIP(ctx, hipMalloc(&ctx->wave_block, bytes));
+    char *p = static_cast<char *>(ctx->wave_block);
+    auto take = [&](size_t each) {
+        char *q = p;
+        p += (n * each + 255) & ~size_t(255);
+        return q;
+    };
+    WaveArgs &w = ctx->wave;
+    w.ray_o = reinterpret_cast<double4 *>(take(32));
+    w.prev_o = reinterpret_cast<double4 *>(take(32));
+    w.ray_d = reinterpret_cast<float4 *>(take(16));
+    w.thr = reinterpret_cast<float4 *>(take(16));
+    w.rad = reinterpret_cast<float4 *>(take(16));
+    w.rbase = reinterpret_cast<float4 *>(take(16));
+    w.h0 = reinterpret_cast<float4 *>(take(16));
+    w.h1 = reinterpret_cast<float4 *>(take(16));
+    w.h2 = reinterpret_cast<float4 *>(take(16));
+    w.value = reinterpret_cast<float4 *>(take(16));
+    w.backdrop = reinterpret_cast<float4 *>(take(16));
+    w.rng = reinterpret_cast<uint32_t *>(take(4));
+    SG_HIP(ctx, hipMalloc(&ctx->queues[0], (2 * n + 64) * sizeof(uint32_t)));
+    ctx->queues[1] = ctx->queues[0] + n;
+    ctx->counter = ctx->queues[0] + 2 * n;
+    if (!ctx->counter_host) SG_HIP(ctx, hipHostMalloc(&ctx->counter_host, 64, hipHostMallocDefault));
+    ctx->wave_capacity = paths;
+    return SG_OK;
+}
+
+// One progressive pass through the wavefront kernels, chunked over pixels so
+// the path state stays within wave_capacity.
+int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
+    const uint32_t count = ta.count;
+    const uint64_t pixels = (uint64_t)ta.width * ta.height;
+    const uint32_t chunk_pixels = std::max<uint32_t>(1u, ctx->chunk_paths / count);
+    const uint32_t capacity = (uint32_t)std::min<uint64_t>(pixels, chunk_pixels) * count;
+    int rc = ensure_wave(ctx, capacity);
+    if (rc != SG_OK) return rc;
+    WaveArgs wa = ctx->wave;
+    wa.width = ta.width;
+    wa.height = ta.height;
+    wa.count = count;
+    wa.accumulated = ta.accumulated;
+    wa.sequence = ta.sequence;
+    wa.has_air = ta.has_air;
+    const uint32_t block = 256;
+    for (uint64_t p0 = 0; p0 < pixels; p0 += chunk_pixels) {
+        const uint32_t chunk = (uint32_t)std::min<uint64_t>(chunk_pixels, pixels - p0);
+        wa.pixel_begin = (uint32_t)p0;
+        wa.paths = chunk * count;
+        uint32_t n = wa.paths;
+        int q = 0;
+        hipLaunchKernelGGL(k_generate, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
+                           ctx->queues[q]);
+        SG_HIP(ctx, hipGetLastError());
+        for (uint32_t iteration = 0; n > 0; ++iteration) {
+            const dim3 grid((n + block - 1) / block);
+            hipLaunchKernelGGL(k_extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
+            SG_HIP(ctx, hipGetLastError());
+            if (iteration == 0 && ta.has_air) {
+                hipLaunchKernelGGL(k_camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
+                SG_HIP(ctx, hipGetLastError());
+            }
+            SG_HIP(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(uint32_t), ctx->stream));
+            hipLaunchKernelGGL(k_shade, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n,
+                               ctx->queues[q ^ 1], ctx->counter);
+            SG_HIP(ctx, hipGetLastError());
+            SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, sizeof(uint32_t), hipMemcpyDeviceToHost,
+                                       ctx->stream));
+            SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            n = *ctx->counter_host;
+            q ^= 1;
+        }
+        hipLaunchKernelGGL(k_accumulate, dim3((chunk + block - 1) / block), dim3(block), 0, ctx->stream, wa,
+                           ta.accumulation, ta.backdrop, ta.moments, chunk);
+        SG_HIP(ctx, hipGetLastError());
+    }
+    return SG_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -452,7 +786,8 @@ int sg_destroy(sg_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
-                       ctx->probe_results};
+                       ctx->probe_results, ctx->wave_block, ctx->queues[0]};
+    if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
@@ -651,8 +986,13 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
         const uint64_t pixels = (uint64_t)ctx->width * ctx->height;
         const uint32_t grid = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);
         if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
-        hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, sc, ta);
-        SG_HIP(ctx, hipGetLastError());
+        if (ctx->integrator == SG_INTEGRATOR_FUSED) {
+            hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, sc, ta);
+            SG_HIP(ctx, hipGetLastError());
+        } else {
+            int wrc = encode_wavefront(ctx, sc, ta);
+            if (wrc != SG_OK) return wrc;
+        }
         if (ctx->profiling) {
             SG_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
             SG_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
@@ -700,6 +1040,18 @@ int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
     if (!ctx || !milliseconds || !launches) return SG_E_INVALID;
     *milliseconds = ctx->kernel_ms;
     *launches = ctx->launches;
+    return SG_OK;
+}
+
+int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths) {
+    if (!ctx) return SG_E_INVALID;
+    if (integrator != SG_INTEGRATOR_FUSED && integrator != SG_INTEGRATOR_WAVEFRONT)
+        return fail(ctx, SG_E_INVALID, "unknown integrator");
+    if (chunk_paths != 0 && (chunk_paths < 64u || chunk_paths > (1u << 28)))
+        return fail(ctx, SG_E_INVALID, "chunk_paths must be 0 (keep) or in [64, 2^28]");
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->integrator = integrator;
+    if (chunk_paths) ctx->chunk_paths = chunk_paths;
     return SG_OK;
 }
 

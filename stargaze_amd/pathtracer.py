@@ -176,6 +176,10 @@ class PathTracer:
         self._check(self._lib.sg_kernel_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "kernel_time")
         return float(ms.value), int(n.value)
 
+    def set_integrator(self, integrator: int, chunk_paths: int = 0) -> None:
+        """INTEGRATOR_FUSED or INTEGRATOR_WAVEFRONT (see include/stargaze_amd.h)."""
+        self._check(self._lib.sg_set_integrator(self._ctx, int(integrator), int(chunk_paths)), "set_integrator")
+
     def probe(self, frame: Frame, kind: int, rays: np.ndarray) -> np.ndarray:
         """Run production device helpers on ``rays`` (PROBE_RAY records)."""
         r = np.ascontiguousarray(rays, PROBE_RAY).reshape(-1)
@@ -187,4 +191,5 @@ class PathTracer:
         return out
 
 
+INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
 PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE = range(7)

@@ -174,3 +174,163 @@ WORKLOADS = {
     "airless_triple_256": (airless_triple, dict(width=256, height=256), 16, 4),
     "ringed_giant_1080p": (ringed_giant, dict(width=1920, height=1080), 64, 6),
 }
+
+
+# -- the Halo observatory ----------------------------------------------------
+def _ground_frame(pitch_deg: float):
+    """(east, up, north) in telescope space for a camera looking north,
+    pitched up by ``pitch_deg`` (src/camera.rs:74-91 conventions)."""
+    t = math.radians(pitch_deg)
+    east = np.array([1.0, 0.0, 0.0])
+    up = np.array([0.0, math.cos(t), -math.sin(t)])
+    north = np.array([0.0, -math.sin(t), -math.cos(t)])
+    return east, up, north
+
+
+def sky_direction(east, up, north, azimuth_deg, elevation_deg):
+    a, e = math.radians(azimuth_deg), math.radians(elevation_deg)
+    return math.cos(e) * (math.cos(a) * north + math.sin(a) * east) + math.sin(e) * up
+
+
+def _rotation_z_extent(extent, angle):
+    """``Primitive::world_extent`` for a rotation about the north (z) axis."""
+    c, s = abs(math.cos(angle)), abs(math.sin(angle))
+    return np.array([c * extent[0] + s * extent[1], s * extent[0] + c * extent[1], extent[2]])
+
+
+def _group(center_lo, center_hi, count):
+    g = np.zeros((), PRIMITIVE)
+    g["center"] = [*((center_lo + center_hi) * 0.5), -float(count)]
+    g["extent"] = [*((center_hi - center_lo) * 0.5 + 0.02), 0.0]
+    return g
+
+
+def cluster_props(props: np.ndarray, levels=((8, 8), (4, 4), (4, 4))) -> np.ndarray:
+    """Nested groups in the reference's flat skip-count format
+    (src/ground.rs:198-225, src/shaders/ground.wgsl:167-178): a group marker
+    (center.w = -descendants) bounds every following entry it owns; a ray
+    that misses it skips them all."""
+    centre = props["center"][:, :3].astype(np.float64)
+    e = props["extent"].astype(np.float64)
+    c, s_ = np.abs(np.cos(e[:, 3])), np.abs(np.sin(e[:, 3]))
+    ext = np.stack([c * e[:, 0] + s_ * e[:, 1], s_ * e[:, 0] + c * e[:, 1], e[:, 2]], -1)
+    lo_all, hi_all = centre - ext, centre + ext
+
+    def build(idx, depth, x0, x1, z0, z1):
+        if depth == len(levels):
+            return list(props[idx])
+        nx, nz = levels[depth]
+        out = []
+        xs = np.linspace(x0, x1, nx + 1)
+        zs = np.linspace(z0, z1, nz + 1)
+        cx = np.clip(np.searchsorted(xs, centre[idx, 0], side="right") - 1, 0, nx - 1)
+        cz = np.clip(np.searchsorted(zs, centre[idx, 2], side="right") - 1, 0, nz - 1)
+        for j in range(nz):
+            for i in range(nx):
+                sel = idx[(cx == i) & (cz == j)]
+                if sel.size == 0:
+                    continue
+                children = build(sel, depth + 1, xs[i], xs[i + 1], zs[j], zs[j + 1])
+                single_group = children[0]["center"][3] < 0 and -children[0]["center"][3] == len(children) - 1
+                if len(children) == 1 or single_group:
+                    out.extend(children)
+                    continue
+                lo = lo_all[sel].min(axis=0)
+                hi = hi_all[sel].max(axis=0)
+                out.append(_group(lo, hi, len(children)))
+                out.extend(children)
+        return out
+
+    x0, z0 = centre[:, 0].min() - 1, centre[:, 2].min() - 1
+    x1, z1 = centre[:, 0].max() + 1, centre[:, 2].max() + 1
+    flat = build(np.arange(len(props)), 0, x0, x1, z0, z1)
+    return np.array(flat, PRIMITIVE)
+
+
+def scatter_props(heights, cells, radius_m, rocks, conifers, seed=0x1234ABCD, reach=2500.0, clear=12.0):
+    """Boxes (rocks, stone material) and ellipsoids (conifers, foliage) on
+    the terrain surface, uniform in area between ``clear`` and ``reach`` m."""
+    from . import terrain as tr
+
+    rng = np.random.default_rng(seed)
+    s = tr.scale(radius_m)
+    n = rocks + conifers
+    r = np.sqrt(rng.uniform(clear**2, reach**2, n))
+    a = rng.uniform(0, 2 * math.pi, n)
+    xs, zs = r * np.cos(a), r * np.sin(a)
+    d2 = xs * xs + zs * zs
+    curvature = -d2 / (radius_m + np.sqrt(radius_m * radius_m - d2))
+    ground = tr.heights_at(heights, cells, xs / s, zs / s) * s + curvature
+    props = np.zeros(n, PRIMITIVE)
+    e = rng.uniform(0.25, 1.4, (rocks, 3)) * np.array([1.0, 0.6, 1.0])
+    angle = rng.uniform(-0.35, 0.35, rocks)
+    grey = rng.uniform(0.18, 0.34, rocks)
+    props["center"][:rocks] = np.stack([xs[:rocks], ground[:rocks] + e[:, 1] * 0.6, zs[:rocks], np.zeros(rocks)], -1)
+    props["extent"][:rocks] = np.concatenate([e, angle[:, None]], -1)
+    props["albedo"][:rocks] = np.stack([grey, grey * 0.97, grey * 0.92, np.full(rocks, 2.0)], -1)
+    rad = rng.uniform(1.2, 2.8, conifers)
+    half = rng.uniform(4.0, 11.0, conifers)
+    green = rng.uniform(0.09, 0.13, conifers)
+    c = slice(rocks, n)
+    props["center"][c] = np.stack([xs[c], ground[c] + half * 0.92, zs[c], np.ones(conifers)], -1)
+    props["extent"][c] = np.stack([rad, half, rad, np.zeros(conifers)], -1)
+    props["albedo"][c] = np.stack([np.full(conifers, 0.045), green, np.full(conifers, 0.035), np.full(conifers, 3.0)], -1)
+    return props
+
+
+def halo_observatory(width=3840, height=2160, cells=4096, rocks=20000, conifers=50000, seed=0x743337A9,
+                     reach=2500.0, levels=((8, 8), (4, 4), (4, 4))):
+    """Halo observatory: the camera 2 m above a 1.2e6 m host at 35 deg N,
+    ridged-fBm terrain, rocks and conifers, Earth-like air with Rayleigh
+    beta (5.8, 13.5, 33.1)e-6 /m and Mie 2e-5 /m (g = 0.76, one 8 km scale
+    height as in the reference shader), two inflated stars low in the sky
+    and the ringed giant overhead.
+
+    Returns ``(frame, terrain_data, cells)``; upload the terrain with
+    ``PathTracer.upload_terrain(terrain_data, cells)`` before encoding.
+    """
+    from . import terrain as tr
+
+    radius_m = 1.2e6
+    f = frame()
+    camera(f.globals, 70.0, width, height)
+    east, up, north = _ground_frame(15.0)
+    r_host = metres(radius_m)
+    host_c = -up * (r_host + metres(2.0))
+    host = place(body([0, 0, 0], r_host, [0.25, 0.22, 0.18], False), host_c)
+    sun_a = star(sky_direction(east, up, north, -40.0, 27.0) * 2.0, metres(2.2e9), 3.0)
+    sun_b = star(sky_direction(east, up, north, -49.0, 31.0) * 2.0, metres(1.3e9), 1.2)
+    giant_frame = ringed_giant(width, height)
+    giant = giant_frame.bodies[1].copy()
+    giant_c = sky_direction(east, up, north, 12.0, 40.0) * metres(1.6e9)
+    pole = up * 0.8 + north * 0.6
+    pole /= np.linalg.norm(pole)
+    giant["ring_plane"][:3] = pole
+    x_axis = np.cross(pole, east)
+    x_axis /= np.linalg.norm(x_axis)
+    giant["axis_x"][:3] = x_axis
+    giant["axis_y"][:3] = np.cross(pole, x_axis)
+    place(giant, giant_c)
+    f.bodies = bodies(sun_a, sun_b, host, giant)
+    host_index = 2
+    au = METRES_PER_AU
+    g = f.globals
+    g["atmo_center"] = [*split(host_c)[0], r_host]
+    g["atmo_rayleigh"] = [5.8e-6 * au, 13.5e-6 * au, 33.1e-6 * au, 2.0e-5 * au]
+    g["atmo_params"] = [0.76, 8000.0 / au, 80000.0 / au, host_index + 1]
+    g["ground_east"] = [*east, 2.0]
+    g["ground_up"] = [*up, radius_m]
+    g["ground_north"] = [*north, host_index + 1]
+    heights = tr.ridged_fbm(cells, seed=seed, amplitude=800.0)
+    data = tr.terrain_data(heights, cells)
+    props = scatter_props(heights, cells, radius_m, rocks, conifers, reach=reach)
+    ground = cluster_props(props, levels) if len(props) > 64 else props
+    f.ground = ground
+    g["ground_counts"] = [len(ground), 1, 0, 0]
+    return f, data, cells
+
+
+def halo_parity(width=96, height=54):
+    """Halo scene inside the reference's capacities: 512-cell terrain and at
+    most 256 primitives (src/terrain.rs:20, src/ground.rs:8)."""
+    return halo_observatory(width, height, cells=512, rocks=50, conifers=130, reach=400.0, levels=((6, 6),))

@@ -180,3 +180,104 @@ def test_seed_determinism_and_reset():
     f.scene_time = 0.0
     assert np.array_equal(first, cases.sample(make(0), f, 1))
     assert not np.array_equal(first, cases.sample(make(12345), f, 1))
+
+
+@pytest.mark.parametrize("radius", [6_371_000.0, 5_000.0])
+def test_terrain_probe_matches_exhaustive_triangles(radius):
+    import terrain_cases as tc
+
+    cells = 64
+    heights, data = tc.make_terrain(cells)
+    rays = tc.probe_rays(radius, extra=200)
+    t = PathTracer((4, 4), options=Options(milky_way=0.0))
+    t.upload_terrain(data, cells)
+    res = t.probe(tc.probe_frame(cells, radius), ptm.PROBE_TERRAIN, rays)
+    tc.check(res, tc.exhaustive(heights, cells, radius, rays))
+
+
+def _render_halo(size, spp, bounces, batches):
+    frame, data, cells = scenes.halo_parity(*size)
+    opts = dict(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0)
+    gpu = PathTracer(size, options=Options(**opts))
+    gpu.upload_terrain(data, cells)
+    gpu.enable_moments(True)
+    orc = oracle.OracleTracer(size, Options(**opts), "f32")
+    orc.upload_terrain(data, cells)
+    orc.enable_moments(True)
+    for _ in range(batches):
+        gpu.encode(frame)
+        orc.encode(frame)
+    return frame, (gpu.read_accumulation(), gpu.read_moments()), (orc.read_accumulation(), orc.read_moments())
+
+
+def test_halo_scene_matches_oracle_per_tile():
+    """Atmosphere, sky cache, terrain, props and two suns at reference capacity."""
+    frame, (ga, gm), (oa, om) = _render_halo((96, 54), 8, 4, 2)
+    assert np.all(np.isfinite(ga)) and np.all(ga[..., :3] >= 0)
+    agree = parity.pixel_agreement(ga, oa)
+    z = parity.tile_z(ga, gm, oa, om)
+    over, worst = parity.assert_tiles(z)
+    print(f"halo: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}, "
+          f"mean {ga[..., :3].mean():.4g} vs {oa[..., :3].mean():.4g}")
+    assert agree >= 0.5, agree
+
+
+def test_landscape_probe_matches_oracle():
+    """Closest landscape hits (host sphere in metres, terrain, props) from the
+    camera, against the FP64 oracle."""
+    frame, data, cells = scenes.halo_parity(32, 18)
+    rng = np.random.default_rng(11)
+    n = 2048
+    rays = np.zeros(n, PROBE_RAY)
+    dirs = rng.normal(size=(n, 3))
+    dirs[:, 1] = -np.abs(dirs[:, 1]) * 0.3 - 0.02
+    up = frame.globals["ground_up"][:3].astype(np.float64)
+    east = frame.globals["ground_east"][:3].astype(np.float64)
+    north = frame.globals["ground_north"][:3].astype(np.float64)
+    world = dirs[:, :1] * east + dirs[:, 1:2] * up + dirs[:, 2:3] * north
+    world /= np.linalg.norm(world, axis=1, keepdims=True)
+    rays["anchor"] = MISS
+    rays["direction"] = world
+    gpu = PathTracer((4, 4), options=Options(milky_way=0.0))
+    gpu.upload_terrain(data, cells)
+    g = gpu.probe(frame, ptm.PROBE_LANDSCAPE, rays)
+    orc = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64")
+    orc.upload_terrain(data, cells)
+    r = orc.probe(frame, ptm.PROBE_LANDSCAPE, rays)
+    same = g["index"] == r["index"].astype(np.uint32)
+    assert same.mean() > 0.99, same.mean()
+    hit = same & (r["index"] != MISS)
+    rel = np.abs(g["distance"][hit] - r["distance"][hit]) / r["distance"][hit]
+    print(f"landscape: {hit.sum()} hits, same index {same.mean():.4f}, median rel {np.median(rel):.2e}, max {rel.max():.2e}")
+    assert np.median(rel) < 1e-5 and rel.max() < 1e-3
+
+
+@pytest.mark.parametrize("scene", ["airless_triple", "ringed_giant", "halo"])
+def test_wavefront_and_fused_integrators_are_bit_identical(scene):
+    """Both integrators run the same step functions in the same draw order.
+
+    Airless scenes give bit-identical images. With air, the camera-air step
+    is compiled in a different kernel in each integrator and the compiler
+    fuses multiply-adds differently, so in-scattered sky radiance may differ
+    in the last bits: those images must agree to 1e-5 relative.
+    """
+    size = (48, 27)
+    if scene == "halo":
+        frame, data, cells = scenes.halo_parity(*size)
+    else:
+        frame, data, cells = getattr(scenes, scene)(*size), None, None
+    images = []
+    for integrator, chunk in ((ptm.INTEGRATOR_FUSED, 0), (ptm.INTEGRATOR_WAVEFRONT, 0), (ptm.INTEGRATOR_WAVEFRONT, 1000)):
+        t = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=5, milky_way=0.0))
+        if data is not None:
+            t.upload_terrain(data, cells)
+        t.set_integrator(integrator, chunk)
+        t.encode(frame)
+        t.encode(frame)
+        images.append(t.read_accumulation())
+    if scene == "halo":
+        for other in images[1:]:
+            np.testing.assert_allclose(other, images[0], rtol=1e-5, atol=1e-9)
+    else:
+        assert np.array_equal(images[0], images[1])
+        assert np.array_equal(images[0], images[2])

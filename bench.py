@@ -22,6 +22,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
+# The oracle module lives in oracle/oracle.py; put its directory first so the
+# directory itself is not taken for a namespace package.
+sys.path.insert(0, str(ROOT / "oracle"))
 
 from stargaze_amd import scenes  # noqa: E402
 from stargaze_amd.pathtracer import Options, PathTracer  # noqa: E402
@@ -53,10 +56,20 @@ FLOP_COST = {
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32, dense
 PEAK_HBM_GBS = 8000.0
 
+def _halo(width, height):
+    frame, data, cells = scenes.halo_observatory(width, height)
+    return frame, (data, cells)
+
+
+def _plain(builder):
+    return lambda width, height: (builder(width, height), None)
+
+
 WORKLOADS = {
-    # name: (builder, width, height, spp per pass, max bounces)
-    "ringed_giant_1080p": (scenes.ringed_giant, 1920, 1080, 64, 6),
-    "airless_triple_256": (scenes.airless_triple, 256, 256, 16, 4),
+    # name: (builder -> (frame, terrain), width, height, spp per pass, max bounces)
+    "halo_4k": (_halo, 3840, 2160, 32, 4),
+    "ringed_giant_1080p": (_plain(scenes.ringed_giant), 1920, 1080, 64, 6),
+    "airless_triple_256": (_plain(scenes.airless_triple), 256, 256, 16, 4),
 }
 
 
@@ -64,30 +77,41 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def work_model(builder, width, height, spp, bounces, rows=4):
-    """Event counts per path sample from the oracle on a central band."""
-    sys.path.insert(0, str(ROOT / "oracle"))
+def oracle_tracer(width, height, spp, bounces, terrain):
     import oracle
 
     t = oracle.OracleTracer((width, height), Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
-    y0 = height // 2 - rows // 2
-    ev = oracle.count_events(t, builder(width, height), (y0, y0 + rows))
+    if terrain is not None:
+        t.upload_terrain(*terrain)
+    return t
+
+
+def work_model(frame, terrain, width, height, spp, bounces, rows=4):
+    """Event counts per path sample from the oracle on bands of rows (the
+    image's quarter, half and three-quarter heights)."""
+    import oracle
+
+    t = oracle_tracer(width, height, spp, bounces, terrain)
+    ev = None
+    bands = [height // 4, height // 2, 3 * height // 4]
+    per_band = max(1, rows // len(bands))
+    for y0 in bands:
+        e = oracle.count_events(t, frame, (y0, y0 + per_band))
+        ev = e if ev is None else {k: ev[k] + e[k] for k in ev}
+        t.history_samples = 0
+        t.sequence = 0
+    rows = per_band * len(bands)
     n = width * rows * spp
     per = {k: v / n for k, v in ev.items()}
     flops = sum(per[k] * FLOP_COST[k] for k in FLOP_COST)
     return per, flops
 
 
-def cpu_baseline(builder, width, height, spp, bounces, target_s=12.0):
+def cpu_baseline(frame, terrain, width, height, spp, bounces, target_s=12.0):
     """FP32 oracle on all host cores over a band of rows sized to ~target_s."""
-    sys.path.insert(0, str(ROOT / "oracle"))
-    import oracle
-
-    frame = builder(width, height)
-    opts = Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0)
     rows = 8
     while True:
-        t = oracle.OracleTracer((width, height), opts)
+        t = oracle_tracer(width, height, spp, bounces, terrain)
         y0 = max(0, height // 2 - rows // 2)
         t0 = time.perf_counter()
         t.encode(frame, rows=(y0, min(height, y0 + rows)))
@@ -112,15 +136,20 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="ringed_giant_1080p", choices=sorted(WORKLOADS))
+    ap.add_argument("--workload", default="halo_4k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
         return 2
     builder, width, height, spp, bounces = WORKLOADS[args.workload]
-    frame = builder(width, height)
+    t0 = time.perf_counter()
+    frame, terrain = builder(width, height)
+    log(f"scene built in {time.perf_counter() - t0:.1f} s")
     tracer = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
+    if terrain is not None:
+        tracer.upload_terrain(*terrain)
     log(f"workload {args.workload}: {width}x{height}, {spp} spp/step, {bounces} bounces")
     for i in range(args.warmup):
         tracer.encode(frame)
@@ -138,7 +167,11 @@ def main() -> int:
     assert np.all(np.isfinite(img)), "non-finite radiance"
     samples_per_step = width * height * spp
     value = samples_per_step * args.steps / elapsed / 1e6
-    per_event, flops = work_model(builder, width, height, spp, bounces)
+    if args.no_work_model:
+        per_event, flops = {}, float("nan")
+    else:
+        log("work model")
+        per_event, flops = work_model(frame, terrain, width, height, spp, bounces)
     kernel_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = flops * samples_per_step / kernel_s / 1e12
     traffic_algorithmic = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
@@ -169,7 +202,7 @@ def main() -> int:
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
             "traffic": None,
-            "kernel": "k_trace",
+            "kernel": "integrator (all kernels of one pass)",
             "kernel_ms": round(kernel_s * 1e3, 3),
             "flop_per_path_sample": round(flops, 1),
             "algorithmic_bytes_per_launch": traffic_algorithmic,
@@ -177,7 +210,7 @@ def main() -> int:
     }
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(builder, width, height, spp, bounces)
+        line["cpu_baseline"] = cpu_baseline(frame, terrain, width, height, spp, bounces)
     print(json.dumps(line), flush=True)
     return 0
 

@@ -19,6 +19,7 @@
 #define SG_LOCAL_BASE 0x80000000u   // src/shaders/ground.wgsl:14
 #define SG_TERRAIN_ID 0x80100000u   // src/shaders/terrain.wgsl:23
 #define SG_TERRAIN_STACK 40         // 12 internal levels (8192 cells) need 37
+#define SG_BVH_STACK 48             // prop BVH depth bound enforced by the host builder
 
 namespace sg {
 
@@ -78,6 +79,8 @@ struct Scene {
     const sg_ray_star *catalogue;
     const uint32_t *sky_cells;   // 256*128+1 offsets, then indices
     const uint32_t *milky_way;
+    const float4 *bvh;           // prop BVH nodes (2 float4 each) or null
+    const uint32_t *bvh_prims;   // leaf slots -> primitive index
     TerrainLayout t;
     uint32_t catalogue_count;
 };

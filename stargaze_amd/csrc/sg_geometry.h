@@ -347,6 +347,125 @@ SG_FN float landscape_top(const Scene &sc) {
     return top;
 }
 
+// One non-group primitive of the landscape loop (src/shaders/ground.wgsl:179-239).
+// Updates `closest`; returns true when an occlusion query is answered.
+SG_FN bool prop_test(const Scene &sc, uint32_t i, f3 o, f3 d, Hit &closest, bool occlusion) {
+    const sg_primitive &object = sc.objects[i];
+    f3 oc = {object.center[0], object.center[1], object.center[2]};
+    f3 e = {object.extent[0], object.extent[1], object.extent[2]};
+    float kind = object.center[3];
+    f3 p = o - oc, v = d;
+    f3 angles = f3s(0.f);
+    if (kind == 2.25f) {
+        angles = rock_angles(object.extent[3]);
+        p = rock_to_local(p, angles);
+        v = rock_to_local(v, angles);
+    } else {
+        p = rotate_north(p, -object.extent[3]);
+        v = rotate_north(v, -object.extent[3]);
+    }
+    float distance = 1.0e30f;
+    f3 n = f3s(0.f);
+    if (kind > 2.5f) {
+        float2 bounds = box_interval(p, v, e);
+        if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) return false;
+        float4 veg = kind < 3.5f ? conifer_hit(p, v, e) : (kind < 4.5f ? grass_hit(p, v, e) : trunk_hit(p, v, e));
+        distance = veg.w;
+        n = f3{veg.x, veg.y, veg.z};
+    } else if (kind > 1.5f) {
+        float2 bounds = box_interval(p, v, e);
+        if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) return false;
+        float4 rock = rock_hit(p, v, e);
+        distance = rock.w;
+        n = f3{rock.x, rock.y, rock.z};
+    } else if (kind > 0.5f) {
+        f3 sp = p / e, sd = v / e;
+        float aa = dot(sd, sd), bb = dot(sp, sd), cc = dot(sp, sp) - 1.0f;
+        float hh = bb * bb - aa * cc;
+        if (hh >= 0.0f) {
+            float root = sqrtf(hh);
+            distance = (-bb - root) / aa;
+            if (distance <= 0.001f) distance = (-bb + root) / aa;
+            n = normalize((p + v * distance) / (e * e));
+        }
+    } else {
+        float2 interval = box_interval(p, v, e);
+        if (interval.x <= interval.y && interval.y > 0.001f) {
+            distance = interval.x > 0.001f ? interval.x : interval.y;
+            f3 point = (p + v * distance) / e;
+            f3 ap = fabs3(point);
+            if (ap.x >= ap.y && ap.x >= ap.z) n.x = signf(point.x);
+            else if (ap.y >= ap.z) n.y = signf(point.y);
+            else n.z = signf(point.z);
+        }
+    }
+    if (distance > 0.001f && distance < 1.0e29f && distance / METRES_PER_AU < closest.distance) {
+        if (occlusion) {
+            closest = Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, f3s(0.f)};
+            return true;
+        }
+        n = kind == 2.25f ? rock_from_local(n, angles) : rotate_north(n, object.extent[3]);
+        closest = Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, from_ground(sc, n)};
+    }
+    return false;
+}
+
+// Prop BVH over the leaf primitives (built by the host library from the
+// frame's primitive list; group markers only cull, so dropping them keeps
+// the closest-hit and any-hit answers of the flat skip-count loop). Nodes:
+// lo.xyz + first child / first leaf slot, hi.xyz + leaf count (0 = inner,
+// children adjacent). Boxes are tested with the loop's own slab test and
+// culling rule.
+SG_FN bool prop_bvh_query(const Scene &sc, f3 o, f3 d, Hit &closest, bool occlusion) {
+    uint32_t stack[SG_BVH_STACK];
+    uint32_t count = 0;
+    uint32_t node = 0;
+    for (;;) {
+        const float4 a = sc.bvh[2 * node], b = sc.bvh[2 * node + 1];
+        const uint32_t first = __float_as_uint(a.w), leaves = __float_as_uint(b.w);
+        if (leaves != 0u) {
+            for (uint32_t k = 0; k < leaves; ++k)
+                if (prop_test(sc, sc.bvh_prims[first + k], o, d, closest, occlusion)) return true;
+        } else {
+            float tn[2];
+            bool hit[2];
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const float4 ca = sc.bvh[2 * (first + c)], cb = sc.bvh[2 * (first + c) + 1];
+                f3 centre = {0.5f * (ca.x + cb.x), 0.5f * (ca.y + cb.y), 0.5f * (ca.z + cb.z)};
+                f3 half = {0.5f * (cb.x - ca.x), 0.5f * (cb.y - ca.y), 0.5f * (cb.z - ca.z)};
+                float2 bounds = box_interval(o - centre, d, half);
+                hit[c] = !(bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance);
+                tn[c] = bounds.x;
+            }
+            if (hit[0] && hit[1]) {
+                const uint32_t near = tn[1] < tn[0] ? 1u : 0u;
+                if (count < SG_BVH_STACK) stack[count++] = first + (near ^ 1u);
+                node = first + near;
+                continue;
+            }
+            if (hit[0] || hit[1]) {
+                node = first + (hit[0] ? 0u : 1u);
+                continue;
+            }
+        }
+        // Pop, skipping nodes the closest hit has since moved in front of.
+        bool found = false;
+        while (count > 0u) {
+            const uint32_t cand = stack[--count];
+            const float4 ca = sc.bvh[2 * cand], cb = sc.bvh[2 * cand + 1];
+            f3 centre = {0.5f * (ca.x + cb.x), 0.5f * (ca.y + cb.y), 0.5f * (ca.z + cb.z)};
+            f3 half = {0.5f * (cb.x - ca.x), 0.5f * (cb.y - ca.y), 0.5f * (cb.z - ca.z)};
+            float2 bounds = box_interval(o - centre, d, half);
+            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) continue;
+            node = cand;
+            found = true;
+            break;
+        }
+        if (!found) return false;
+    }
+}
+
 // src/shaders/ground.wgsl:123-242. The host-sphere quadratic runs in FP64.
 SG_FN Hit landscape_query(const Scene &sc, const Ray &ray, float limit, bool occlusion) {
     Hit closest{limit, SG_MISS, f3s(0.f)};
@@ -389,70 +508,25 @@ SG_FN Hit landscape_query(const Scene &sc, const Ray &ray, float limit, bool occ
             if (occlusion) return closest;
         }
     }
+    if (sc.bvh) {
+        if (prop_bvh_query(sc, o, d, closest, occlusion)) return closest;
+        return closest;
+    }
     for (uint32_t i = 0; i < g.ground_counts[0]; ++i) {
         const sg_primitive &object = sc.objects[i];
-        f3 oc = {object.center[0], object.center[1], object.center[2]};
-        f3 e = {object.extent[0], object.extent[1], object.extent[2]};
-        float kind = object.center[3];
-        if (kind < 0.0f) {
+        if (object.center[3] < 0.0f) {
+            f3 oc = {object.center[0], object.center[1], object.center[2]};
+            f3 e = {object.extent[0], object.extent[1], object.extent[2]};
             float2 bounds = box_interval(o - oc, d, e);
             if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance)
-                i += (uint32_t)(-kind);
+                i += (uint32_t)(-object.center[3]);
             continue;
         }
-        f3 p = o - oc, v = d;
-        f3 angles = f3s(0.f);
-        if (kind == 2.25f) {
-            angles = rock_angles(object.extent[3]);
-            p = rock_to_local(p, angles);
-            v = rock_to_local(v, angles);
-        } else {
-            p = rotate_north(p, -object.extent[3]);
-            v = rotate_north(v, -object.extent[3]);
-        }
-        float distance = 1.0e30f;
-        f3 n = f3s(0.f);
-        if (kind > 2.5f) {
-            float2 bounds = box_interval(p, v, e);
-            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) continue;
-            float4 veg = kind < 3.5f ? conifer_hit(p, v, e) : (kind < 4.5f ? grass_hit(p, v, e) : trunk_hit(p, v, e));
-            distance = veg.w;
-            n = f3{veg.x, veg.y, veg.z};
-        } else if (kind > 1.5f) {
-            float2 bounds = box_interval(p, v, e);
-            if (bounds.x > bounds.y || bounds.y <= 0.001f || bounds.x / METRES_PER_AU > closest.distance) continue;
-            float4 rock = rock_hit(p, v, e);
-            distance = rock.w;
-            n = f3{rock.x, rock.y, rock.z};
-        } else if (kind > 0.5f) {
-            f3 sp = p / e, sd = v / e;
-            float aa = dot(sd, sd), bb = dot(sp, sd), cc = dot(sp, sp) - 1.0f;
-            float hh = bb * bb - aa * cc;
-            if (hh >= 0.0f) {
-                float root = sqrtf(hh);
-                distance = (-bb - root) / aa;
-                if (distance <= 0.001f) distance = (-bb + root) / aa;
-                n = normalize((p + v * distance) / (e * e));
-            }
-        } else {
-            float2 interval = box_interval(p, v, e);
-            if (interval.x <= interval.y && interval.y > 0.001f) {
-                distance = interval.x > 0.001f ? interval.x : interval.y;
-                f3 point = (p + v * distance) / e;
-                f3 ap = fabs3(point);
-                if (ap.x >= ap.y && ap.x >= ap.z) n.x = signf(point.x);
-                else if (ap.y >= ap.z) n.y = signf(point.y);
-                else n.z = signf(point.z);
-            }
-        }
-        if (distance > 0.001f && distance < 1.0e29f && distance / METRES_PER_AU < closest.distance) {
-            if (occlusion) return Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, f3s(0.f)};
-            n = kind == 2.25f ? rock_from_local(n, angles) : rotate_north(n, object.extent[3]);
-            closest = Hit{distance / METRES_PER_AU, SG_LOCAL_BASE + 1u + i, from_ground(sc, n)};
-        }
+        if (prop_test(sc, i, o, d, closest, occlusion)) return closest;
     }
     return closest;
 }
+
 SG_FN Hit landscape_hit(const Scene &sc, const Ray &ray) { return landscape_query(sc, ray, 1.0e30f, false); }
 
 // src/shaders/pathtrace.wgsl:173-185

@@ -25,6 +25,12 @@ namespace {
 
 constexpr uint32_t kTraceBlock = 256;
 
+// Register budget of the heavy wavefront kernels, as waves per SIMD.
+#ifndef SG_HEAVY_WAVES
+#define SG_HEAVY_WAVES 1
+#endif
+#define SG_HEAVY __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_HEAVY_WAVES, 8)))
+
 struct TraceArgs {
     float4 *accumulation;
     float4 *backdrop;
@@ -226,7 +232,7 @@ __global__ __launch_bounds__(256) void k_generate(Scene sc, WaveArgs wa, uint32_
     queue[id] = id;
 }
 
-__global__ __launch_bounds__(256) void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+__global__ SG_HEAVY void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = queue[i];
@@ -243,7 +249,7 @@ __global__ __launch_bounds__(256) void k_extend(Scene sc, WaveArgs wa, const uin
     wa.h2[id] = make_float4(h.ring.point.x, h.ring.point.y, h.ring.point.z, 0.f);
 }
 
-__global__ __launch_bounds__(256) void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+__global__ SG_HEAVY void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = queue[i];
@@ -265,7 +271,7 @@ __global__ __launch_bounds__(256) void k_camera_air(Scene sc, WaveArgs wa, const
     wa.rng[id] = st.rng;
 }
 
-__global__ __launch_bounds__(256) void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
+__global__ SG_HEAVY void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
                                                uint32_t *next, uint32_t *next_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
@@ -511,6 +517,12 @@ struct sg_ctx {
     sg_probe_ray *probe_rays = nullptr;
     sg_probe_result *probe_results = nullptr;
     uint32_t probe_capacity = 0;
+    // Prop BVH, rebuilt when the primitive list changes.
+    std::vector<uint8_t> bvh_key;
+    float4 *bvh_nodes = nullptr;
+    uint32_t *bvh_prims = nullptr;
+    size_t bvh_node_capacity = 0, bvh_prim_capacity = 0;
+    bool bvh_valid = false;
     // Integrator selection and wavefront buffers
     uint32_t integrator = SG_INTEGRATOR_WAVEFRONT;
     uint32_t chunk_paths = 1u << 23;
@@ -564,6 +576,153 @@ int alloc_image(sg_ctx *ctx) {
     return SG_OK;
 }
 
+// --- prop BVH ---------------------------------------------------------------
+// Binary BVH over the non-group primitives of the frame's list. Leaf boxes
+// are the world box of each primitive's local extent box under its rotation
+// (north-axis angle, or the packed Rz*Ry*Rx of tumbled rocks; the same
+// conservative box src/ground.rs:172-196 uses for its groups) plus the
+// reference's 0.02 m group margin. Every primitive kind lies inside its
+// extent box or is culled by it in the shader, so the BVH changes no answer.
+constexpr uint32_t kBvhLeaf = 4;
+
+float u2f(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+constexpr size_t kBvhMinPrims = 16;
+
+struct BvhPrim {
+    float lo[3], hi[3], c[3];
+    uint32_t index;
+};
+
+void prim_bounds(const sg_primitive &p, BvhPrim &out) {
+    double r[3][3];
+    if (p.center[3] == 2.25f) {
+        uint32_t bits = (uint32_t)p.extent[3];
+        const double k = 6.283185307179586 / 256.0;
+        double ax = (bits & 255u) * k, ay = ((bits >> 8) & 255u) * k, az = ((bits >> 16) & 255u) * k;
+        double cx = std::cos(ax), sx = std::sin(ax), cy = std::cos(ay), sy = std::sin(ay), cz = std::cos(az),
+               sz = std::sin(az);
+        // Rz(az) * Ry(ay) * Rx(ax)
+        double rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
+        double ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
+        double rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
+        double t[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                t[i][j] = 0;
+                for (int m = 0; m < 3; ++m) t[i][j] += ry[i][m] * rx[m][j];
+            }
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                r[i][j] = 0;
+                for (int m = 0; m < 3; ++m) r[i][j] += rz[i][m] * t[m][j];
+            }
+    } else {
+        double c = std::cos((double)p.extent[3]), s = std::sin((double)p.extent[3]);
+        double rz[3][3] = {{c, -s, 0}, {s, c, 0}, {0, 0, 1}};
+        std::memcpy(r, rz, sizeof(r));
+    }
+    for (int i = 0; i < 3; ++i) {
+        double h = 0;
+        for (int j = 0; j < 3; ++j) h += std::fabs(r[i][j]) * p.extent[j];
+        h = h * (1.0 + 1e-5) + 0.02;
+        out.lo[i] = (float)(p.center[i] - h);
+        out.hi[i] = (float)(p.center[i] + h);
+        out.c[i] = p.center[i];
+    }
+}
+
+// Median split on the longest centroid axis; children stored adjacently.
+void build_bvh(const sg_primitive *objects, uint32_t n, std::vector<float4> &nodes, std::vector<uint32_t> &order) {
+    std::vector<BvhPrim> prims;
+    prims.reserve(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (objects[i].center[3] < 0.0f) continue;
+        BvhPrim b;
+        prim_bounds(objects[i], b);
+        b.index = i;
+        prims.push_back(b);
+    }
+    nodes.clear();
+    order.clear();
+    nodes.resize(2);
+    if (prims.empty()) {
+        nodes[0] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, u2f(0u));
+        nodes[1] = make_float4(-1.0e30f, -1.0e30f, -1.0e30f, u2f(0u));
+        return;
+    }
+    struct Task { uint32_t node, begin, end; };
+    std::vector<Task> tasks{{0u, 0u, (uint32_t)prims.size()}};
+    while (!tasks.empty()) {
+        Task t = tasks.back();
+        tasks.pop_back();
+        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
+        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
+        for (uint32_t k = t.begin; k < t.end; ++k)
+            for (int a = 0; a < 3; ++a) {
+                lo[a] = std::min(lo[a], prims[k].lo[a]);
+                hi[a] = std::max(hi[a], prims[k].hi[a]);
+                clo[a] = std::min(clo[a], prims[k].c[a]);
+                chi[a] = std::max(chi[a], prims[k].c[a]);
+            }
+        const uint32_t count = t.end - t.begin;
+        if (count <= kBvhLeaf) {
+            nodes[2 * t.node] = make_float4(lo[0], lo[1], lo[2], u2f((uint32_t)order.size()));
+            nodes[2 * t.node + 1] = make_float4(hi[0], hi[1], hi[2], u2f(count));
+            for (uint32_t k = t.begin; k < t.end; ++k) order.push_back(prims[k].index);
+            continue;
+        }
+        int axis = 0;
+        for (int a = 1; a < 3; ++a)
+            if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
+        const uint32_t mid = t.begin + count / 2;
+        std::nth_element(prims.begin() + t.begin, prims.begin() + mid, prims.begin() + t.end,
+                         [axis](const BvhPrim &x, const BvhPrim &y) { return x.c[axis] < y.c[axis]; });
+        const uint32_t left = (uint32_t)(nodes.size() / 2);
+        nodes.resize(nodes.size() + 4);
+        nodes[2 * t.node] = make_float4(lo[0], lo[1], lo[2], u2f(left));
+        nodes[2 * t.node + 1] = make_float4(hi[0], hi[1], hi[2], u2f(0u));
+        tasks.push_back({left + 1, mid, t.end});
+        tasks.push_back({left, t.begin, mid});
+    }
+}
+
+// Rebuild and upload the BVH when the primitive list changed.
+int ensure_bvh(sg_ctx *ctx, const sg_frame *frame) {
+    if (frame->ground_count < kBvhMinPrims) {
+        ctx->bvh_valid = false;
+        return SG_OK;
+    }
+    const uint8_t *bytes = reinterpret_cast<const uint8_t *>(frame->ground);
+    const size_t n = (size_t)frame->ground_count * sizeof(sg_primitive);
+    if (ctx->bvh_valid && ctx->bvh_key.size() == n && std::memcmp(ctx->bvh_key.data(), bytes, n) == 0) return SG_OK;
+    std::vector<float4> nodes;
+    std::vector<uint32_t> order;
+    build_bvh(frame->ground, frame->ground_count, nodes, order);
+    if (order.empty()) order.push_back(0u);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (nodes.size() > ctx->bvh_node_capacity) {
+        if (ctx->bvh_nodes) SG_HIP(ctx, hipFree(ctx->bvh_nodes));
+        ctx->bvh_nodes = nullptr;
+        SG_HIP(ctx, hipMalloc(&ctx->bvh_nodes, nodes.size() * sizeof(float4)));
+        ctx->bvh_node_capacity = nodes.size();
+    }
+    if (order.size() > ctx->bvh_prim_capacity) {
+        if (ctx->bvh_prims) SG_HIP(ctx, hipFree(ctx->bvh_prims));
+        ctx->bvh_prims = nullptr;
+        SG_HIP(ctx, hipMalloc(&ctx->bvh_prims, order.size() * sizeof(uint32_t)));
+        ctx->bvh_prim_capacity = order.size();
+    }
+    SG_HIP(ctx, hipMemcpy(ctx->bvh_nodes, nodes.data(), nodes.size() * sizeof(float4), hipMemcpyHostToDevice));
+    SG_HIP(ctx, hipMemcpy(ctx->bvh_prims, order.data(), order.size() * sizeof(uint32_t), hipMemcpyHostToDevice));
+    ctx->bvh_key.assign(bytes, bytes + n);
+    ctx->bvh_valid = true;
+    return SG_OK;
+}
+
 Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     Scene sc;
     std::memset(&sc, 0, sizeof(sc));
@@ -576,6 +735,8 @@ Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     sc.sky_cells = ctx->sky_cells;
     sc.catalogue_count = ctx->catalogue_count;
     sc.milky_way = ctx->milky_way;
+    sc.bvh = ctx->bvh_valid ? ctx->bvh_nodes : nullptr;
+    sc.bvh_prims = ctx->bvh_valid ? ctx->bvh_prims : nullptr;
     sc.t = terrain_layout(settings.globals.ground_counts[2]);
     return sc;
 }
@@ -786,7 +947,8 @@ int sg_destroy(sg_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
-                       ctx->probe_results, ctx->wave_block, ctx->queues[0]};
+                       ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
+                       ctx->bvh_prims};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
@@ -956,6 +1118,8 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
         if (frame->ground_count)
             SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
                                        hipMemcpyHostToDevice, ctx->stream));
+        int brc = ensure_bvh(ctx, frame);
+        if (brc != SG_OK) return brc;
         Scene sc = make_scene(ctx, st);
         const bool terrain_on = g.ground_counts[1] != 0u && g.ground_north[3] != 0.0f;
         const uint32_t radius_bits = float_bits(g.ground_up[3]);
@@ -1113,6 +1277,8 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     if (frame->ground_count)
         SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
                                    hipMemcpyHostToDevice, ctx->stream));
+    int brc = ensure_bvh(ctx, frame);
+    if (brc != SG_OK) return brc;
     Scene sc = make_scene(ctx, st);
     const uint32_t radius_bits = float_bits(st.globals.ground_up[3]);
     if (ctx->terrain && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {

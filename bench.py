@@ -86,20 +86,21 @@ def oracle_tracer(width, height, spp, bounces, terrain):
     return t
 
 
-def work_model(frame, terrain, width, height, spp, bounces, rows=4):
-    """Event counts per path sample from the oracle on bands of rows (the
-    image's quarter, half and three-quarter heights)."""
+def work_model(frame, terrain, width, height, spp, bounces, rows=16):
+    """Event counts per path sample of a steady-state pass (not the first,
+    which takes extra light strata near limbs) from the oracle on bands of
+    rows at the image's quarter, half and three-quarter heights."""
     import oracle
 
     t = oracle_tracer(width, height, spp, bounces, terrain)
+    t.encode(frame, rows=(0, 1))  # first pass: fills the sky and terrain caches
     ev = None
-    bands = [height // 4, height // 2, 3 * height // 4]
+    bands = [height * (2 * k + 1) // 8 for k in range(4)]
     per_band = max(1, rows // len(bands))
     for y0 in bands:
+        t.history_samples = spp
         e = oracle.count_events(t, frame, (y0, y0 + per_band))
         ev = e if ev is None else {k: ev[k] + e[k] for k in ev}
-        t.history_samples = 0
-        t.sequence = 0
     rows = per_band * len(bands)
     n = width * rows * spp
     per = {k: v / n for k, v in ev.items()}

@@ -126,10 +126,20 @@ enum {
     EV_VERTEX, EV_RNG, EV_COUNT
 };
 static uint64_t counters[EV_COUNT];
+static _Thread_local uint64_t local_counters[EV_COUNT];
 static int counting = 0;
-#define COUNT(e) do { if (counting) counters[e] += 1; } while (0)
+#define COUNT(e) do { if (counting) local_counters[e] += 1; } while (0)
 void orc_set_counting(int on) { counting = on; memset(counters, 0, sizeof(counters)); }
 void orc_counters(uint64_t *out) { memcpy(out, counters, sizeof(counters)); }
+/* Folds the calling thread's counts into the totals. */
+static void flush_counters(void) {
+    if (!counting) return;
+#pragma omp critical(orc_counters)
+    for (int i = 0; i < EV_COUNT; ++i) {
+        counters[i] += local_counters[i];
+        local_counters[i] = 0;
+    }
+}
 uint32_t orc_counter_count(void) { return EV_COUNT; }
 
 /* --- RNG (src/shaders/pathtrace.wgsl:87-98) --- */
@@ -1110,7 +1120,7 @@ static R shell_exit(const Ctx *c, v3 origin, v3 dir) {
 static R density_integral(const Ctx *c, v3 origin, v3 dir, uint32_t steps) {
     R distance = shell_exit(c, origin, dir);
     if (distance <= 0.0) return 0.0;
-    if (counting) counters[EV_DENSITY_STEP] += steps;
+    if (counting) local_counters[EV_DENSITY_STEP] += steps;
     R dt = distance / (R)steps;
     R inv_h = 1.0 / scale_height(c);
     R r = host_radius(c);
@@ -1675,7 +1685,9 @@ void orc_encode_rows(const orc_scene *sc, uint32_t row_begin, uint32_t row_end, 
     const uint32_t old_count = s->counts[1];
     const int has_air = s->atmo_params.v[3] >= 0.5f;
     const uint32_t base_sequence = s->sampling[0];
-#pragma omp parallel for schedule(dynamic, 1) if (!counting)
+#pragma omp parallel
+    {
+#pragma omp for schedule(dynamic, 1)
     for (uint32_t y = row_begin; y < row_end; ++y) {
         for (uint32_t x = 0; x < width; ++x) {
             Ctx c = c0;
@@ -1738,6 +1750,8 @@ void orc_encode_rows(const orc_scene *sc, uint32_t row_begin, uint32_t row_end, 
                 }
             }
         }
+    }
+    flush_counters();
     }
 }
 

@@ -18,8 +18,11 @@
 #define SG_MISS 0xffffffffu
 #define SG_LOCAL_BASE 0x80000000u   // src/shaders/ground.wgsl:14
 #define SG_TERRAIN_ID 0x80100000u   // src/shaders/terrain.wgsl:23
-#define SG_TERRAIN_STACK 40         // 12 internal levels (8192 cells) need 37
-#define SG_BVH_STACK 48             // prop BVH depth bound enforced by the host builder
+
+// Per-lane traversal stacks (scratch memory). The terrain stack holds
+// 3 * levels + 1 entries: 12 internal levels (8192 cells) need 37.
+#define SG_TERRAIN_STACK 40
+#define SG_BVH_STACK 48  // the host builder's median split stays far shallower
 
 namespace sg {
 
@@ -47,9 +50,13 @@ static_assert(sizeof(sg_ray_star) == 32, "RayStar layout");
 static_assert(sizeof(sg_camera_air) == 16, "CameraAir layout");
 
 // Terrain buffer geometry derived from ground_counts.z (src/shaders/terrain.wgsl:6-20).
+// `data` floats come from the caller in the reference's layout (heights,
+// derivatives when cells <= 1024, min/max per node). The caches after them
+// are this library's own: node AABBs padded to 8 floats (two aligned
+// float4 loads per node) and vertices with the reference's stride.
 struct TerrainLayout {
     uint32_t cells, width, compact, stride;
-    uint64_t vertices, nodes, bounds, cache, vertex_cache;
+    uint64_t vertices, nodes, bounds, data, cache, vertex_cache, total;
 };
 
 SG_FN TerrainLayout terrain_layout(uint32_t ground_counts_z) {
@@ -63,8 +70,10 @@ SG_FN TerrainLayout terrain_layout(uint32_t ground_counts_z) {
     uint64_t leaves = t.cells / 2u;
     t.nodes = (4u * leaves * leaves - 1u) / 3u;
     t.bounds = (t.compact ? 1u : 3u) * t.vertices;
-    t.cache = t.bounds + 2u * t.nodes;
-    t.vertex_cache = t.cache + 6u * t.nodes;
+    t.data = t.bounds + 2u * t.nodes;
+    t.cache = (t.data + 3u) & ~(uint64_t)3u;
+    t.vertex_cache = t.cache + 8u * t.nodes;
+    t.total = t.vertex_cache + (uint64_t)t.stride * t.vertices;
     return t;
 }
 
@@ -274,8 +283,8 @@ SG_FN f3 environment(const Scene &sc, f3 view_direction, bool primary) {
     f3 direction = world_direction(sc, view_direction);
     float phi = atan2f(direction.y, direction.x) + PI;
     float theta = acosf(clampf(direction.z, -1.0f, 1.0f));
-    uint32_t x = min((uint32_t)(phi * (256.0f / TAU)), 255u);
-    uint32_t y = min((uint32_t)(theta * (128.0f / PI)), 127u);
+    uint32_t x = umin((uint32_t)(phi * (256.0f / TAU)), 255u);
+    uint32_t y = umin((uint32_t)(theta * (128.0f / PI)), 127u);
     uint32_t cell = y * 256u + x;
     for (uint32_t j = sc.sky_cells[cell]; j < sc.sky_cells[cell + 1u]; ++j) {
         const sg_ray_star &star = sc.catalogue[sc.sky_cells[32769u + j]];
@@ -292,7 +301,7 @@ SG_FN f3 milky_way_texel(const Scene &sc, int px, int py) {
     int width = (int)sc.s.background[1];
     int height = (int)sc.s.background[2];
     int x = (px + width) % width;
-    int y = min(max(py, 0), height - 1);
+    int y = imin(imax(py, 0), height - 1);
     uint32_t packed = sc.milky_way[(uint32_t)(y * width + x)];
     return f3{(float)(packed & 1023u), (float)((packed >> 10) & 1023u), (float)((packed >> 20) & 1023u)} /
            1023.0f;

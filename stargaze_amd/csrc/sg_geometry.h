@@ -225,8 +225,9 @@ SG_FN f3 terrain_normal(const Scene &sc, uint32_t x, uint32_t z) {
 }
 // Slab test against a cached node AABB (src/shaders/terrain.wgsl:118-130).
 SG_FN float2 terrain_interval(const Scene &sc, f3 o, f3 inv, uint32_t node) {
-    const float *c = sc.terrain + sc.t.cache + (uint64_t)node * 6u;
-    f3 lo = {c[0], c[1], c[2]}, hi = {c[3], c[4], c[5]};
+    const float4 *c = reinterpret_cast<const float4 *>(sc.terrain + sc.t.cache) + 2u * (uint64_t)node;
+    const float4 l = c[0], h = c[1];
+    f3 lo = {l.x, l.y, l.z}, hi = {h.x, h.y, h.z};
     bool px = fabsf(inv.x) >= 1.0e29f, py = fabsf(inv.y) >= 1.0e29f, pz = fabsf(inv.z) >= 1.0e29f;
     if ((px && (o.x < lo.x || o.x > hi.x)) || (py && (o.y < lo.y || o.y > hi.y)) ||
         (pz && (o.z < lo.z || o.z > hi.z)))
@@ -247,6 +248,60 @@ SG_FN uint32_t terrain_compact_bits(uint32_t value) {
     return (x | (x >> 8)) & 0x0000ffffu;
 }
 SG_FN uint32_t first_leading_bit(uint32_t v) { return 31u - (uint32_t)__builtin_clz(v); }
+
+// One entry of cache_terrain_geometry (src/shaders/terrain.wgsl:242-285):
+// node `index` (if it is a node) gets its curved, 0.02 m-padded AABB and
+// vertex `index` its curved position and normal (12-bit packed above 1024
+// cells). `terrain` is the scene's terrain buffer.
+SG_FN void terrain_cache_entry(const Scene &sc, float *terrain, uint64_t index) {
+    const TerrainLayout &t = sc.t;
+    if (index >= t.vertices) return;
+    const float scale = terrain_scale(sc);
+    if (index < t.nodes) {
+        uint32_t node = (uint32_t)index;
+        uint32_t level = first_leading_bit(3u * node + 1u) / 2u;
+        uint32_t size = t.cells >> level;
+        uint32_t morton = node - ((1u << (2u * level)) - 1u) / 3u;
+        uint32_t nx = terrain_compact_bits(morton) * size, nz = terrain_compact_bits(morton >> 1) * size;
+        float lox = terrain_coordinate(sc, nx), loz = terrain_coordinate(sc, nz);
+        float hix = terrain_coordinate(sc, nx + size), hiz = terrain_coordinate(sc, nz + size);
+        float nearx = clampf(0.0f, lox, hix), nearz = clampf(0.0f, loz, hiz);
+        float farx = fmaxf(fabsf(lox), fabsf(hix)), farz = fmaxf(fabsf(loz), fabsf(hiz));
+        float lower = terrain[t.bounds + (uint64_t)node * 2u] * scale + terrain_curvature(sc, farx, farz) - 0.02f;
+        float upper = terrain[t.bounds + (uint64_t)node * 2u + 1u] * scale + terrain_curvature(sc, nearx, nearz) + 0.02f;
+        float4 *c = reinterpret_cast<float4 *>(terrain + t.cache) + 2u * (uint64_t)node;
+        c[0] = make_float4(lox, lower, loz, 0.0f);
+        c[1] = make_float4(hix, upper, hiz, 0.0f);
+    }
+    const uint32_t x = (uint32_t)(index % t.width), z = (uint32_t)(index / t.width);
+    const float px = terrain_coordinate(sc, x), pz = terrain_coordinate(sc, z);
+    const float vy = terrain[index] * scale + terrain_curvature(sc, px, pz);
+    const float r = sc.s.globals.ground_up[3];
+    const float inv = 1.0f / sqrtf(fmaxf(r * r - (px * px + pz * pz), 1.0f));
+    const float cgx = px * inv, cgz = pz * inv;
+    f3 normal;
+    if (t.compact) {
+        uint32_t lx = x > 0u ? x - 1u : 0u, lz = z > 0u ? z - 1u : 0u;
+        uint32_t hx = umin(x + 1u, t.cells), hz = umin(z + 1u, t.cells);
+        float dx = (terrain[(uint64_t)z * t.width + hx] - terrain[(uint64_t)z * t.width + lx]) /
+                   ((terrain_coordinate(sc, hx) - terrain_coordinate(sc, lx)) / scale);
+        float dz = (terrain[(uint64_t)hz * t.width + x] - terrain[(uint64_t)lz * t.width + x]) /
+                   ((terrain_coordinate(sc, hz) - terrain_coordinate(sc, lz)) / scale);
+        normal = normalize(f3{-dx + cgx, 1.0f, -dz + cgz});
+    } else {
+        normal = normalize(f3{-terrain[t.vertices + index] + cgx, 1.0f, -terrain[2u * t.vertices + index] + cgz});
+    }
+    float *v = terrain + t.vertex_cache + index * t.stride;
+    if (t.compact) {
+        uint32_t qx = (uint32_t)rintf(clampf(normal.x * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
+        uint32_t qz = (uint32_t)rintf(clampf(normal.z * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
+        v[0] = vy;
+        v[1] = (float)(qx | (qz << 12));
+        return;
+    }
+    v[0] = px; v[1] = vy; v[2] = pz;
+    v[3] = normal.x; v[4] = normal.y; v[5] = normal.z;
+}
 
 // Near-to-far stack traversal (src/shaders/terrain.wgsl:152-234). Theme 0
 // shading: the smooth barycentric normal (terrain_shading is identity there).
@@ -342,8 +397,8 @@ SG_FN float4 terrain_query(const Scene &sc, f3 o, f3 d, float initial_limit, boo
 
 // Highest landscape point (src/shaders/ground.wgsl:244-251).
 SG_FN float landscape_top(const Scene &sc) {
-    float top = __uint_as_float(sc.s.globals.ground_counts[3]);
-    if (sc.s.globals.ground_counts[1] != 0u) top = fmaxf(top, sc.terrain[sc.t.cache + 4u]);
+    float top = sg_u2f(sc.s.globals.ground_counts[3]);
+    if (sc.s.globals.ground_counts[1] != 0u) top = fmaxf(top, sc.terrain[sc.t.cache + 5u]);
     return top;
 }
 
@@ -413,16 +468,18 @@ SG_FN bool prop_test(const Scene &sc, uint32_t i, f3 o, f3 d, Hit &closest, bool
 // Prop BVH over the leaf primitives (built by the host library from the
 // frame's primitive list; group markers only cull, so dropping them keeps
 // the closest-hit and any-hit answers of the flat skip-count loop). Nodes:
-// lo.xyz + first child / first leaf slot, hi.xyz + leaf count (0 = inner,
-// children adjacent). Boxes are tested with the loop's own slab test and
-// culling rule.
+// lo.xyz + first child / first leaf slot, hi.xyz + leaf count (inner nodes:
+// 0x80000000 | split axis; children adjacent). Boxes are tested with the
+// loop's own slab test and culling rule; nearer child first. A per-lane
+// stack in scratch measured faster on MI355X than stackless (parent-link)
+// traversal, which re-tests boxes on the way up.
 SG_FN bool prop_bvh_query(const Scene &sc, f3 o, f3 d, Hit &closest, bool occlusion) {
     uint32_t stack[SG_BVH_STACK];
     uint32_t count = 0;
     uint32_t node = 0;
     for (;;) {
         const float4 a = sc.bvh[2 * node], b = sc.bvh[2 * node + 1];
-        const uint32_t first = __float_as_uint(a.w), leaves = __float_as_uint(b.w);
+        const uint32_t first = sg_f2u(a.w), leaves = (sg_f2u(b.w) & 0x80000000u) ? 0u : sg_f2u(b.w);
         if (leaves != 0u) {
             for (uint32_t k = 0; k < leaves; ++k)
                 if (prop_test(sc, sc.bvh_prims[first + k], o, d, closest, occlusion)) return true;

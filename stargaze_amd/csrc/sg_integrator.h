@@ -214,7 +214,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     float r = host_radius(sc);
     f3 center = atmo_center(sc);
     float H = atmo_scale_height(sc);
-    uint32_t steps = min(max((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
     float dt = t_max / (float)steps;
     uint32_t jitter_rng = hash_u32(rng ^ 0x3c6ef372u);
     float jitter_draw = rnd(jitter_rng);
@@ -237,7 +237,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         float sun = brightest_risen_light(sc, origin);
         if (sun > 0.0f) {
             float ratio = maxc(reflected) / sun;
-            float minimum = rsqrtf(clampf((float)accumulated + 1.0f, 1.0f, 4096.0f));
+            float minimum = 1.0f / sqrtf(clampf((float)accumulated + 1.0f, 1.0f, 4096.0f));
             float keep = clampf(ratio * 1.0e3f, minimum, 1.0f);
             reflected = rnd(reflected_rng) < keep ? reflected / keep : f3s(0.0f);
         }
@@ -292,7 +292,7 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
     float t_max = fminf(limit, exit);
     if (t_max <= 0.0f) return f3s(1.0f);
     float H = atmo_scale_height(sc);
-    uint32_t steps = min(max((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
     float dt = t_max / (float)steps;
     uint32_t jitter_rng = hash_u32(seed ^ 0x3c6ef372u);
     float jitter_draw = rnd(jitter_rng);
@@ -312,7 +312,7 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
 
 // --- surface sky cache (src/shaders/forest.wgsl:3-54) ------------------------
 SG_FN f3 surface_sky_texel(const Scene &sc, int x, int y) {
-    float4 t = sc.sky_light[(uint32_t)min(max(y, 0), 7) * 16u + (uint32_t)((x + 16) % 16)];
+    float4 t = sc.sky_light[(uint32_t)imin(imax(y, 0), 7) * 16u + (uint32_t)((x + 16) % 16)];
     return f3{t.x, t.y, t.z};
 }
 SG_FN f3 surface_sky_lookup(const Scene &sc, f3 dir) {

@@ -17,6 +17,7 @@
 #include <string>
 #include <vector>
 
+#include "sg_bvh.h"
 #include "sg_integrator.h"
 
 using namespace sg;
@@ -369,54 +370,7 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveArgs wa, float4 *accumul
 
 // cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
 __global__ void k_terrain_cache(Scene sc, float *terrain) {
-    const uint64_t index = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const TerrainLayout &t = sc.t;
-    if (index >= t.vertices) return;
-    const float scale = terrain_scale(sc);
-    if (index < t.nodes) {
-        uint32_t node = (uint32_t)index;
-        uint32_t level = first_leading_bit(3u * node + 1u) / 2u;
-        uint32_t size = t.cells >> level;
-        uint32_t morton = node - ((1u << (2u * level)) - 1u) / 3u;
-        uint32_t nx = terrain_compact_bits(morton) * size, nz = terrain_compact_bits(morton >> 1) * size;
-        float lox = terrain_coordinate(sc, nx), loz = terrain_coordinate(sc, nz);
-        float hix = terrain_coordinate(sc, nx + size), hiz = terrain_coordinate(sc, nz + size);
-        float nearx = clampf(0.0f, lox, hix), nearz = clampf(0.0f, loz, hiz);
-        float farx = fmaxf(fabsf(lox), fabsf(hix)), farz = fmaxf(fabsf(loz), fabsf(hiz));
-        float lower = terrain[t.bounds + (uint64_t)node * 2u] * scale + terrain_curvature(sc, farx, farz) - 0.02f;
-        float upper = terrain[t.bounds + (uint64_t)node * 2u + 1u] * scale + terrain_curvature(sc, nearx, nearz) + 0.02f;
-        float *c = terrain + t.cache + (uint64_t)node * 6u;
-        c[0] = lox; c[1] = lower; c[2] = loz;
-        c[3] = hix; c[4] = upper; c[5] = hiz;
-    }
-    const uint32_t x = (uint32_t)(index % t.width), z = (uint32_t)(index / t.width);
-    const float px = terrain_coordinate(sc, x), pz = terrain_coordinate(sc, z);
-    const float vy = terrain[index] * scale + terrain_curvature(sc, px, pz);
-    const float r = sc.s.globals.ground_up[3];
-    const float inv = 1.0f / sqrtf(fmaxf(r * r - (px * px + pz * pz), 1.0f));
-    const float cgx = px * inv, cgz = pz * inv;
-    f3 normal;
-    if (t.compact) {
-        uint32_t lx = x > 0u ? x - 1u : 0u, lz = z > 0u ? z - 1u : 0u;
-        uint32_t hx = min(x + 1u, t.cells), hz = min(z + 1u, t.cells);
-        float dx = (terrain[(uint64_t)z * t.width + hx] - terrain[(uint64_t)z * t.width + lx]) /
-                   ((terrain_coordinate(sc, hx) - terrain_coordinate(sc, lx)) / scale);
-        float dz = (terrain[(uint64_t)hz * t.width + x] - terrain[(uint64_t)lz * t.width + x]) /
-                   ((terrain_coordinate(sc, hz) - terrain_coordinate(sc, lz)) / scale);
-        normal = normalize(f3{-dx + cgx, 1.0f, -dz + cgz});
-    } else {
-        normal = normalize(f3{-terrain[t.vertices + index] + cgx, 1.0f, -terrain[2u * t.vertices + index] + cgz});
-    }
-    float *v = terrain + t.vertex_cache + index * t.stride;
-    if (t.compact) {
-        uint32_t qx = (uint32_t)rintf(clampf(normal.x * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
-        uint32_t qz = (uint32_t)rintf(clampf(normal.z * 0.5f + 0.5f, 0.0f, 1.0f) * 4095.0f);
-        v[0] = vy;
-        v[1] = (float)(qx | (qz << 12));
-        return;
-    }
-    v[0] = px; v[1] = vy; v[2] = pz;
-    v[3] = normal.x; v[4] = normal.y; v[5] = normal.z;
+    terrain_cache_entry(sc, terrain, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
 }
 
 // cache_surface_sky (src/shaders/forest.wgsl:3-17): 16x8 texels.
@@ -576,123 +530,9 @@ int alloc_image(sg_ctx *ctx) {
     return SG_OK;
 }
 
-// --- prop BVH ---------------------------------------------------------------
-// Binary BVH over the non-group primitives of the frame's list. Leaf boxes
-// are the world box of each primitive's local extent box under its rotation
-// (north-axis angle, or the packed Rz*Ry*Rx of tumbled rocks; the same
-// conservative box src/ground.rs:172-196 uses for its groups) plus the
-// reference's 0.02 m group margin. Every primitive kind lies inside its
-// extent box or is culled by it in the shader, so the BVH changes no answer.
-constexpr uint32_t kBvhLeaf = 4;
-
-float u2f(uint32_t u) {
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-constexpr size_t kBvhMinPrims = 16;
-
-struct BvhPrim {
-    float lo[3], hi[3], c[3];
-    uint32_t index;
-};
-
-void prim_bounds(const sg_primitive &p, BvhPrim &out) {
-    double r[3][3];
-    if (p.center[3] == 2.25f) {
-        uint32_t bits = (uint32_t)p.extent[3];
-        const double k = 6.283185307179586 / 256.0;
-        double ax = (bits & 255u) * k, ay = ((bits >> 8) & 255u) * k, az = ((bits >> 16) & 255u) * k;
-        double cx = std::cos(ax), sx = std::sin(ax), cy = std::cos(ay), sy = std::sin(ay), cz = std::cos(az),
-               sz = std::sin(az);
-        // Rz(az) * Ry(ay) * Rx(ax)
-        double rz[3][3] = {{cz, -sz, 0}, {sz, cz, 0}, {0, 0, 1}};
-        double ry[3][3] = {{cy, 0, sy}, {0, 1, 0}, {-sy, 0, cy}};
-        double rx[3][3] = {{1, 0, 0}, {0, cx, -sx}, {0, sx, cx}};
-        double t[3][3];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                t[i][j] = 0;
-                for (int m = 0; m < 3; ++m) t[i][j] += ry[i][m] * rx[m][j];
-            }
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                r[i][j] = 0;
-                for (int m = 0; m < 3; ++m) r[i][j] += rz[i][m] * t[m][j];
-            }
-    } else {
-        double c = std::cos((double)p.extent[3]), s = std::sin((double)p.extent[3]);
-        double rz[3][3] = {{c, -s, 0}, {s, c, 0}, {0, 0, 1}};
-        std::memcpy(r, rz, sizeof(r));
-    }
-    for (int i = 0; i < 3; ++i) {
-        double h = 0;
-        for (int j = 0; j < 3; ++j) h += std::fabs(r[i][j]) * p.extent[j];
-        h = h * (1.0 + 1e-5) + 0.02;
-        out.lo[i] = (float)(p.center[i] - h);
-        out.hi[i] = (float)(p.center[i] + h);
-        out.c[i] = p.center[i];
-    }
-}
-
-// Median split on the longest centroid axis; children stored adjacently.
-void build_bvh(const sg_primitive *objects, uint32_t n, std::vector<float4> &nodes, std::vector<uint32_t> &order) {
-    std::vector<BvhPrim> prims;
-    prims.reserve(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        if (objects[i].center[3] < 0.0f) continue;
-        BvhPrim b;
-        prim_bounds(objects[i], b);
-        b.index = i;
-        prims.push_back(b);
-    }
-    nodes.clear();
-    order.clear();
-    nodes.resize(2);
-    if (prims.empty()) {
-        nodes[0] = make_float4(1.0e30f, 1.0e30f, 1.0e30f, u2f(0u));
-        nodes[1] = make_float4(-1.0e30f, -1.0e30f, -1.0e30f, u2f(0u));
-        return;
-    }
-    struct Task { uint32_t node, begin, end; };
-    std::vector<Task> tasks{{0u, 0u, (uint32_t)prims.size()}};
-    while (!tasks.empty()) {
-        Task t = tasks.back();
-        tasks.pop_back();
-        float lo[3] = {1e30f, 1e30f, 1e30f}, hi[3] = {-1e30f, -1e30f, -1e30f};
-        float clo[3] = {1e30f, 1e30f, 1e30f}, chi[3] = {-1e30f, -1e30f, -1e30f};
-        for (uint32_t k = t.begin; k < t.end; ++k)
-            for (int a = 0; a < 3; ++a) {
-                lo[a] = std::min(lo[a], prims[k].lo[a]);
-                hi[a] = std::max(hi[a], prims[k].hi[a]);
-                clo[a] = std::min(clo[a], prims[k].c[a]);
-                chi[a] = std::max(chi[a], prims[k].c[a]);
-            }
-        const uint32_t count = t.end - t.begin;
-        if (count <= kBvhLeaf) {
-            nodes[2 * t.node] = make_float4(lo[0], lo[1], lo[2], u2f((uint32_t)order.size()));
-            nodes[2 * t.node + 1] = make_float4(hi[0], hi[1], hi[2], u2f(count));
-            for (uint32_t k = t.begin; k < t.end; ++k) order.push_back(prims[k].index);
-            continue;
-        }
-        int axis = 0;
-        for (int a = 1; a < 3; ++a)
-            if (chi[a] - clo[a] > chi[axis] - clo[axis]) axis = a;
-        const uint32_t mid = t.begin + count / 2;
-        std::nth_element(prims.begin() + t.begin, prims.begin() + mid, prims.begin() + t.end,
-                         [axis](const BvhPrim &x, const BvhPrim &y) { return x.c[axis] < y.c[axis]; });
-        const uint32_t left = (uint32_t)(nodes.size() / 2);
-        nodes.resize(nodes.size() + 4);
-        nodes[2 * t.node] = make_float4(lo[0], lo[1], lo[2], u2f(left));
-        nodes[2 * t.node + 1] = make_float4(hi[0], hi[1], hi[2], u2f(0u));
-        tasks.push_back({left + 1, mid, t.end});
-        tasks.push_back({left, t.begin, mid});
-    }
-}
-
 // Rebuild and upload the BVH when the primitive list changed.
 int ensure_bvh(sg_ctx *ctx, const sg_frame *frame) {
-    if (frame->ground_count < kBvhMinPrims) {
+    if (frame->ground_count < sg_host::kBvhMinPrims) {
         ctx->bvh_valid = false;
         return SG_OK;
     }
@@ -701,7 +541,7 @@ int ensure_bvh(sg_ctx *ctx, const sg_frame *frame) {
     if (ctx->bvh_valid && ctx->bvh_key.size() == n && std::memcmp(ctx->bvh_key.data(), bytes, n) == 0) return SG_OK;
     std::vector<float4> nodes;
     std::vector<uint32_t> order;
-    build_bvh(frame->ground, frame->ground_count, nodes, order);
+    sg_host::build_bvh(frame->ground, frame->ground_count, nodes, order);
     if (order.empty()) order.push_back(0u);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (nodes.size() > ctx->bvh_node_capacity) {
@@ -1035,8 +875,8 @@ int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint
     uint32_t log2 = 0;
     while ((1u << log2) < cells) ++log2;
     TerrainLayout t = terrain_layout(log2 << 8);
-    if (float_count != t.cache) return fail(ctx, SG_E_INVALID, "terrain data size does not match its cell count");
-    const uint64_t total = t.vertex_cache + (uint64_t)t.stride * t.vertices;
+    if (float_count != t.data) return fail(ctx, SG_E_INVALID, "terrain data size does not match its cell count");
+    const uint64_t total = t.total;
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->terrain && ctx->terrain_floats != total) {
         SG_HIP(ctx, hipFree(ctx->terrain));

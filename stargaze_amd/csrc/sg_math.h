@@ -29,6 +29,12 @@ template <class T> SG_FN T dot(V3<T> a, V3<T> b) { return a.x * b.x + a.y * b.y 
 template <class T> SG_FN V3<T> cross(V3<T> a, V3<T> b) {
     return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
 }
+SG_FN uint32_t sg_f2u(float f) { return __builtin_bit_cast(uint32_t, f); }
+SG_FN float sg_u2f(uint32_t u) { return __builtin_bit_cast(float, u); }
+SG_FN uint32_t umin(uint32_t a, uint32_t b) { return a < b ? a : b; }
+SG_FN uint32_t umax(uint32_t a, uint32_t b) { return a > b ? a : b; }
+SG_FN int imin(int a, int b) { return a < b ? a : b; }
+SG_FN int imax(int a, int b) { return a > b ? a : b; }
 SG_FN float sqrtT(float x) { return sqrtf(x); }
 SG_FN double sqrtT(double x) { return sqrt(x); }
 SG_FN float length(f3 a) { return sqrtf(dot(a, a)); }

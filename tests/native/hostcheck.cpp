@@ -1,0 +1,81 @@
+// Host build of the product's device helpers, for CPU tests of their logic
+// (termination and answers of the stackless traversals). Test code only:
+// the product library is stargaze_amd/libstargaze_amd.so.
+#include <cstring>
+
+#include "../../stargaze_amd/csrc/sg_integrator.h"
+
+using namespace sg;
+
+extern "C" {
+
+// Product terrain layout for `settings` (floats of caller data and of the
+// whole buffer) and the product's cache pass over `terrain` on the host.
+void hc_terrain_layout(const sg_settings *settings, uint64_t *data, uint64_t *total) {
+    TerrainLayout t = terrain_layout(settings->globals.ground_counts[2]);
+    *data = t.data;
+    *total = t.total;
+}
+void hc_terrain_cache(const sg_settings *settings, float *terrain) {
+    Scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.s = *settings;
+    sc.t = terrain_layout(settings->globals.ground_counts[2]);
+    for (uint64_t i = 0; i < sc.t.vertices; ++i) terrain_cache_entry(sc, terrain, i);
+}
+
+// Landscape query (kind 5) or terrain query (kind 4) for `count` probe rays.
+// `bvh` may be null (flat primitive loop). Terrain may be null when off.
+int hc_query(const sg_settings *settings, const sg_sphere *bodies, const sg_primitive *objects,
+             const float *terrain, const float *bvh_nodes, const uint32_t *bvh_prims,
+             uint32_t kind, const sg_probe_ray *rays, uint32_t count, sg_probe_result *out) {
+    Scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.s = *settings;
+    sc.bodies = reinterpret_cast<const Body *>(bodies);
+    sc.objects = objects;
+    sc.terrain = terrain;
+    sc.bvh = reinterpret_cast<const float4 *>(bvh_nodes);
+    sc.bvh_prims = bvh_prims;
+    sc.t = terrain_layout(settings->globals.ground_counts[2]);
+    for (uint32_t i = 0; i < count; ++i) {
+        const sg_probe_ray &pr = rays[i];
+        Ray ray{d3{pr.offset[0], pr.offset[1], pr.offset[2]},
+                f3{pr.direction[0], pr.direction[1], pr.direction[2]}, pr.anchor};
+        sg_probe_result r;
+        std::memset(&r, 0, sizeof(r));
+        if (kind == SG_PROBE_TERRAIN) {
+            float4 h = terrain_query(sc, tof(ray.offset), ray.direction, 1.0e30f, pr.target != 0u);
+            r.distance = h.w;
+            r.index = h.w < 1.0e29f ? SG_TERRAIN_ID : SG_MISS;
+            r.normal[0] = h.x; r.normal[1] = h.y; r.normal[2] = h.z;
+        } else {
+            Hit h = landscape_query(sc, ray, 1.0e30f, pr.target != 0u);
+            r.distance = h.distance;
+            r.index = h.index;
+            r.normal[0] = h.normal.x; r.normal[1] = h.normal.y; r.normal[2] = h.normal.z;
+        }
+        out[i] = r;
+    }
+    return 0;
+}
+}
+
+#include "../../stargaze_amd/csrc/sg_bvh.h"
+
+extern "C" {
+// The library's BVH builder on a primitive list: returns the node count
+// (2 float4 per node) and fills up to `capacity` nodes / prims.
+uint32_t hc_build_bvh(const sg_primitive *objects, uint32_t n, float *nodes, uint32_t *prims, uint32_t capacity,
+                      uint32_t *prim_count) {
+    std::vector<float4> nv;
+    std::vector<uint32_t> order;
+    sg_host::build_bvh(objects, n, nv, order);
+    const uint32_t count = (uint32_t)(nv.size() / 2);
+    *prim_count = (uint32_t)order.size();
+    if (count > capacity || order.size() > capacity) return count;
+    std::memcpy(nodes, nv.data(), nv.size() * sizeof(float4));
+    std::memcpy(prims, order.data(), order.size() * sizeof(uint32_t));
+    return count;
+}
+}

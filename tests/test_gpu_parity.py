@@ -245,7 +245,12 @@ def test_landscape_probe_matches_oracle():
     orc.upload_terrain(data, cells)
     r = orc.probe(frame, ptm.PROBE_LANDSCAPE, rays)
     same = g["index"] == r["index"].astype(np.uint32)
-    assert same.mean() > 0.99, same.mean()
+    # Host sphere vs coincident terrain near the levelled lookout is a tie
+    # broken differently by the FP64 and FP32 quadratics (distances agree).
+    tie = ~same & np.isin(g["index"], [0x80000000, 0x80100000]) & np.isin(r["index"], [0x80000000, 0x80100000])
+    assert np.all(same | tie)
+    tie_rel = np.abs(g["distance"][tie] - r["distance"][tie]) / r["distance"][tie]
+    assert np.all(tie_rel < 1e-5)
     hit = same & (r["index"] != MISS)
     rel = np.abs(g["distance"][hit] - r["distance"][hit]) / r["distance"][hit]
     print(f"landscape: {hit.sum()} hits, same index {same.mean():.4f}, median rel {np.median(rel):.2e}, max {rel.max():.2e}")

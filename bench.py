@@ -132,6 +132,78 @@ def cpu_baseline(frame, terrain, width, height, spp, bounces, target_s=12.0):
     }
 
 
+def batch_leg(steps: int, warmup: int, count: int = 96, tune: bool = True) -> dict:
+    """Batched views against per-view launches on the same views.
+
+    Both forms render identical images pass for pass (same trace code, RNG
+    streams and blend order), so equal passes are equal RMSE. Latency is a
+    view's completion time within a pass, measured from the pass start on
+    the GPU clock (batch) or from the first launch with HIP events
+    (per-view)."""
+    from stargaze_amd.pathtracer import INTEGRATOR_FUSED, Batch
+
+    t0 = time.perf_counter()
+    views = scenes.batch_views(count)
+    frames = [v["frame"] for v in views]
+    tracers = []
+    for v in views:
+        t = PathTracer((v["width"], v["height"]),
+                       options=Options(samples_per_frame=v["spp"], max_bounces=v["bounces"], milky_way=0.0))
+        if v["terrain"] is not None:
+            t.upload_terrain(*v["terrain"])
+        t.set_integrator(INTEGRATOR_FUSED)
+        tracers.append(t)
+    batch = Batch(tracers)
+    log(f"batch: {count} views built in {time.perf_counter() - t0:.1f} s")
+    tuned = None
+    if tune:
+        best, best_ms = batch.autotune(frames, repeats=1)
+        tuned = {"launch": best, "pass_ms": round(best_ms, 3)}
+        log(f"batch autotune: {best} {best_ms:.2f} ms")
+    paths = sum(v["width"] * v["height"] * v["spp"] for v in views)
+    for _ in range(warmup):
+        batch.encode(frames)
+    batch.synchronize()
+    lat_b = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.encode(frames)
+        ms, span = batch.view_times()
+        lat_b.append(ms)
+    batch.synchronize()
+    t_batch = (time.perf_counter() - t0) / steps
+    for _ in range(warmup):
+        batch.encode_views(frames)
+    lat_v = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        lat_v.append(batch.encode_views(frames))
+    t_views = (time.perf_counter() - t0) / steps
+    lat_b, lat_v = np.concatenate(lat_b), np.concatenate(lat_v)
+    batch.close()
+    for t in tracers:
+        t.close()
+
+    def pct(a, q):
+        return round(float(np.percentile(a, q)), 3)
+
+    return {
+        "views": count,
+        "path_samples_per_pass": paths,
+        "views_per_s": round(count / t_batch, 2),
+        "per_view_launch_views_per_s": round(count / t_views, 2),
+        "speedup": round(t_views / t_batch, 3),
+        "pass_ms": round(t_batch * 1e3, 3),
+        "per_view_launch_pass_ms": round(t_views * 1e3, 3),
+        "latency_ms": {"p50": pct(lat_b, 50), "p99": pct(lat_b, 99)},
+        "per_view_launch_latency_ms": {"p50": pct(lat_v, 50), "p99": pct(lat_v, 99)},
+        "autotune": tuned,
+        "views_desc": "log-uniform 128^2..1920x1080 px, spp {1,2,4,8,16} per pass, 4 bounces; "
+        "airless triple / ringed giant / planet with air from orbit, every 6th the Halo observatory "
+        "(512-cell terrain, 180 props, air)",
+    }
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,6 +212,7 @@ def main() -> int:
     ap.add_argument("--workload", default="halo_4k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
+    ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
@@ -209,6 +282,9 @@ def main() -> int:
             "algorithmic_bytes_per_launch": traffic_algorithmic,
         },
     }
+    if args.batch_steps > 0:
+        log("batch leg")
+        line["batch"] = batch_leg(args.batch_steps, 1)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(frame, terrain, width, height, spp, bounces)

@@ -207,6 +207,55 @@ const char *sg_last_error(const sg_ctx *ctx);
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind,
              const sg_probe_ray *rays, uint32_t count, sg_probe_result *results);
 
+/* ------------------------------------------------------------------------
+ * Batched views (no reference counterpart: the reference renders one view
+ * per PathTracer, src/pathtracer.rs:190). A batch borrows existing contexts
+ * as views of uneven resolution and sample count and renders one progressive
+ * pass of all of them per sg_batch_encode, as a single persistent launch
+ * replayed from a hipGraph while the set of views and their sample counts
+ * stay the same. Each view keeps its own history, options and images: read
+ * them with sg_read_accumulation on the view context, as after sg_encode.
+ * A context belongs to at most one batch and cannot be destroyed before it.
+ * ---------------------------------------------------------------------- */
+typedef struct sg_batch sg_batch;
+
+#define SG_BATCH_ORDER_GIVEN 0u    /* views' blocks in the order given */
+#define SG_BATCH_ORDER_SHORTEST 1u /* fewest pixel blocks first (lowest mean latency) */
+#define SG_BATCH_ORDER_LONGEST 2u  /* most pixel blocks first */
+
+typedef struct sg_batch_launch {
+    uint32_t block;           /* lanes per workgroup: 64, 128 or 256 */
+    uint32_t workgroups;      /* persistent grid; 0 = two per compute unit */
+    uint32_t blocks_per_pull; /* pixel blocks claimed per atomic, 1..64 */
+    uint32_t order;           /* SG_BATCH_ORDER_* */
+    uint32_t use_graph;       /* 1 = replay each pass from a hipGraph, 0 = plain launches */
+} sg_batch_launch;
+
+int sg_batch_create(sg_batch **out, sg_ctx *const *views, uint32_t count);
+int sg_batch_destroy(sg_batch *batch);
+int sg_batch_set_launch(sg_batch *batch, const sg_batch_launch *launch);
+int sg_batch_get_launch(const sg_batch *batch, sg_batch_launch *launch);
+/* One progressive pass of every view; frames[i] is view i's frame. Same
+ * history, sample-limit and Settings semantics per view as sg_encode. */
+int sg_batch_encode(sg_batch *batch, const sg_frame *frames);
+int sg_batch_synchronize(sg_batch *batch);
+/* Completion time of each view in the last batch pass, in ms after the first
+ * workgroup started (GPU constant-rate clock); -1 for views without samples
+ * in that pass. `span_ms` (optional) receives the whole pass. */
+int sg_batch_view_times(sg_batch *batch, double *milliseconds, uint32_t count, double *span_ms);
+int sg_batch_set_profiling(sg_batch *batch, int32_t enable);
+int sg_batch_kernel_time(sg_batch *batch, double *milliseconds, uint64_t *launches);
+/* The per-view form for comparison: one sg_encode per view in order, each on
+ * its view's stream; milliseconds[i] (optional) = completion of view i after
+ * the first launch was queued. */
+int sg_batch_encode_views(sg_batch *batch, const sg_frame *frames, double *milliseconds);
+/* Time trial passes over launch settings (workgroup size, grid size, blocks
+ * per pull) and keep the fastest. Trial passes render into the views, whose
+ * progressive history is reset afterwards. */
+int sg_batch_autotune(sg_batch *batch, const sg_frame *frames, uint32_t repeats, sg_batch_launch *best,
+                      double *best_ms);
+const char *sg_batch_last_error(const sg_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -39,6 +39,18 @@ EXPORTS = (
     "sg_set_integrator",
     "sg_last_error",
     "sg_probe",
+    "sg_batch_create",
+    "sg_batch_destroy",
+    "sg_batch_set_launch",
+    "sg_batch_get_launch",
+    "sg_batch_encode",
+    "sg_batch_synchronize",
+    "sg_batch_view_times",
+    "sg_batch_set_profiling",
+    "sg_batch_kernel_time",
+    "sg_batch_encode_views",
+    "sg_batch_autotune",
+    "sg_batch_last_error",
 )
 
 
@@ -53,6 +65,21 @@ class SgFrame(ctypes.Structure):
         ("ground_count", ctypes.c_uint32),
         ("scene_time", ctypes.c_double),
     ]
+
+
+class SgBatchLaunch(ctypes.Structure):
+    """``sg_batch_launch`` (include/stargaze_amd.h)."""
+
+    _fields_ = [
+        ("block", ctypes.c_uint32),
+        ("workgroups", ctypes.c_uint32),
+        ("blocks_per_pull", ctypes.c_uint32),
+        ("order", ctypes.c_uint32),
+        ("use_graph", ctypes.c_uint32),
+    ]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 class NativeError(RuntimeError):
@@ -97,6 +124,18 @@ def load() -> ctypes.CDLL:
         "sg_set_integrator": ([P, U32, U32], I32),
         "sg_last_error": ([P], ctypes.c_char_p),
         "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
+        "sg_batch_create": ([ctypes.POINTER(P), ctypes.POINTER(P), U32], I32),
+        "sg_batch_destroy": ([P], I32),
+        "sg_batch_set_launch": ([P, ctypes.POINTER(SgBatchLaunch)], I32),
+        "sg_batch_get_launch": ([P, ctypes.POINTER(SgBatchLaunch)], I32),
+        "sg_batch_encode": ([P, P], I32),
+        "sg_batch_synchronize": ([P], I32),
+        "sg_batch_view_times": ([P, P, U32, ctypes.POINTER(ctypes.c_double)], I32),
+        "sg_batch_set_profiling": ([P, I32], I32),
+        "sg_batch_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
+        "sg_batch_encode_views": ([P, P, P], I32),
+        "sg_batch_autotune": ([P, P, U32, ctypes.POINTER(SgBatchLaunch), ctypes.POINTER(ctypes.c_double)], I32),
+        "sg_batch_last_error": ([P], ctypes.c_char_p),
     }
     for name in EXPORTS:
         fn = getattr(lib, name)
@@ -105,8 +144,9 @@ def load() -> ctypes.CDLL:
     return lib
 
 
-def check(lib: ctypes.CDLL, ctx, rc: int, what: str) -> int:
+def check(lib: ctypes.CDLL, ctx, rc: int, what: str, batch: bool = False) -> int:
     if rc < 0:
-        detail = lib.sg_last_error(ctx).decode() if ctx else ""
+        last = lib.sg_batch_last_error if batch else lib.sg_last_error
+        detail = last(ctx).decode() if ctx else ""
         raise NativeError(f"{what} failed: {ERRORS.get(rc, rc)}: {detail}")
     return rc

@@ -44,38 +44,20 @@ struct TraceArgs {
     uint32_t pixels_per_block;
 };
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
-    __shared__ float4 s_value[kTraceBlock];
-    __shared__ float4 s_backdrop[kTraceBlock];
-    const uint32_t t = threadIdx.x;
-    const uint32_t lp = t / ta.count;
-    const uint32_t s = t - lp * ta.count;
-    const uint32_t pixel = blockIdx.x * ta.pixels_per_block + lp;
-    const uint32_t pixels = ta.width * ta.height;
-    const bool active = lp < ta.pixels_per_block && pixel < pixels;
-    float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
-    f3 backdrop = f3s(0.f);
-    if (active) {
-        const uint32_t x = pixel % ta.width, y = pixel / ta.width;
-        const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
-        uint32_t rng = path_seed(pixel, ta.sequence + s);
-        f3 direction = camera_direction(sc, x, y, rng);
-        PathResult r = trace_path(sc, direction, rng, accumulated);
-        value = r.value;
-        backdrop = r.backdrop;
-    }
-    s_value[t] = value;
-    s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
-    __syncthreads();
-    if (!active || s != 0u) return;
-    const uint32_t base = lp * ta.count;
+// The per-pixel epilogue of `main` (src/shaders/pathtrace.wgsl:1653-1674):
+// ordered mean of this pass's samples and the running-average blend, or,
+// with air, one blend per sample (src/pathtracer.rs:861-882,971-976).
+__device__ __forceinline__ void blend_pixel(const TraceArgs &ta, uint32_t pixel, const float4 *values,
+                                            const float4 *backdrops) {
+    const uint32_t base = 0;
+
     float4 mean, bmean;
     float4 m2 = make_float4(0.f, 0.f, 0.f, 0.f);
     if (!ta.has_air) {
         // One dispatch of counts.z samples (src/shaders/pathtrace.wgsl:1653-1674).
         float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), bsum = sum, sq = sum;
         for (uint32_t k = 0; k < ta.count; ++k) {
-            float4 v = s_value[base + k], b = s_backdrop[base + k];
+            float4 v = values[base + k], b = backdrops[base + k];
             sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
             bsum.x += b.x; bsum.y += b.y; bsum.z += b.z;
             sq.x += v.x * v.x; sq.y += v.y * v.y; sq.z += v.z * v.z;
@@ -107,7 +89,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
             if (ta.moments) m2 = ta.moments[pixel];
         }
         for (uint32_t k = 0; k < ta.count; ++k) {
-            float4 v = s_value[base + k], b = s_backdrop[base + k];
+            float4 v = values[base + k], b = backdrops[base + k];
             float4 v2 = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0.f);
             const uint32_t old_count = ta.accumulated + k;
             if (old_count > 0u) {
@@ -130,6 +112,41 @@ __global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
     ta.backdrop[pixel] = bmean;
     if (ta.moments) ta.moments[pixel] = m2;
 }
+// One block of pixel samples: lane t traces sample t % count of pixel
+// block * pixels_per_block + t / count, then one lane per pixel forms the
+// ordered mean and blend. Every lane of the workgroup must call it.
+__device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta, uint32_t block, float4 *s_value,
+                                            float4 *s_backdrop) {
+    const uint32_t t = threadIdx.x;
+    const uint32_t lp = t / ta.count;
+    const uint32_t s = t - lp * ta.count;
+    const uint32_t pixel = block * ta.pixels_per_block + lp;
+    const uint32_t pixels = ta.width * ta.height;
+    const bool active = lp < ta.pixels_per_block && pixel < pixels;
+    float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
+    f3 backdrop = f3s(0.f);
+    if (active) {
+        const uint32_t x = pixel % ta.width, y = pixel / ta.width;
+        const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
+        uint32_t rng = path_seed(pixel, ta.sequence + s);
+        f3 direction = camera_direction(sc, x, y, rng);
+        PathResult r = trace_path(sc, direction, rng, accumulated);
+        value = r.value;
+        backdrop = r.backdrop;
+    }
+    s_value[t] = value;
+    s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
+    __syncthreads();
+    if (!active || s != 0u) return;
+    blend_pixel(ta, pixel, s_value + lp * ta.count, s_backdrop + lp * ta.count);
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
+    __shared__ float4 s_value[kTraceBlock];
+    __shared__ float4 s_backdrop[kTraceBlock];
+    trace_block(sc, ta, blockIdx.x, s_value, s_backdrop);
+}
+
 
 // ---------------------------------------------------------------------------
 // Wavefront integrator: the loop of trace() run as one kernel per step over
@@ -368,6 +385,99 @@ __global__ __launch_bounds__(256) void k_accumulate(WaveArgs wa, float4 *accumul
     if (moments) moments[pixel] = m2;
 }
 
+// ---------------------------------------------------------------------------
+// Batched views: one persistent launch renders a progressive pass of many
+// views (own scene, resolution, sample count). The views' pixel blocks are
+// laid end to end in table order; workgroups pull runs of blocks from an
+// atomic counter and trace each with the fused integrator's trace_block,
+// reading the view's scene from device memory.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxBatchViews = 1024;
+
+struct BatchView {
+    Scene scene;
+    TraceArgs ta;          // pixels_per_block = workgroup size / samples
+    uint32_t block_begin;  // first block of this view in the pass
+    uint32_t blocks;
+    uint32_t view;         // index of the view in the batch
+    uint32_t pad;
+};
+
+struct BatchArgs {
+    BatchView *table;             // views with samples to take, in launch order
+    uint32_t entries;
+    uint32_t total_blocks;
+    uint32_t blocks_per_pull;
+    uint32_t *counter;            // [0] next block; [1 + e] blocks left of entry e
+    unsigned long long *done;     // [e] clock at which entry e finished
+    unsigned long long *clock;    // [0] first workgroup start, [1] last workgroup end
+};
+
+__global__ void k_batch_reset(BatchArgs ba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ba.counter[0] = 0u;
+        ba.clock[0] = ~0ull;
+        ba.clock[1] = 0ull;
+    }
+    if (i < ba.entries) {
+        ba.counter[1 + i] = ba.table[i].blocks;
+        ba.done[i] = 0ull;
+    }
+}
+
+template <uint32_t B>
+__global__ __launch_bounds__(B) void k_batch(BatchArgs ba) {
+    __shared__ float4 s_value[B];
+    __shared__ float4 s_backdrop[B];
+    __shared__ uint32_t s_begin[kMaxBatchViews + 1];
+    __shared__ uint32_t s_first;
+    const uint32_t t = threadIdx.x;
+    for (uint32_t e = t; e < ba.entries; e += B) s_begin[e] = ba.table[e].block_begin;
+    if (t == 0) {
+        s_begin[ba.entries] = ba.total_blocks;
+        atomicMin(&ba.clock[0], (unsigned long long)wall_clock64());
+    }
+    for (;;) {
+        __syncthreads();
+        if (t == 0) s_first = atomicAdd(&ba.counter[0], ba.blocks_per_pull);
+        __syncthreads();
+        const uint32_t first = s_first;
+        if (first >= ba.total_blocks) break;
+        const uint32_t last = umin(first + ba.blocks_per_pull, ba.total_blocks);
+        for (uint32_t block = first; block < last; ++block) {
+            // Entry owning the block: last e with s_begin[e] <= block.
+            uint32_t lo = 0, hi = ba.entries;
+            while (hi - lo > 1u) {
+                const uint32_t mid = (lo + hi) >> 1;
+                if (s_begin[mid] <= block) lo = mid;
+                else hi = mid;
+            }
+            const uint32_t e = __builtin_amdgcn_readfirstlane(lo);
+            const BatchView &v = ba.table[e];
+            trace_block(v.scene, v.ta, block - s_begin[e], s_value, s_backdrop);
+            __syncthreads();
+            if (t == 0) {
+                __threadfence();
+                if (atomicSub(&ba.counter[1 + e], 1u) == 1u) ba.done[e] = wall_clock64();
+            }
+        }
+    }
+    if (t == 0) atomicMax(&ba.clock[1], (unsigned long long)wall_clock64());
+}
+
+// End of a pass: each entry's accumulated count and noise sequence advance
+// as the host history does, so an unchanged table can be replayed.
+__global__ void k_batch_advance(BatchArgs ba) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ba.entries) return;
+    BatchView &v = ba.table[e];
+    v.ta.accumulated += v.ta.count;
+    v.ta.sequence += v.ta.count;
+    v.scene.s.counts[1] += v.ta.count;
+    v.scene.s.sampling[0] += v.ta.count;
+}
+
 // cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
 __global__ void k_terrain_cache(Scene sc, float *terrain) {
     terrain_cache_entry(sc, terrain, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -492,6 +602,13 @@ struct sg_ctx {
     double kernel_ms = 0.0;
     uint64_t launches = 0;
     std::string error;
+    // Device copies of the last encoded frame's bodies and primitives.
+    bool frame_on_device = false;
+    // Batch this context is a view of, and the event of its last queued pass,
+    // which work on the context stream must wait for.
+    sg_batch *batch = nullptr;
+    hipEvent_t batch_event = nullptr;
+    hipEvent_t ev_ready = nullptr;
 };
 
 namespace {
@@ -499,6 +616,14 @@ namespace {
 int fail(sg_ctx *ctx, int code, const std::string &message) {
     if (ctx) ctx->error = message;
     return code;
+}
+
+// Order the context stream after the last batch pass that rendered this view.
+void join_batch(sg_ctx *ctx) {
+    if (ctx->batch_event) {
+        (void)hipStreamWaitEvent(ctx->stream, ctx->batch_event, 0);
+        ctx->batch_event = nullptr;
+    }
 }
 #define SG_HIP(ctx, call)                                                                        \
     do {                                                                                         \
@@ -599,6 +724,7 @@ void history_update(sg_ctx *ctx, const sg_frame *frame) {
     if (key != ctx->key) {
         ctx->key.swap(key);
         ctx->samples = 0;
+        ctx->frame_on_device = false;
     }
 }
 
@@ -726,6 +852,122 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
     return SG_OK;
 }
 
+// The host half of PathTracer::encode (src/pathtracer.rs:789-960): history,
+// sample count, the Settings uniform, uploads and the cache kernels, queued
+// on the context stream. Leaves the integrator launch to the caller.
+struct PassPlan {
+    Scene sc;
+    TraceArgs ta;
+    uint32_t count = 0;
+    bool queued = false;  // work was queued on the context stream
+};
+
+int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
+    int rc = validate_frame(ctx, frame);
+    if (rc != SG_OK) return rc;
+    if (frame->ground_count > 0) {
+        rc = ensure_objects(ctx, frame->ground_count);
+        if (rc != SG_OK) return rc;
+    }
+    history_update(ctx, frame);
+    const sg_options &o = ctx->options;
+    uint32_t remaining = o.sample_limit > ctx->samples ? o.sample_limit - ctx->samples : 0u;
+    uint32_t count = std::min(std::min(o.samples_per_frame, SG_MAX_BATCH_SAMPLES), remaining);
+    sg_settings st;
+    std::memset(&st, 0, sizeof(st));
+    st.globals = *frame->globals;
+    sg_globals &g = st.globals;
+    g.viewport[3] = g.viewport[0] / std::max(g.viewport[1], 1.0f);
+    g.viewport[0] = (float)ctx->width;
+    g.viewport[1] = (float)ctx->height;
+    g.viewport[2] = o.exposure;
+    float prop_top = 0.0f;
+    for (uint32_t i = 0; i < frame->ground_count; ++i) {
+        const sg_primitive &p = frame->ground[i];
+        float e = std::sqrt(p.extent[0] * p.extent[0] + p.extent[1] * p.extent[1] + p.extent[2] * p.extent[2]);
+        prop_top = std::max(prop_top, p.center[1] + e);
+    }
+    g.ground_counts[3] = float_bits(prop_top);
+    uint32_t cells = ctx->terrain_cells ? ctx->terrain_cells : 512u;
+    uint32_t log2 = 0;
+    while ((1u << log2) < cells) ++log2;
+    g.ground_counts[2] |= log2 << 8;
+    uint32_t lights[4] = {0, 0, 0, 0};
+    for (uint32_t i = 0; i < frame->body_count; ++i) {
+        if (frame->bodies[i].emissive >= 0.5f) {
+            if (lights[3] < 3u) lights[lights[3]] = i;
+            lights[3] += 1u;
+        }
+    }
+    std::memcpy(st.lights, lights, sizeof(lights));
+    st.sampling[0] = o.noise_seed + ctx->sequence;
+    st.counts[0] = frame->body_count;
+    st.counts[1] = ctx->samples;
+    st.counts[2] = count;
+    st.counts[3] = o.max_bounces;
+    st.background[0] = o.milky_way;
+    st.background[1] = ctx->milky_way ? (float)ctx->milky_w : 1024.0f;
+    st.background[2] = ctx->milky_way ? (float)ctx->milky_h : 512.0f;
+    st.background[3] = 100.0f;  // stars::POINT_SOURCE_GAIN (src/stars.rs:130)
+    const bool has_air = frame->globals->atmo_params[3] >= 0.5f;
+    plan.count = count;
+    plan.queued = false;
+    if (count == 0) return SG_OK;
+    // The device copies of bodies and primitives stay valid while the
+    // history key (which covers them) is unchanged.
+    if (!ctx->frame_on_device) {
+        if (frame->body_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        if (frame->ground_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        ctx->frame_on_device = true;
+        plan.queued = true;
+    }
+    int brc = ensure_bvh(ctx, frame);
+    if (brc != SG_OK) return brc;
+    plan.sc = make_scene(ctx, st);
+    const Scene &sc = plan.sc;
+    const bool terrain_on = g.ground_counts[1] != 0u && g.ground_north[3] != 0.0f;
+    const uint32_t radius_bits = float_bits(g.ground_up[3]);
+    if (terrain_on && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {
+        const uint64_t n = sc.t.vertices;
+        hipLaunchKernelGGL(k_terrain_cache, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, sc,
+                           ctx->terrain);
+        SG_HIP(ctx, hipGetLastError());
+        ctx->terrain_cached = true;
+        ctx->terrain_radius_bits = radius_bits;
+        plan.queued = true;
+    }
+    if (ctx->samples == 0 && has_air && terrain_on) {
+        hipLaunchKernelGGL(k_sky_cache, dim3(1), dim3(128), 0, ctx->stream, sc, ctx->sky_light);
+        SG_HIP(ctx, hipGetLastError());
+        plan.queued = true;
+    }
+    TraceArgs &ta = plan.ta;
+    ta.accumulation = ctx->accumulation;
+    ta.backdrop = ctx->backdrop;
+    ta.moments = ctx->moments_on ? ctx->moments : nullptr;
+    ta.width = ctx->width;
+    ta.height = ctx->height;
+    ta.count = count;
+    ta.accumulated = ctx->samples;
+    ta.sequence = st.sampling[0];
+    ta.has_air = has_air ? 1u : 0u;
+    ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
+    return SG_OK;
+}
+
+// The history bookkeeping after a pass (src/pathtracer.rs:1002-1008).
+void finish_pass(sg_ctx *ctx, uint32_t count) {
+    const sg_options &o = ctx->options;
+    ctx->samples += count;
+    ctx->sequence += count;
+    ctx->exposure_idle_frames = count > 0 ? 0u : std::min<uint32_t>(ctx->exposure_idle_frames + 1u, 0xffffffffu);
+    ctx->exposure_was_enabled = o.auto_exposure != 0;
+}
+
 }  // namespace
 
 extern "C" {
@@ -750,7 +992,8 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
     };
     if (hipGetDevice(&ctx->device) != hipSuccess) return bail(SG_E_DEVICE);
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(SG_E_DEVICE);
-    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess)
+    if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->ev_ready, hipEventDisableTiming) != hipSuccess)
         return bail(SG_E_DEVICE);
     if (alloc_image(ctx) != SG_OK) return bail(SG_E_DEVICE);
     if (hipMalloc(&ctx->bodies, SG_BODY_CAPACITY * sizeof(Body)) != hipSuccess) return bail(SG_E_DEVICE);
@@ -784,6 +1027,7 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
 
 int sg_destroy(sg_ctx *ctx) {
     if (!ctx) return SG_OK;
+    if (ctx->batch) return fail(ctx, SG_E_STATE, "context is a view of a batch; destroy the batch first");
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
@@ -794,6 +1038,7 @@ int sg_destroy(sg_ctx *ctx) {
         if (b) (void)hipFree(b);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    if (ctx->ev_ready) (void)hipEventDestroy(ctx->ev_ready);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return SG_OK;
@@ -801,6 +1046,8 @@ int sg_destroy(sg_ctx *ctx) {
 
 int sg_set_milky_way(sg_ctx *ctx, const uint32_t *texels, uint32_t width, uint32_t height) {
     if (!ctx || !texels || width == 0 || height == 0) return SG_E_INVALID;
+    join_batch(ctx);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->milky_way) SG_HIP(ctx, hipFree(ctx->milky_way));
     ctx->milky_way = nullptr;
     SG_HIP(ctx, hipMalloc(&ctx->milky_way, (size_t)width * height * 4u));
@@ -813,6 +1060,7 @@ int sg_set_milky_way(sg_ctx *ctx, const uint32_t *texels, uint32_t width, uint32
 int sg_resize(sg_ctx *ctx, uint32_t width, uint32_t height) {
     if (!ctx || width == 0 || height == 0) return SG_E_INVALID;
     if (width == ctx->width && height == ctx->height) return SG_OK;
+    join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->width = width;
     ctx->height = height;
@@ -877,6 +1125,7 @@ int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint
     TerrainLayout t = terrain_layout(log2 << 8);
     if (float_count != t.data) return fail(ctx, SG_E_INVALID, "terrain data size does not match its cell count");
     const uint64_t total = t.total;
+    join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     if (ctx->terrain && ctx->terrain_floats != total) {
         SG_HIP(ctx, hipFree(ctx->terrain));
@@ -894,6 +1143,7 @@ int sg_enable_moments(sg_ctx *ctx, int32_t enable) {
     if (!ctx) return SG_E_INVALID;
     bool on = enable != 0;
     if (on == ctx->moments_on) return SG_OK;
+    join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->moments_on = on;
     ctx->key.clear();
@@ -904,97 +1154,21 @@ int sg_enable_moments(sg_ctx *ctx, int32_t enable) {
 // PathTracer::encode (src/pathtracer.rs:789-1009).
 int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
     if (!ctx) return SG_E_INVALID;
-    int rc = validate_frame(ctx, frame);
+    join_batch(ctx);
+    PassPlan plan;
+    int rc = prepare_pass(ctx, frame, plan);
     if (rc != SG_OK) return rc;
-    if (frame->ground_count > 0) {
-        rc = ensure_objects(ctx, frame->ground_count);
-        if (rc != SG_OK) return rc;
-    }
-    history_update(ctx, frame);
-    const sg_options &o = ctx->options;
-    uint32_t remaining = o.sample_limit > ctx->samples ? o.sample_limit - ctx->samples : 0u;
-    uint32_t count = std::min(std::min(o.samples_per_frame, SG_MAX_BATCH_SAMPLES), remaining);
-    sg_settings st;
-    std::memset(&st, 0, sizeof(st));
-    st.globals = *frame->globals;
-    sg_globals &g = st.globals;
-    g.viewport[3] = g.viewport[0] / std::max(g.viewport[1], 1.0f);
-    g.viewport[0] = (float)ctx->width;
-    g.viewport[1] = (float)ctx->height;
-    g.viewport[2] = o.exposure;
-    float prop_top = 0.0f;
-    for (uint32_t i = 0; i < frame->ground_count; ++i) {
-        const sg_primitive &p = frame->ground[i];
-        float e = std::sqrt(p.extent[0] * p.extent[0] + p.extent[1] * p.extent[1] + p.extent[2] * p.extent[2]);
-        prop_top = std::max(prop_top, p.center[1] + e);
-    }
-    g.ground_counts[3] = float_bits(prop_top);
-    uint32_t cells = ctx->terrain_cells ? ctx->terrain_cells : 512u;
-    uint32_t log2 = 0;
-    while ((1u << log2) < cells) ++log2;
-    g.ground_counts[2] |= log2 << 8;
-    uint32_t lights[4] = {0, 0, 0, 0};
-    for (uint32_t i = 0; i < frame->body_count; ++i) {
-        if (frame->bodies[i].emissive >= 0.5f) {
-            if (lights[3] < 3u) lights[lights[3]] = i;
-            lights[3] += 1u;
-        }
-    }
-    std::memcpy(st.lights, lights, sizeof(lights));
-    st.sampling[0] = o.noise_seed + ctx->sequence;
-    st.counts[0] = frame->body_count;
-    st.counts[1] = ctx->samples;
-    st.counts[2] = count;
-    st.counts[3] = o.max_bounces;
-    st.background[0] = o.milky_way;
-    st.background[1] = ctx->milky_way ? (float)ctx->milky_w : 1024.0f;
-    st.background[2] = ctx->milky_way ? (float)ctx->milky_h : 512.0f;
-    st.background[3] = 100.0f;  // stars::POINT_SOURCE_GAIN (src/stars.rs:130)
-    const bool has_air = frame->globals->atmo_params[3] >= 0.5f;
-    if (count > 0) {
-        if (frame->body_count)
-            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
-                                       hipMemcpyHostToDevice, ctx->stream));
-        if (frame->ground_count)
-            SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
-                                       hipMemcpyHostToDevice, ctx->stream));
-        int brc = ensure_bvh(ctx, frame);
-        if (brc != SG_OK) return brc;
-        Scene sc = make_scene(ctx, st);
-        const bool terrain_on = g.ground_counts[1] != 0u && g.ground_north[3] != 0.0f;
-        const uint32_t radius_bits = float_bits(g.ground_up[3]);
-        if (terrain_on && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {
-            const uint64_t n = sc.t.vertices;
-            hipLaunchKernelGGL(k_terrain_cache, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, ctx->stream, sc,
-                               ctx->terrain);
-            SG_HIP(ctx, hipGetLastError());
-            ctx->terrain_cached = true;
-            ctx->terrain_radius_bits = radius_bits;
-        }
-        if (ctx->samples == 0 && has_air && terrain_on) {
-            hipLaunchKernelGGL(k_sky_cache, dim3(1), dim3(128), 0, ctx->stream, sc, ctx->sky_light);
-            SG_HIP(ctx, hipGetLastError());
-        }
-        TraceArgs ta;
-        ta.accumulation = ctx->accumulation;
-        ta.backdrop = ctx->backdrop;
-        ta.moments = ctx->moments_on ? ctx->moments : nullptr;
-        ta.width = ctx->width;
-        ta.height = ctx->height;
-        ta.count = count;
-        ta.accumulated = ctx->samples;
-        ta.sequence = st.sampling[0];
-        ta.has_air = has_air ? 1u : 0u;
-        ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
-        const uint32_t block = ta.pixels_per_block * count;
+    if (plan.count > 0) {
+        const TraceArgs &ta = plan.ta;
+        const uint32_t block = ta.pixels_per_block * plan.count;
         const uint64_t pixels = (uint64_t)ctx->width * ctx->height;
         const uint32_t grid = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);
         if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
         if (ctx->integrator == SG_INTEGRATOR_FUSED) {
-            hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, sc, ta);
+            hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, plan.sc, ta);
             SG_HIP(ctx, hipGetLastError());
         } else {
-            int wrc = encode_wavefront(ctx, sc, ta);
+            int wrc = encode_wavefront(ctx, plan.sc, ta);
             if (wrc != SG_OK) return wrc;
         }
         if (ctx->profiling) {
@@ -1006,10 +1180,7 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
             ctx->launches += 1;
         }
     }
-    ctx->samples += count;
-    ctx->sequence += count;
-    ctx->exposure_idle_frames = count > 0 ? 0u : std::min<uint32_t>(ctx->exposure_idle_frames + 1u, 0xffffffffu);
-    ctx->exposure_was_enabled = o.auto_exposure != 0;
+    finish_pass(ctx, plan.count);
     return SG_OK;
 }
 
@@ -1018,6 +1189,7 @@ static int read_image(sg_ctx *ctx, const float4 *src, float *dst, uint64_t pixel
     if (!src) return fail(ctx, SG_E_STATE, "buffer not allocated");
     if (pixel_count != (uint64_t)ctx->width * ctx->height)
         return fail(ctx, SG_E_INVALID, "pixel count does not match the image size");
+    join_batch(ctx);
     SG_HIP(ctx, hipMemcpyAsync(dst, src, pixel_count * 16u, hipMemcpyDeviceToHost, ctx->stream));
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return SG_OK;
@@ -1028,6 +1200,7 @@ int sg_read_moments(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ct
 
 int sg_synchronize(sg_ctx *ctx) {
     if (!ctx) return SG_E_INVALID;
+    join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return SG_OK;
 }
@@ -1053,6 +1226,7 @@ int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths) {
         return fail(ctx, SG_E_INVALID, "unknown integrator");
     if (chunk_paths != 0 && (chunk_paths < 64u || chunk_paths > (1u << 28)))
         return fail(ctx, SG_E_INVALID, "chunk_paths must be 0 (keep) or in [64, 2^28]");
+    join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->integrator = integrator;
     if (chunk_paths) ctx->chunk_paths = chunk_paths;
@@ -1076,6 +1250,7 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     }
     if ((kind == SG_PROBE_TERRAIN || frame->globals->ground_counts[1] != 0u) && !ctx->terrain)
         return fail(ctx, SG_E_STATE, "terrain probe without uploaded terrain");
+    join_batch(ctx);
     if (frame->ground_count) {
         int rc = ensure_objects(ctx, frame->ground_count);
         if (rc != SG_OK) return rc;
@@ -1111,6 +1286,8 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
             lights[3] += 1u;
         }
     std::memcpy(st.lights, lights, sizeof(lights));
+    // The probe's frame replaces the encoded frame's device copies.
+    ctx->frame_on_device = false;
     if (frame->body_count)
         SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body), hipMemcpyHostToDevice,
                                    ctx->stream));
@@ -1138,5 +1315,438 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     return SG_OK;
 }
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched views (no reference counterpart: the reference renders one view per
+// PathTracer). A batch borrows existing contexts as its views; each pass runs
+// every view's host preparation, then one persistent k_batch launch, replayed
+// from a hipGraph while the view table stays the same.
+// ---------------------------------------------------------------------------
+struct sg_batch {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::vector<sg_ctx *> views;
+    sg_batch_launch launch{};
+    uint32_t compute_units = 0;
+    double clock_khz = 0.0;
+    // Device state: table, counters, completion clocks.
+    BatchView *table = nullptr;
+    uint32_t *counter = nullptr;
+    unsigned long long *done = nullptr;  // entries + 2 clock words
+    BatchView *staging = nullptr;        // pinned copy of the table
+    std::vector<BatchView> expected;     // device table after the last pass
+    std::vector<uint32_t> entry_view;
+    uint32_t total_blocks = 0;
+    bool table_valid = false;
+    bool pass_queued = false;
+    // Graph of one pass for the current table shape and launch.
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool graph_valid = false;
+    hipEvent_t ev_done = nullptr, ev_start = nullptr, ev_stop = nullptr;
+    bool profiling = false;
+    double kernel_ms = 0.0;
+    uint64_t launches = 0;
+    std::vector<hipEvent_t> view_events;  // per-view launch completion
+    std::string error;
+};
+
+namespace {
+
+int bfail(sg_batch *b, int code, const std::string &message) {
+    if (b) b->error = message;
+    return code;
+}
+#define SG_BHIP(b, call)                                                                         \
+    do {                                                                                         \
+        hipError_t e_ = (call);                                                                  \
+        if (e_ != hipSuccess)                                                                    \
+            return bfail((b), SG_E_DEVICE, std::string(#call) + ": " + hipGetErrorString(e_));   \
+    } while (0)
+
+bool valid_launch(const sg_batch_launch &l) {
+    return (l.block == 64u || l.block == 128u || l.block == 256u) && l.workgroups >= 1u && l.workgroups <= 65536u &&
+           l.blocks_per_pull >= 1u && l.blocks_per_pull <= 64u && l.order <= SG_BATCH_ORDER_LONGEST &&
+           l.use_graph <= 1u;
+}
+
+void drop_graph(sg_batch *b) {
+    if (b->exec) (void)hipGraphExecDestroy(b->exec);
+    if (b->graph) (void)hipGraphDestroy(b->graph);
+    b->exec = nullptr;
+    b->graph = nullptr;
+    b->graph_valid = false;
+}
+
+BatchArgs batch_args(const sg_batch *b) {
+    BatchArgs ba;
+    ba.table = b->table;
+    ba.entries = (uint32_t)b->entry_view.size();
+    ba.total_blocks = b->total_blocks;
+    ba.blocks_per_pull = b->launch.blocks_per_pull;
+    ba.counter = b->counter;
+    ba.done = b->done;
+    ba.clock = b->done + kMaxBatchViews;
+    return ba;
+}
+
+// Queue one pass (reset, persistent trace, advance) on the batch stream.
+int queue_pass(sg_batch *b) {
+    const BatchArgs ba = batch_args(b);
+    const dim3 small((ba.entries + 255) / 256 + 1);
+    hipLaunchKernelGGL(k_batch_reset, small, dim3(256), 0, b->stream, ba);
+    SG_BHIP(b, hipGetLastError());
+    const dim3 grid(b->launch.workgroups);
+    switch (b->launch.block) {
+    case 64: hipLaunchKernelGGL(k_batch<64>, grid, dim3(64), 0, b->stream, ba); break;
+    case 128: hipLaunchKernelGGL(k_batch<128>, grid, dim3(128), 0, b->stream, ba); break;
+    default: hipLaunchKernelGGL(k_batch<256>, grid, dim3(256), 0, b->stream, ba); break;
+    }
+    SG_BHIP(b, hipGetLastError());
+    hipLaunchKernelGGL(k_batch_advance, small, dim3(256), 0, b->stream, ba);
+    SG_BHIP(b, hipGetLastError());
+    return SG_OK;
+}
+
+int capture_pass(sg_batch *b) {
+    drop_graph(b);
+    SG_BHIP(b, hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+    int rc = queue_pass(b);
+    hipGraph_t g = nullptr;
+    hipError_t e = hipStreamEndCapture(b->stream, &g);
+    if (rc != SG_OK) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc;
+    }
+    if (e != hipSuccess) return bfail(b, SG_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    b->graph = g;
+    SG_BHIP(b, hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    b->graph_valid = true;
+    return SG_OK;
+}
+
+// The table a pass of `plans` would use under the current launch settings.
+void build_table(const sg_batch *b, std::vector<PassPlan> &plans, std::vector<BatchView> &table,
+                 std::vector<uint32_t> &entry_view, uint32_t &total_blocks) {
+    table.clear();
+    entry_view.clear();
+    const uint32_t n = (uint32_t)plans.size();
+    std::vector<uint32_t> order;
+    std::vector<uint64_t> blocks(n, 0);
+    for (uint32_t i = 0; i < n; ++i) {
+        if (plans[i].count == 0) continue;
+        TraceArgs &ta = plans[i].ta;
+        ta.pixels_per_block = std::max<uint32_t>(1u, b->launch.block / ta.count);
+        const uint64_t pixels = (uint64_t)ta.width * ta.height;
+        blocks[i] = (pixels + ta.pixels_per_block - 1) / ta.pixels_per_block;
+        order.push_back(i);
+    }
+    if (b->launch.order == SG_BATCH_ORDER_SHORTEST)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return blocks[x] < blocks[y]; });
+    else if (b->launch.order == SG_BATCH_ORDER_LONGEST)
+        std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return blocks[x] > blocks[y]; });
+    total_blocks = 0;
+    for (uint32_t i : order) {
+        BatchView v;
+        std::memset(&v, 0, sizeof(v));
+        v.scene = plans[i].sc;
+        v.ta = plans[i].ta;
+        v.block_begin = total_blocks;
+        v.blocks = (uint32_t)blocks[i];
+        v.view = i;
+        total_blocks += v.blocks;
+        table.push_back(v);
+        entry_view.push_back(i);
+    }
+}
+
+// One batch pass over `frames` (one per view).
+int batch_pass(sg_batch *b, const sg_frame *frames) {
+    const uint32_t n = (uint32_t)b->views.size();
+    std::vector<PassPlan> plans(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        sg_ctx *ctx = b->views[i];
+        join_batch(ctx);
+        int rc = prepare_pass(ctx, &frames[i], plans[i]);
+        if (rc != SG_OK) return bfail(b, rc, "view " + std::to_string(i) + ": " + ctx->error);
+    }
+    std::vector<BatchView> table;
+    std::vector<uint32_t> entry_view;
+    uint32_t total_blocks = 0;
+    build_table(b, plans, table, entry_view, total_blocks);
+    // 32-bit block indices; each view is below 2^32 pixels by its size types.
+    uint64_t check = 0;
+    for (const BatchView &v : table) check += v.blocks;
+    if (check >= (1ull << 31)) return bfail(b, SG_E_CAPACITY, "batch pass exceeds 2^31 pixel blocks");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (!plans[i].queued) continue;
+        sg_ctx *ctx = b->views[i];
+        SG_BHIP(b, hipEventRecord(ctx->ev_ready, ctx->stream));
+        SG_BHIP(b, hipStreamWaitEvent(b->stream, ctx->ev_ready, 0));
+    }
+    if (!table.empty()) {
+        const bool same = b->table_valid && table.size() == b->expected.size() &&
+                          std::memcmp(table.data(), b->expected.data(), table.size() * sizeof(BatchView)) == 0;
+        if (!same) {
+            // The staging buffer may still feed the previous pass's copy.
+            SG_BHIP(b, hipStreamSynchronize(b->stream));
+            std::memcpy(b->staging, table.data(), table.size() * sizeof(BatchView));
+            SG_BHIP(b, hipMemcpyAsync(b->table, b->staging, table.size() * sizeof(BatchView), hipMemcpyHostToDevice,
+                                      b->stream));
+            if (table.size() != b->entry_view.size() || total_blocks != b->total_blocks) b->graph_valid = false;
+            b->entry_view = entry_view;
+            b->total_blocks = total_blocks;
+            b->table_valid = true;
+        }
+        if (b->profiling) SG_BHIP(b, hipEventRecord(b->ev_start, b->stream));
+        if (b->launch.use_graph) {
+            if (!b->graph_valid) {
+                int rc = capture_pass(b);
+                if (rc != SG_OK) return rc;
+            }
+            SG_BHIP(b, hipGraphLaunch(b->exec, b->stream));
+        } else {
+            int rc = queue_pass(b);
+            if (rc != SG_OK) return rc;
+        }
+        if (b->profiling) {
+            SG_BHIP(b, hipEventRecord(b->ev_stop, b->stream));
+            SG_BHIP(b, hipEventSynchronize(b->ev_stop));
+            float ms = 0.f;
+            SG_BHIP(b, hipEventElapsedTime(&ms, b->ev_start, b->ev_stop));
+            b->kernel_ms += ms;
+            b->launches += 1;
+        }
+        for (BatchView &v : table) {
+            v.ta.accumulated += v.ta.count;
+            v.ta.sequence += v.ta.count;
+            v.scene.s.counts[1] += v.ta.count;
+            v.scene.s.sampling[0] += v.ta.count;
+        }
+        b->expected.swap(table);
+        b->pass_queued = true;
+        SG_BHIP(b, hipEventRecord(b->ev_done, b->stream));
+        for (uint32_t i : b->entry_view) b->views[i]->batch_event = b->ev_done;
+    }
+    for (uint32_t i = 0; i < n; ++i) finish_pass(b->views[i], plans[i].count);
+    return SG_OK;
+}
+
+// Forget the views' progressive history so the next pass starts afresh.
+void reset_history(sg_batch *b) {
+    for (sg_ctx *ctx : b->views) {
+        ctx->key.clear();
+        ctx->samples = 0;
+        ctx->sequence = 0;
+        ctx->frame_on_device = false;
+    }
+    b->table_valid = false;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sg_batch_create(sg_batch **out, sg_ctx *const *views, uint32_t count) {
+    if (!out) return SG_E_INVALID;
+    *out = nullptr;
+    if (!views || count == 0 || count > kMaxBatchViews) return SG_E_INVALID;
+    for (uint32_t i = 0; i < count; ++i) {
+        if (!views[i] || views[i]->batch) return SG_E_INVALID;
+        for (uint32_t j = 0; j < i; ++j)
+            if (views[j] == views[i]) return SG_E_INVALID;
+    }
+    sg_batch *b = new sg_batch();
+    auto bail = [&](int code) {
+        sg_batch_destroy(b);
+        return code;
+    };
+    if (hipGetDevice(&b->device) != hipSuccess) return bail(SG_E_DEVICE);
+    for (uint32_t i = 0; i < count; ++i)
+        if (views[i]->device != b->device) return bail(SG_E_INVALID);
+    int cus = 0, khz = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device) != hipSuccess ||
+        hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, b->device) != hipSuccess || cus <= 0 || khz <= 0)
+        return bail(SG_E_DEVICE);
+    b->compute_units = (uint32_t)cus;
+    b->clock_khz = (double)khz;
+    b->launch = sg_batch_launch{256u, 2u * (uint32_t)cus, 1u, SG_BATCH_ORDER_GIVEN, 1u};
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipEventCreateWithFlags(&b->ev_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreate(&b->ev_start) != hipSuccess || hipEventCreate(&b->ev_stop) != hipSuccess)
+        return bail(SG_E_DEVICE);
+    if (hipMalloc(&b->table, kMaxBatchViews * sizeof(BatchView)) != hipSuccess ||
+        hipMalloc(&b->counter, (kMaxBatchViews + 1) * sizeof(uint32_t)) != hipSuccess ||
+        hipMalloc(&b->done, (kMaxBatchViews + 2) * sizeof(unsigned long long)) != hipSuccess ||
+        hipHostMalloc(&b->staging, kMaxBatchViews * sizeof(BatchView), hipHostMallocDefault) != hipSuccess)
+        return bail(SG_E_DEVICE);
+    b->views.assign(views, views + count);
+    for (sg_ctx *ctx : b->views) ctx->batch = b;
+    *out = b;
+    return SG_OK;
+}
+
+int sg_batch_destroy(sg_batch *b) {
+    if (!b) return SG_OK;
+    if (b->stream) (void)hipStreamSynchronize(b->stream);
+    for (sg_ctx *ctx : b->views) {
+        ctx->batch = nullptr;
+        if (ctx->batch_event == b->ev_done) ctx->batch_event = nullptr;
+    }
+    drop_graph(b);
+    for (hipEvent_t e : b->view_events) (void)hipEventDestroy(e);
+    if (b->table) (void)hipFree(b->table);
+    if (b->counter) (void)hipFree(b->counter);
+    if (b->done) (void)hipFree(b->done);
+    if (b->staging) (void)hipHostFree(b->staging);
+    if (b->ev_done) (void)hipEventDestroy(b->ev_done);
+    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+    if (b->ev_stop) (void)hipEventDestroy(b->ev_stop);
+    if (b->stream) (void)hipStreamDestroy(b->stream);
+    delete b;
+    return SG_OK;
+}
+
+int sg_batch_set_launch(sg_batch *b, const sg_batch_launch *launch) {
+    if (!b || !launch) return SG_E_INVALID;
+    sg_batch_launch l = *launch;
+    if (l.workgroups == 0) l.workgroups = 2u * b->compute_units;
+    if (!valid_launch(l)) return bfail(b, SG_E_INVALID, "invalid batch launch settings");
+    if (l.block != b->launch.block || l.order != b->launch.order) b->table_valid = false;
+    if (std::memcmp(&l, &b->launch, sizeof(l)) != 0) b->graph_valid = false;
+    b->launch = l;
+    return SG_OK;
+}
+
+int sg_batch_get_launch(const sg_batch *b, sg_batch_launch *launch) {
+    if (!b || !launch) return SG_E_INVALID;
+    *launch = b->launch;
+    return SG_OK;
+}
+
+int sg_batch_encode(sg_batch *b, const sg_frame *frames) {
+    if (!b || !frames) return SG_E_INVALID;
+    return batch_pass(b, frames);
+}
+
+int sg_batch_synchronize(sg_batch *b) {
+    if (!b) return SG_E_INVALID;
+    SG_BHIP(b, hipStreamSynchronize(b->stream));
+    return SG_OK;
+}
+
+int sg_batch_view_times(sg_batch *b, double *milliseconds, uint32_t count, double *span_ms) {
+    if (!b || !milliseconds || count != b->views.size()) return SG_E_INVALID;
+    SG_BHIP(b, hipStreamSynchronize(b->stream));
+    for (uint32_t i = 0; i < count; ++i) milliseconds[i] = -1.0;
+    if (span_ms) *span_ms = -1.0;
+    if (!b->pass_queued) return SG_OK;
+    const size_t entries = b->entry_view.size();
+    std::vector<unsigned long long> done(entries), clock(2);
+    if (entries)
+        SG_BHIP(b, hipMemcpy(done.data(), b->done, entries * sizeof(unsigned long long), hipMemcpyDeviceToHost));
+    SG_BHIP(b, hipMemcpy(clock.data(), b->done + kMaxBatchViews, 2 * sizeof(unsigned long long),
+                         hipMemcpyDeviceToHost));
+    for (size_t e = 0; e < entries; ++e)
+        milliseconds[b->entry_view[e]] = (double)(done[e] - clock[0]) / b->clock_khz;
+    if (span_ms) *span_ms = (double)(clock[1] - clock[0]) / b->clock_khz;
+    return SG_OK;
+}
+
+int sg_batch_set_profiling(sg_batch *b, int32_t enable) {
+    if (!b) return SG_E_INVALID;
+    b->profiling = enable != 0;
+    b->kernel_ms = 0.0;
+    b->launches = 0;
+    return SG_OK;
+}
+
+int sg_batch_kernel_time(sg_batch *b, double *milliseconds, uint64_t *launches) {
+    if (!b || !milliseconds || !launches) return SG_E_INVALID;
+    *milliseconds = b->kernel_ms;
+    *launches = b->launches;
+    return SG_OK;
+}
+
+int sg_batch_encode_views(sg_batch *b, const sg_frame *frames, double *milliseconds) {
+    if (!b || !frames) return SG_E_INVALID;
+    const uint32_t n = (uint32_t)b->views.size();
+    if (b->view_events.size() != n) {
+        for (hipEvent_t e : b->view_events) (void)hipEventDestroy(e);
+        b->view_events.assign(n, nullptr);
+        for (uint32_t i = 0; i < n; ++i) SG_BHIP(b, hipEventCreate(&b->view_events[i]));
+    }
+    SG_BHIP(b, hipStreamSynchronize(b->stream));
+    for (sg_ctx *ctx : b->views) {
+        join_batch(ctx);
+        SG_BHIP(b, hipStreamSynchronize(ctx->stream));
+    }
+    SG_BHIP(b, hipEventRecord(b->ev_start, b->views[0]->stream));
+    for (uint32_t i = 0; i < n; ++i) {
+        sg_ctx *ctx = b->views[i];
+        int rc = sg_encode(ctx, &frames[i]);
+        if (rc != SG_OK) return bfail(b, rc, "view " + std::to_string(i) + ": " + ctx->error);
+        SG_BHIP(b, hipEventRecord(b->view_events[i], ctx->stream));
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        SG_BHIP(b, hipEventSynchronize(b->view_events[i]));
+        if (milliseconds) {
+            float ms = 0.f;
+            SG_BHIP(b, hipEventElapsedTime(&ms, b->ev_start, b->view_events[i]));
+            milliseconds[i] = ms;
+        }
+    }
+    return SG_OK;
+}
+
+int sg_batch_autotune(sg_batch *b, const sg_frame *frames, uint32_t repeats, sg_batch_launch *best,
+                      double *best_ms) {
+    if (!b || !frames) return SG_E_INVALID;
+    repeats = std::max<uint32_t>(1u, std::min<uint32_t>(repeats, 16u));
+    const sg_batch_launch keep = b->launch;
+    const bool keep_profiling = b->profiling;
+    sg_batch_launch winner = keep;
+    double winner_ms = 1e300;
+    const uint32_t cus = b->compute_units;
+    const uint32_t blocks[] = {64u, 128u, 256u};
+    const uint32_t per_cu[] = {1u, 2u, 4u, 8u};
+    const uint32_t pulls[] = {1u, 4u};
+    int rc = SG_OK;
+    for (uint32_t block : blocks)
+        for (uint32_t wpc : per_cu)
+            for (uint32_t pull : pulls) {
+                sg_batch_launch l{block, wpc * cus, pull, keep.order, 1u};
+                if (sg_batch_set_launch(b, &l) != SG_OK) continue;
+                // One untimed pass captures the graph and warms the caches.
+                rc = batch_pass(b, frames);
+                if (rc != SG_OK) goto out;
+                sg_batch_set_profiling(b, 1);
+                for (uint32_t r = 0; r < repeats; ++r) {
+                    rc = batch_pass(b, frames);
+                    if (rc != SG_OK) goto out;
+                }
+                {
+                    const double ms = b->kernel_ms / (double)std::max<uint64_t>(b->launches, 1);
+                    if (ms < winner_ms) {
+                        winner_ms = ms;
+                        winner = b->launch;
+                    }
+                }
+                sg_batch_set_profiling(b, 0);
+            }
+out:
+    if (rc == SG_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfail(b, SG_E_DEVICE, "autotune sync");
+    reset_history(b);
+    sg_batch_set_launch(b, rc == SG_OK ? &winner : &keep);
+    sg_batch_set_profiling(b, keep_profiling ? 1 : 0);
+    if (rc != SG_OK) return rc;
+    if (best) *best = winner;
+    if (best_ms) *best_ms = winner_ms;
+    return SG_OK;
+}
+
+const char *sg_batch_last_error(const sg_batch *b) { return b ? b->error.c_str() : "null batch"; }
 
 }  // extern "C"

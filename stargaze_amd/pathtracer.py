@@ -193,3 +193,104 @@ class PathTracer:
 
 INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
 PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE = range(7)
+
+BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST, BATCH_ORDER_LONGEST = 0, 1, 2
+
+
+class Batch:
+    """Many views rendered one progressive pass at a time in one launch.
+
+    No reference counterpart (the reference has one ``PathTracer`` per view,
+    src/pathtracer.rs:190); each view is a ``PathTracer`` whose images are
+    read as usual. The batch must be closed before its views.
+    """
+
+    def __init__(self, views: list[PathTracer]):
+        self._lib = _native.load()
+        self.views = list(views)
+        handles = (ctypes.c_void_p * len(self.views))(*[v._ctx.value for v in self.views])
+        b = ctypes.c_void_p()
+        rc = self._lib.sg_batch_create(ctypes.byref(b), handles, len(self.views))
+        _native.check(self._lib, None, rc, "sg_batch_create", batch=True)
+        self._b = b
+
+    def close(self) -> None:
+        if getattr(self, "_b", None):
+            self._lib.sg_batch_destroy(self._b)
+            self._b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc: int, what: str) -> int:
+        return _native.check(self._lib, self._b, rc, what, batch=True)
+
+    def _frames(self, frames: list[Frame]):
+        if len(frames) != len(self.views):
+            raise ValueError(f"{len(frames)} frames for {len(self.views)} views")
+        arr = (_native.SgFrame * len(frames))()
+        keep = []
+        for i, f in enumerate(frames):
+            native, k = f.native()
+            arr[i] = native
+            keep.append(k)
+        return arr, keep
+
+    @property
+    def launch(self) -> dict:
+        l = _native.SgBatchLaunch()
+        self._check(self._lib.sg_batch_get_launch(self._b, ctypes.byref(l)), "sg_batch_get_launch")
+        return l.as_dict()
+
+    def set_launch(self, **kw) -> None:
+        cur = self.launch
+        cur.update(kw)
+        l = _native.SgBatchLaunch(**cur)
+        self._check(self._lib.sg_batch_set_launch(self._b, ctypes.byref(l)), "sg_batch_set_launch")
+
+    def encode(self, frames: list[Frame]) -> None:
+        arr, keep = self._frames(frames)
+        self._check(self._lib.sg_batch_encode(self._b, arr), "sg_batch_encode")
+
+    def encode_views(self, frames: list[Frame]) -> np.ndarray:
+        """The per-view form: one ``encode`` per view; returns each view's
+        completion time (ms) after the first launch was queued."""
+        arr, keep = self._frames(frames)
+        ms = np.zeros(len(self.views), np.float64)
+        self._check(self._lib.sg_batch_encode_views(self._b, arr, ms.ctypes.data), "sg_batch_encode_views")
+        return ms
+
+    def synchronize(self) -> None:
+        self._check(self._lib.sg_batch_synchronize(self._b), "sg_batch_synchronize")
+
+    def view_times(self) -> tuple[np.ndarray, float]:
+        """Completion time (ms) of each view in the last batch pass and the pass span."""
+        ms = np.zeros(len(self.views), np.float64)
+        span = ctypes.c_double()
+        self._check(
+            self._lib.sg_batch_view_times(self._b, ms.ctypes.data, len(self.views), ctypes.byref(span)),
+            "sg_batch_view_times",
+        )
+        return ms, float(span.value)
+
+    def set_profiling(self, enable: bool) -> None:
+        self._check(self._lib.sg_batch_set_profiling(self._b, int(enable)), "sg_batch_set_profiling")
+
+    def kernel_time(self) -> tuple[float, int]:
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        self._check(self._lib.sg_batch_kernel_time(self._b, ctypes.byref(ms), ctypes.byref(n)), "sg_batch_kernel_time")
+        return float(ms.value), int(n.value)
+
+    def autotune(self, frames: list[Frame], repeats: int = 2) -> tuple[dict, float]:
+        """Pick the fastest launch settings; resets every view's history."""
+        arr, keep = self._frames(frames)
+        best = _native.SgBatchLaunch()
+        ms = ctypes.c_double()
+        self._check(
+            self._lib.sg_batch_autotune(self._b, arr, int(repeats), ctypes.byref(best), ctypes.byref(ms)),
+            "sg_batch_autotune",
+        )
+        return best.as_dict(), float(ms.value)

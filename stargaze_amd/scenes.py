@@ -334,3 +334,70 @@ def halo_parity(width=96, height=54):
     """Halo scene inside the reference's capacities: 512-cell terrain and at
     most 256 primitives (src/terrain.rs:20, src/ground.rs:8)."""
     return halo_observatory(width, height, cells=512, rocks=50, conifers=130, reach=400.0, levels=((6, 6),))
+
+
+# -- batched views -------------------------------------------------------------
+def orbit_air(width: int = 320, height: int = 240, seed: int = 0x0A11CE) -> Frame:
+    """A 6.4e6 m planet with Earth-like air seen from 4e6 m above its
+    surface, limb in view, one star; no ground scene."""
+    rng = Pcg(seed)
+    f = frame()
+    camera(f.globals, 55.0, width, height)
+    r = metres(6.4e6)
+    up = np.array([0.0, 1.0, 0.0])
+    centre = -up * (r + metres(4.0e6)) + np.array([0.0, 0.0, -metres(rng.uniform(2e6, 5e6))])
+    planet = place(body([0, 0, 0], r, [0.28, 0.3, 0.33], False), centre)
+    sun = star(direction(rng.uniform(-90.0, 90.0), rng.uniform(5.0, 40.0)) * metres(1.5e11), metres(7.0e8))
+    f.bodies = bodies(sun, planet)
+    au = METRES_PER_AU
+    g = f.globals
+    g["atmo_center"] = [*split(centre)[0], r]
+    g["atmo_rayleigh"] = [5.8e-6 * au, 13.5e-6 * au, 33.1e-6 * au, 2.0e-5 * au]
+    g["atmo_params"] = [0.76, 8000.0 / au, 80000.0 / au, 2.0]
+    return f
+
+
+def batch_views(count: int = 96, seed: int = 0xBA7C4, min_pixels: int = 128 * 128, max_pixels: int = 1920 * 1080,
+                spp=(1, 2, 4, 8, 16), bounces: int = 4, halo_every: int = 6, terrain_cells: int = 512):
+    """``count`` views of uneven resolution and sample count: pixel counts
+    log-uniform in [min_pixels, max_pixels] at aspect 1, 4:3 or 16:9,
+    samples per pass drawn from ``spp``, scenes cycling through the airless
+    triple, the ringed giant (own moon seed), a planet with air seen from
+    orbit and, every ``halo_every``-th view, the Halo observatory at the
+    reference's terrain and primitive capacities.
+
+    Returns a list of dicts with ``frame``, ``terrain`` (``(data, cells)`` or
+    None), ``width``, ``height``, ``spp`` and ``bounces``.
+    """
+    rng = np.random.default_rng(seed)
+    aspects = (1.0, 4.0 / 3.0, 16.0 / 9.0)
+    halo_cache = {}
+    views = []
+    for i in range(count):
+        pixels = math.exp(rng.uniform(math.log(min_pixels), math.log(max_pixels)))
+        aspect = aspects[int(rng.integers(len(aspects)))]
+        height = max(8, int(round(math.sqrt(pixels / aspect))))
+        width = max(8, int(round(height * aspect)))
+        samples = int(spp[int(rng.integers(len(spp)))])
+        terrain = None
+        if halo_every and i % halo_every == halo_every - 1:
+            if "halo" not in halo_cache:
+                f, data, cells = halo_observatory(64, 36, cells=terrain_cells, rocks=50, conifers=130, reach=400.0,
+                                                  levels=((6, 6),))
+                halo_cache["halo"] = (f, data, cells)
+            f0, data, cells = halo_cache["halo"]
+            f = Frame(f0.globals.copy(), f0.bodies.copy(), f0.ground.copy())
+            camera(f.globals, 70.0, width, height)
+            terrain = (data, cells)
+            kind = "halo"
+        else:
+            kind = ("triple", "giant", "orbit_air")[i % 3]
+            if kind == "triple":
+                f = airless_triple(width, height)
+            elif kind == "giant":
+                f = ringed_giant(width, height, seed=int(rng.integers(1, 2**31)))
+            else:
+                f = orbit_air(width, height, seed=int(rng.integers(1, 2**31)))
+        views.append(dict(kind=kind, frame=f, terrain=terrain, width=width, height=height, spp=samples,
+                          bounces=bounces))
+    return views

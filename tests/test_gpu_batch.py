@@ -1,0 +1,138 @@
+"""Batched views on MI355X: one persistent launch per pass must give each
+view the image its own per-view ``encode`` gives (same trace code, same RNG
+streams, same blend order)."""
+import numpy as np
+import pytest
+
+from stargaze_amd import scenes
+from stargaze_amd.pathtracer import INTEGRATOR_FUSED, Batch, Options, PathTracer
+
+pytestmark = pytest.mark.gpu
+
+
+def small_views(count=8, seed=11):
+    return scenes.batch_views(count, seed=seed, min_pixels=16 * 16, max_pixels=96 * 64, spp=(1, 3, 4, 16),
+                              halo_every=4)
+
+
+def tracers(views, limit=None):
+    out = []
+    for v in views:
+        o = Options(samples_per_frame=v["spp"], max_bounces=v["bounces"], milky_way=0.0)
+        if limit is not None:
+            o.sample_limit = limit(v)
+        t = PathTracer((v["width"], v["height"]), options=o)
+        if v["terrain"] is not None:
+            t.upload_terrain(*v["terrain"])
+        t.set_integrator(INTEGRATOR_FUSED)
+        out.append(t)
+    return out
+
+
+def compare(views, a, b, label):
+    worst = 0.0
+    for i, (v, ta, tb) in enumerate(zip(views, a, b)):
+        assert ta.samples() == tb.samples(), f"{label}: view {i} sample count"
+        x, y = ta.read_accumulation(), tb.read_accumulation()
+        bx, by = ta.read_backdrop(), tb.read_backdrop()
+        assert np.all(np.isfinite(x)), f"{label}: view {i} non-finite"
+        scale = max(1e-30, float(np.abs(y).max()))
+        diff = max(float(np.abs(x - y).max()), float(np.abs(bx - by).max())) / scale
+        worst = max(worst, diff)
+        # Same code and RNG: identical up to the compiler's fused multiply-add
+        # choices, which may differ between the two kernels.
+        assert diff <= 1e-5, f"{label}: view {i} ({v['kind']} {v['width']}x{v['height']} spp {v['spp']}): {diff:.3e}"
+    print(f"{label}: {len(views)} views, worst relative difference {worst:.3e}", flush=True)
+
+
+def close(batch, *groups):
+    batch.close()
+    for g in groups:
+        for t in g:
+            t.close()
+
+
+@pytest.mark.parametrize("block", [64, 128, 256])
+@pytest.mark.parametrize("use_graph", [1, 0])
+def test_batch_matches_per_view(block, use_graph):
+    print(f"batch parity block={block} graph={use_graph}", flush=True)
+    views = small_views()
+    frames = [v["frame"] for v in views]
+    mine, ref = tracers(views), tracers(views)
+    batch = Batch(mine)
+    batch.set_launch(block=block, use_graph=use_graph, workgroups=64, blocks_per_pull=2)
+    for _ in range(3):
+        batch.encode(frames)
+        for t, f in zip(ref, frames):
+            t.encode(f)
+    compare(views, mine, ref, f"block {block} graph {use_graph}")
+    close(batch, mine, ref)
+
+
+def test_batch_follows_frame_changes_and_limits():
+    """A moved camera restarts one view's history and a sample limit shrinks
+    another's last pass: the table is re-uploaded and the graph re-captured."""
+    print("batch table changes", flush=True)
+    views = small_views(6, seed=5)
+    frames = [v["frame"] for v in views]
+    limit = lambda v: 2 * v["spp"] + 1  # third pass takes one sample
+    mine, ref = tracers(views, limit), tracers(views, limit)
+    batch = Batch(mine)
+    for p in range(5):
+        if p == 2:
+            g = frames[1].globals.copy()
+            g["cam_forward"] = [0.05, 0.0, -1.0, g["cam_forward"][3]]
+            frames[1] = scenes.Frame(g, frames[1].bodies, frames[1].ground)
+        batch.encode(frames)
+        for t, f in zip(ref, frames):
+            t.encode(f)
+    compare(views, mine, ref, "changes")
+    close(batch, mine, ref)
+
+
+def test_batch_view_times_and_orders():
+    print("batch view times", flush=True)
+    views = small_views(8, seed=3)
+    frames = [v["frame"] for v in views]
+    mine = tracers(views)
+    batch = Batch(mine)
+    for order in (0, 1, 2):
+        batch.set_launch(order=order)
+        batch.encode(frames)
+        ms, span = batch.view_times()
+        assert span > 0.0
+        assert np.all(ms > 0.0) and np.all(ms <= span * 1.0001), (ms, span)
+    per_view = batch.encode_views(frames)
+    assert np.all(per_view > 0.0)
+    close(batch, mine)
+
+
+def test_batch_autotune_then_parity():
+    print("batch autotune", flush=True)
+    views = small_views(6, seed=9)
+    frames = [v["frame"] for v in views]
+    mine, ref = tracers(views), tracers(views)
+    batch = Batch(mine)
+    best, ms = batch.autotune(frames, repeats=1)
+    assert best["block"] in (64, 128, 256) and best["workgroups"] >= 1 and ms > 0.0
+    assert batch.launch == best
+    assert all(t.samples() == 0 for t in mine)
+    for _ in range(2):
+        batch.encode(frames)
+        for t, f in zip(ref, frames):
+            t.encode(f)
+    compare(views, mine, ref, "after autotune")
+    close(batch, mine, ref)
+
+
+def test_batch_rejects_misuse():
+    views = small_views(2, seed=1)
+    mine = tracers(views)
+    batch = Batch(mine)
+    with pytest.raises(Exception):
+        Batch([mine[0]])  # already a view of a batch
+    with pytest.raises(Exception):
+        batch.set_launch(block=96)
+    with pytest.raises(ValueError):
+        batch.encode([views[0]["frame"]])
+    close(batch, mine)

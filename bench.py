@@ -13,6 +13,7 @@ from __future__ import annotations
 
 import argparse
 import json
+import math
 import os
 import sys
 import time
@@ -132,7 +133,14 @@ def cpu_baseline(frame, terrain, width, height, spp, bounces, target_s=12.0):
     }
 
 
-def batch_leg(steps: int, warmup: int, count: int = 96, tune: bool = True) -> dict:
+BATCH_WORKLOADS = {
+    # name: batch_views keyword arguments
+    "mixed_96": dict(count=96, min_pixels=128 * 128, max_pixels=1920 * 1080),
+    "small_96": dict(count=96, min_pixels=64 * 64, max_pixels=640 * 360),
+}
+
+
+def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = True) -> dict:
     """Batched views against per-view launches on the same views.
 
     Both forms render identical images pass for pass (same trace code, RNG
@@ -143,7 +151,9 @@ def batch_leg(steps: int, warmup: int, count: int = 96, tune: bool = True) -> di
     from stargaze_amd.pathtracer import INTEGRATOR_FUSED, Batch
 
     t0 = time.perf_counter()
-    views = scenes.batch_views(count)
+    spec = BATCH_WORKLOADS[workload]
+    count = spec["count"]
+    views = scenes.batch_views(**spec)
     frames = [v["frame"] for v in views]
     tracers = []
     for v in views:
@@ -172,14 +182,16 @@ def batch_leg(steps: int, warmup: int, count: int = 96, tune: bool = True) -> di
         lat_b.append(ms)
     batch.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
-    for _ in range(warmup):
-        batch.encode_views(frames)
-    lat_v = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        lat_v.append(batch.encode_views(frames))
-    t_views = (time.perf_counter() - t0) / steps
-    lat_b, lat_v = np.concatenate(lat_b), np.concatenate(lat_v)
+    per_view = {}
+    for serial in (False, True):
+        for _ in range(warmup):
+            batch.encode_views(frames, serial=serial)
+        lat = []
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            lat.append(batch.encode_views(frames, serial=serial))
+        per_view[serial] = ((time.perf_counter() - t0) / steps, np.concatenate(lat))
+    lat_b = np.concatenate(lat_b)
     batch.close()
     for t in tracers:
         t.close()
@@ -187,21 +199,74 @@ def batch_leg(steps: int, warmup: int, count: int = 96, tune: bool = True) -> di
     def pct(a, q):
         return round(float(np.percentile(a, q)), 3)
 
+    def form(t, lat):
+        return {"views_per_s": round(count / t, 2), "pass_ms": round(t * 1e3, 3),
+                "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)}}
+
+    lo, hi = int(math.sqrt(spec["min_pixels"])), spec["max_pixels"]
     return {
+        "workload": workload,
         "views": count,
         "path_samples_per_pass": paths,
-        "views_per_s": round(count / t_batch, 2),
-        "per_view_launch_views_per_s": round(count / t_views, 2),
-        "speedup": round(t_views / t_batch, 3),
-        "pass_ms": round(t_batch * 1e3, 3),
-        "per_view_launch_pass_ms": round(t_views * 1e3, 3),
-        "latency_ms": {"p50": pct(lat_b, 50), "p99": pct(lat_b, 99)},
-        "per_view_launch_latency_ms": {"p50": pct(lat_v, 50), "p99": pct(lat_v, 99)},
+        "batched": form(t_batch, lat_b),
+        "per_view_one_queue": form(*per_view[True]),
+        "per_view_own_streams": form(*per_view[False]),
+        "speedup_vs_one_queue": round(per_view[True][0] / t_batch, 3),
+        "speedup_vs_own_streams": round(per_view[False][0] / t_batch, 3),
         "autotune": tuned,
-        "views_desc": "log-uniform 128^2..1920x1080 px, spp {1,2,4,8,16} per pass, 4 bounces; "
+        "views_desc": f"{count} views, log-uniform {lo}^2..{hi} px, spp {{1,2,4,8,16}} per pass, 4 bounces; "
         "airless triple / ringed giant / planet with air from orbit, every 6th the Halo observatory "
         "(512-cell terrain, 180 props, air)",
     }
+
+
+ADJOINT_WORKLOADS = {
+    # name: (builder, width, height, spp, bounces, tabulated ring profile on body 1)
+    "ringed_giant_1080p": (scenes.ringed_giant, 1920, 1080, 64, 6, True),
+    "orbit_air_1080p": (scenes.orbit_air, 1920, 1080, 16, 4, False),
+}
+
+
+def adjoint_leg(steps: int) -> list:
+    """Adjoint against forward kernel time on the same pass (HIP events on
+    the tracer's stream). The adjoint runs each path twice (primal, then
+    replay with derivatives)."""
+    from stargaze_amd.pathtracer import INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT
+
+    out = []
+    for name, (builder, width, height, spp, bounces, profiled) in ADJOINT_WORKLOADS.items():
+        frame = builder(width, height)
+        t = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
+        if profiled:
+            t.set_ring_profile(1, np.full(512, float(frame.bodies["ring_params"][1][1]), np.float32))
+        d = np.random.default_rng(1).uniform(0.5, 1.5, (height, width, 3)).astype(np.float32)
+        times = {}
+        for integrator, label in ((INTEGRATOR_FUSED, "fused"), (INTEGRATOR_WAVEFRONT, "wavefront")):
+            t.set_integrator(integrator)
+            t.restart()
+            t.encode(frame)
+            t.adjoint(frame, d)
+            t.set_profiling(True)
+            for _ in range(steps):
+                t.restart()
+                t.encode(frame)
+                t.adjoint(frame, d)
+            fwd_ms, n = t.kernel_time()
+            adj_ms, m = t.adjoint_time()
+            times[label] = fwd_ms / max(n, 1)
+            times["adjoint"] = adj_ms / max(m, 1)
+            t.set_profiling(False)
+        grad = t.adjoint(frame, d)
+        assert np.all(np.isfinite(grad)), "non-finite gradient"
+        t.close()
+        best = min(times["fused"], times["wavefront"])
+        out.append({"workload": name, "spp": spp, "max_bounces": bounces,
+                    "forward_ms": {"fused": round(times["fused"], 3), "wavefront": round(times["wavefront"], 3)},
+                    "adjoint_ms": round(times["adjoint"], 3),
+                    "ratio_vs_fused": round(times["adjoint"] / times["fused"], 3),
+                    "ratio_vs_best_forward": round(times["adjoint"] / best, 3)})
+        log(f"adjoint {name}: {out[-1]}")
+    return out
 
 
 def main() -> int:
@@ -213,6 +278,7 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
+    ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
@@ -283,8 +349,13 @@ def main() -> int:
         },
     }
     if args.batch_steps > 0:
-        log("batch leg")
-        line["batch"] = batch_leg(args.batch_steps, 1)
+        line["batch"] = []
+        for name in BATCH_WORKLOADS:
+            log(f"batch leg {name}")
+            line["batch"].append(batch_leg(args.batch_steps, 1, name))
+    if args.adjoint_steps > 0:
+        log("adjoint leg")
+        line["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(frame, terrain, width, height, spp, bounces)

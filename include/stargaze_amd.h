@@ -34,6 +34,8 @@ extern "C" {
 #define SG_MAX_SAMPLES (1u << 24)
 /* Finest terrain grid this build traverses (the reference stops at 2048). */
 #define SG_MAX_TERRAIN_CELLS 8192u
+/* Radial bins of a tabulated ring optical-depth profile (sg_set_ring_profile). */
+#define SG_RING_PROFILE_BINS 512u
 
 #define SG_OK 0
 #define SG_E_INVALID (-1)   /* bad argument, size or layout */
@@ -198,11 +200,47 @@ int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
  * its megakernel). SG_INTEGRATOR_WAVEFRONT (default) runs extend / camera-air
  * / shade kernels over compacted queues of at most `chunk_paths` live paths;
  * SG_INTEGRATOR_FUSED runs one lane per path through the whole loop. Both
- * give bit-identical images. `chunk_paths` 0 keeps the current value. */
+ * draw the same samples; images agree to the last bits (the compiler fuses
+ * multiply-adds differently in the two kernels). `chunk_paths` 0 keeps the current value. */
 #define SG_INTEGRATOR_FUSED 1u
 #define SG_INTEGRATOR_WAVEFRONT 2u
 int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths);
 const char *sg_last_error(const sg_ctx *ctx);
+/* ------------------------------------------------------------------------
+ * Differentiable rendering (no reference counterpart).
+ *
+ * sg_set_ring_profile replaces the banded optical-depth profile of one
+ * body's ring (src/shaders/rings.wgsl:50-81) by SG_RING_PROFILE_BINS values
+ * over the fractional radius from the inner to the outer edge (bin =
+ * floor(x * bins)); the ring colour keeps its banded tint. tau = NULL
+ * restores the analytic profile. Either call restarts the accumulation.
+ *
+ * sg_adjoint differentiates the last pass sg_encode rendered (same frame,
+ * same draws): it returns d L / d parameters for
+ *     L = sum over pixels p of dot(d_image[p], pass_mean[p].rgb),
+ * where pass_mean is the mean radiance of that pass's samples (the image
+ * itself after a pass from a fresh history). The gradient vector has
+ * SG_GRAD_COUNT entries, laid out as below; parameters are the stored values
+ * (sg_sphere.color, tau bins, atmo_rayleigh rgb and w, atmo_params x).
+ * Scenes with a landscape are rejected. Body albedos are differentiated for
+ * surface style 0; gas-giant albedos get zero. `pass_mean` (optional,
+ * W*H*4 floats) receives the pass mean the adjoint recomputed.
+ * ---------------------------------------------------------------------- */
+#define SG_GRAD_ALBEDO(body, channel) (3u * (body) + (channel))
+#define SG_GRAD_RING(bin) (3u * SG_BODY_CAPACITY + (bin))
+#define SG_GRAD_AIR (3u * SG_BODY_CAPACITY + SG_RING_PROFILE_BINS) /* +0..2 Rayleigh rgb, +3 Mie, +4 Mie g */
+#define SG_GRAD_COUNT (SG_GRAD_AIR + 5u)
+
+int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t bins);
+/* Restart the accumulation with the next pass while keeping the noise
+ * sequence running, as a changed frame does (src/pathtracer.rs:172-188):
+ * every restarted pass draws fresh samples. */
+int sg_restart(sg_ctx *ctx);
+int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_t pixel_count,
+               float *gradient, uint32_t gradient_count, float *pass_mean);
+/* Device time of sg_adjoint kernels since sg_set_profiling (HIP events). */
+int sg_adjoint_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
+
 /* Run production device helpers on a batch of probe rays (see sg_probe_kind). */
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind,
              const sg_probe_ray *rays, uint32_t count, sg_probe_result *results);
@@ -246,9 +284,11 @@ int sg_batch_view_times(sg_batch *batch, double *milliseconds, uint32_t count, d
 int sg_batch_set_profiling(sg_batch *batch, int32_t enable);
 int sg_batch_kernel_time(sg_batch *batch, double *milliseconds, uint64_t *launches);
 /* The per-view form for comparison: one sg_encode per view in order, each on
- * its view's stream; milliseconds[i] (optional) = completion of view i after
- * the first launch was queued. */
-int sg_batch_encode_views(sg_batch *batch, const sg_frame *frames, double *milliseconds);
+ * its view's stream (serial = 0, so views may overlap on the device) or each
+ * waiting for the previous view (serial = 1, one queue in submission order);
+ * milliseconds[i] (optional) = completion of view i after the first launch
+ * was queued. */
+int sg_batch_encode_views(sg_batch *batch, const sg_frame *frames, double *milliseconds, int32_t serial);
 /* Time trial passes over launch settings (workgroup size, grid size, blocks
  * per pull) and keep the fastest. Trial passes render into the views, whose
  * progressive history is reset afterwards. */

@@ -39,6 +39,10 @@ EXPORTS = (
     "sg_set_integrator",
     "sg_last_error",
     "sg_probe",
+    "sg_set_ring_profile",
+    "sg_restart",
+    "sg_adjoint",
+    "sg_adjoint_time",
     "sg_batch_create",
     "sg_batch_destroy",
     "sg_batch_set_launch",
@@ -124,6 +128,10 @@ def load() -> ctypes.CDLL:
         "sg_set_integrator": ([P, U32, U32], I32),
         "sg_last_error": ([P], ctypes.c_char_p),
         "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
+        "sg_set_ring_profile": ([P, U32, P, U32], I32),
+        "sg_restart": ([P], I32),
+        "sg_adjoint": ([P, ctypes.POINTER(SgFrame), P, U64, P, U32, P], I32),
+        "sg_adjoint_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
         "sg_batch_create": ([ctypes.POINTER(P), ctypes.POINTER(P), U32], I32),
         "sg_batch_destroy": ([P], I32),
         "sg_batch_set_launch": ([P, ctypes.POINTER(SgBatchLaunch)], I32),
@@ -133,7 +141,7 @@ def load() -> ctypes.CDLL:
         "sg_batch_view_times": ([P, P, U32, ctypes.POINTER(ctypes.c_double)], I32),
         "sg_batch_set_profiling": ([P, I32], I32),
         "sg_batch_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
-        "sg_batch_encode_views": ([P, P, P], I32),
+        "sg_batch_encode_views": ([P, P, P, I32], I32),
         "sg_batch_autotune": ([P, P, U32, ctypes.POINTER(SgBatchLaunch), ctypes.POINTER(ctypes.c_double)], I32),
         "sg_batch_last_error": ([P], ctypes.c_char_p),
     }

@@ -92,6 +92,10 @@ struct Scene {
     const uint32_t *bvh_prims;   // leaf slots -> primitive index
     TerrainLayout t;
     uint32_t catalogue_count;
+    // Tabulated ring optical depth (SG_RING_PROFILE_BINS radial bins from
+    // the inner to the outer edge) replacing the banded profile of one body.
+    const float *ring_profile;
+    uint32_t ring_profile_body;  // SG_MISS when no table is set
 };
 
 struct Ray {
@@ -425,10 +429,8 @@ SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
 }
 // src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1 (gas giant).
 // Styles 2-6 are rejected by the host library before any launch.
-SG_FN f3 surface_albedo(const Body &b, f3 normal) {
-    f3 local = body_local(b, normal);
-    uint32_t style = (uint32_t)(b.axis_x.w + 0.5f);
-    if (style > 0u) return clamp3(gas_giant_albedo(b, local), 0.0f, 1.0f);
+// Style 0: the albedo is material.rgb times this scalar mottle, clamped.
+SG_FN float surface_mottle(const Body &b, f3 local) {
     float mottle = 1.0f;
     if (b.low.w > 0.0f) {
         f3 p = local * 2.4f + seed_offset(b.axis_y.w);
@@ -440,7 +442,22 @@ SG_FN f3 surface_albedo(const Body &b, f3 normal) {
         }
         mottle = mixf(1.0f, 0.68f + 0.64f * noise, b.low.w);
     }
-    return clamp3(xyz(b.material) * mottle, 0.0f, 1.0f);
+    return mottle;
+}
+SG_FN uint32_t surface_style(const Body &b) { return (uint32_t)(b.axis_x.w + 0.5f); }
+SG_FN f3 surface_albedo(const Body &b, f3 normal) {
+    f3 local = body_local(b, normal);
+    if (surface_style(b) > 0u) return clamp3(gas_giant_albedo(b, local), 0.0f, 1.0f);
+    return clamp3(xyz(b.material) * surface_mottle(b, local), 0.0f, 1.0f);
+}
+// d albedo / d material.rgb for style 0 (zero where clamped, and for the
+// gas-giant style, whose albedo the adjoint does not differentiate).
+SG_FN f3 surface_albedo_derivative(const Body &b, f3 normal) {
+    if (surface_style(b) > 0u) return f3s(0.0f);
+    const float m = surface_mottle(b, body_local(b, normal));
+    f3 a = xyz(b.material) * m;
+    return f3{(a.x > 0.0f && a.x < 1.0f) ? m : 0.0f, (a.y > 0.0f && a.y < 1.0f) ? m : 0.0f,
+              (a.z > 0.0f && a.z < 1.0f) ? m : 0.0f};
 }
 
 }  // namespace sg

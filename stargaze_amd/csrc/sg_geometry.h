@@ -629,12 +629,21 @@ SG_FN RingHit closest_ring(const Scene &sc, const Ray &ray, float limit) {
     }
     return closest;
 }
-// Analytic banded profile (src/shaders/rings.wgsl:50-81).
+// Radial bin of the tabulated profile for fractional radius x in [0, 1].
+SG_FN uint32_t ring_profile_bin(float x) {
+    return umin((uint32_t)fmaxf(x * (float)SG_RING_PROFILE_BINS, 0.0f), SG_RING_PROFILE_BINS - 1u);
+}
+SG_FN float ring_fraction(const Body &b, f3 point) {
+    float radius = length(point) / b.center.w;
+    return (radius - b.ring_plane.w) / (b.ring_params.x - b.ring_plane.w);
+}
+// Analytic banded profile (src/shaders/rings.wgsl:50-81); the optical depth
+// comes from the tabulated profile instead for the body that has one.
 SG_FN float4 ring_optics(const Scene &sc, uint32_t index, f3 point) {
     const Body &b = body(sc, index);
-    float radius = length(point) / b.center.w;
-    float x = (radius - b.ring_plane.w) / (b.ring_params.x - b.ring_plane.w);
+    float x = ring_fraction(b, point);
     float tau = b.ring_params.y;
+    const bool tabulated = index == sc.ring_profile_body;
     f3 color = xyz(b.ring_color);
     if (b.ring_params.z > 0.5f) {
         float broad = 0.5f + 0.5f * sinf(43.0f * x + 1.8f * sinf(17.0f * x));
@@ -657,6 +666,7 @@ SG_FN float4 ring_optics(const Scene &sc, uint32_t index, f3 point) {
         tau *= density;
         color = color * tint * (0.94f + 0.06f * broad);
     }
+    if (tabulated) tau = sc.ring_profile[ring_profile_bin(x)];
     return make_float4(color.x, color.y, color.z, tau);
 }
 SG_FN float ring_transmission(const Scene &sc, const Ray &ray, const RingHit &hit) {
@@ -709,6 +719,45 @@ SG_FN float light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
         if (hit.index != SG_MISS && hit.distance < surface.distance) transmission *= ring_transmission(sc, ray, hit);
     }
     return transmission;
+}
+// The shadow ray's crossing of the tabulated ring, if it lies before the
+// light: its profile bin and d log(transmission) / d tau (= -1 / mu).
+SG_FN bool ring_profile_crossing(const Scene &sc, const Ray &ray, uint32_t light, uint32_t &bin, float &dlog) {
+    const uint32_t index = sc.ring_profile_body;
+    if (index == SG_MISS) return false;
+    RingHit hit = ring_hit(sc, ray, index);
+    if (hit.index == SG_MISS) return false;
+    Hit surface = sphere_hit(sc, ray, light);
+    if (surface.index == SG_MISS || hit.distance >= surface.distance) return false;
+    const Body &b = body(sc, index);
+    bin = ring_profile_bin(ring_fraction(b, hit.point));
+    dlog = -1.0f / fmaxf(fabsf(dot(ray.direction, xyz(b.ring_plane))), 1.0e-7f);
+    return true;
+}
+// ring_bsdf = color * factor / (4 pi): the factor and d factor / d tau.
+SG_FN float2 ring_bsdf_factor(float tau, float outgoing, float incoming) {
+    float mo = fmaxf(fabsf(outgoing), 1.0e-7f), mi = fmaxf(fabsf(incoming), 1.0e-7f);
+    float a = tau / mo, b = tau / mi;
+    if (outgoing * incoming < 0.0f) {
+        float delta = fabsf(a - b);
+        float k = fabsf(1.0f / mo - 1.0f / mi), m = fminf(1.0f / mo, 1.0f / mi);
+        float integral, slope;  // I(delta) and dI/d delta
+        if (delta < 1.0e-3f) {
+            integral = 1.0f - 0.5f * delta;
+            slope = -0.5f;
+        } else {
+            integral = one_minus_exp(delta) / fmaxf(delta, 1.0e-20f);
+            slope = (expf(-delta) - integral) / delta;
+        }
+        float e = expf(-fminf(a, b)) / (mo * mi);
+        float factor = tau * e * integral;
+        float dfactor = e * (integral - tau * m * integral + tau * k * slope);
+        return make_float2(factor, dfactor);
+    }
+    float q = 1.0f / mo + 1.0f / mi, x = a + b;
+    float factor = one_minus_exp(x) / (mo + mi);
+    float dfactor = q * (x < 1.0e-3f ? 1.0f - x : expf(-x)) / (mo + mi);
+    return make_float2(factor, dfactor);
 }
 // src/shaders/rings.wgsl:121-141
 SG_FN float air_light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {

@@ -5,6 +5,65 @@
 
 namespace sg {
 
+// --- gradient accumulation for the adjoint ----------------------------------
+// Slots of the gradient vector (the C ABI's SG_GRAD_* layout): material rgb
+// per body, one per ring profile bin, then Rayleigh rgb, Mie and Mie g.
+constexpr uint32_t GRAD_ALBEDO = 0u;
+constexpr uint32_t GRAD_RING = 3u * SG_BODY_CAPACITY;
+constexpr uint32_t GRAD_AIR = GRAD_RING + SG_RING_PROFILE_BINS;
+constexpr uint32_t GRAD_COUNT = GRAD_AIR + 5u;
+
+// The plain integrator: every gradient hook compiles away.
+struct NoGrad {
+    static constexpr bool on = false;
+};
+// Adds weighted derivatives into a GRAD_COUNT accumulator (workgroup memory
+// on the GPU). Every method takes d(loss) already multiplied out.
+struct GradSink {
+    static constexpr bool on = true;
+    float *g;
+    SG_FN void add(uint32_t i, float v) const {
+        if (v == 0.0f) return;
+#if defined(__HIP_DEVICE_COMPILE__)
+        atomicAdd(&g[i], v);
+#else
+        g[i] += v;
+#endif
+    }
+    SG_FN void albedo(uint32_t body, f3 d) const {
+        add(GRAD_ALBEDO + 3u * body, d.x);
+        add(GRAD_ALBEDO + 3u * body + 1u, d.y);
+        add(GRAD_ALBEDO + 3u * body + 2u, d.z);
+    }
+    SG_FN void ring(uint32_t bin, float d) const { add(GRAD_RING + bin, d); }
+    // d / d(beta_ext) per component; beta_ext = beta_rayleigh + beta_mie.
+    SG_FN void extinction(f3 d) const {
+        add(GRAD_AIR, d.x);
+        add(GRAD_AIR + 1u, d.y);
+        add(GRAD_AIR + 2u, d.z);
+        add(GRAD_AIR + 3u, d.x + d.y + d.z);
+    }
+    // A weighted contribution proportional to beta_phase = beta_r * rayleigh
+    // + beta_m * mie(g): its derivatives through the phase coefficients.
+    SG_FN void phase(f3 wc, f3 beta_phase, float rayleigh, float mie, float dmie_dg, float beta_m) const {
+        f3 q = f3{beta_phase.x > 0.0f ? wc.x / beta_phase.x : 0.0f, beta_phase.y > 0.0f ? wc.y / beta_phase.y : 0.0f,
+                  beta_phase.z > 0.0f ? wc.z / beta_phase.z : 0.0f};
+        add(GRAD_AIR, q.x * rayleigh);
+        add(GRAD_AIR + 1u, q.y * rayleigh);
+        add(GRAD_AIR + 2u, q.z * rayleigh);
+        float sum = q.x + q.y + q.z;
+        add(GRAD_AIR + 3u, sum * mie);
+        add(GRAD_AIR + 4u, sum * beta_m * dmie_dg);
+    }
+    // A weighted contribution behind a shadow ray that may cross the
+    // tabulated ring.
+    SG_FN void shadow(const Scene &sc, const Ray &ray, uint32_t light, float weighted) const {
+        uint32_t bin;
+        float dlog;
+        if (weighted != 0.0f && ring_profile_crossing(sc, ray, light, bin, dlog)) ring(bin, weighted * dlog);
+    }
+};
+
 // --- ground materials, theme 0 (src/shaders/ground.wgsl:256-281, 597-666) ----
 SG_FN float footprint_of(const Scene &sc, float distance) {
     return distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
@@ -129,16 +188,36 @@ SG_FN bool inside_air(const Scene &sc, const Ray &origin) {
     float top = atmosphere_top(sc);
     return dot(oc, oc) < top * top;
 }
-SG_FN f3 air_transmittance(const Scene &sc, const Ray &origin, f3 dir) {
-    if (!inside_air(sc, origin)) return f3s(1.0f);
-    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
+// Density column from an anchored origin to the top of the air; negative
+// when the origin is outside the air (no extinction).
+SG_FN float air_column(const Scene &sc, const Ray &origin, f3 dir) {
+    if (!inside_air(sc, origin)) return -1.0f;
     f3 p = atmo_center(sc) + tof(host_offset(sc, origin));
-    return exp3(-beta_ext * density_integral(sc, p, dir, ATMO_SUN_STEPS));
+    return density_integral(sc, p, dir, ATMO_SUN_STEPS);
+}
+SG_FN f3 air_extinction(const Scene &sc, float column) {
+    if (column < 0.0f) return f3s(1.0f);
+    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
+    return exp3(-beta_ext * column);
+}
+SG_FN f3 air_transmittance(const Scene &sc, const Ray &origin, f3 dir) {
+    return air_extinction(sc, air_column(sc, origin, dir));
 }
 SG_FN float rayleigh_phase(float c) { return 3.0f / (16.0f * PI) * (1.0f + c * c); }
 SG_FN float mie_phase(float c, float g) {
     float g2 = g * g;
     return (1.0f - g2) / (4.0f * PI * powf(1.0f + g2 - 2.0f * g * c, 1.5f));
+}
+SG_FN float mie_phase_dg(float c, float g) {
+    float u = 1.0f + g * g - 2.0f * g * c;
+    float s = sqrtf(u);
+    return (-2.0f * g / (u * s) - 3.0f * (1.0f - g * g) * (g - c) / (u * u * s)) / (4.0f * PI);
+}
+// d log(segment) / d(beta_ext) for the segment integral of one view step.
+SG_FN float segment_dlog(float tau, float column, float segment, float beta_ext) {
+    if (segment <= 0.0f) return 0.0f;
+    if (tau < 1.0e-3f) return -0.5f * column * column / segment;
+    return (column * expf(-tau) / segment - 1.0f) / fmaxf(beta_ext, 1.0e-20f);
 }
 
 struct ReflectorSample {
@@ -201,7 +280,12 @@ SG_FN float brightest_risen_light(const Scene &sc, f3 origin) {
 }
 
 // Single scattering along a camera ray (src/shaders/pathtrace.wgsl:1092-1213).
-SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t &rng, uint32_t accumulated) {
+// With a gradient sink, `wt` is d(loss)/d(inscatter) and the derivatives of
+// the in-scatter with respect to the air coefficients, the reflector's
+// albedo and the tabulated ring profile (shadow rays) are added to `grad`.
+template <class G = NoGrad>
+SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t &rng, uint32_t accumulated,
+                         const G &grad = G(), f3 wt = f3s(0.0f)) {
     Scatter result{f3s(1.0f), f3s(0.0f)};
     if (sc.s.globals.atmo_params[3] < 0.5f) return result;
     float exit = shell_exit(sc, origin, dir);
@@ -223,6 +307,8 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     uint32_t reflected_rng = hash_u32(rng ^ 0xa511e9b3u);
     ReflectorSample reflector = sample_reflector(sc, origin, reflected_rng);
     f3 reflected = f3s(0.0f), reflected_direction = dir, reflected_phase = f3s(0.0f);
+    float reflected_cos = 0.0f;
+    f3 reflector_dalbedo = f3s(0.0f);  // d log(albedo) / d material, per component
     if (reflector.index != SG_MISS) {
         Ray reflected_ray = camera_ray(origin, dir);
         LightSample light = sample_light(sc, reflected_ray, reflector.index, reflected_rng);
@@ -231,7 +317,17 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                     (reflector.inverse_probability / light.pdf);
         reflected_direction = light.direction;
         float cos_theta = dot(dir, light.direction);
+        reflected_cos = cos_theta;
         reflected_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
+        if constexpr (G::on) {
+            Hit rh = sphere_hit(sc, reflected_ray, reflector.index);
+            if (rh.index != SG_MISS) {
+                const Body &rb = body(sc, reflector.index);
+                f3 a = surface_albedo(rb, rh.normal), d = surface_albedo_derivative(rb, rh.normal);
+                reflector_dalbedo = f3{a.x > 0.0f ? d.x / a.x : 0.0f, a.y > 0.0f ? d.y / a.y : 0.0f,
+                                       a.z > 0.0f ? d.z / a.z : 0.0f};
+            }
+        }
     }
     if (reflected.x > 0.0f || reflected.y > 0.0f || reflected.z > 0.0f) {
         float sun = brightest_risen_light(sc, origin);
@@ -245,6 +341,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     bool planetshine = reflected.x > 0.0f || reflected.y > 0.0f || reflected.z > 0.0f;
 
     f3 t_view = f3s(1.0f), inscatter = f3s(0.0f);
+    float view_column = 0.0f;  // density column from the origin to this step
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
@@ -256,6 +353,13 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         segment.x = tau.x < 1.0e-3f ? column * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) / fmaxf(beta_ext.x, 1.0e-20f);
         segment.y = tau.y < 1.0e-3f ? column * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) / fmaxf(beta_ext.y, 1.0e-20f);
         segment.z = tau.z < 1.0e-3f ? column * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) / fmaxf(beta_ext.z, 1.0e-20f);
+        f3 step_dlog = f3s(0.0f);  // d log(t_view * segment) / d(beta_ext)
+        if constexpr (G::on) {
+            step_dlog = f3{segment_dlog(tau.x, column, segment.x, beta_ext.x),
+                           segment_dlog(tau.y, column, segment.y, beta_ext.y),
+                           segment_dlog(tau.z, column, segment.z, beta_ext.z)} -
+                        f3s(view_column);
+        }
         for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
             uint32_t b = light_index(sc, slot);
             if (body(sc, b).material.w < 0.5f) continue;
@@ -267,27 +371,49 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                 float visibility = air_light_visibility(sc, light_ray, b);
                 if (visibility == 0.0f) continue;
                 f3 e_sun = xyz(body(sc, b).material) * (visibility / ((float)light_samples * light.pdf));
-                f3 sun_t = exp3(-beta_ext * density_integral(sc, p, light.direction, ATMO_SUN_STEPS));
+                float sun_column = density_integral(sc, p, light.direction, ATMO_SUN_STEPS);
+                f3 sun_t = exp3(-beta_ext * sun_column);
                 float cos_theta = dot(dir, light.direction);
                 f3 beta_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
-                inscatter += t_view * segment * beta_phase * e_sun * sun_t;
+                f3 term = t_view * segment * beta_phase * e_sun * sun_t;
+                inscatter += term;
+                if constexpr (G::on) {
+                    f3 wc = wt * term;
+                    grad.extinction(wc * (step_dlog - f3s(sun_column)));
+                    grad.phase(wc, beta_phase, rayleigh_phase(cos_theta), mie_phase(cos_theta, g),
+                               mie_phase_dg(cos_theta, g), beta_m);
+                    grad.shadow(sc, light_ray, b, wc.x + wc.y + wc.z);
+                }
             }
         }
         if (planetshine) {
-            float visibility = air_light_visibility(sc, camera_ray(p, reflected_direction), reflector.index);
+            Ray shine_ray = camera_ray(p, reflected_direction);
+            float visibility = air_light_visibility(sc, shine_ray, reflector.index);
             if (visibility > 0.0f) {
-                f3 light_t = exp3(-beta_ext * density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS));
-                inscatter += t_view * segment * reflected_phase * reflected * (light_t * visibility);
+                float light_column = density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS);
+                f3 light_t = exp3(-beta_ext * light_column);
+                f3 term = t_view * segment * reflected_phase * reflected * (light_t * visibility);
+                inscatter += term;
+                if constexpr (G::on) {
+                    f3 wc = wt * term;
+                    grad.extinction(wc * (step_dlog - f3s(light_column)));
+                    grad.phase(wc, reflected_phase, rayleigh_phase(reflected_cos), mie_phase(reflected_cos, g),
+                               mie_phase_dg(reflected_cos, g), beta_m);
+                    grad.albedo(reflector.index, wc * reflector_dalbedo);
+                    grad.shadow(sc, shine_ray, reflector.index, wc.x + wc.y + wc.z);
+                }
             }
         }
         t_view *= step_t;
+        if constexpr (G::on) view_column += column;
     }
     result.transmittance = t_view;
     result.inscatter = inscatter;
     return result;
 }
 // src/shaders/pathtrace.wgsl:1217-1234
-SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t seed) {
+SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t seed,
+                                   float *column_total = nullptr) {
     float exit = shell_exit(sc, origin, dir);
     float t_max = fminf(limit, exit);
     if (t_max <= 0.0f) return f3s(1.0f);
@@ -301,12 +427,15 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
     f3 center = atmo_center(sc);
     float r = host_radius(sc);
     f3 transmittance = f3s(1.0f);
+    float total = 0.0f;
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
         float column = expf(-altitude / H) * dt;
         transmittance *= exp3(-beta_ext * column);
+        total += column;
     }
+    if (column_total) *column_total = total;
     return transmittance;
 }
 
@@ -368,7 +497,12 @@ SG_FN float ring_area(const Scene &sc, uint32_t index) {
     float r = b.center.w;
     return PI * r * r * (b.ring_params.x * b.ring_params.x - b.ring_plane.w * b.ring_plane.w);
 }
-SG_FN f3 ringshine(const Scene &sc, uint32_t index, f3 origin, f3 normal, bool last_vertex, uint32_t &rng) {
+// With a gradient sink, `wt` is d(loss)/d(ringshine) and the derivatives
+// through the tabulated ring profile (at the sampled ring point and on its
+// shadow rays) are added to `grad`.
+template <class G = NoGrad>
+SG_FN f3 ringshine(const Scene &sc, uint32_t index, f3 origin, f3 normal, bool last_vertex, uint32_t &rng,
+                   const G &grad = G(), f3 wt = f3s(0.0f)) {
     const Body &b = body(sc, index);
     if (b.ring_params.x <= b.ring_plane.w || b.ring_params.y <= 0.0f) return f3s(0.0f);
     f3 n = xyz(b.ring_plane);
@@ -389,18 +523,38 @@ SG_FN f3 ringshine(const Scene &sc, uint32_t index, f3 origin, f3 normal, bool l
     float4 optics = ring_optics(sc, index, point);
     if (optics.w == 0.0f) return f3s(0.0f);
     float mu_out = dot(-direction, n);
+    float pdf = distance2 / (ring_area(sc, index) * cos_ring);
+    float weight = last_vertex ? 1.0f : power_weight(pdf, cos_surface / PI);
+    const float scale = cos_surface * weight / (PI * pdf);
     f3 shine = f3s(0.0f);
+    const bool tabulated = G::on && index == sc.ring_profile_body;
+    f3 shine_tau = f3s(0.0f);  // d shine / d tau at the sampled point
     for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
         uint32_t i = light_index(sc, slot);
         if (body(sc, i).material.w < 0.5f) continue;
         LightSample light = sample_light(sc, Ray{tod(point), n, index}, i, rng);
         float mu_in = dot(light.direction, n);
-        float visibility = light_visibility(sc, ring_origin(sc, RingHit{0.0f, index, point}, light.direction), i);
-        shine += ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) * (fabsf(mu_in) * visibility / light.pdf);
+        Ray shadow = ring_origin(sc, RingHit{0.0f, index, point}, light.direction);
+        float visibility = light_visibility(sc, shadow, i);
+        f3 term = ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) * (fabsf(mu_in) * visibility / light.pdf);
+        shine += term;
+        if constexpr (G::on) {
+            if (tabulated) {
+                float2 f = ring_bsdf_factor(optics.w, mu_out, mu_in);
+                shine_tau += xyz(optics) * xyz(body(sc, i).material) *
+                             (f.y / (4.0f * PI) * fabsf(mu_in) * visibility / light.pdf);
+            }
+            f3 wc = wt * term * scale;
+            grad.shadow(sc, shadow, i, wc.x + wc.y + wc.z);
+        }
     }
-    float pdf = distance2 / (ring_area(sc, index) * cos_ring);
-    float weight = last_vertex ? 1.0f : power_weight(pdf, cos_surface / PI);
-    return shine * (cos_surface * weight / (PI * pdf));
+    if constexpr (G::on) {
+        if (tabulated) {
+            f3 wc = wt * shine_tau * scale;
+            grad.ring(ring_profile_bin(ring_fraction(b, point)), wc.x + wc.y + wc.z);
+        }
+    }
+    return shine * scale;
 }
 SG_FN float pixel_angle(const Scene &sc) {
     return 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
@@ -457,6 +611,22 @@ struct PathState {
     float previous_pdf, ring_cap;
     uint32_t rng, depth, null_crossings, shine_body, flags, accumulated;
 };
+// Path replay adjoint state: the primal run's returned radiance, the loss
+// weight of the path's radiance, and the ring cap's scale on the indirect
+// part (taken as a constant). rest() is d(loss)/d log(a throughput factor
+// applied now): every later contribution carries that factor.
+struct PathGrad : GradSink {
+    f3 w;
+    f3 value;
+    float cap_scale;
+    SG_FN bool capped(const PathState &st) const { return st.ring_cap < 1.0e29f; }
+    SG_FN f3 so_far(const PathState &st) const {
+        return capped(st) ? st.ring_base + (st.radiance - st.ring_base) * cap_scale : st.radiance;
+    }
+    SG_FN f3 weight(const PathState &st) const { return capped(st) ? w * cap_scale : w; }
+    SG_FN f3 rest(const PathState &st) const { return w * (value - so_far(st)); }
+};
+
 struct HitRecord {
     Hit hit;
     float disc_scale;
@@ -494,7 +664,8 @@ SG_FN HitRecord path_extend(const Scene &sc, PathState &st) {
     return h;
 }
 
-SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h) {
+template <class G = NoGrad>
+SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, const G &grad = G()) {
     if (!(st.flags & PATH_CAMERA_SEGMENT)) return;
     st.flags &= ~PATH_CAMERA_SEGMENT;
     if (sc.s.globals.atmo_params[3] < 0.5f) return;
@@ -503,15 +674,32 @@ SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h) {
     if (h.ring.index != SG_MISS) limit = fminf(limit, h.ring.distance);
     f3 origin = tof(st.ray.offset);
     uint32_t air_rng = st.rng;
-    Scatter air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
-                             st.accumulated);
-    f3 transmittance = camera_air_transmittance(sc, origin, st.ray.direction, limit, st.rng);
+    Scatter air;
+    float column = 0.0f;
+    if constexpr (G::on) {
+        air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
+                         st.accumulated, static_cast<const GradSink &>(grad), grad.weight(st) * st.throughput);
+    } else {
+        air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
+                         st.accumulated);
+    }
+    f3 transmittance = camera_air_transmittance(sc, origin, st.ray.direction, limit, st.rng,
+                                                G::on ? &column : nullptr);
     st.radiance += st.throughput * air.inscatter;
+    if constexpr (G::on) grad.extinction(grad.rest(st) * (-column));
     st.throughput *= transmittance;
     st.rng = air_rng;
 }
 
-SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
+// With a PathGrad, replays the step of the primal run and adds the
+// derivatives of every contribution and throughput factor with respect to
+// body materials, the tabulated ring profile and the air coefficients.
+// Discrete choices whose probability the weights compensate (Russian
+// roulette, reflector and point-disc selection, ring scattering) carry no
+// score; the ring pass-through, weighted 1 with probability T, carries
+// d log T. MIS weights are taken as constants.
+template <class G = NoGrad>
+SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const G &grad = G()) {
     const sg_settings &s = sc.s;
     const uint32_t host = (uint32_t)s.globals.atmo_params[3] - 1u;
     const uint32_t max_vertices = s.counts[3];
@@ -520,8 +708,13 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
     Ray &ray = st.ray;
     if (st.flags & PATH_IN_AIR) {
         st.flags &= ~PATH_IN_AIR;
-        if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host))
-            st.throughput *= air_transmittance(sc, ray, ray.direction);
+        if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host)) {
+            float column = air_column(sc, ray, ray.direction);
+            if constexpr (G::on) {
+                if (column > 0.0f) grad.extinction(grad.rest(st) * (-column));
+            }
+            st.throughput *= air_extinction(sc, column);
+        }
     }
     if (ring.index != SG_MISS) {
         const Body &planet = body(sc, ring.index);
@@ -543,21 +736,41 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
             ring_mis = power_weight(st.previous_pdf, shine_pdf);
         }
         st.shine_body = SG_MISS;
+        const bool tabulated = G::on && ring.index == sc.ring_profile_body;
+        uint32_t bin = 0u;
+        if constexpr (G::on) bin = ring_profile_bin(ring_fraction(planet, ring.point));
         float unshadowed = 0.0f, direct = 0.0f;
         for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
             uint32_t i = light_index(sc, slot);
             if (body(sc, i).material.w < 0.5f || optics.w == 0.0f) continue;
             LightSample light = sample_light(sc, Ray{tod(ring.point), n, ring.index}, i, st.rng);
             float mu_in = dot(light.direction, n);
-            float visibility = light_visibility(sc, ring_origin(sc, ring, light.direction), i);
+            Ray shadow = ring_origin(sc, ring, light.direction);
+            float visibility = light_visibility(sc, shadow, i);
             float weight = last_vertex ? 1.0f : power_weight(light.pdf, bsdf_pdf);
             f3 lit = st.throughput * ring_bsdf(optics, mu_out, mu_in) * xyz(body(sc, i).material) *
                      (fabsf(mu_in) / light.pdf);
+            if constexpr (G::on) {
+                f3 wc = grad.weight(st) * lit * (visibility * weight * ring_mis);
+                grad.shadow(sc, shadow, i, wc.x + wc.y + wc.z);
+                if (tabulated) {
+                    float2 f = ring_bsdf_factor(optics.w, mu_out, mu_in);
+                    f3 d = grad.weight(st) * st.throughput * xyz(optics) * xyz(body(sc, i).material) *
+                           (f.y / (4.0f * PI) * fabsf(mu_in) / light.pdf * visibility * weight * ring_mis);
+                    grad.ring(bin, d.x + d.y + d.z);
+                }
+            }
             st.radiance += lit * (visibility * weight * ring_mis);
             unshadowed += luminance(lit);
             direct += luminance(lit) * visibility;
         }
         if (rnd(st.rng) < transmission) {
+            if constexpr (G::on) {
+                if (tabulated) {
+                    f3 r = grad.rest(st);
+                    grad.ring(bin, (r.x + r.y + r.z) * (-1.0f / fmaxf(fabsf(dot(ray.direction, n)), 1.0e-7f)));
+                }
+            }
             ray = ring_origin(sc, ring, ray.direction);
             st.null_crossings += 1u;
             return st.null_crossings <= s.counts[0];
@@ -571,6 +784,15 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
         float phi = TAU * rnd(st.rng);
         float radial = sqrtf(fmaxf(0.0f, 1.0f - z * z));
         f3 direction = basis_direction(n, radial * cosf(phi), radial * sinf(phi), z);
+        if constexpr (G::on) {
+            if (tabulated) {
+                float2 f = ring_bsdf_factor(optics.w, mu_out, z);
+                if (f.x > 0.0f) {
+                    f3 r = grad.rest(st);
+                    grad.ring(bin, (r.x + r.y + r.z) * (f.y / f.x));
+                }
+            }
+        }
         st.throughput *= ring_bsdf(optics, mu_out, z) * (fabsf(z) / bsdf_pdf);
         st.previous_pdf = bsdf_pdf;
         st.flags &= ~PATH_SKY_ON_ESCAPE;
@@ -614,6 +836,13 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
     f3 normal = local ? landscape_shading_normal(sc, ray, hit) : hit.normal;
     Ray outgoing = local ? landscape_outgoing(sc, ray, hit)
                          : Ray{tod(normal * (hb->center.w * (1.0f + 2.0e-6f))), normal, hit.index};
+    // Material derivative and the weight of this vertex's direct light.
+    f3 dalbedo = f3s(0.0f), wt_direct = f3s(0.0f);
+    if constexpr (G::on) {
+        f3 a = local ? f3s(0.0f) : surface_albedo(*hb, normal);
+        if (!local) dalbedo = surface_albedo_derivative(*hb, normal);
+        wt_direct = grad.weight(st) * st.throughput * a;
+    }
     f3 direct_light = f3s(0.0f);
     bool last_vertex = st.depth + 1u == max_vertices;
     for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
@@ -627,15 +856,33 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
         if (visibility > 0.0f) {
             float bsdf_pdf = cosine / PI;
             float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
-            direct_light += xyz(body(sc, i).material) * air_transmittance(sc, outgoing, sample.direction) *
-                            (bsdf_pdf * weight * visibility / sample.pdf);
+            float column = air_column(sc, outgoing, sample.direction);
+            f3 term = xyz(body(sc, i).material) * air_extinction(sc, column) *
+                      (bsdf_pdf * weight * visibility / sample.pdf);
+            direct_light += term;
+            if constexpr (G::on) {
+                f3 wc = wt_direct * term;
+                if (column > 0.0f) grad.extinction(wc * (-column));
+                grad.shadow(sc, outgoing, i, wc.x + wc.y + wc.z);
+            }
         }
     }
     f3 albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
+    if constexpr (G::on) {
+        if (!local) grad.albedo(hit.index, grad.weight(st) * st.throughput * dalbedo * direct_light);
+    }
     st.radiance += st.throughput * albedo * direct_light;
     st.shine_body = SG_MISS;
     if (!local) {
-        st.radiance += st.throughput * albedo * ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, st.rng);
+        f3 shine;
+        if constexpr (G::on) {
+            shine = ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, st.rng,
+                              static_cast<const GradSink &>(grad), wt_direct);
+            grad.albedo(hit.index, grad.weight(st) * st.throughput * dalbedo * shine);
+        } else {
+            shine = ringshine(sc, hit.index, tof(outgoing.offset), normal, last_vertex, st.rng);
+        }
+        st.radiance += st.throughput * albedo * shine;
         st.shine_body = hit.index;
     }
     bool sky_lit = local && s.globals.ground_counts[1] != 0u && s.globals.atmo_params[3] > 0.5f;
@@ -646,6 +893,14 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h) {
     float cosine = sqrtf(1.0f - u);
     outgoing.direction = basis_direction(normal, sqrtf(u) * cosf(phi), sqrtf(u) * sinf(phi), cosine);
     st.previous_pdf = cosine / PI;
+    if constexpr (G::on) {
+        if (!local) {
+            f3 r = grad.rest(st);
+            grad.albedo(hit.index, f3{albedo.x > 0.0f ? r.x * dalbedo.x / albedo.x : 0.0f,
+                                      albedo.y > 0.0f ? r.y * dalbedo.y / albedo.y : 0.0f,
+                                      albedo.z > 0.0f ? r.z * dalbedo.z / albedo.z : 0.0f});
+        }
+    }
     st.throughput *= albedo;
     if (st.depth >= 2u) {
         float survival = clampf(maxc(st.throughput), 0.05f, 0.95f);
@@ -675,16 +930,44 @@ SG_FN PathResult path_finish(const PathState &st) {
     return out;
 }
 
+// The loop of trace() over one path (its state already begun).
+template <class G = NoGrad>
+SG_FN void run_path(const Scene &sc, PathState &st, const G &grad = G()) {
+    for (;;) {
+        HitRecord h = path_extend(sc, st);
+        path_camera_air(sc, st, h, grad);
+        if (!path_shade(sc, st, h, grad)) break;
+    }
+}
 // Whole path in one lane.
 SG_FN PathResult trace_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated) {
     PathState st;
     path_begin(st, direction, rng, accumulated);
-    for (;;) {
-        HitRecord h = path_extend(sc, st);
-        path_camera_air(sc, st, h);
-        if (!path_shade(sc, st, h)) break;
-    }
+    run_path(sc, st);
     return path_finish(st);
+}
+// Scale path_finish applies to the indirect radiance after the ring cap.
+SG_FN float ring_cap_scale(const PathState &st) {
+    if (st.ring_cap >= 1.0e29f) return 1.0f;
+    float lum = luminance(st.radiance - st.ring_base);
+    return lum > st.ring_cap ? st.ring_cap / lum : 1.0f;
+}
+// One path sample's adjoint: the primal run, then its replay with the same
+// draws adding d(loss)/d(parameters) for d(loss)/d(radiance) = w.
+SG_FN PathResult adjoint_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated, f3 w,
+                              float *gradient) {
+    PathState st;
+    path_begin(st, direction, rng, accumulated);
+    run_path(sc, st);
+    PathResult primal = path_finish(st);
+    PathGrad grad;
+    grad.g = gradient;
+    grad.w = w;
+    grad.value = f3{primal.value.x, primal.value.y, primal.value.z};
+    grad.cap_scale = ring_cap_scale(st);
+    path_begin(st, direction, rng, accumulated);
+    run_path(sc, st, grad);
+    return primal;
 }
 
 // src/shaders/pathtrace.wgsl:1619-1626

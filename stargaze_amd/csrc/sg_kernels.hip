@@ -478,6 +478,70 @@ __global__ void k_batch_advance(BatchArgs ba) {
     v.scene.s.sampling[0] += v.ta.count;
 }
 
+// ---------------------------------------------------------------------------
+// Adjoint of one progressive pass (no reference counterpart). Lane layout as
+// k_trace; each lane runs its path sample's primal and replay (adjoint_path),
+// adding d(loss)/d(parameters) into a workgroup copy of the gradient that is
+// flushed to the device gradient once, at the end of the persistent loop.
+// ---------------------------------------------------------------------------
+struct AdjointArgs {
+    TraceArgs ta;            // pass layout; accumulation/backdrop receive the pass mean (or null)
+    const float *d_image;    // d(loss)/d(pass mean), rgb per pixel
+    float *gradient;         // GRAD_COUNT
+    uint32_t blocks;
+};
+
+__global__ void k_fill(float *p, uint32_t n, float v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs aa) {
+    __shared__ float s_grad[GRAD_COUNT];
+    __shared__ float4 s_value[kTraceBlock];
+    __shared__ float4 s_backdrop[kTraceBlock];
+    const uint32_t t = threadIdx.x;
+    for (uint32_t i = t; i < GRAD_COUNT; i += blockDim.x) s_grad[i] = 0.0f;
+    __syncthreads();
+    const TraceArgs &ta = aa.ta;
+    const uint32_t lp = t / ta.count;
+    const uint32_t s = t - lp * ta.count;
+    const uint32_t pixels = ta.width * ta.height;
+    const float inv_count = 1.0f / (float)ta.count;
+    for (uint32_t block = blockIdx.x; block < aa.blocks; block += gridDim.x) {
+        const uint32_t pixel = block * ta.pixels_per_block + lp;
+        const bool active = lp < ta.pixels_per_block && pixel < pixels;
+        float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
+        f3 backdrop = f3s(0.f);
+        if (active) {
+            const uint32_t x = pixel % ta.width, y = pixel / ta.width;
+            const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
+            uint32_t rng = path_seed(pixel, ta.sequence + s);
+            f3 direction = camera_direction(sc, x, y, rng);
+            const float *d = aa.d_image + 3u * (size_t)pixel;
+            f3 w = f3{d[0], d[1], d[2]} * inv_count;
+            PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad);
+            value = r.value;
+            backdrop = r.backdrop;
+        }
+        if (ta.accumulation) {
+            s_value[t] = value;
+            s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
+            __syncthreads();
+            if (active && s == 0u) {
+                TraceArgs mean = ta;
+                mean.accumulated = 0u;
+                mean.moments = nullptr;
+                blend_pixel(mean, pixel, s_value + lp * ta.count, s_backdrop + lp * ta.count);
+            }
+            __syncthreads();
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < GRAD_COUNT; i += blockDim.x)
+        if (s_grad[i] != 0.0f) atomicAdd(&aa.gradient[i], s_grad[i]);
+}
+
 // cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
 __global__ void k_terrain_cache(Scene sc, float *terrain) {
     terrain_cache_entry(sc, terrain, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
@@ -609,6 +673,20 @@ struct sg_ctx {
     sg_batch *batch = nullptr;
     hipEvent_t batch_event = nullptr;
     hipEvent_t ev_ready = nullptr;
+    // Tabulated ring profile (SG_RING_PROFILE_BINS floats) and its body.
+    float *ring_profile = nullptr;
+    uint32_t ring_profile_body = SG_MISS;
+    // The last queued pass, which sg_adjoint differentiates.
+    struct {
+        uint32_t count = 0, samples = 0, sequence = 0;
+    } last;
+    // Adjoint buffers: d(loss)/d(image) rgb, gradient, recomputed pass mean.
+    float *adj_image = nullptr;
+    float *adj_gradient = nullptr;
+    float4 *adj_primal = nullptr;
+    uint64_t adj_pixels = 0;
+    double adjoint_ms = 0.0;
+    uint64_t adjoint_launches = 0;
 };
 
 namespace {
@@ -703,11 +781,15 @@ Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     sc.bvh = ctx->bvh_valid ? ctx->bvh_nodes : nullptr;
     sc.bvh_prims = ctx->bvh_valid ? ctx->bvh_prims : nullptr;
     sc.t = terrain_layout(settings.globals.ground_counts[2]);
+    const bool profiled = ctx->ring_profile && ctx->ring_profile_body < settings.counts[0];
+    sc.ring_profile = profiled ? ctx->ring_profile : nullptr;
+    sc.ring_profile_body = profiled ? ctx->ring_profile_body : SG_MISS;
     return sc;
 }
 
-// History::update (src/pathtracer.rs:172-188).
-void history_update(sg_ctx *ctx, const sg_frame *frame) {
+// History key of a frame (src/pathtracer.rs:172-181): everything whose change
+// restarts the accumulation.
+std::vector<uint8_t> history_key(const sg_ctx *ctx, const sg_frame *frame) {
     std::vector<uint8_t> key;
     const uint8_t *g = reinterpret_cast<const uint8_t *>(frame->globals);
     key.insert(key.end(), g, g + sizeof(sg_globals));
@@ -721,6 +803,12 @@ void history_update(sg_ctx *ctx, const sg_frame *frame) {
     key.insert(key.end(), mb, mb + 4);
     const uint8_t *mw = reinterpret_cast<const uint8_t *>(&ctx->options.milky_way);
     key.insert(key.end(), mw, mw + 4);
+    return key;
+}
+
+// History::update (src/pathtracer.rs:172-188).
+void history_update(sg_ctx *ctx, const sg_frame *frame) {
+    std::vector<uint8_t> key = history_key(ctx, frame);
     if (key != ctx->key) {
         ctx->key.swap(key);
         ctx->samples = 0;
@@ -862,17 +950,11 @@ struct PassPlan {
     bool queued = false;  // work was queued on the context stream
 };
 
-int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
-    int rc = validate_frame(ctx, frame);
-    if (rc != SG_OK) return rc;
-    if (frame->ground_count > 0) {
-        rc = ensure_objects(ctx, frame->ground_count);
-        if (rc != SG_OK) return rc;
-    }
-    history_update(ctx, frame);
+// The Settings uniform of a pass of `count` samples over `samples` already
+// accumulated, at noise sequence `sequence` (src/pathtracer.rs:829-905).
+sg_settings pass_settings(const sg_ctx *ctx, const sg_frame *frame, uint32_t count, uint32_t samples,
+                          uint32_t sequence) {
     const sg_options &o = ctx->options;
-    uint32_t remaining = o.sample_limit > ctx->samples ? o.sample_limit - ctx->samples : 0u;
-    uint32_t count = std::min(std::min(o.samples_per_frame, SG_MAX_BATCH_SAMPLES), remaining);
     sg_settings st;
     std::memset(&st, 0, sizeof(st));
     st.globals = *frame->globals;
@@ -900,15 +982,31 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
         }
     }
     std::memcpy(st.lights, lights, sizeof(lights));
-    st.sampling[0] = o.noise_seed + ctx->sequence;
+    st.sampling[0] = sequence;
     st.counts[0] = frame->body_count;
-    st.counts[1] = ctx->samples;
+    st.counts[1] = samples;
     st.counts[2] = count;
     st.counts[3] = o.max_bounces;
     st.background[0] = o.milky_way;
     st.background[1] = ctx->milky_way ? (float)ctx->milky_w : 1024.0f;
     st.background[2] = ctx->milky_way ? (float)ctx->milky_h : 512.0f;
     st.background[3] = 100.0f;  // stars::POINT_SOURCE_GAIN (src/stars.rs:130)
+    return st;
+}
+
+int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
+    int rc = validate_frame(ctx, frame);
+    if (rc != SG_OK) return rc;
+    if (frame->ground_count > 0) {
+        rc = ensure_objects(ctx, frame->ground_count);
+        if (rc != SG_OK) return rc;
+    }
+    history_update(ctx, frame);
+    const sg_options &o = ctx->options;
+    uint32_t remaining = o.sample_limit > ctx->samples ? o.sample_limit - ctx->samples : 0u;
+    uint32_t count = std::min(std::min(o.samples_per_frame, SG_MAX_BATCH_SAMPLES), remaining);
+    sg_settings st = pass_settings(ctx, frame, count, ctx->samples, o.noise_seed + ctx->sequence);
+    const sg_globals &g = st.globals;
     const bool has_air = frame->globals->atmo_params[3] >= 0.5f;
     plan.count = count;
     plan.queued = false;
@@ -956,6 +1054,9 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     ta.sequence = st.sampling[0];
     ta.has_air = has_air ? 1u : 0u;
     ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
+    ctx->last.count = count;
+    ctx->last.samples = ctx->samples;
+    ctx->last.sequence = st.sampling[0];
     return SG_OK;
 }
 
@@ -1032,7 +1133,7 @@ int sg_destroy(sg_ctx *ctx) {
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
-                       ctx->bvh_prims};
+                       ctx->bvh_prims, ctx->ring_profile, ctx->adj_image, ctx->adj_gradient, ctx->adj_primal};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
@@ -1210,6 +1311,8 @@ int sg_set_profiling(sg_ctx *ctx, int32_t enable) {
     ctx->profiling = enable != 0;
     ctx->kernel_ms = 0.0;
     ctx->launches = 0;
+    ctx->adjoint_ms = 0.0;
+    ctx->adjoint_launches = 0;
     return SG_OK;
 }
 
@@ -1234,6 +1337,126 @@ int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths) {
 }
 
 const char *sg_last_error(const sg_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
+
+int sg_restart(sg_ctx *ctx) {
+    if (!ctx) return SG_E_INVALID;
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->last.count = 0;
+    ctx->exposure_idle_frames = 0;
+    return SG_OK;
+}
+
+int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t bins) {
+    if (!ctx) return SG_E_INVALID;
+    join_batch(ctx);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!tau) {
+        ctx->ring_profile_body = SG_MISS;
+    } else {
+        if (bins != SG_RING_PROFILE_BINS) return fail(ctx, SG_E_INVALID, "the ring profile has SG_RING_PROFILE_BINS bins");
+        if (body >= SG_BODY_CAPACITY) return fail(ctx, SG_E_INVALID, "ring profile body index out of range");
+        for (uint32_t i = 0; i < bins; ++i)
+            if (!(tau[i] >= 0.0f) || !std::isfinite(tau[i]))
+                return fail(ctx, SG_E_INVALID, "ring optical depths must be finite and non-negative");
+        if (!ctx->ring_profile) SG_HIP(ctx, hipMalloc(&ctx->ring_profile, SG_RING_PROFILE_BINS * sizeof(float)));
+        SG_HIP(ctx, hipMemcpy(ctx->ring_profile, tau, bins * sizeof(float), hipMemcpyHostToDevice));
+        ctx->ring_profile_body = body;
+    }
+    // A new profile is a new scene: restart the accumulation.
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->last.count = 0;
+    return SG_OK;
+}
+
+int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_t pixel_count, float *gradient,
+               uint32_t gradient_count, float *pass_mean) {
+    if (!ctx || !frame || !frame->globals || !d_image || !gradient) return SG_E_INVALID;
+    if (gradient_count != SG_GRAD_COUNT) return fail(ctx, SG_E_INVALID, "gradient_count must be SG_GRAD_COUNT");
+    if (pixel_count != (uint64_t)ctx->width * ctx->height)
+        return fail(ctx, SG_E_INVALID, "pixel count does not match the image size");
+    int rc = validate_frame(ctx, frame);
+    if (rc != SG_OK) return rc;
+    if (frame->globals->ground_north[3] != 0.0f)
+        return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate scenes with a landscape");
+    if (ctx->last.count == 0 || ctx->last.samples + ctx->last.count != ctx->samples || history_key(ctx, frame) != ctx->key)
+        return fail(ctx, SG_E_STATE, "sg_adjoint needs the frame of the last pass sg_encode rendered");
+    join_batch(ctx);
+    const uint64_t pixels = pixel_count;
+    if (pixels > ctx->adj_pixels) {
+        SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+        if (ctx->adj_image) SG_HIP(ctx, hipFree(ctx->adj_image));
+        if (ctx->adj_primal) SG_HIP(ctx, hipFree(ctx->adj_primal));
+        ctx->adj_image = nullptr;
+        ctx->adj_primal = nullptr;
+        SG_HIP(ctx, hipMalloc(&ctx->adj_image, pixels * 3u * sizeof(float)));
+        SG_HIP(ctx, hipMalloc(&ctx->adj_primal, pixels * 2u * sizeof(float4)));
+        ctx->adj_pixels = pixels;
+    }
+    if (!ctx->adj_gradient) SG_HIP(ctx, hipMalloc(&ctx->adj_gradient, SG_GRAD_COUNT * sizeof(float)));
+    if (!ctx->frame_on_device) {
+        if (frame->body_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        if (frame->ground_count)
+            SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
+                                       hipMemcpyHostToDevice, ctx->stream));
+        ctx->frame_on_device = true;
+    }
+    rc = ensure_bvh(ctx, frame);
+    if (rc != SG_OK) return rc;
+    const uint32_t count = ctx->last.count;
+    Scene sc = make_scene(ctx, pass_settings(ctx, frame, count, ctx->last.samples, ctx->last.sequence));
+    AdjointArgs aa;
+    TraceArgs &ta = aa.ta;
+    ta.accumulation = pass_mean ? ctx->adj_primal : nullptr;
+    ta.backdrop = pass_mean ? ctx->adj_primal + pixels : nullptr;
+    ta.moments = nullptr;
+    ta.width = ctx->width;
+    ta.height = ctx->height;
+    ta.count = count;
+    ta.accumulated = ctx->last.samples;
+    ta.sequence = ctx->last.sequence;
+    ta.has_air = frame->globals->atmo_params[3] >= 0.5f ? 1u : 0u;
+    ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
+    aa.d_image = ctx->adj_image;
+    aa.gradient = ctx->adj_gradient;
+    aa.blocks = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);
+    SG_HIP(ctx, hipMemcpyAsync(ctx->adj_image, d_image, pixels * 3u * sizeof(float), hipMemcpyHostToDevice,
+                               ctx->stream));
+    hipLaunchKernelGGL(k_fill, dim3((SG_GRAD_COUNT + 255) / 256), dim3(256), 0, ctx->stream, ctx->adj_gradient,
+                       (uint32_t)SG_GRAD_COUNT, 0.0f);
+    SG_HIP(ctx, hipGetLastError());
+    int cus = 0;
+    SG_HIP(ctx, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const uint32_t grid = std::min<uint32_t>(aa.blocks, 2u * (uint32_t)std::max(cus, 1));
+    if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
+    hipLaunchKernelGGL(k_adjoint, dim3(grid), dim3(ta.pixels_per_block * count), 0, ctx->stream, sc, aa);
+    SG_HIP(ctx, hipGetLastError());
+    if (ctx->profiling) {
+        SG_HIP(ctx, hipEventRecord(ctx->ev_stop, ctx->stream));
+        SG_HIP(ctx, hipEventSynchronize(ctx->ev_stop));
+        float ms = 0.f;
+        SG_HIP(ctx, hipEventElapsedTime(&ms, ctx->ev_start, ctx->ev_stop));
+        ctx->adjoint_ms += ms;
+        ctx->adjoint_launches += 1;
+    }
+    SG_HIP(ctx, hipMemcpyAsync(gradient, ctx->adj_gradient, SG_GRAD_COUNT * sizeof(float), hipMemcpyDeviceToHost,
+                               ctx->stream));
+    if (pass_mean)
+        SG_HIP(ctx, hipMemcpyAsync(pass_mean, ctx->adj_primal, pixels * sizeof(float4), hipMemcpyDeviceToHost,
+                                   ctx->stream));
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+int sg_adjoint_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
+    if (!ctx || !milliseconds || !launches) return SG_E_INVALID;
+    *milliseconds = ctx->adjoint_ms;
+    *launches = ctx->adjoint_launches;
+    return SG_OK;
+}
 
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
              sg_probe_result *results) {
@@ -1670,7 +1893,7 @@ int sg_batch_kernel_time(sg_batch *b, double *milliseconds, uint64_t *launches) 
     return SG_OK;
 }
 
-int sg_batch_encode_views(sg_batch *b, const sg_frame *frames, double *milliseconds) {
+int sg_batch_encode_views(sg_batch *b, const sg_frame *frames, double *milliseconds, int32_t serial) {
     if (!b || !frames) return SG_E_INVALID;
     const uint32_t n = (uint32_t)b->views.size();
     if (b->view_events.size() != n) {
@@ -1686,6 +1909,8 @@ int sg_batch_encode_views(sg_batch *b, const sg_frame *frames, double *milliseco
     SG_BHIP(b, hipEventRecord(b->ev_start, b->views[0]->stream));
     for (uint32_t i = 0; i < n; ++i) {
         sg_ctx *ctx = b->views[i];
+        // One queue, as one device queue submits the views in order.
+        if (serial && i > 0) SG_BHIP(b, hipStreamWaitEvent(ctx->stream, b->view_events[i - 1], 0));
         int rc = sg_encode(ctx, &frames[i]);
         if (rc != SG_OK) return bfail(b, rc, "view " + std::to_string(i) + ": " + ctx->error);
         SG_BHIP(b, hipEventRecord(b->view_events[i], ctx->stream));

@@ -118,6 +118,16 @@ SKY_WIDTH, SKY_HEIGHT = 1024, 512  # src/sky.rs:8-9
 AU_KM = 149_597_870.7  # src/sim.rs:13
 METRES_PER_AU = 149_597_870_700.0  # src/shaders/ground.wgsl:15
 
+# Differentiable rendering (include/stargaze_amd.h SG_GRAD_*).
+RING_PROFILE_BINS = 512
+GRAD_RING = 3 * BODY_CAPACITY
+GRAD_AIR = GRAD_RING + RING_PROFILE_BINS  # Rayleigh rgb, Mie, Mie g
+GRAD_COUNT = GRAD_AIR + 5
+
+
+def grad_albedo(body: int, channel: int) -> int:
+    return 3 * body + channel
+
 
 def zeros(dtype: np.dtype, n: int | None = None) -> np.ndarray:
     """A zero-filled record (``n is None``) or array of ``dtype``."""

@@ -12,7 +12,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _native
-from .layouts import GLOBALS, OPTIONS, PRIMITIVE, PROBE_RAY, PROBE_RESULT, RAY_STAR, SPHERE, MAX_SAMPLES
+from .layouts import GLOBALS, GRAD_COUNT, OPTIONS, PRIMITIVE, PROBE_RAY, PROBE_RESULT, RAY_STAR, SPHERE, MAX_SAMPLES
 
 
 @dataclass
@@ -180,6 +180,42 @@ class PathTracer:
         """INTEGRATOR_FUSED or INTEGRATOR_WAVEFRONT (see include/stargaze_amd.h)."""
         self._check(self._lib.sg_set_integrator(self._ctx, int(integrator), int(chunk_paths)), "set_integrator")
 
+    def restart(self) -> None:
+        """Start a fresh accumulation with the next pass (new draws)."""
+        self._check(self._lib.sg_restart(self._ctx), "restart")
+
+    def set_ring_profile(self, body: int, tau: np.ndarray | None) -> None:
+        """Tabulated ring optical depth (RING_PROFILE_BINS bins) for ``body``;
+        ``None`` restores the banded profile."""
+        if tau is None:
+            self._check(self._lib.sg_set_ring_profile(self._ctx, int(body), None, 0), "set_ring_profile")
+            return
+        t = np.ascontiguousarray(tau, np.float32).reshape(-1)
+        self._check(self._lib.sg_set_ring_profile(self._ctx, int(body), t.ctypes.data, t.size), "set_ring_profile")
+
+    def adjoint(self, frame: Frame, d_image: np.ndarray, want_mean: bool = False):
+        """Gradient (GRAD_COUNT floats) of sum(d_image * pass_mean.rgb) for the
+        last encoded pass of ``frame``; with ``want_mean`` also the pass mean
+        (H, W, 4) the adjoint recomputed."""
+        w, h = self.size()
+        d = np.ascontiguousarray(d_image, np.float32)
+        if d.shape != (h, w, 3):
+            raise ValueError(f"d_image must be ({h}, {w}, 3), got {d.shape}")
+        grad = np.zeros(GRAD_COUNT, np.float32)
+        mean = np.zeros((h, w, 4), np.float32) if want_mean else None
+        f, keep = frame.native()
+        self._check(
+            self._lib.sg_adjoint(self._ctx, ctypes.byref(f), d.ctypes.data, w * h, grad.ctypes.data, GRAD_COUNT,
+                                 mean.ctypes.data if want_mean else None),
+            "adjoint",
+        )
+        return (grad, mean) if want_mean else grad
+
+    def adjoint_time(self) -> tuple[float, int]:
+        ms, n = ctypes.c_double(), ctypes.c_uint64()
+        self._check(self._lib.sg_adjoint_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "adjoint_time")
+        return float(ms.value), int(n.value)
+
     def probe(self, frame: Frame, kind: int, rays: np.ndarray) -> np.ndarray:
         """Run production device helpers on ``rays`` (PROBE_RAY records)."""
         r = np.ascontiguousarray(rays, PROBE_RAY).reshape(-1)
@@ -255,12 +291,14 @@ class Batch:
         arr, keep = self._frames(frames)
         self._check(self._lib.sg_batch_encode(self._b, arr), "sg_batch_encode")
 
-    def encode_views(self, frames: list[Frame]) -> np.ndarray:
-        """The per-view form: one ``encode`` per view; returns each view's
-        completion time (ms) after the first launch was queued."""
+    def encode_views(self, frames: list[Frame], serial: bool = False) -> np.ndarray:
+        """The per-view form: one ``encode`` per view, overlapping on the
+        device or (``serial``) one after another as on one queue; returns
+        each view's completion time (ms) after the first launch was queued."""
         arr, keep = self._frames(frames)
         ms = np.zeros(len(self.views), np.float64)
-        self._check(self._lib.sg_batch_encode_views(self._b, arr, ms.ctypes.data), "sg_batch_encode_views")
+        self._check(self._lib.sg_batch_encode_views(self._b, arr, ms.ctypes.data, int(serial)),
+                    "sg_batch_encode_views")
         return ms
 
     def synchronize(self) -> None:

@@ -339,13 +339,15 @@ def halo_parity(width=96, height=54):
 # -- batched views -------------------------------------------------------------
 def orbit_air(width: int = 320, height: int = 240, seed: int = 0x0A11CE) -> Frame:
     """A 6.4e6 m planet with Earth-like air seen from 4e6 m above its
-    surface, limb in view, one star; no ground scene."""
+    surface, the limb across the view, one star; no ground scene."""
     rng = Pcg(seed)
     f = frame()
     camera(f.globals, 55.0, width, height)
     r = metres(6.4e6)
-    up = np.array([0.0, 1.0, 0.0])
-    centre = -up * (r + metres(4.0e6)) + np.array([0.0, 0.0, -metres(rng.uniform(2e6, 5e6))])
+    # Planet centre 35-50 deg below the view axis: from 4e6 m up its limb
+    # (38 deg from the centre) crosses the view.
+    below = math.radians(rng.uniform(35.0, 50.0))
+    centre = np.array([0.0, -math.sin(below), -math.cos(below)]) * (r + metres(4.0e6))
     planet = place(body([0, 0, 0], r, [0.28, 0.3, 0.33], False), centre)
     sun = star(direction(rng.uniform(-90.0, 90.0), rng.uniform(5.0, 40.0)) * metres(1.5e11), metres(7.0e8))
     f.bodies = bodies(sun, planet)
@@ -401,3 +403,103 @@ def batch_views(count: int = 96, seed: int = 0xBA7C4, min_pixels: int = 128 * 12
         views.append(dict(kind=kind, frame=f, terrain=terrain, width=width, height=height, spp=samples,
                           bounces=bounces))
     return views
+
+
+# -- differentiable-rendering scenes ---------------------------------------------
+def moons_over_plain(width: int = 48, height: int = 32) -> Frame:
+    """Star, a planet under the camera (2 m up) and three close moons high
+    in a 60 deg view, the third with procedural mottle; albedos in (0, 1)."""
+    f = frame()
+    camera(f.globals, 60.0, width, height)
+    r_planet = metres(6.4e6)
+    up = np.array([0.0, 1.0, 0.0])
+    planet_c = -up * (r_planet + metres(2.0))
+    planet = place(body([0, 0, 0], r_planet, [0.31, 0.27, 0.22], False), planet_c)
+    moons = []
+    specs = [((-14.0, 14.0), 2.4e6, 2.0e7, [0.55, 0.5, 0.45]), ((6.0, 20.0), 1.6e6, 1.4e7, [0.2, 0.35, 0.6]),
+             ((18.0, 9.0), 2.0e6, 1.8e7, [0.7, 0.62, 0.4])]
+    for k, ((az, el), radius, dist, color) in enumerate(specs):
+        m = place(body([0, 0, 0], metres(radius), color, False), direction(az, el) * metres(dist))
+        if k == 2:
+            m["center_low"][3] = 0.8  # procedural mottle strength
+            m["axis_x"] = [1, 0, 0, 0.0]
+            m["axis_y"] = [0, 1, 0, 3.0]
+        moons.append(m)
+    sun = star(direction(-60.0, 35.0) * metres(1.5e11), metres(7.0e8))
+    f.bodies = bodies(sun, planet, *moons)
+    return f
+
+
+def ring_shadow(width: int = 48, height: int = 32, tau: float = 0.6, outer: float = 2.3) -> Frame:
+    """A ringed giant seen from 1.5 radii above 60 deg latitude, the view
+    filled by its northern hemisphere; the star lies 30 deg below the ring
+    plane, on the camera's side, so shadow rays from the lit surface meet
+    the ring (1.1 to ``outer`` radii) near two radii. Camera rays never
+    reach the ring plane."""
+    f = frame()
+    camera(f.globals, 40.0, width, height)
+    r = metres(6.0e7)
+    lat = math.radians(60.0)
+    # Camera frame: forward -z; the planet centre straight ahead.
+    centre = np.array([0.0, 0.0, -1.5 * r])
+    pole = np.array([0.0, math.cos(lat), math.sin(lat)])  # n . (camera - centre) / |..| = sin(lat)
+    giant = body([0, 0, 0], r, [0.5, 0.45, 0.4], False)
+    giant["ring_plane"] = [*pole, 1.1]
+    giant["ring_params"] = [outer, tau, 0.0, 0.0]
+    giant["ring_color"] = [0.7, 0.65, 0.6, 0.0]
+    x_axis = np.array([1.0, 0.0, 0.0])
+    giant["axis_x"] = [*x_axis, 0.0]
+    giant["axis_y"] = [*np.cross(pole, x_axis), 0.0]
+    place(giant, centre)
+    # In-plane direction towards the camera's sub-point, then the star:
+    # 30 deg below the ring plane, on the camera's side.
+    towards = np.array([0.0, 0.0, 1.0]) - pole * math.sin(lat)
+    towards /= np.linalg.norm(towards)
+    light = towards * math.cos(math.radians(30.0)) - pole * math.sin(math.radians(30.0))
+    sun = star(light * metres(1.5e11), metres(7.0e8))
+    f.bodies = bodies(sun, giant)
+    return f
+
+
+def ring_window(width: int = 32, height: int = 32, tau: float = 0.5) -> Frame:
+    """Looking straight through a black (colour 0) ring at a star that fills
+    the 1 deg view: every camera ray crosses the ring annulus near normal
+    incidence, so a pixel's expected radiance is L exp(-tau / mu)."""
+    f = frame()
+    camera(f.globals, 1.0, width, height)
+    r = metres(6.0e7)
+    pole = np.array([0.0, 0.0, 1.0])
+    giant = body([0, 0, 0], r, [0.5, 0.5, 0.5], False)
+    giant["ring_plane"] = [*pole, 1.1]
+    giant["ring_params"] = [2.3, tau, 0.0, 0.0]
+    giant["ring_color"] = [0.0, 0.0, 0.0, 0.0]
+    # Ring centre off to the side so the line of sight crosses at 1.7 radii.
+    place(giant, np.array([1.7 * r, 0.0, -5.0 * r]))
+    sun = star(np.array([0.0, 0.0, -1.0]) * metres(1.5e11), metres(7.0e9))
+    f.bodies = bodies(sun, giant)
+    return f
+
+
+def ringed_disc(width: int = 64, height: int = 48, tau: float = 0.6) -> Frame:
+    """A ringed giant filling half of a 40 deg view from four radii, ring
+    plane tilted 20 deg from edge-on, lit from above and behind the camera:
+    the lit ring crosses the disc and its shadow falls on the disc below it."""
+    f = frame()
+    camera(f.globals, 40.0, width, height)
+    r = metres(6.0e7)
+    centre = np.array([0.0, 0.0, -4.0 * r])
+    tilt = math.radians(20.0)
+    pole = np.array([0.0, math.cos(tilt), math.sin(tilt)])
+    giant = body([0, 0, 0], r, [0.5, 0.45, 0.4], False)
+    giant["ring_plane"] = [*pole, 1.2]
+    giant["ring_params"] = [2.2, tau, 0.0, 0.0]
+    giant["ring_color"] = [0.75, 0.7, 0.62, 0.0]
+    x_axis = np.array([1.0, 0.0, 0.0])
+    giant["axis_x"] = [*x_axis, 0.0]
+    giant["axis_y"] = [*np.cross(pole, x_axis), 0.0]
+    place(giant, centre)
+    light = np.array([0.25, 0.55, 1.0])
+    light /= np.linalg.norm(light)
+    sun = star(light * metres(1.5e11), metres(7.0e8))
+    f.bodies = bodies(sun, giant)
+    return f

@@ -1,0 +1,156 @@
+"""Adjoint checks shared by the host-build and GPU tests.
+
+A renderer gives the pass mean of a fresh history (``render``) and the
+adjoint of that pass (``adjoint``); the checks compare the adjoint with
+central finite differences of ``render`` with the same draws, which is the
+derivative of the same Monte Carlo estimate wherever no discrete choice
+depends on the parameter. Scenes are picked so that holds: no Russian
+roulette (at most 3 vertices), no reflector selection (no air with other
+lit bodies), camera rays that never meet the ring when its profile is
+perturbed. The ring pass-through choice, which does depend on tau, is
+checked against its closed form instead.
+"""
+import math
+
+import numpy as np
+
+from stargaze_amd import scenes
+from stargaze_amd.layouts import GRAD_AIR, GRAD_COUNT, GRAD_RING, RING_PROFILE_BINS, grad_albedo
+
+H_ALBEDO = 1e-3
+H_TAU = 1e-3
+H_AIR_REL = 1e-3
+TOL = 1e-2
+
+
+class Params:
+    """Mutable copy of the differentiable parameters of a frame."""
+
+    def __init__(self, frame, ring_body=None, tau=None):
+        self.frame = scenes.Frame(frame.globals.copy(), frame.bodies.copy(), frame.ground.copy())
+        self.ring_body = ring_body
+        self.tau = None if tau is None else np.array(tau, np.float32)
+
+    def copy(self):
+        return Params(self.frame, self.ring_body, self.tau)
+
+    def get(self, p):
+        kind, i = p[0], p[1]
+        if kind == "albedo":
+            return float(self.frame.bodies["color"][i][p[2]])
+        if kind == "ring":
+            return float(self.tau[i])
+        return float(self.frame.globals["atmo_rayleigh"][i] if i < 4 else self.frame.globals["atmo_params"][0])
+
+    def set(self, p, v):
+        kind, i = p[0], p[1]
+        if kind == "albedo":
+            c = self.frame.bodies["color"][i].copy()
+            c[p[2]] = v
+            self.frame.bodies["color"][i] = c
+        elif kind == "ring":
+            self.tau[i] = v
+        elif i < 4:
+            a = self.frame.globals["atmo_rayleigh"].copy()
+            a[i] = v
+            self.frame.globals["atmo_rayleigh"] = a
+        else:
+            a = self.frame.globals["atmo_params"].copy()
+            a[0] = v
+            self.frame.globals["atmo_params"] = a
+
+
+def slot(p):
+    if p[0] == "albedo":
+        return grad_albedo(p[1], p[2])
+    if p[0] == "ring":
+        return GRAD_RING + p[1]
+    return GRAD_AIR + p[1]
+
+
+def step(params, p):
+    if p[0] == "albedo":
+        return H_ALBEDO
+    if p[0] == "ring":
+        return H_TAU
+    return H_AIR_REL * abs(params.get(p)) if p[1] < 4 else H_ALBEDO
+
+
+def loss_weights(width, height, seed=3):
+    return np.random.default_rng(seed).uniform(0.5, 1.5, (height, width, 3)).astype(np.float32)
+
+
+def finite_differences(renderer, params, plist, d):
+    out = []
+    for p in plist:
+        h = step(params, p)
+        lo, hi = params.copy(), params.copy()
+        v = params.get(p)
+        # The perturbed value as stored (f32) sets the actual step.
+        hi.set(p, v + h)
+        lo.set(p, v - h)
+        span = hi.get(p) - lo.get(p)
+        out.append((float(np.sum(d * renderer.render(hi))) - float(np.sum(d * renderer.render(lo)))) / span)
+    return np.array(out)
+
+
+def compare(renderer, params, plist, label, seed=3):
+    """Adjoint against central differences on every parameter in plist."""
+    w, h = renderer.size
+    d = loss_weights(w, h, seed)
+    grad = renderer.adjoint(params, d)
+    assert grad.shape == (GRAD_COUNT,)
+    adj = np.array([grad[slot(p)] for p in plist], np.float64)
+    fd = finite_differences(renderer, params, plist, d)
+    assert np.all(np.abs(fd) > 0), "every checked parameter must move the loss"
+    rel = np.abs(adj - fd) / np.abs(fd)
+    for p, a, f, r in zip(plist, adj, fd, rel):
+        print(f"{label} {p}: adjoint {a:.6e} fd {f:.6e} rel {r:.2e}")
+    assert rel.max() <= TOL, f"{label}: worst relative difference {rel.max():.3e}"
+    return adj, fd
+
+
+# -- the cases ---------------------------------------------------------------
+def albedo_case(width=48, height=32):
+    frame = scenes.moons_over_plain(width, height)
+    plist = [("albedo", b, c) for b in (1, 2, 3, 4) for c in range(3)]
+    return Params(frame), plist, dict(spp=4, bounces=3)
+
+
+def air_case(width=40, height=30):
+    frame = scenes.orbit_air(width, height, seed=0x0A11CE)
+    plist = [("air", k) for k in range(5)]
+    return Params(frame), plist, dict(spp=4, bounces=3)
+
+
+def ring_case(width=48, height=32, bins=16):
+    frame = scenes.ring_shadow(width, height)
+    tau = np.full(RING_PROFILE_BINS, 0.6, np.float32)
+    tau[::7] = 0.3
+    params = Params(frame, ring_body=1, tau=tau)
+    return params, bins, dict(spp=4, bounces=1)
+
+
+def ring_bins_with_signal(grad, count):
+    """The `count` profile bins with the largest gradient (the shadowed ones)."""
+    g = np.abs(grad[GRAD_RING:GRAD_RING + RING_PROFILE_BINS])
+    order = np.argsort(-g)
+    chosen = [int(b) for b in order[:count] if g[b] > 0]
+    assert len(chosen) == count, f"only {len(chosen)} ring bins carry a gradient"
+    return [("ring", b) for b in sorted(chosen)]
+
+
+def window_case(width=32, height=32, tau=0.5):
+    frame = scenes.ring_window(width, height, tau)
+    params = Params(frame, ring_body=1, tau=np.full(RING_PROFILE_BINS, tau, np.float32))
+    return params, dict(spp=16, bounces=1)
+
+
+def window_expectation(params, d):
+    """d/d(tau, all bins) of E[sum d * mean] for the ring window: every ray
+    meets the black ring at mu ~ 1 before the star, so E[pixel] = L exp(-tau/mu)."""
+    radiance = params.frame.bodies["color"][0].astype(np.float64)
+    tau = float(params.tau[0])
+    fov = math.radians(1.0)
+    mu = math.cos(fov / 2) ** 0.5  # mean over the view is within 4e-5 of 1
+    return float(np.sum(d * radiance)) * (-1.0 / mu) * math.exp(-tau / mu)

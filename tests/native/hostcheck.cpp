@@ -61,6 +61,49 @@ int hc_query(const sg_settings *settings, const sg_sphere *bodies, const sg_prim
 }
 }
 
+extern "C" {
+// One pass of counts.z samples per pixel from `settings` (no landscape, no
+// catalogue or sky map): the per-pixel mean radiance (rgb) into `mean`, and,
+// when `gradient` is not null, the product adjoint for d(loss)/d(mean) =
+// d_image added into `gradient` (GRAD_COUNT floats).
+int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *ring_profile, uint32_t ring_body,
+            const float *d_image, double *mean, float *gradient) {
+    Scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.s = *settings;
+    sc.bodies = reinterpret_cast<const Body *>(bodies);
+    sc.t = terrain_layout(settings->globals.ground_counts[2]);
+    sc.ring_profile = ring_profile;
+    sc.ring_profile_body = ring_profile ? ring_body : SG_MISS;
+    const uint32_t width = (uint32_t)settings->globals.viewport[0];
+    const uint32_t height = (uint32_t)settings->globals.viewport[1];
+    const uint32_t count = settings->counts[2];
+    const bool has_air = settings->globals.atmo_params[3] >= 0.5f;
+    for (uint32_t pixel = 0; pixel < width * height; ++pixel) {
+        const uint32_t x = pixel % width, y = pixel / width;
+        double sum[3] = {0.0, 0.0, 0.0};
+        for (uint32_t s = 0; s < count; ++s) {
+            const uint32_t accumulated = has_air ? settings->counts[1] + s : settings->counts[1];
+            uint32_t rng = path_seed(pixel, settings->sampling[0] + s);
+            f3 direction = camera_direction(sc, x, y, rng);
+            PathResult r;
+            if (gradient) {
+                const float *d = d_image + 3u * pixel;
+                r = adjoint_path(sc, direction, rng, accumulated, f3{d[0], d[1], d[2]} * (1.0f / (float)count),
+                                 gradient);
+            } else {
+                r = trace_path(sc, direction, rng, accumulated);
+            }
+            sum[0] += r.value.x;
+            sum[1] += r.value.y;
+            sum[2] += r.value.z;
+        }
+        for (int c = 0; c < 3; ++c) mean[3u * pixel + c] = sum[c] / count;
+    }
+    return 0;
+}
+}
+
 #include "../../stargaze_amd/csrc/sg_bvh.h"
 
 extern "C" {

@@ -1,0 +1,91 @@
+"""The product adjoint, built for the host (the same integrator source the
+GPU library compiles), against central finite differences of the host
+build's own renderer with the same draws."""
+import ctypes
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import adjoint_cases as ac
+import oracle
+from stargaze_amd.layouts import GRAD_COUNT, GRAD_RING, RING_PROFILE_BINS
+from stargaze_amd.pathtracer import Options
+
+sys.path.insert(0, str(Path(__file__).resolve().parent / "native"))
+import build as hostbuild  # noqa: E402
+
+
+class HostRenderer:
+    """hc_pass of tests/native/hostcheck.cpp: one pass from a fresh history."""
+
+    def __init__(self, width, height, spp, bounces):
+        self.size = (width, height)
+        self.spp, self.bounces = spp, bounces
+        lib = ctypes.CDLL(str(hostbuild.build()))
+        P = ctypes.c_void_p
+        lib.hc_pass.argtypes = [P, P, P, ctypes.c_uint32, P, P, P]
+        lib.hc_pass.restype = ctypes.c_int
+        self.lib = lib
+
+    def _run(self, params, d=None):
+        w, h = self.size
+        tracer = oracle.OracleTracer((w, h), Options(samples_per_frame=self.spp, max_bounces=self.bounces,
+                                                    milky_way=0.0))
+        st = np.ascontiguousarray(tracer.settings(params.frame, self.spp))
+        bodies = np.ascontiguousarray(params.frame.bodies)
+        table = None if params.tau is None else np.ascontiguousarray(params.tau, np.float32)
+        mean = np.zeros((h, w, 3), np.float64)
+        grad = np.zeros(GRAD_COUNT, np.float32) if d is not None else None
+        dd = None if d is None else np.ascontiguousarray(d, np.float32)
+        self.lib.hc_pass(st.ctypes.data, bodies.ctypes.data, None if table is None else table.ctypes.data,
+                         params.ring_body or 0, None if dd is None else dd.ctypes.data, mean.ctypes.data,
+                         None if grad is None else grad.ctypes.data)
+        return mean, grad
+
+    def render(self, params):
+        return self._run(params)[0]
+
+    def adjoint(self, params, d):
+        return self._run(params, d)[1]
+
+
+def test_adjoint_albedo_host():
+    params, plist, q = ac.albedo_case(24, 16)
+    r = HostRenderer(24, 16, q["spp"], q["bounces"])
+    ac.compare(r, params, plist, "host albedo")
+
+
+def test_adjoint_air_host():
+    params, plist, q = ac.air_case(20, 15)
+    r = HostRenderer(20, 15, q["spp"], q["bounces"])
+    ac.compare(r, params, plist, "host air")
+
+
+def test_adjoint_ring_profile_host():
+    params, bins, q = ac.ring_case(24, 16, bins=8)
+    r = HostRenderer(24, 16, q["spp"], q["bounces"])
+    d = ac.loss_weights(24, 16)
+    plist = ac.ring_bins_with_signal(r.adjoint(params, d), bins)
+    ac.compare(r, params, plist, "host ring")
+
+
+def test_adjoint_ring_pass_through_host():
+    """Score of the pass-through choice against its closed form."""
+    params, q = ac.window_case(16, 16)
+    r = HostRenderer(16, 16, 64, q["bounces"])
+    d = ac.loss_weights(16, 16)
+    grad = r.adjoint(params, d)
+    total = float(np.sum(grad[GRAD_RING:GRAD_RING + RING_PROFILE_BINS].astype(np.float64)))
+    expected = ac.window_expectation(params, d)
+    # 16 x 16 x 64 draws of a Bernoulli(exp(-0.5)) pass-through.
+    n = 16 * 16 * 64
+    t = np.exp(-0.5)
+    sigma = abs(expected) * np.sqrt((1 - t) / (t * n))
+    print(f"pass-through: adjoint {total:.6e} closed form {expected:.6e} sigma {sigma:.2e}")
+    assert abs(total - expected) <= 5 * sigma + 1e-4 * abs(expected)
+    # The mean image itself is L exp(-tau) in expectation.
+    mean = r.render(params)
+    lum = params.frame.bodies["color"][0][0]
+    assert abs(mean[..., 0].mean() / (lum * t) - 1) <= 5 * np.sqrt((1 - t) / (t * n))

@@ -1,0 +1,193 @@
+"""HIP adjoint on MI355X: against central finite differences of the plain
+HIP renderer (same draws), against the closed form of the ring
+pass-through, through the torch.autograd harness, and in 200 Adam steps of
+an inverse problem."""
+import numpy as np
+import pytest
+
+import adjoint_cases as ac
+from stargaze_amd import scenes
+from stargaze_amd.layouts import GRAD_COUNT, GRAD_RING, RING_PROFILE_BINS, grad_albedo
+from stargaze_amd.pathtracer import INTEGRATOR_FUSED, Options, PathTracer
+
+pytestmark = pytest.mark.gpu
+
+
+class GpuRenderer:
+    """A fresh tracer per call, so every render draws the same samples."""
+
+    def __init__(self, width, height, spp, bounces):
+        self.size = (width, height)
+        self.spp, self.bounces = spp, bounces
+
+    def tracer(self, params):
+        t = PathTracer(self.size, options=Options(samples_per_frame=self.spp, max_bounces=self.bounces,
+                                                  milky_way=0.0))
+        if params.tau is not None:
+            t.set_ring_profile(params.ring_body, params.tau)
+        return t
+
+    def render(self, params):
+        t = self.tracer(params)
+        t.encode(params.frame)
+        img = t.read_accumulation()[..., :3].astype(np.float64)
+        t.close()
+        return img
+
+    def adjoint(self, params, d):
+        t = self.tracer(params)
+        t.encode(params.frame)
+        g = t.adjoint(params.frame, d)
+        t.close()
+        return g
+
+
+def test_adjoint_matches_fd_on_33_parameters():
+    """12 albedos, 5 air coefficients, 16 ring-profile bins: adjoint against
+    central differences (h = 1e-3) of the plain renderer, within 1e-2."""
+    print("adjoint vs fd: albedo", flush=True)
+    params, plist, q = ac.albedo_case(96, 64)
+    worst = []
+    adj, fd = ac.compare(GpuRenderer(96, 64, q["spp"], q["bounces"]), params, plist, "gpu albedo")
+    worst.append(np.max(np.abs(adj - fd) / np.abs(fd)))
+    print("adjoint vs fd: air", flush=True)
+    params, plist, q = ac.air_case(80, 60)
+    adj, fd = ac.compare(GpuRenderer(80, 60, q["spp"], q["bounces"]), params, plist, "gpu air")
+    worst.append(np.max(np.abs(adj - fd) / np.abs(fd)))
+    print("adjoint vs fd: ring", flush=True)
+    params, bins, q = ac.ring_case(96, 64, bins=16)
+    r = GpuRenderer(96, 64, q["spp"], q["bounces"])
+    plist = ac.ring_bins_with_signal(r.adjoint(params, ac.loss_weights(96, 64)), bins)
+    adj, fd = ac.compare(r, params, plist, "gpu ring")
+    worst.append(np.max(np.abs(adj - fd) / np.abs(fd)))
+    print(f"worst relative difference per group: {worst}")
+
+
+@pytest.mark.parametrize("case", ["albedo", "air", "ring"])
+def test_adjoint_pass_mean_matches_forward(case):
+    """The adjoint's own primal is the plain renderer's image (fused kernel,
+    same draws, same blend order)."""
+    print(f"adjoint primal {case}", flush=True)
+    params, _, q = {"albedo": ac.albedo_case, "air": ac.air_case, "ring": ac.ring_case}[case](64, 48)
+    t = PathTracer((64, 48), options=Options(samples_per_frame=q["spp"], max_bounces=q["bounces"], milky_way=0.0))
+    if params.tau is not None:
+        t.set_ring_profile(params.ring_body, params.tau)
+    t.set_integrator(INTEGRATOR_FUSED)
+    t.encode(params.frame)
+    image = t.read_accumulation()
+    grad, mean = t.adjoint(params.frame, ac.loss_weights(64, 48), want_mean=True)
+    assert np.all(np.isfinite(grad))
+    scale = float(np.abs(image[..., :3]).max())
+    diff = float(np.abs(mean[..., :3] - image[..., :3]).max()) / scale
+    print(f"{case}: pass mean vs image {diff:.3e}")
+    assert diff <= 1e-5
+    t.close()
+
+
+def test_adjoint_pass_through_closed_form():
+    params, q = ac.window_case(64, 64)
+    r = GpuRenderer(64, 64, 64, q["bounces"])
+    d = ac.loss_weights(64, 64)
+    grad = r.adjoint(params, d)
+    total = float(np.sum(grad[GRAD_RING:GRAD_RING + RING_PROFILE_BINS].astype(np.float64)))
+    expected = ac.window_expectation(params, d)
+    n = 64 * 64 * 64
+    t = np.exp(-0.5)
+    sigma = abs(expected) * np.sqrt((1 - t) / (t * n))
+    print(f"pass-through: adjoint {total:.6e} closed form {expected:.6e} sigma {sigma:.2e}")
+    assert abs(total - expected) <= 5 * sigma + 1e-4 * abs(expected)
+
+
+def test_adjoint_errors():
+    params, _, q = ac.albedo_case(16, 16)
+    t = PathTracer((16, 16), options=Options(samples_per_frame=2, max_bounces=3, milky_way=0.0))
+    d = np.ones((16, 16, 3), np.float32)
+    with pytest.raises(Exception):
+        t.adjoint(params.frame, d)  # nothing rendered yet
+    t.encode(params.frame)
+    other = ac.Params(params.frame)
+    other.set(("albedo", 1, 0), 0.9)
+    with pytest.raises(Exception):
+        t.adjoint(other.frame, d)  # not the frame of the last pass
+    with pytest.raises(Exception):
+        t.adjoint(params.frame, np.ones((8, 8, 3), np.float32))
+    g = t.adjoint(params.frame, d)
+    assert g.shape == (GRAD_COUNT,)
+    frame, data, cells = scenes.halo_parity(16, 16)
+    t2 = PathTracer((16, 16), options=Options(samples_per_frame=1, max_bounces=2, milky_way=0.0))
+    t2.upload_terrain(data, cells)
+    t2.encode(frame)
+    with pytest.raises(Exception):
+        t2.adjoint(frame, d)  # landscapes are not differentiated
+    t.close()
+    t2.close()
+
+
+def test_torch_function_matches_adjoint():
+    import torch
+
+    from stargaze_amd import diff
+
+    params, _, q = ac.ring_case(48, 32)
+    scene = diff.DiffScene(params.frame, (48, 32), q["spp"], q["bounces"], albedo_bodies=[1], ring_body=1)
+    albedo, tau, air = scene.initial()
+    tau = torch.from_numpy(params.tau.copy())
+    for p in (albedo, tau, air):
+        p.requires_grad_(True)
+    w = torch.from_numpy(ac.loss_weights(48, 32))
+    image = diff.render(scene, albedo, tau, air)
+    (image * w).sum().backward()
+    # The same pass through the C ABI directly.
+    t = PathTracer((48, 32), options=Options(samples_per_frame=q["spp"], max_bounces=q["bounces"], milky_way=0.0))
+    t.set_ring_profile(1, params.tau)
+    t.encode(params.frame)
+    g = t.adjoint(params.frame, ac.loss_weights(48, 32))
+    assert np.allclose(albedo.grad.numpy()[0], g[grad_albedo(1, 0):grad_albedo(1, 0) + 3], rtol=1e-5, atol=0)
+    assert np.allclose(tau.grad.numpy(), g[GRAD_RING:GRAD_RING + RING_PROFILE_BINS], rtol=1e-5, atol=1e-12)
+    assert np.abs(tau.grad.numpy()).max() > 0
+    t.close()
+    scene.close()
+
+
+def test_adam_recovers_ring_profile_and_albedo():
+    """200 Adam steps from a flat profile and grey albedo to a target image
+    rendered with a banded 16-segment profile: the lit ring crosses the
+    disc and shadows part of it, so ring brightness, shadow depth and the
+    unshadowed disc pin the profile and the albedo apart."""
+    import torch
+
+    from stargaze_amd import diff
+
+    width, height = 64, 48
+    frame = scenes.ringed_disc(width, height)
+    segments = 16
+    spread = RING_PROFILE_BINS // segments
+    true_coarse = torch.tensor([0.2 + 0.6 * (k % 3 == 0) + 0.05 * k for k in range(segments)], dtype=torch.float32)
+    true_albedo = torch.tensor([[0.5, 0.45, 0.4]])
+    target_scene = diff.DiffScene(frame, (width, height), 64, 1, albedo_bodies=[1], ring_body=1)
+    _, _, air = target_scene.initial()
+    with torch.no_grad():
+        target = diff.render(target_scene, true_albedo, true_coarse.repeat_interleave(spread), air)
+    target_scene.close()
+    scene = diff.DiffScene(frame, (width, height), 16, 1, albedo_bodies=[1], ring_body=1)
+    coarse = torch.full((segments,), 0.5, requires_grad=True)
+    albedo = torch.full((1, 3), 0.3, requires_grad=True)
+    opt = torch.optim.Adam([coarse, albedo], lr=0.02)
+    losses = []
+    for it in range(200):
+        opt.zero_grad()
+        image = diff.render(scene, albedo, coarse.clamp(min=0.0).repeat_interleave(spread), air)
+        loss = ((image - target) ** 2).mean()
+        loss.backward()
+        opt.step()
+        losses.append(float(loss))
+        if it % 50 == 0:
+            print(f"adam {it}: loss {losses[-1]:.4e} albedo {albedo.detach().numpy().round(3)}", flush=True)
+    err0 = float((0.5 - true_coarse).abs().mean())
+    err1 = float((coarse.detach() - true_coarse).abs().mean())
+    print(f"loss {losses[0]:.4e} -> {np.mean(losses[-10:]):.4e}; |tau error| {err0:.3f} -> {err1:.3f}; "
+          f"albedo {albedo.detach().numpy().round(4)}")
+    assert np.mean(losses[-10:]) < 0.05 * losses[0]
+    assert np.allclose(albedo.detach().numpy(), true_albedo.numpy(), rtol=0.03)
+    assert err1 < 0.4 * err0
+    scene.close()

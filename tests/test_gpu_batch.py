@@ -102,8 +102,11 @@ def test_batch_view_times_and_orders():
         ms, span = batch.view_times()
         assert span > 0.0
         assert np.all(ms > 0.0) and np.all(ms <= span * 1.0001), (ms, span)
-    per_view = batch.encode_views(frames)
-    assert np.all(per_view > 0.0)
+    for serial in (False, True):
+        per_view = batch.encode_views(frames, serial=serial)
+        assert np.all(per_view > 0.0)
+    # One queue: completion times follow submission order.
+    assert np.all(np.diff(per_view) >= -1e-3), per_view
     close(batch, mine)
 
 

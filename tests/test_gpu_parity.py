@@ -258,13 +258,13 @@ def test_landscape_probe_matches_oracle():
 
 
 @pytest.mark.parametrize("scene", ["airless_triple", "ringed_giant", "halo"])
-def test_wavefront_and_fused_integrators_are_bit_identical(scene):
+def test_wavefront_and_fused_integrators_agree(scene):
     """Both integrators run the same step functions in the same draw order.
 
-    Airless scenes give bit-identical images. With air, the camera-air step
-    is compiled in a different kernel in each integrator and the compiler
-    fuses multiply-adds differently, so in-scattered sky radiance may differ
-    in the last bits: those images must agree to 1e-5 relative.
+    The steps are compiled into different kernels in each integrator and the
+    compiler fuses multiply-adds differently, so radiance may differ in the
+    last bits: the images must agree to 1e-5 relative. Chunking the
+    wavefront differently changes nothing: those images are bit-identical.
     """
     size = (48, 27)
     if scene == "halo":
@@ -280,9 +280,5 @@ def test_wavefront_and_fused_integrators_are_bit_identical(scene):
         t.encode(frame)
         t.encode(frame)
         images.append(t.read_accumulation())
-    if scene == "halo":
-        for other in images[1:]:
-            np.testing.assert_allclose(other, images[0], rtol=1e-5, atol=1e-9)
-    else:
-        assert np.array_equal(images[0], images[1])
-        assert np.array_equal(images[0], images[2])
+    np.testing.assert_allclose(images[1], images[0], rtol=1e-5, atol=1e-9)
+    assert np.array_equal(images[1], images[2])

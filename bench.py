@@ -59,17 +59,23 @@ PEAK_HBM_GBS = 8000.0
 
 def _halo(width, height):
     frame, data, cells = scenes.halo_observatory(width, height)
-    return frame, (data, cells)
+    return frame, (data, cells), None
+
+
+def _giant(width, height):
+    """The ringed giant with the 512-bin C/B/A optical-depth table."""
+    frame = scenes.ringed_giant(width, height)
+    return frame, None, (1, scenes.ring_profile_cba(frame, 1))
 
 
 def _plain(builder):
-    return lambda width, height: (builder(width, height), None)
+    return lambda width, height: (builder(width, height), None, None)
 
 
 WORKLOADS = {
-    # name: (builder -> (frame, terrain), width, height, spp per pass, max bounces)
+    # name: (builder -> (frame, terrain, ring profile), width, height, spp per pass, max bounces)
     "halo_4k": (_halo, 3840, 2160, 32, 4),
-    "ringed_giant_1080p": (_plain(scenes.ringed_giant), 1920, 1080, 64, 6),
+    "ringed_giant_1080p": (_giant, 1920, 1080, 64, 6),
     "airless_triple_256": (_plain(scenes.airless_triple), 256, 256, 16, 4),
 }
 
@@ -78,22 +84,24 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def oracle_tracer(width, height, spp, bounces, terrain):
+def oracle_tracer(width, height, spp, bounces, terrain, profile=None):
     import oracle
 
     t = oracle.OracleTracer((width, height), Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
     if terrain is not None:
         t.upload_terrain(*terrain)
+    if profile is not None:
+        t.set_ring_profile(*profile)
     return t
 
 
-def work_model(frame, terrain, width, height, spp, bounces, rows=16):
+def work_model(frame, terrain, profile, width, height, spp, bounces, rows=16):
     """Event counts per path sample of a steady-state pass (not the first,
     which takes extra light strata near limbs) from the oracle on bands of
     rows at the image's quarter, half and three-quarter heights."""
     import oracle
 
-    t = oracle_tracer(width, height, spp, bounces, terrain)
+    t = oracle_tracer(width, height, spp, bounces, terrain, profile)
     t.encode(frame, rows=(0, 1))  # first pass: fills the sky and terrain caches
     ev = None
     bands = [height * (2 * k + 1) // 8 for k in range(4)]
@@ -109,11 +117,11 @@ def work_model(frame, terrain, width, height, spp, bounces, rows=16):
     return per, flops
 
 
-def cpu_baseline(frame, terrain, width, height, spp, bounces, target_s=12.0):
+def cpu_baseline(frame, terrain, profile, width, height, spp, bounces, target_s=12.0):
     """FP32 oracle on all host cores over a band of rows sized to ~target_s."""
     rows = 8
     while True:
-        t = oracle_tracer(width, height, spp, bounces, terrain)
+        t = oracle_tracer(width, height, spp, bounces, terrain, profile)
         y0 = max(0, height // 2 - rows // 2)
         t0 = time.perf_counter()
         t.encode(frame, rows=(y0, min(height, y0 + rows)))
@@ -137,6 +145,7 @@ BATCH_WORKLOADS = {
     # name: batch_views keyword arguments
     "mixed_96": dict(count=96, min_pixels=128 * 128, max_pixels=1920 * 1080),
     "small_96": dict(count=96, min_pixels=64 * 64, max_pixels=640 * 360),
+    "tiny_96": dict(count=96, min_pixels=48 * 48, max_pixels=256 * 256),
 }
 
 
@@ -221,9 +230,9 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
 
 
 ADJOINT_WORKLOADS = {
-    # name: (builder, width, height, spp, bounces, tabulated ring profile on body 1)
-    "ringed_giant_1080p": (scenes.ringed_giant, 1920, 1080, 64, 6, True),
-    "orbit_air_1080p": (scenes.orbit_air, 1920, 1080, 16, 4, False),
+    # name: (builder, width, height, spp, bounces, body with a tabulated ring, tabulated density)
+    "diff_system_512": (scenes.diff_system, 512, 512, 64, 4, 2, True),
+    "ringed_giant_1080p": (scenes.ringed_giant, 1920, 1080, 64, 6, 1, False),
 }
 
 
@@ -234,12 +243,36 @@ def adjoint_leg(steps: int) -> list:
     from stargaze_amd.pathtracer import INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT
 
     out = []
-    for name, (builder, width, height, spp, bounces, profiled) in ADJOINT_WORKLOADS.items():
+    for name, (builder, width, height, spp, bounces, ring_body, dense) in ADJOINT_WORKLOADS.items():
         frame = builder(width, height)
-        t = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
-        if profiled:
-            t.set_ring_profile(1, np.full(512, float(frame.bodies["ring_params"][1][1]), np.float32))
-        d = np.random.default_rng(1).uniform(0.5, 1.5, (height, width, 3)).astype(np.float32)
+        tau = scenes.ring_profile_cba(frame, ring_body)
+        density = scenes.exponential_density(frame) if dense else None
+
+        def tracer_for(f, tau, density):
+            t = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces,
+                                                            milky_way=0.0))
+            t.set_ring_profile(ring_body, tau)
+            if density is not None:
+                t.set_density_profile(density)
+            return t
+
+        # Loss: L2 distance to a target rendered with every parameter
+        # perturbed by up to 10% (PCG seed 2).
+        rng = scenes.Pcg(2)
+        target_frame = scenes.Frame(frame.globals.copy(), frame.bodies.copy(), frame.ground.copy())
+        for b in range(len(target_frame.bodies)):
+            if target_frame.bodies["emissive"][b] < 0.5:
+                target_frame.bodies["color"][b] = target_frame.bodies["color"][b] * np.float32(
+                    1.0 + rng.uniform(-0.1, 0.1))
+        t = tracer_for(target_frame, tau * np.float32(1.0 + rng.uniform(-0.1, 0.1)),
+                       None if density is None else density * np.float32(1.0 + rng.uniform(-0.1, 0.1)))
+        t.encode(target_frame)
+        target = t.read_accumulation()[..., :3]
+        t.close()
+        t = tracer_for(frame, tau, density)
+        t.encode(frame)
+        image = t.read_accumulation()[..., :3]
+        d = (2.0 * (image - target) / image.size).astype(np.float32)
         times = {}
         for integrator, label in ((INTEGRATOR_FUSED, "fused"), (INTEGRATOR_WAVEFRONT, "wavefront")):
             t.set_integrator(integrator)
@@ -260,7 +293,9 @@ def adjoint_leg(steps: int) -> list:
         assert np.all(np.isfinite(grad)), "non-finite gradient"
         t.close()
         best = min(times["fused"], times["wavefront"])
-        out.append({"workload": name, "spp": spp, "max_bounces": bounces,
+        params = 3 * int(np.sum(frame.bodies["emissive"] < 0.5)) + 512 + 5 + (16 if dense else 0)
+        out.append({"workload": name, "width": width, "height": height, "spp": spp, "max_bounces": bounces,
+                    "parameters": params, "loss": "L2 to a target with parameters perturbed up to 10%",
                     "forward_ms": {"fused": round(times["fused"], 3), "wavefront": round(times["wavefront"], 3)},
                     "adjoint_ms": round(times["adjoint"], 3),
                     "ratio_vs_fused": round(times["adjoint"] / times["fused"], 3),
@@ -276,6 +311,7 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="halo_4k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default integrator launch")
     ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
@@ -285,12 +321,18 @@ def main() -> int:
         return 2
     builder, width, height, spp, bounces = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    frame, terrain = builder(width, height)
+    frame, terrain, profile = builder(width, height)
     log(f"scene built in {time.perf_counter() - t0:.1f} s")
     tracer = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
     if terrain is not None:
         tracer.upload_terrain(*terrain)
+    if profile is not None:
+        tracer.set_ring_profile(*profile)
     log(f"workload {args.workload}: {width}x{height}, {spp} spp/step, {bounces} bounces")
+    launch = None
+    if not args.no_autotune:
+        launch, tuned_ms = tracer.autotune(frame, repeats=1)
+        log(f"autotune: {launch} {tuned_ms:.2f} ms")
     for i in range(args.warmup):
         tracer.encode(frame)
     tracer.synchronize()
@@ -311,7 +353,7 @@ def main() -> int:
         per_event, flops = {}, float("nan")
     else:
         log("work model")
-        per_event, flops = work_model(frame, terrain, width, height, spp, bounces)
+        per_event, flops = work_model(frame, terrain, profile, width, height, spp, bounces)
     kernel_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = flops * samples_per_step / kernel_s / 1e12
     traffic_algorithmic = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
@@ -335,6 +377,7 @@ def main() -> int:
             "spp_per_step": spp,
             "max_bounces": bounces,
         },
+        "launch": launch if launch is not None else tracer.launch,
         "roofline": {
             "bound": "valu",
             "achieved": round(achieved, 3),
@@ -358,7 +401,7 @@ def main() -> int:
         line["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(frame, terrain, width, height, spp, bounces)
+        line["cpu_baseline"] = cpu_baseline(frame, terrain, profile, width, height, spp, bounces)
     print(json.dumps(line), flush=True)
     return 0
 

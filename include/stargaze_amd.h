@@ -36,6 +36,8 @@ extern "C" {
 #define SG_MAX_TERRAIN_CELLS 8192u
 /* Radial bins of a tabulated ring optical-depth profile (sg_set_ring_profile). */
 #define SG_RING_PROFILE_BINS 512u
+/* Altitude bins of a tabulated air density profile (sg_set_density_profile). */
+#define SG_DENSITY_BINS 16u
 
 #define SG_OK 0
 #define SG_E_INVALID (-1)   /* bad argument, size or layout */
@@ -205,6 +207,22 @@ int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
 #define SG_INTEGRATOR_FUSED 1u
 #define SG_INTEGRATOR_WAVEFRONT 2u
 int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths);
+/* Compaction threshold of the wavefront integrator: once fewer than
+ * `tail_paths` paths of a chunk are alive, one launch runs each of them
+ * through the rest of its loop (same draws) instead of further
+ * extend / shade rounds. 0 (default) compacts to the end. */
+int sg_set_tail(sg_ctx *ctx, uint32_t tail_paths);
+typedef struct sg_launch {
+    uint32_t integrator;  /* SG_INTEGRATOR_* */
+    uint32_t chunk_paths; /* wavefront queue capacity (paths in flight) */
+    uint32_t tail_paths;  /* compaction threshold, see sg_set_tail */
+} sg_launch;
+int sg_get_launch(const sg_ctx *ctx, sg_launch *launch);
+/* Time trial passes of `frame` over queue capacities, compaction thresholds
+ * and the fused integrator, keep the fastest and return it. Trial passes
+ * render into the image; the accumulation restarts with the next pass, at
+ * the noise sequence it would have had without them. */
+int sg_autotune(sg_ctx *ctx, const sg_frame *frame, uint32_t repeats, sg_launch *best, double *best_ms);
 const char *sg_last_error(const sg_ctx *ctx);
 /* ------------------------------------------------------------------------
  * Differentiable rendering (no reference counterpart).
@@ -221,7 +239,8 @@ const char *sg_last_error(const sg_ctx *ctx);
  * where pass_mean is the mean radiance of that pass's samples (the image
  * itself after a pass from a fresh history). The gradient vector has
  * SG_GRAD_COUNT entries, laid out as below; parameters are the stored values
- * (sg_sphere.color, tau bins, atmo_rayleigh rgb and w, atmo_params x).
+ * (sg_sphere.color, tau bins, atmo_rayleigh rgb and w, atmo_params x,
+ * density bins; the density gradient is zero unless a profile is set).
  * Scenes with a landscape are rejected. Body albedos are differentiated for
  * surface style 0; gas-giant albedos get zero. `pass_mean` (optional,
  * W*H*4 floats) receives the pass mean the adjoint recomputed.
@@ -229,9 +248,15 @@ const char *sg_last_error(const sg_ctx *ctx);
 #define SG_GRAD_ALBEDO(body, channel) (3u * (body) + (channel))
 #define SG_GRAD_RING(bin) (3u * SG_BODY_CAPACITY + (bin))
 #define SG_GRAD_AIR (3u * SG_BODY_CAPACITY + SG_RING_PROFILE_BINS) /* +0..2 Rayleigh rgb, +3 Mie, +4 Mie g */
-#define SG_GRAD_COUNT (SG_GRAD_AIR + 5u)
+#define SG_GRAD_DENSITY(bin) (SG_GRAD_AIR + 5u + (bin))
+#define SG_GRAD_COUNT (SG_GRAD_AIR + 5u + SG_DENSITY_BINS)
 
 int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t bins);
+/* Relative air density per altitude slice (SG_DENSITY_BINS equal slices from
+ * the surface to atmo_params.z) replacing exp(-h / H) in every density
+ * sample (src/shaders/pathtrace.wgsl:970,1166,1230); NULL restores the
+ * exponential. Restarts the accumulation. */
+int sg_set_density_profile(sg_ctx *ctx, const float *density, uint32_t bins);
 /* Restart the accumulation with the next pass while keeping the noise
  * sequence running, as a changed frame does (src/pathtracer.rs:172-188):
  * every restarted pass draws fresh samples. */
@@ -291,7 +316,8 @@ int sg_batch_kernel_time(sg_batch *batch, double *milliseconds, uint64_t *launch
 int sg_batch_encode_views(sg_batch *batch, const sg_frame *frames, double *milliseconds, int32_t serial);
 /* Time trial passes over launch settings (workgroup size, grid size, blocks
  * per pull) and keep the fastest. Trial passes render into the views, whose
- * progressive history is reset afterwards. */
+ * progressive history restarts afterwards at the noise sequence each view
+ * had before. */
 int sg_batch_autotune(sg_batch *batch, const sg_frame *frames, uint32_t repeats, sg_batch_launch *best,
                       double *best_ms);
 const char *sg_batch_last_error(const sg_batch *batch);

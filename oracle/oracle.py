@@ -52,6 +52,8 @@ class OrcScene(ctypes.Structure):
         ("catalogue_count", ctypes.c_uint32),
         ("sky_cells", ctypes.c_void_p),
         ("milky_way", ctypes.c_void_p),
+        ("ring_profile", ctypes.c_void_p),
+        ("ring_profile_body", ctypes.c_uint32),
     ]
 
 
@@ -125,6 +127,7 @@ class OracleTracer:
         self.terrain_cells = 0
         self.terrain_radius = None
         self.sky = np.zeros((128, 4), np.float32)
+        self.ring_profile = None
         self.moments_on = False
         self._alloc()
 
@@ -139,6 +142,12 @@ class OracleTracer:
 
     def enable_moments(self, on=True):
         self.moments_on = on
+
+    def set_ring_profile(self, body: int, tau):
+        """``PathTracer.set_ring_profile``: 512 bins, or None; restarts."""
+        self.ring_profile = None if tau is None else (int(body), np.array(tau, np.float32).reshape(512))
+        self.key = b""
+        self.history_samples = 0
 
     def upload_terrain(self, data: np.ndarray, cells: int):
         lay = terrain_layout(cells)
@@ -230,6 +239,13 @@ class OracleTracer:
             mw = np.ascontiguousarray(self.milky_way, np.uint32)
             keep.append(mw)
             sc.milky_way = mw.ctypes.data
+        if self.ring_profile is not None:
+            body, tau = self.ring_profile
+            if body < len(np.asarray(frame.bodies).reshape(-1)):
+                t = np.ascontiguousarray(tau, np.float32)
+                keep.append(t)
+                sc.ring_profile = t.ctypes.data
+                sc.ring_profile_body = body
         return sc, keep
 
     def encode(self, frame, rows=None):

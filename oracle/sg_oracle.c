@@ -113,6 +113,8 @@ typedef struct {
     uint32_t ncat;
     const uint32_t *cells;
     const uint32_t *milky;
+    const float *ring_profile;
+    uint32_t ring_profile_body;
     /* per-dispatch counts.y (the sample index semantics of the host slices) */
     uint32_t accumulated;
 } Ctx;
@@ -960,6 +962,12 @@ static R ring_optics(const Ctx *c, uint32_t index, v3 point, v3 *color) {
         tau *= density;
         col = scl(mul(col, tint), 0.94 + 0.06 * broad);
     }
+    if (index == c->ring_profile_body) {
+        /* 512 radial bins over [inner, outer], clamped at both ends. */
+        R bin = x * 512.0;
+        uint32_t b = bin > 0.0 ? (uint32_t)bin : 0u;
+        tau = c->ring_profile[b < 511u ? b : 511u];
+    }
     if (color) *color = col;
     return tau;
 }
@@ -1664,6 +1672,8 @@ static Ctx make_ctx(const orc_scene *sc) {
     c.ncat = sc->catalogue_count;
     c.cells = sc->sky_cells;
     c.milky = sc->milky_way;
+    c.ring_profile = sc->ring_profile;
+    c.ring_profile_body = sc->ring_profile ? sc->ring_profile_body : MISS;
     c.accumulated = ((const Settings *)sc->settings)->counts[1];
     return c;
 }

@@ -27,6 +27,10 @@ typedef struct orc_scene {
     uint32_t catalogue_count;
     const uint32_t *sky_cells;
     const uint32_t *milky_way;
+    /* Tabulated ring optical depth (512 radial bins) of one body, or null:
+     * the product's extension of src/shaders/rings.wgsl:50-81, restated. */
+    const float *ring_profile;
+    uint32_t ring_profile_body;
 } orc_scene;
 
 typedef struct orc_probe_ray {

@@ -37,9 +37,13 @@ EXPORTS = (
     "sg_set_profiling",
     "sg_kernel_time",
     "sg_set_integrator",
+    "sg_set_tail",
+    "sg_get_launch",
+    "sg_autotune",
     "sg_last_error",
     "sg_probe",
     "sg_set_ring_profile",
+    "sg_set_density_profile",
     "sg_restart",
     "sg_adjoint",
     "sg_adjoint_time",
@@ -69,6 +73,15 @@ class SgFrame(ctypes.Structure):
         ("ground_count", ctypes.c_uint32),
         ("scene_time", ctypes.c_double),
     ]
+
+
+class SgLaunch(ctypes.Structure):
+    """``sg_launch`` (include/stargaze_amd.h)."""
+
+    _fields_ = [("integrator", ctypes.c_uint32), ("chunk_paths", ctypes.c_uint32), ("tail_paths", ctypes.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
 class SgBatchLaunch(ctypes.Structure):
@@ -126,10 +139,15 @@ def load() -> ctypes.CDLL:
         "sg_set_profiling": ([P, I32], I32),
         "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
         "sg_set_integrator": ([P, U32, U32], I32),
+        "sg_set_tail": ([P, U32], I32),
+        "sg_get_launch": ([P, ctypes.POINTER(SgLaunch)], I32),
+        "sg_autotune": ([P, ctypes.POINTER(SgFrame), U32, ctypes.POINTER(SgLaunch), ctypes.POINTER(ctypes.c_double)],
+                        I32),
         "sg_last_error": ([P], ctypes.c_char_p),
         "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
         "sg_set_ring_profile": ([P, U32, P, U32], I32),
         "sg_restart": ([P], I32),
+        "sg_set_density_profile": ([P, P, U32], I32),
         "sg_adjoint": ([P, ctypes.POINTER(SgFrame), P, U64, P, U32, P], I32),
         "sg_adjoint_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
         "sg_batch_create": ([ctypes.POINTER(P), ctypes.POINTER(P), U32], I32),

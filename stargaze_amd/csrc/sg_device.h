@@ -96,6 +96,9 @@ struct Scene {
     // the inner to the outer edge) replacing the banded profile of one body.
     const float *ring_profile;
     uint32_t ring_profile_body;  // SG_MISS when no table is set
+    // Tabulated relative air density (SG_DENSITY_BINS altitude slices up to
+    // the top of the air) replacing exp(-h / H); null when not set.
+    const float *density_profile;
 };
 
 struct Ray {

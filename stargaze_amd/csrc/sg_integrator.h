@@ -7,11 +7,29 @@ namespace sg {
 
 // --- gradient accumulation for the adjoint ----------------------------------
 // Slots of the gradient vector (the C ABI's SG_GRAD_* layout): material rgb
-// per body, one per ring profile bin, then Rayleigh rgb, Mie and Mie g.
+// per body, one per ring profile bin, Rayleigh rgb, Mie and Mie g, then one
+// per density profile bin.
 constexpr uint32_t GRAD_ALBEDO = 0u;
 constexpr uint32_t GRAD_RING = 3u * SG_BODY_CAPACITY;
 constexpr uint32_t GRAD_AIR = GRAD_RING + SG_RING_PROFILE_BINS;
-constexpr uint32_t GRAD_COUNT = GRAD_AIR + 5u;
+constexpr uint32_t GRAD_DENSITY = GRAD_AIR + 5u;
+constexpr uint32_t GRAD_COUNT = GRAD_DENSITY + SG_DENSITY_BINS;
+constexpr uint32_t GRAD_PRIVATE = SG_DENSITY_BINS + 5u;  // lane-private slots per path
+
+// Profile bin of an altitude: SG_DENSITY_BINS equal slices up to the top of
+// the air.
+SG_FN uint32_t density_bin(const Scene &sc, float altitude) {
+    return umin((uint32_t)fmaxf(altitude / sc.s.globals.atmo_params[2] * (float)SG_DENSITY_BINS, 0.0f),
+                SG_DENSITY_BINS - 1u);
+}
+// Where the samples of a density column fell (for the adjoint): the profile
+// bin of each sample, four bits apiece, their spacing and their sum.
+struct ColumnBins {
+    float total = 0.0f;
+    float dt = 0.0f;
+    uint32_t bins = 0u;
+    uint32_t count = 0u;
+};
 
 // The plain integrator: every gradient hook compiles away.
 struct NoGrad {
@@ -22,6 +40,11 @@ struct NoGrad {
 struct GradSink {
     static constexpr bool on = true;
     float *g;
+    // Lane-private accumulators, slot k at dens[k * dstride]: the density
+    // profile bins, then the five air coefficients (every contribution in
+    // the air touches those, so they are summed per lane and flushed once).
+    float *dens;
+    uint32_t dstride;
     SG_FN void add(uint32_t i, float v) const {
         if (v == 0.0f) return;
 #if defined(__HIP_DEVICE_COMPILE__)
@@ -36,24 +59,43 @@ struct GradSink {
         add(GRAD_ALBEDO + 3u * body + 2u, d.z);
     }
     SG_FN void ring(uint32_t bin, float d) const { add(GRAD_RING + bin, d); }
+    SG_FN void air(uint32_t k, float v) const { dens[(SG_DENSITY_BINS + k) * dstride] += v; }
     // d / d(beta_ext) per component; beta_ext = beta_rayleigh + beta_mie.
     SG_FN void extinction(f3 d) const {
-        add(GRAD_AIR, d.x);
-        add(GRAD_AIR + 1u, d.y);
-        add(GRAD_AIR + 2u, d.z);
-        add(GRAD_AIR + 3u, d.x + d.y + d.z);
+        air(0u, d.x);
+        air(1u, d.y);
+        air(2u, d.z);
+        air(3u, d.x + d.y + d.z);
     }
     // A weighted contribution proportional to beta_phase = beta_r * rayleigh
     // + beta_m * mie(g): its derivatives through the phase coefficients.
     SG_FN void phase(f3 wc, f3 beta_phase, float rayleigh, float mie, float dmie_dg, float beta_m) const {
         f3 q = f3{beta_phase.x > 0.0f ? wc.x / beta_phase.x : 0.0f, beta_phase.y > 0.0f ? wc.y / beta_phase.y : 0.0f,
                   beta_phase.z > 0.0f ? wc.z / beta_phase.z : 0.0f};
-        add(GRAD_AIR, q.x * rayleigh);
-        add(GRAD_AIR + 1u, q.y * rayleigh);
-        add(GRAD_AIR + 2u, q.z * rayleigh);
+        air(0u, q.x * rayleigh);
+        air(1u, q.y * rayleigh);
+        air(2u, q.z * rayleigh);
         float sum = q.x + q.y + q.z;
-        add(GRAD_AIR + 3u, sum * mie);
-        add(GRAD_AIR + 4u, sum * beta_m * dmie_dg);
+        air(3u, sum * mie);
+        air(4u, sum * beta_m * dmie_dg);
+    }
+    // d / d(profile bin) of a factor exp(-beta_ext * column) for weight
+    // e = d(loss)/d log(factor) * (-beta_ext): each sample adds dt.
+    SG_FN void column(const Scene &sc, const ColumnBins &c, float e) const {
+        if (!sc.density_profile || e == 0.0f) return;
+        for (uint32_t k = 0; k < c.count; ++k) dens[((c.bins >> (4u * k)) & 15u) * dstride] += e * c.dt;
+    }
+    SG_FN void density(uint32_t bin, float v) const { dens[bin * dstride] += v; }
+    // Move the lane's private accumulators into the gradient.
+    SG_FN void flush_private() const {
+        for (uint32_t b = 0; b < SG_DENSITY_BINS; ++b) {
+            add(GRAD_DENSITY + b, dens[b * dstride]);
+            dens[b * dstride] = 0.0f;
+        }
+        for (uint32_t k = 0; k < 5u; ++k) {
+            add(GRAD_AIR + k, dens[(SG_DENSITY_BINS + k) * dstride]);
+            dens[(SG_DENSITY_BINS + k) * dstride] = 0.0f;
+        }
     }
     // A weighted contribution behind a shadow ray that may cross the
     // tabulated ring.
@@ -161,7 +203,7 @@ SG_FN float shell_exit(const Scene &sc, f3 origin, f3 dir) {
     if (disc < 0.0f) return 0.0f;
     return -b + sqrtf(disc);
 }
-SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps) {
+SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps, ColumnBins *trace = nullptr) {
     float distance = shell_exit(sc, origin, dir);
     if (distance <= 0.0f) return 0.0f;
     float dt = distance / (float)steps;
@@ -172,7 +214,18 @@ SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps)
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + 0.5f) * dt);
         float h = fmaxf(length(p - c) - r, 0.0f);
-        sum += expf(-h * inv_h);
+        if (sc.density_profile) {
+            const uint32_t bin = density_bin(sc, h);
+            sum += sc.density_profile[bin];
+            if (trace) trace->bins |= bin << (4u * i);
+        } else {
+            sum += expf(-h * inv_h);
+        }
+    }
+    if (trace) {
+        trace->dt = dt;
+        trace->count = steps;
+        trace->total = sum * dt;
     }
     return sum * dt;
 }
@@ -190,10 +243,10 @@ SG_FN bool inside_air(const Scene &sc, const Ray &origin) {
 }
 // Density column from an anchored origin to the top of the air; negative
 // when the origin is outside the air (no extinction).
-SG_FN float air_column(const Scene &sc, const Ray &origin, f3 dir) {
+SG_FN float air_column(const Scene &sc, const Ray &origin, f3 dir, ColumnBins *trace = nullptr) {
     if (!inside_air(sc, origin)) return -1.0f;
     f3 p = atmo_center(sc) + tof(host_offset(sc, origin));
-    return density_integral(sc, p, dir, ATMO_SUN_STEPS);
+    return density_integral(sc, p, dir, ATMO_SUN_STEPS, trace);
 }
 SG_FN f3 air_extinction(const Scene &sc, float column) {
     if (column < 0.0f) return f3s(1.0f);
@@ -341,11 +394,20 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     bool planetshine = reflected.x > 0.0f || reflected.y > 0.0f || reflected.z > 0.0f;
 
     f3 t_view = f3s(1.0f), inscatter = f3s(0.0f);
+    f3 reflector_grad = f3s(0.0f);  // d(loss)/d(reflector material), summed over the steps
     float view_column = 0.0f;  // density column from the origin to this step
+    uint32_t view_bins = 0u;   // density profile bins of the steps so far
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
-        float density = expf(-altitude / H);
+        uint32_t bin = 0u;
+        float density;
+        if (sc.density_profile) {
+            bin = density_bin(sc, altitude);
+            density = sc.density_profile[bin];
+        } else {
+            density = expf(-altitude / H);
+        }
         float column = density * dt;
         f3 tau = beta_ext * column;
         f3 step_t = exp3(-tau);
@@ -354,12 +416,29 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         segment.y = tau.y < 1.0e-3f ? column * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) / fmaxf(beta_ext.y, 1.0e-20f);
         segment.z = tau.z < 1.0e-3f ? column * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) / fmaxf(beta_ext.z, 1.0e-20f);
         f3 step_dlog = f3s(0.0f);  // d log(t_view * segment) / d(beta_ext)
+        f3 seg_dcol = f3s(0.0f);   // d log(segment) / d(column)
         if constexpr (G::on) {
             step_dlog = f3{segment_dlog(tau.x, column, segment.x, beta_ext.x),
                            segment_dlog(tau.y, column, segment.y, beta_ext.y),
                            segment_dlog(tau.z, column, segment.z, beta_ext.z)} -
                         f3s(view_column);
+            if (sc.density_profile) {
+                seg_dcol = f3{segment.x > 0.0f ? (tau.x < 1.0e-3f ? 1.0f - tau.x : step_t.x) / segment.x : 0.0f,
+                              segment.y > 0.0f ? (tau.y < 1.0e-3f ? 1.0f - tau.y : step_t.y) / segment.y : 0.0f,
+                              segment.z > 0.0f ? (tau.z < 1.0e-3f ? 1.0f - tau.z : step_t.z) / segment.z : 0.0f};
+            }
         }
+        // d(loss)/d(profile) of a weighted term of this step: the view path
+        // before the step, the step's own segment, and the light column.
+        auto density_terms = [&](f3 wc, const ColumnBins &light_bins) {
+            if constexpr (G::on) {
+                if (!sc.density_profile) return;
+                const float e = -(beta_ext.x * wc.x + beta_ext.y * wc.y + beta_ext.z * wc.z);
+                for (uint32_t j = 0; j < i; ++j) grad.density((view_bins >> (4u * j)) & 15u, e * dt);
+                grad.density(bin, (wc.x * seg_dcol.x + wc.y * seg_dcol.y + wc.z * seg_dcol.z) * dt);
+                grad.column(sc, light_bins, e);
+            }
+        };
         for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
             uint32_t b = light_index(sc, slot);
             if (body(sc, b).material.w < 0.5f) continue;
@@ -371,7 +450,9 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                 float visibility = air_light_visibility(sc, light_ray, b);
                 if (visibility == 0.0f) continue;
                 f3 e_sun = xyz(body(sc, b).material) * (visibility / ((float)light_samples * light.pdf));
-                float sun_column = density_integral(sc, p, light.direction, ATMO_SUN_STEPS);
+                ColumnBins sun_bins;
+                float sun_column = density_integral(sc, p, light.direction, ATMO_SUN_STEPS,
+                                                   G::on ? &sun_bins : nullptr);
                 f3 sun_t = exp3(-beta_ext * sun_column);
                 float cos_theta = dot(dir, light.direction);
                 f3 beta_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
@@ -383,6 +464,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                     grad.phase(wc, beta_phase, rayleigh_phase(cos_theta), mie_phase(cos_theta, g),
                                mie_phase_dg(cos_theta, g), beta_m);
                     grad.shadow(sc, light_ray, b, wc.x + wc.y + wc.z);
+                    density_terms(wc, sun_bins);
                 }
             }
         }
@@ -390,7 +472,9 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             Ray shine_ray = camera_ray(p, reflected_direction);
             float visibility = air_light_visibility(sc, shine_ray, reflector.index);
             if (visibility > 0.0f) {
-                float light_column = density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS);
+                ColumnBins light_bins;
+                float light_column = density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS,
+                                                      G::on ? &light_bins : nullptr);
                 f3 light_t = exp3(-beta_ext * light_column);
                 f3 term = t_view * segment * reflected_phase * reflected * (light_t * visibility);
                 inscatter += term;
@@ -399,13 +483,20 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                     grad.extinction(wc * (step_dlog - f3s(light_column)));
                     grad.phase(wc, reflected_phase, rayleigh_phase(reflected_cos), mie_phase(reflected_cos, g),
                                mie_phase_dg(reflected_cos, g), beta_m);
-                    grad.albedo(reflector.index, wc * reflector_dalbedo);
+                    reflector_grad += wc * reflector_dalbedo;
                     grad.shadow(sc, shine_ray, reflector.index, wc.x + wc.y + wc.z);
+                    density_terms(wc, light_bins);
                 }
             }
         }
         t_view *= step_t;
-        if constexpr (G::on) view_column += column;
+        if constexpr (G::on) {
+            view_column += column;
+            view_bins |= bin << (4u * i);
+        }
+    }
+    if constexpr (G::on) {
+        if (planetshine) grad.albedo(reflector.index, reflector_grad);
     }
     result.transmittance = t_view;
     result.inscatter = inscatter;
@@ -413,7 +504,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
 }
 // src/shaders/pathtrace.wgsl:1217-1234
 SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t seed,
-                                   float *column_total = nullptr) {
+                                   ColumnBins *trace = nullptr) {
     float exit = shell_exit(sc, origin, dir);
     float t_max = fminf(limit, exit);
     if (t_max <= 0.0f) return f3s(1.0f);
@@ -428,14 +519,27 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
     float r = host_radius(sc);
     f3 transmittance = f3s(1.0f);
     float total = 0.0f;
+    uint32_t bins = 0u;
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
-        float column = expf(-altitude / H) * dt;
+        float column;
+        if (sc.density_profile) {
+            const uint32_t bin = density_bin(sc, altitude);
+            column = sc.density_profile[bin] * dt;
+            bins |= bin << (4u * i);
+        } else {
+            column = expf(-altitude / H) * dt;
+        }
         transmittance *= exp3(-beta_ext * column);
         total += column;
     }
-    if (column_total) *column_total = total;
+    if (trace) {
+        trace->total = total;
+        trace->dt = dt;
+        trace->bins = bins;
+        trace->count = steps;
+    }
     return transmittance;
 }
 
@@ -675,7 +779,7 @@ SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, c
     f3 origin = tof(st.ray.offset);
     uint32_t air_rng = st.rng;
     Scatter air;
-    float column = 0.0f;
+    ColumnBins column;
     if constexpr (G::on) {
         air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
                          st.accumulated, static_cast<const GradSink &>(grad), grad.weight(st) * st.throughput);
@@ -686,7 +790,12 @@ SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, c
     f3 transmittance = camera_air_transmittance(sc, origin, st.ray.direction, limit, st.rng,
                                                 G::on ? &column : nullptr);
     st.radiance += st.throughput * air.inscatter;
-    if constexpr (G::on) grad.extinction(grad.rest(st) * (-column));
+    if constexpr (G::on) {
+        f3 r = grad.rest(st);
+        grad.extinction(r * (-column.total));
+        f3 be = beta_rayleigh(sc) + f3s(beta_mie(sc));
+        grad.column(sc, column, -(be.x * r.x + be.y * r.y + be.z * r.z));
+    }
     st.throughput *= transmittance;
     st.rng = air_rng;
 }
@@ -709,9 +818,15 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
     if (st.flags & PATH_IN_AIR) {
         st.flags &= ~PATH_IN_AIR;
         if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host)) {
-            float column = air_column(sc, ray, ray.direction);
+            ColumnBins bins;
+            float column = air_column(sc, ray, ray.direction, G::on ? &bins : nullptr);
             if constexpr (G::on) {
-                if (column > 0.0f) grad.extinction(grad.rest(st) * (-column));
+                if (column > 0.0f) {
+                    f3 r = grad.rest(st);
+                    grad.extinction(r * (-column));
+                    f3 be = beta_rayleigh(sc) + f3s(beta_mie(sc));
+                    grad.column(sc, bins, -(be.x * r.x + be.y * r.y + be.z * r.z));
+                }
             }
             st.throughput *= air_extinction(sc, column);
         }
@@ -856,13 +971,18 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
         if (visibility > 0.0f) {
             float bsdf_pdf = cosine / PI;
             float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
-            float column = air_column(sc, outgoing, sample.direction);
+            ColumnBins bins;
+            float column = air_column(sc, outgoing, sample.direction, G::on ? &bins : nullptr);
             f3 term = xyz(body(sc, i).material) * air_extinction(sc, column) *
                       (bsdf_pdf * weight * visibility / sample.pdf);
             direct_light += term;
             if constexpr (G::on) {
                 f3 wc = wt_direct * term;
-                if (column > 0.0f) grad.extinction(wc * (-column));
+                if (column > 0.0f) {
+                    grad.extinction(wc * (-column));
+                    f3 be = beta_rayleigh(sc) + f3s(beta_mie(sc));
+                    grad.column(sc, bins, -(be.x * wc.x + be.y * wc.y + be.z * wc.z));
+                }
                 grad.shadow(sc, outgoing, i, wc.x + wc.y + wc.z);
             }
         }
@@ -954,19 +1074,25 @@ SG_FN float ring_cap_scale(const PathState &st) {
 }
 // One path sample's adjoint: the primal run, then its replay with the same
 // draws adding d(loss)/d(parameters) for d(loss)/d(radiance) = w.
+// `density` holds GRAD_PRIVATE zeroed lane-private floats, `density_stride`
+// apart, for the density-profile and air-coefficient terms; they are moved
+// into `gradient` (and zeroed) at the end.
 SG_FN PathResult adjoint_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated, f3 w,
-                              float *gradient) {
+                              float *gradient, float *density, uint32_t density_stride) {
     PathState st;
     path_begin(st, direction, rng, accumulated);
     run_path(sc, st);
     PathResult primal = path_finish(st);
     PathGrad grad;
     grad.g = gradient;
+    grad.dens = density;
+    grad.dstride = density_stride;
     grad.w = w;
     grad.value = f3{primal.value.x, primal.value.y, primal.value.z};
     grad.cap_scale = ring_cap_scale(st);
     path_begin(st, direction, rng, accumulated);
     run_path(sc, st, grad);
+    grad.flush_private();
     return primal;
 }
 

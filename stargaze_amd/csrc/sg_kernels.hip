@@ -318,6 +318,19 @@ __global__ SG_HEAVY void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, u
     if (alive) next[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = id;
 }
 
+// The tail of a chunk: once fewer paths than the compaction threshold are
+// alive, each lane runs its queued path through the remaining loop of
+// trace() in one launch (same steps, same draws).
+__global__ SG_HEAVY void k_tail(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = queue[i];
+    PathState st;
+    load_state(wa, id, st);
+    run_path(sc, st);
+    finish(wa, id, st);
+}
+
 // Ordered per-pixel blend of the chunk's path results
 // (src/shaders/pathtrace.wgsl:1653-1674; per-sample dispatches when air is on).
 __global__ __launch_bounds__(256) void k_accumulate(WaveArgs wa, float4 *accumulation, float4 *backdrop,
@@ -498,10 +511,12 @@ __global__ void k_fill(float *p, uint32_t n, float v) {
 
 __global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs aa) {
     __shared__ float s_grad[GRAD_COUNT];
+    __shared__ float s_density[GRAD_PRIVATE * kTraceBlock];  // lane-private, slot-major
     __shared__ float4 s_value[kTraceBlock];
     __shared__ float4 s_backdrop[kTraceBlock];
     const uint32_t t = threadIdx.x;
     for (uint32_t i = t; i < GRAD_COUNT; i += blockDim.x) s_grad[i] = 0.0f;
+    for (uint32_t b = 0; b < GRAD_PRIVATE; ++b) s_density[b * blockDim.x + t] = 0.0f;
     __syncthreads();
     const TraceArgs &ta = aa.ta;
     const uint32_t lp = t / ta.count;
@@ -520,7 +535,7 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs a
             f3 direction = camera_direction(sc, x, y, rng);
             const float *d = aa.d_image + 3u * (size_t)pixel;
             f3 w = f3{d[0], d[1], d[2]} * inv_count;
-            PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad);
+            PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad, s_density + t, blockDim.x);
             value = r.value;
             backdrop = r.backdrop;
         }
@@ -654,6 +669,7 @@ struct sg_ctx {
     // Integrator selection and wavefront buffers
     uint32_t integrator = SG_INTEGRATOR_WAVEFRONT;
     uint32_t chunk_paths = 1u << 23;
+    uint32_t tail_paths = 0;  // compaction threshold: fused tail below this many live paths
     uint32_t wave_capacity = 0;
     void *wave_block = nullptr;
     WaveArgs wave{};
@@ -676,6 +692,9 @@ struct sg_ctx {
     // Tabulated ring profile (SG_RING_PROFILE_BINS floats) and its body.
     float *ring_profile = nullptr;
     uint32_t ring_profile_body = SG_MISS;
+    // Tabulated air density (SG_DENSITY_BINS floats) while density_on.
+    float *density_profile = nullptr;
+    bool density_on = false;
     // The last queued pass, which sg_adjoint differentiates.
     struct {
         uint32_t count = 0, samples = 0, sequence = 0;
@@ -784,6 +803,7 @@ Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     const bool profiled = ctx->ring_profile && ctx->ring_profile_body < settings.counts[0];
     sc.ring_profile = profiled ? ctx->ring_profile : nullptr;
     sc.ring_profile_body = profiled ? ctx->ring_profile_body : SG_MISS;
+    sc.density_profile = ctx->density_on ? ctx->density_profile : nullptr;
     return sc;
 }
 
@@ -916,6 +936,12 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
                            ctx->queues[q]);
         SG_HIP(ctx, hipGetLastError());
         for (uint32_t iteration = 0; n > 0; ++iteration) {
+            if (n < ctx->tail_paths) {
+                hipLaunchKernelGGL(k_tail, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
+                                   ctx->queues[q], n);
+                SG_HIP(ctx, hipGetLastError());
+                break;
+            }
             const dim3 grid((n + block - 1) / block);
             hipLaunchKernelGGL(k_extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
             SG_HIP(ctx, hipGetLastError());
@@ -1133,7 +1159,8 @@ int sg_destroy(sg_ctx *ctx) {
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
-                       ctx->bvh_prims, ctx->ring_profile, ctx->adj_image, ctx->adj_gradient, ctx->adj_primal};
+                       ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
+                       ctx->adj_gradient, ctx->adj_primal};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
@@ -1336,6 +1363,85 @@ int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths) {
     return SG_OK;
 }
 
+int sg_set_tail(sg_ctx *ctx, uint32_t tail_paths) {
+    if (!ctx) return SG_E_INVALID;
+    ctx->tail_paths = tail_paths;
+    return SG_OK;
+}
+
+int sg_get_launch(const sg_ctx *ctx, sg_launch *launch) {
+    if (!ctx || !launch) return SG_E_INVALID;
+    *launch = sg_launch{ctx->integrator, ctx->chunk_paths, ctx->tail_paths};
+    return SG_OK;
+}
+
+int sg_autotune(sg_ctx *ctx, const sg_frame *frame, uint32_t repeats, sg_launch *best, double *best_ms) {
+    if (!ctx || !frame) return SG_E_INVALID;
+    repeats = std::max<uint32_t>(1u, std::min<uint32_t>(repeats, 16u));
+    const sg_launch keep{ctx->integrator, ctx->chunk_paths, ctx->tail_paths};
+    const uint32_t keep_sequence = ctx->sequence;
+    const bool keep_profiling = ctx->profiling;
+    const double keep_ms = ctx->kernel_ms;
+    const uint64_t keep_launches = ctx->launches;
+    sg_launch winner = keep;
+    double winner_ms = 1e300;
+    int rc = SG_OK;
+    auto trial = [&](const sg_launch &l) -> int {
+        ctx->integrator = l.integrator;
+        ctx->chunk_paths = l.chunk_paths;
+        ctx->tail_paths = l.tail_paths;
+        int r = sg_encode(ctx, frame);  // warm: buffers sized, caches filled
+        if (r != SG_OK) return r;
+        ctx->profiling = true;
+        ctx->kernel_ms = 0.0;
+        ctx->launches = 0;
+        for (uint32_t i = 0; i < repeats && r == SG_OK; ++i) r = sg_encode(ctx, frame);
+        ctx->profiling = false;
+        if (r != SG_OK) return r;
+        if (ctx->launches == 0) return SG_OK;  // sample limit reached: nothing to time
+        const double ms = ctx->kernel_ms / (double)ctx->launches;
+        if (ms < winner_ms) {
+            winner_ms = ms;
+            winner = l;
+        }
+        return SG_OK;
+    };
+    // Queue capacity first (no tail), then the compaction threshold at the
+    // best capacity, then the fused integrator.
+    const uint32_t chunks[] = {1u << 21, 1u << 22, 1u << 23, 1u << 24};
+    for (uint32_t c : chunks) {
+        rc = trial(sg_launch{SG_INTEGRATOR_WAVEFRONT, c, 0u});
+        if (rc != SG_OK) break;
+    }
+    if (rc == SG_OK) {
+        const uint32_t capacity = winner.chunk_paths;
+        const uint32_t tails[] = {capacity / 64u, capacity / 16u, capacity / 4u};
+        for (uint32_t t : tails) {
+            rc = trial(sg_launch{SG_INTEGRATOR_WAVEFRONT, capacity, t});
+            if (rc != SG_OK) break;
+        }
+    }
+    if (rc == SG_OK) rc = trial(sg_launch{SG_INTEGRATOR_FUSED, keep.chunk_paths, keep.tail_paths});
+    if (rc == SG_OK && hipStreamSynchronize(ctx->stream) != hipSuccess) rc = fail(ctx, SG_E_DEVICE, "autotune sync");
+    const sg_launch chosen = (rc == SG_OK && winner_ms < 1e300) ? winner : keep;
+    ctx->integrator = chosen.integrator;
+    ctx->chunk_paths = chosen.chunk_paths;
+    ctx->tail_paths = chosen.tail_paths;
+    ctx->profiling = keep_profiling;
+    ctx->kernel_ms = keep_ms;
+    ctx->launches = keep_launches;
+    // Trial passes rendered into the image: start the next pass afresh, with
+    // the draws it would have had without them.
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->sequence = keep_sequence;
+    ctx->last.count = 0;
+    if (rc != SG_OK) return rc;
+    if (best) *best = chosen;
+    if (best_ms) *best_ms = winner_ms;
+    return SG_OK;
+}
+
 const char *sg_last_error(const sg_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
 int sg_restart(sg_ctx *ctx) {
@@ -1364,6 +1470,27 @@ int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t b
         ctx->ring_profile_body = body;
     }
     // A new profile is a new scene: restart the accumulation.
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->last.count = 0;
+    return SG_OK;
+}
+
+int sg_set_density_profile(sg_ctx *ctx, const float *density, uint32_t bins) {
+    if (!ctx) return SG_E_INVALID;
+    join_batch(ctx);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    if (!density) {
+        ctx->density_on = false;
+    } else {
+        if (bins != SG_DENSITY_BINS) return fail(ctx, SG_E_INVALID, "the density profile has SG_DENSITY_BINS bins");
+        for (uint32_t i = 0; i < bins; ++i)
+            if (!(density[i] >= 0.0f) || !std::isfinite(density[i]))
+                return fail(ctx, SG_E_INVALID, "air densities must be finite and non-negative");
+        if (!ctx->density_profile) SG_HIP(ctx, hipMalloc(&ctx->density_profile, SG_DENSITY_BINS * sizeof(float)));
+        SG_HIP(ctx, hipMemcpy(ctx->density_profile, density, bins * sizeof(float), hipMemcpyHostToDevice));
+        ctx->density_on = true;
+    }
     ctx->key.clear();
     ctx->samples = 0;
     ctx->last.count = 0;
@@ -1757,12 +1884,15 @@ int batch_pass(sg_batch *b, const sg_frame *frames) {
     return SG_OK;
 }
 
-// Forget the views' progressive history so the next pass starts afresh.
-void reset_history(sg_batch *b) {
-    for (sg_ctx *ctx : b->views) {
+// Forget the views' progressive history so the next pass starts afresh,
+// with each view's noise sequence back at `sequences`.
+void reset_history(sg_batch *b, const std::vector<uint32_t> &sequences) {
+    for (size_t i = 0; i < b->views.size(); ++i) {
+        sg_ctx *ctx = b->views[i];
         ctx->key.clear();
         ctx->samples = 0;
-        ctx->sequence = 0;
+        ctx->sequence = sequences[i];
+        ctx->last.count = 0;
         ctx->frame_on_device = false;
     }
     b->table_valid = false;
@@ -1932,6 +2062,8 @@ int sg_batch_autotune(sg_batch *b, const sg_frame *frames, uint32_t repeats, sg_
     repeats = std::max<uint32_t>(1u, std::min<uint32_t>(repeats, 16u));
     const sg_batch_launch keep = b->launch;
     const bool keep_profiling = b->profiling;
+    std::vector<uint32_t> sequences;
+    for (const sg_ctx *ctx : b->views) sequences.push_back(ctx->sequence);
     sg_batch_launch winner = keep;
     double winner_ms = 1e300;
     const uint32_t cus = b->compute_units;
@@ -1963,7 +2095,7 @@ int sg_batch_autotune(sg_batch *b, const sg_frame *frames, uint32_t repeats, sg_
             }
 out:
     if (rc == SG_OK && hipStreamSynchronize(b->stream) != hipSuccess) rc = bfail(b, SG_E_DEVICE, "autotune sync");
-    reset_history(b);
+    reset_history(b, sequences);
     sg_batch_set_launch(b, rc == SG_OK ? &winner : &keep);
     sg_batch_set_profiling(b, keep_profiling ? 1 : 0);
     if (rc != SG_OK) return rc;

@@ -9,7 +9,9 @@ gradient vector onto the three parameter tensors:
 * ``albedo`` (B, 3): ``sg_sphere.color`` of the bodies in ``scene.albedo_bodies``;
 * ``tau`` (512,): the tabulated ring profile of ``scene.ring_body``;
 * ``air`` (5,): Rayleigh rgb (``atmo_rayleigh.xyz``), Mie (``atmo_rayleigh.w``)
-  and Mie g (``atmo_params.x``), in the frame's units (per AU).
+  and Mie g (``atmo_params.x``), in the frame's units (per AU);
+* ``density`` (16,): the tabulated relative air density per altitude slice
+  (when ``scene.density`` is set; otherwise exp(-h / H) and no gradient).
 
 Tensors live on the CPU: the GPU work happens in the library, and each
 call moves one image and one gradient vector.
@@ -19,20 +21,22 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from .layouts import GRAD_AIR, GRAD_RING, RING_PROFILE_BINS, grad_albedo
+from .layouts import DENSITY_BINS, GRAD_AIR, GRAD_DENSITY, GRAD_RING, RING_PROFILE_BINS, grad_albedo
 from .pathtracer import Frame, Options, PathTracer
 
 
 class DiffScene:
     """A frame, its differentiable parameters and the tracer that renders it."""
 
-    def __init__(self, frame: Frame, size, spp: int, bounces: int, albedo_bodies=(), ring_body=None):
+    def __init__(self, frame: Frame, size, spp: int, bounces: int, albedo_bodies=(), ring_body=None,
+                 density: bool = False):
         self.frame = Frame(frame.globals.copy(), frame.bodies.copy(), frame.ground.copy(), frame.scene_time)
         self.size = (int(size[0]), int(size[1]))
         self.tracer = PathTracer(self.size, options=Options(samples_per_frame=spp, max_bounces=bounces,
                                                             milky_way=0.0))
         self.albedo_bodies = list(albedo_bodies)
         self.ring_body = ring_body
+        self.density = density
 
     def initial(self):
         """Parameter tensors holding the frame's current values (tau: the
@@ -44,7 +48,9 @@ class DiffScene:
             tau = torch.full((RING_PROFILE_BINS,), float(self.frame.bodies["ring_params"][self.ring_body][1]))
         g = self.frame.globals
         air = torch.tensor([*g["atmo_rayleigh"], g["atmo_params"][0]], dtype=torch.float32)
-        return albedo, tau, air
+        top, scale = float(g["atmo_params"][2]), float(g["atmo_params"][1])
+        density = torch.exp(-(torch.arange(DENSITY_BINS) + 0.5) * top / DENSITY_BINS / scale).float()
+        return albedo, tau, air, density
 
     def frame_with(self, albedo: np.ndarray, air: np.ndarray) -> Frame:
         bodies = self.frame.bodies.copy()
@@ -63,11 +69,13 @@ class DiffScene:
 
 class _Render(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, albedo, tau, air, scene: DiffScene):
+    def forward(ctx, albedo, tau, air, density, scene: DiffScene):
         frame = scene.frame_with(albedo.detach().cpu().numpy().astype(np.float32),
                                  air.detach().cpu().numpy().astype(np.float32))
         if scene.ring_body is not None:
             scene.tracer.set_ring_profile(scene.ring_body, tau.detach().cpu().numpy().astype(np.float32))
+        if scene.density:
+            scene.tracer.set_density_profile(density.detach().cpu().numpy().astype(np.float32))
         scene.tracer.restart()
         scene.tracer.encode(frame)
         image = scene.tracer.read_accumulation()[..., :3].copy()
@@ -84,11 +92,18 @@ class _Render(torch.autograd.Function):
             if scene.albedo_bodies else torch.zeros((0, 3))
         tau = g[GRAD_RING:GRAD_RING + RING_PROFILE_BINS].clone() if ctx.has_tau else None
         air = g[GRAD_AIR:GRAD_AIR + 5].clone()
-        return albedo, tau, air, None
+        density = g[GRAD_DENSITY:GRAD_DENSITY + DENSITY_BINS].clone() if scene.density else None
+        return albedo, tau, air, density, None
 
 
-def render(scene: DiffScene, albedo: torch.Tensor, tau: torch.Tensor | None, air: torch.Tensor) -> torch.Tensor:
-    """Pass mean radiance (H, W, 3) of one fresh pass; differentiable in all three."""
+def render(scene: DiffScene, albedo: torch.Tensor, tau: torch.Tensor | None, air: torch.Tensor,
+           density: torch.Tensor | None = None) -> torch.Tensor:
+    """Pass mean radiance (H, W, 3) of one fresh pass; differentiable in every
+    parameter tensor the scene uses."""
     if tau is None:
         tau = torch.zeros(RING_PROFILE_BINS)
-    return _Render.apply(albedo, tau, air, scene)
+    if density is None:
+        if scene.density:
+            raise ValueError("the scene renders with a density profile: pass `density`")
+        density = torch.zeros(DENSITY_BINS)
+    return _Render.apply(albedo, tau, air, density, scene)

@@ -120,9 +120,11 @@ METRES_PER_AU = 149_597_870_700.0  # src/shaders/ground.wgsl:15
 
 # Differentiable rendering (include/stargaze_amd.h SG_GRAD_*).
 RING_PROFILE_BINS = 512
+DENSITY_BINS = 16
 GRAD_RING = 3 * BODY_CAPACITY
 GRAD_AIR = GRAD_RING + RING_PROFILE_BINS  # Rayleigh rgb, Mie, Mie g
-GRAD_COUNT = GRAD_AIR + 5
+GRAD_DENSITY = GRAD_AIR + 5
+GRAD_COUNT = GRAD_DENSITY + DENSITY_BINS
 
 
 def grad_albedo(body: int, channel: int) -> int:

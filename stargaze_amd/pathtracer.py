@@ -193,6 +193,15 @@ class PathTracer:
         t = np.ascontiguousarray(tau, np.float32).reshape(-1)
         self._check(self._lib.sg_set_ring_profile(self._ctx, int(body), t.ctypes.data, t.size), "set_ring_profile")
 
+    def set_density_profile(self, density: np.ndarray | None) -> None:
+        """Relative air density per altitude slice (DENSITY_BINS); ``None``
+        restores exp(-h / H)."""
+        if density is None:
+            self._check(self._lib.sg_set_density_profile(self._ctx, None, 0), "set_density_profile")
+            return
+        d = np.ascontiguousarray(density, np.float32).reshape(-1)
+        self._check(self._lib.sg_set_density_profile(self._ctx, d.ctypes.data, d.size), "set_density_profile")
+
     def adjoint(self, frame: Frame, d_image: np.ndarray, want_mean: bool = False):
         """Gradient (GRAD_COUNT floats) of sum(d_image * pass_mean.rgb) for the
         last encoded pass of ``frame``; with ``want_mean`` also the pass mean
@@ -215,6 +224,26 @@ class PathTracer:
         ms, n = ctypes.c_double(), ctypes.c_uint64()
         self._check(self._lib.sg_adjoint_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "adjoint_time")
         return float(ms.value), int(n.value)
+
+    def set_tail(self, tail_paths: int) -> None:
+        """Wavefront compaction threshold (0: compact to the end)."""
+        self._check(self._lib.sg_set_tail(self._ctx, int(tail_paths)), "set_tail")
+
+    @property
+    def launch(self) -> dict:
+        l = _native.SgLaunch()
+        self._check(self._lib.sg_get_launch(self._ctx, ctypes.byref(l)), "get_launch")
+        return l.as_dict()
+
+    def autotune(self, frame: Frame, repeats: int = 1) -> tuple[dict, float]:
+        """Fastest integrator, queue capacity and compaction threshold for
+        ``frame``; the accumulation restarts with the next pass."""
+        f, keep = frame.native()
+        best = _native.SgLaunch()
+        ms = ctypes.c_double()
+        self._check(self._lib.sg_autotune(self._ctx, ctypes.byref(f), int(repeats), ctypes.byref(best),
+                                          ctypes.byref(ms)), "autotune")
+        return best.as_dict(), float(ms.value)
 
     def probe(self, frame: Frame, kind: int, rays: np.ndarray) -> np.ndarray:
         """Run production device helpers on ``rays`` (PROBE_RAY records)."""

@@ -503,3 +503,71 @@ def ringed_disc(width: int = 64, height: int = 48, tau: float = 0.6) -> Frame:
     sun = star(light * metres(1.5e11), metres(7.0e8))
     f.bodies = bodies(sun, giant)
     return f
+
+
+def diff_system(width: int = 512, height: int = 512, seed: int = 2) -> Frame:
+    """The differentiable-rendering workload: from 3.0e6 m above a 6.4e6 m
+    planet with Earth-like air, its limb across the lower view, a ringed
+    giant (constant-albedo surface, ring 1.2-2.3 radii) and six moons in the
+    sky, one star. Eight lit bodies with style-0 albedos (mottled moons),
+    their albedos drawn with a PCG seeded by ``seed``."""
+    rng = Pcg(seed)
+    f = frame()
+    camera(f.globals, 60.0, width, height)
+    r = metres(6.4e6)
+    below = math.radians(48.0)
+    host_c = np.array([0.0, -math.sin(below), -math.cos(below)]) * (r + metres(3.0e6))
+    host = place(body([0, 0, 0], r, [0.3, 0.32, 0.35], False), host_c)
+    r_giant = metres(6.0e7)
+    giant = body([0, 0, 0], r_giant, [0.6, 0.52, 0.4], False)
+    tilt = math.radians(18.0)
+    pole = np.array([0.0, math.cos(tilt), math.sin(tilt)])
+    giant["ring_plane"] = [*pole, 1.2]
+    giant["ring_params"] = [2.3, 0.7, 0.0, 0.0]
+    giant["ring_color"] = [0.72, 0.68, 0.6, 0.0]
+    x_axis = np.array([1.0, 0.0, 0.0])
+    giant["axis_x"] = [*x_axis, 0.0]
+    giant["axis_y"] = [*np.cross(pole, x_axis), 0.0]
+    place(giant, direction(10.0, 18.0) * metres(4.5e8))
+    moons = []
+    for k in range(6):
+        radius = metres(math.exp(rng.uniform(math.log(6e5), math.log(2.5e6))))
+        albedo = [rng.uniform(0.1, 0.8) for _ in range(3)]
+        m = body([0, 0, 0], radius, albedo, False)
+        m["center_low"][3] = 0.6
+        m["axis_x"] = [1, 0, 0, 0.0]
+        m["axis_y"] = [0, 1, 0, float(k + 1)]
+        az, el = rng.uniform(-35.0, 35.0), rng.uniform(8.0, 28.0)
+        moons.append(place(m, direction(az, el) * metres(rng.uniform(2.0e7, 6.0e7))))
+    sun = star(direction(-55.0, 20.0) * metres(1.5e11), metres(7.0e8))
+    f.bodies = bodies(sun, host, giant, *moons)
+    au = METRES_PER_AU
+    g = f.globals
+    g["atmo_center"] = [*split(host_c)[0], r]
+    g["atmo_rayleigh"] = [5.8e-6 * au, 13.5e-6 * au, 33.1e-6 * au, 2.0e-5 * au]
+    g["atmo_params"] = [0.76, 8000.0 / au, 80000.0 / au, 2.0]
+    return f
+
+
+def exponential_density(frame: Frame, bins: int = 16) -> np.ndarray:
+    """exp(-h / H) at the middle altitude of each of ``bins`` slices."""
+    top, scale = float(frame.globals["atmo_params"][2]), float(frame.globals["atmo_params"][1])
+    return np.exp(-(np.arange(bins) + 0.5) * top / bins / scale).astype(np.float32)
+
+
+def ring_profile_cba(frame: Frame, ring_body: int, bins: int = 512, seed: int = 0x5EED1234) -> np.ndarray:
+    """A 512-bin ring optical-depth profile of C, B and A bands (0.1, 1.5,
+    0.5) with 0.01 gaps (Cassini division and two narrow gaps in A) and
+    +-10% noise from a PCG seeded by ``seed``, over the body's ring radii."""
+    b = frame.bodies[ring_body]
+    inner, outer = float(b["ring_plane"][3]), float(b["ring_params"][0])
+    radius = inner + (np.arange(bins) + 0.5) / bins * (outer - inner)  # in planet radii
+    # Saturn-like band edges scaled onto [inner, outer].
+    edges = np.array([1.24, 1.53, 1.95, 2.03, 2.27]) - 1.24
+    x = (radius - inner) / (outer - inner) * edges[-1]
+    tau = np.where(x < edges[1], 0.1, np.where(x < edges[2], 1.5, np.where(x < edges[3], 0.01, 0.5)))
+    for gap in (2.14 - 1.24, 2.21 - 1.24):
+        tau = np.where(np.abs(x - gap) < 0.004, 0.01, tau)
+    rng = Pcg(seed)
+    noise = np.array([rng.uniform(-0.1, 0.1) for _ in range(bins)])
+    return (tau * (1.0 + noise)).astype(np.float32)

@@ -15,7 +15,7 @@ import math
 import numpy as np
 
 from stargaze_amd import scenes
-from stargaze_amd.layouts import GRAD_AIR, GRAD_COUNT, GRAD_RING, RING_PROFILE_BINS, grad_albedo
+from stargaze_amd.layouts import DENSITY_BINS, GRAD_AIR, GRAD_COUNT, GRAD_DENSITY, GRAD_RING, RING_PROFILE_BINS, grad_albedo
 
 H_ALBEDO = 1e-3
 H_TAU = 1e-3
@@ -26,13 +26,14 @@ TOL = 1e-2
 class Params:
     """Mutable copy of the differentiable parameters of a frame."""
 
-    def __init__(self, frame, ring_body=None, tau=None):
+    def __init__(self, frame, ring_body=None, tau=None, density=None):
         self.frame = scenes.Frame(frame.globals.copy(), frame.bodies.copy(), frame.ground.copy())
         self.ring_body = ring_body
         self.tau = None if tau is None else np.array(tau, np.float32)
+        self.density = None if density is None else np.array(density, np.float32)
 
     def copy(self):
-        return Params(self.frame, self.ring_body, self.tau)
+        return Params(self.frame, self.ring_body, self.tau, self.density)
 
     def get(self, p):
         kind, i = p[0], p[1]
@@ -40,6 +41,8 @@ class Params:
             return float(self.frame.bodies["color"][i][p[2]])
         if kind == "ring":
             return float(self.tau[i])
+        if kind == "density":
+            return float(self.density[i])
         return float(self.frame.globals["atmo_rayleigh"][i] if i < 4 else self.frame.globals["atmo_params"][0])
 
     def set(self, p, v):
@@ -50,6 +53,8 @@ class Params:
             self.frame.bodies["color"][i] = c
         elif kind == "ring":
             self.tau[i] = v
+        elif kind == "density":
+            self.density[i] = v
         elif i < 4:
             a = self.frame.globals["atmo_rayleigh"].copy()
             a[i] = v
@@ -65,6 +70,8 @@ def slot(p):
         return grad_albedo(p[1], p[2])
     if p[0] == "ring":
         return GRAD_RING + p[1]
+    if p[0] == "density":
+        return GRAD_DENSITY + p[1]
     return GRAD_AIR + p[1]
 
 
@@ -73,6 +80,8 @@ def step(params, p):
         return H_ALBEDO
     if p[0] == "ring":
         return H_TAU
+    if p[0] == "density":  # relative: the slices span four decades
+        return H_AIR_REL * abs(params.get(p))
     return H_AIR_REL * abs(params.get(p)) if p[1] < 4 else H_ALBEDO
 
 
@@ -121,6 +130,26 @@ def air_case(width=40, height=30):
     frame = scenes.orbit_air(width, height, seed=0x0A11CE)
     plist = [("air", k) for k in range(5)]
     return Params(frame), plist, dict(spp=4, bounces=3)
+
+
+def density_case(width=40, height=30):
+    """The orbit-air view with a tabulated density: exp(-h / H) at each
+    slice's middle altitude."""
+    frame = scenes.orbit_air(width, height, seed=0x0A11CE)
+    g = frame.globals
+    top, scale = float(g["atmo_params"][2]), float(g["atmo_params"][1])
+    density = np.exp(-(np.arange(DENSITY_BINS) + 0.5) * top / DENSITY_BINS / scale).astype(np.float32)
+    return Params(frame, density=density), dict(spp=4, bounces=3)
+
+
+def density_bins_with_signal(grad, count):
+    """The `count` density slices with the largest gradient. Slices where
+    in-scattering and extinction nearly cancel have gradients so small that
+    central differences at h = 1e-3 measure the curvature instead."""
+    g = np.abs(grad[GRAD_DENSITY:GRAD_DENSITY + DENSITY_BINS])
+    order = np.argsort(-g)[:count]
+    assert np.all(g[order] > 0), "density slices without a gradient"
+    return [("density", int(b)) for b in sorted(order)]
 
 
 def ring_case(width=48, height=32, bins=16):

@@ -67,7 +67,7 @@ extern "C" {
 // when `gradient` is not null, the product adjoint for d(loss)/d(mean) =
 // d_image added into `gradient` (GRAD_COUNT floats).
 int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *ring_profile, uint32_t ring_body,
-            const float *d_image, double *mean, float *gradient) {
+            const float *density_profile, const float *d_image, double *mean, float *gradient) {
     Scene sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.s = *settings;
@@ -75,6 +75,8 @@ int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *r
     sc.t = terrain_layout(settings->globals.ground_counts[2]);
     sc.ring_profile = ring_profile;
     sc.ring_profile_body = ring_profile ? ring_body : SG_MISS;
+    sc.density_profile = density_profile;
+    float density[GRAD_PRIVATE] = {};
     const uint32_t width = (uint32_t)settings->globals.viewport[0];
     const uint32_t height = (uint32_t)settings->globals.viewport[1];
     const uint32_t count = settings->counts[2];
@@ -90,7 +92,7 @@ int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *r
             if (gradient) {
                 const float *d = d_image + 3u * pixel;
                 r = adjoint_path(sc, direction, rng, accumulated, f3{d[0], d[1], d[2]} * (1.0f / (float)count),
-                                 gradient);
+                                 gradient, density, 1u);
             } else {
                 r = trace_path(sc, direction, rng, accumulated);
             }

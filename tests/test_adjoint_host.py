@@ -25,7 +25,7 @@ class HostRenderer:
         self.spp, self.bounces = spp, bounces
         lib = ctypes.CDLL(str(hostbuild.build()))
         P = ctypes.c_void_p
-        lib.hc_pass.argtypes = [P, P, P, ctypes.c_uint32, P, P, P]
+        lib.hc_pass.argtypes = [P, P, P, ctypes.c_uint32, P, P, P, P]
         lib.hc_pass.restype = ctypes.c_int
         self.lib = lib
 
@@ -36,11 +36,13 @@ class HostRenderer:
         st = np.ascontiguousarray(tracer.settings(params.frame, self.spp))
         bodies = np.ascontiguousarray(params.frame.bodies)
         table = None if params.tau is None else np.ascontiguousarray(params.tau, np.float32)
+        dens = None if params.density is None else np.ascontiguousarray(params.density, np.float32)
         mean = np.zeros((h, w, 3), np.float64)
         grad = np.zeros(GRAD_COUNT, np.float32) if d is not None else None
         dd = None if d is None else np.ascontiguousarray(d, np.float32)
         self.lib.hc_pass(st.ctypes.data, bodies.ctypes.data, None if table is None else table.ctypes.data,
-                         params.ring_body or 0, None if dd is None else dd.ctypes.data, mean.ctypes.data,
+                         params.ring_body or 0, None if dens is None else dens.ctypes.data,
+                         None if dd is None else dd.ctypes.data, mean.ctypes.data,
                          None if grad is None else grad.ctypes.data)
         return mean, grad
 
@@ -89,3 +91,29 @@ def test_adjoint_ring_pass_through_host():
     mean = r.render(params)
     lum = params.frame.bodies["color"][0][0]
     assert abs(mean[..., 0].mean() / (lum * t) - 1) <= 5 * np.sqrt((1 - t) / (t * n))
+
+
+def test_adjoint_density_profile_host():
+    params, q = ac.density_case(20, 15)
+    r = HostRenderer(20, 15, q["spp"], q["bounces"])
+    plist = ac.density_bins_with_signal(r.adjoint(params, ac.loss_weights(20, 15)), 8)
+    ac.compare(r, params, plist, "host density")
+
+
+def test_density_profile_scales_against_the_coefficients():
+    """Density and scattering coefficients enter only as their product
+    (optical depth) and the phase-weighted segment (beta * column / beta_ext):
+    doubling the profile and halving every coefficient leaves the image
+    unchanged, which holds only if every density sample reads the profile."""
+    params, _, q = ac.air_case(16, 12)
+    r = HostRenderer(16, 12, q["spp"], q["bounces"])
+    ones = ac.Params(params.frame, density=np.ones(16, np.float32))
+    twos = ac.Params(params.frame, density=np.full(16, 2.0, np.float32))
+    a = twos.frame.globals["atmo_rayleigh"].copy()
+    twos.frame.globals["atmo_rayleigh"] = a * np.float32(0.5)
+    x, y = r.render(ones), r.render(twos)
+    assert np.abs(x).max() > 0
+    np.testing.assert_allclose(y, x, rtol=2e-5, atol=1e-7 * np.abs(x).max())
+    # and the profile is in use: a different profile changes the sky
+    other = ac.Params(params.frame, density=np.linspace(1.0, 0.1, 16).astype(np.float32))
+    assert np.abs(r.render(other) - x).max() > 1e-3

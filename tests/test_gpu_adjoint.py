@@ -25,6 +25,8 @@ class GpuRenderer:
                                                   milky_way=0.0))
         if params.tau is not None:
             t.set_ring_profile(params.ring_body, params.tau)
+        if params.density is not None:
+            t.set_density_profile(params.density)
         return t
 
     def render(self, params):
@@ -42,9 +44,10 @@ class GpuRenderer:
         return g
 
 
-def test_adjoint_matches_fd_on_33_parameters():
-    """12 albedos, 5 air coefficients, 16 ring-profile bins: adjoint against
-    central differences (h = 1e-3) of the plain renderer, within 1e-2."""
+def test_adjoint_matches_fd_on_41_parameters():
+    """12 albedos, 5 air coefficients, 16 ring-profile bins, 8 density
+    slices: adjoint against central differences (h = 1e-3, relative for the
+    air coefficients and densities) of the plain renderer, within 1e-2."""
     print("adjoint vs fd: albedo", flush=True)
     params, plist, q = ac.albedo_case(96, 64)
     worst = []
@@ -60,18 +63,29 @@ def test_adjoint_matches_fd_on_33_parameters():
     plist = ac.ring_bins_with_signal(r.adjoint(params, ac.loss_weights(96, 64)), bins)
     adj, fd = ac.compare(r, params, plist, "gpu ring")
     worst.append(np.max(np.abs(adj - fd) / np.abs(fd)))
+    print("adjoint vs fd: density", flush=True)
+    params, q = ac.density_case(80, 60)
+    r = GpuRenderer(80, 60, q["spp"], q["bounces"])
+    plist = ac.density_bins_with_signal(r.adjoint(params, ac.loss_weights(80, 60)), 8)
+    adj, fd = ac.compare(r, params, plist, "gpu density")
+    worst.append(np.max(np.abs(adj - fd) / np.abs(fd)))
     print(f"worst relative difference per group: {worst}")
 
 
-@pytest.mark.parametrize("case", ["albedo", "air", "ring"])
+@pytest.mark.parametrize("case", ["albedo", "air", "ring", "density"])
 def test_adjoint_pass_mean_matches_forward(case):
     """The adjoint's own primal is the plain renderer's image (fused kernel,
     same draws, same blend order)."""
     print(f"adjoint primal {case}", flush=True)
-    params, _, q = {"albedo": ac.albedo_case, "air": ac.air_case, "ring": ac.ring_case}[case](64, 48)
+    if case == "density":
+        params, q = ac.density_case(64, 48)
+    else:
+        params, _, q = {"albedo": ac.albedo_case, "air": ac.air_case, "ring": ac.ring_case}[case](64, 48)
     t = PathTracer((64, 48), options=Options(samples_per_frame=q["spp"], max_bounces=q["bounces"], milky_way=0.0))
     if params.tau is not None:
         t.set_ring_profile(params.ring_body, params.tau)
+    if params.density is not None:
+        t.set_density_profile(params.density)
     t.set_integrator(INTEGRATOR_FUSED)
     t.encode(params.frame)
     image = t.read_accumulation()
@@ -130,7 +144,7 @@ def test_torch_function_matches_adjoint():
 
     params, _, q = ac.ring_case(48, 32)
     scene = diff.DiffScene(params.frame, (48, 32), q["spp"], q["bounces"], albedo_bodies=[1], ring_body=1)
-    albedo, tau, air = scene.initial()
+    albedo, tau, air, _ = scene.initial()
     tau = torch.from_numpy(params.tau.copy())
     for p in (albedo, tau, air):
         p.requires_grad_(True)
@@ -165,7 +179,7 @@ def test_adam_recovers_ring_profile_and_albedo():
     true_coarse = torch.tensor([0.2 + 0.6 * (k % 3 == 0) + 0.05 * k for k in range(segments)], dtype=torch.float32)
     true_albedo = torch.tensor([[0.5, 0.45, 0.4]])
     target_scene = diff.DiffScene(frame, (width, height), 64, 1, albedo_bodies=[1], ring_body=1)
-    _, _, air = target_scene.initial()
+    _, _, air, _ = target_scene.initial()
     with torch.no_grad():
         target = diff.render(target_scene, true_albedo, true_coarse.repeat_interleave(spread), air)
     target_scene.close()
@@ -190,4 +204,31 @@ def test_adam_recovers_ring_profile_and_albedo():
     assert np.mean(losses[-10:]) < 0.05 * losses[0]
     assert np.allclose(albedo.detach().numpy(), true_albedo.numpy(), rtol=0.03)
     assert err1 < 0.4 * err0
+    scene.close()
+
+
+def test_torch_function_density_and_air():
+    """Density slices and air coefficients through the autograd harness
+    against the C ABI on the 557-parameter system scene (small)."""
+    import torch
+
+    from stargaze_amd import diff
+
+    frame = scenes.diff_system(48, 48)
+    scene = diff.DiffScene(frame, (48, 48), 4, 3, albedo_bodies=list(range(1, 9)), ring_body=2, density=True)
+    albedo, tau, air, density = scene.initial()
+    for p in (albedo, tau, air, density):
+        p.requires_grad_(True)
+    w = torch.from_numpy(ac.loss_weights(48, 48))
+    (diff.render(scene, albedo, tau, air, density) * w).sum().backward()
+    assert albedo.grad.shape == (8, 3) and density.grad.shape == (16,)
+    t = PathTracer((48, 48), options=Options(samples_per_frame=4, max_bounces=3, milky_way=0.0))
+    t.set_ring_profile(2, tau.detach().numpy())
+    t.set_density_profile(density.detach().numpy())
+    t.encode(scene.frame_with(albedo.detach().numpy(), air.detach().numpy()))
+    g = t.adjoint(scene.frame_with(albedo.detach().numpy(), air.detach().numpy()), ac.loss_weights(48, 48))
+    np.testing.assert_allclose(density.grad.numpy(), g[ac.GRAD_DENSITY:ac.GRAD_DENSITY + 16], rtol=1e-5)
+    np.testing.assert_allclose(air.grad.numpy(), g[ac.GRAD_AIR:ac.GRAD_AIR + 5], rtol=1e-5)
+    assert np.abs(density.grad.numpy()).max() > 0 and np.abs(albedo.grad.numpy()).max() > 0
+    t.close()
     scene.close()

@@ -116,12 +116,15 @@ def test_ring_hits_match_fp64_oracle():
     assert np.median(t_rel) < 1e-4
 
 
-def _render_pair(frame, size, spp, bounces, batches):
+def _render_pair(frame, size, spp, bounces, batches, profile=None):
     opts = Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0)
     gpu = PathTracer(size, options=opts)
     gpu.enable_moments(True)
     orc = oracle.OracleTracer(size, Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0), "f32")
     orc.enable_moments(True)
+    if profile is not None:
+        gpu.set_ring_profile(*profile)
+        orc.set_ring_profile(*profile)
     for _ in range(batches):
         gpu.encode(frame)
         orc.encode(frame)
@@ -144,6 +147,40 @@ def test_images_match_oracle_per_tile(scene, size, spp, bounces, batches):
     over, worst = parity.assert_tiles(z)
     print(f"{scene}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
     assert agree >= 0.9, agree
+
+
+def test_tabulated_ring_matches_oracle_per_tile():
+    """The ringed giant with its 512-bin C/B/A optical-depth table."""
+    size = (96, 54)
+    frame = scenes.ringed_giant(*size)
+    profile = (1, scenes.ring_profile_cba(frame, 1))
+    (ga, gm), (oa, om) = _render_pair(frame, size, 16, 6, 2, profile)
+    agree = parity.pixel_agreement(ga, oa)
+    z = parity.tile_z(ga, gm, oa, om)
+    over, worst = parity.assert_tiles(z)
+    print(f"tabulated ring: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
+    assert agree >= 0.9, agree
+
+
+def test_constant_ring_table_is_the_unbanded_ring():
+    """A table holding the body's own optical depth everywhere renders the
+    unbanded ring bit for bit."""
+    size = (64, 36)
+    frame = scenes.ringed_giant(*size)
+    b = frame.bodies.copy()
+    p = b["ring_params"][1].copy()
+    p[2] = 0.0  # unbanded
+    b["ring_params"][1] = p
+    frame = scenes.Frame(frame.globals, b, frame.ground)
+    images = []
+    for table in (None, np.full(512, float(p[1]), np.float32)):
+        t = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=4, milky_way=0.0))
+        if table is not None:
+            t.set_ring_profile(1, table)
+        t.encode(frame)
+        images.append(t.read_accumulation())
+        t.close()
+    assert np.array_equal(images[0], images[1])
 
 
 def test_batch_size_invariance_with_air():
@@ -282,3 +319,40 @@ def test_wavefront_and_fused_integrators_agree(scene):
         images.append(t.read_accumulation())
     np.testing.assert_allclose(images[1], images[0], rtol=1e-5, atol=1e-9)
     assert np.array_equal(images[1], images[2])
+
+
+@pytest.mark.parametrize("scene", ["ringed_giant", "halo"])
+def test_wavefront_tail_and_autotune(scene):
+    """The fused tail below the compaction threshold changes no draw: images
+    agree with the plain wavefront's; the autotuner's choice renders the
+    same image as well."""
+    size = (64, 36)
+    if scene == "halo":
+        frame, data, cells = scenes.halo_parity(*size)
+    else:
+        frame, data, cells = getattr(scenes, scene)(*size), None, None
+
+    def render(configure):
+        t = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=6, milky_way=0.0))
+        if data is not None:
+            t.upload_terrain(data, cells)
+        configure(t)
+        t.encode(frame)
+        t.encode(frame)
+        img = t.read_accumulation()
+        t.close()
+        return img
+
+    base = render(lambda t: t.set_integrator(ptm.INTEGRATOR_WAVEFRONT, 4096))
+    for tail in (64, 1024, 1 << 20):
+        img = render(lambda t: (t.set_integrator(ptm.INTEGRATOR_WAVEFRONT, 4096), t.set_tail(tail)))
+        np.testing.assert_allclose(img, base, rtol=1e-5, atol=1e-9)
+
+    def tuned(t):
+        best, ms = t.autotune(frame, repeats=1)
+        print(f"{scene}: autotune {best} {ms:.3f} ms")
+        assert best == t.launch and ms > 0.0
+        assert t.samples() == 0
+
+    img = render(tuned)
+    np.testing.assert_allclose(img, base, rtol=1e-5, atol=1e-9)

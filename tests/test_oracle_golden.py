@@ -66,3 +66,31 @@ def test_ring_energy_at_most_one(precision):
 def test_fp64_oracle_builds_are_distinct():
     assert oracle.library("f32").orc_real_bytes() == 4
     assert oracle.library("f64").orc_real_bytes() == 8
+
+
+def test_oracle_constant_ring_table_is_the_unbanded_ring():
+    """The oracle's tabulated-profile lookup: a table holding the body's own
+    optical depth everywhere renders the unbanded ring bit for bit, and a
+    different table changes the ring."""
+    import numpy as np
+
+    import oracle
+    from stargaze_amd import scenes
+    from stargaze_amd.pathtracer import Options
+
+    size = (32, 18)
+    frame = scenes.ringed_giant(*size)
+    b = frame.bodies.copy()
+    p = b["ring_params"][1].copy()
+    p[2] = 0.0
+    b["ring_params"][1] = p
+    frame = scenes.Frame(frame.globals, b, frame.ground)
+    images = []
+    for table in (None, np.full(512, float(p[1]), np.float32), np.full(512, 0.05, np.float32)):
+        t = oracle.OracleTracer(size, Options(samples_per_frame=4, max_bounces=3, milky_way=0.0))
+        if table is not None:
+            t.set_ring_profile(1, table)
+        t.encode(frame)
+        images.append(t.read_accumulation())
+    assert np.array_equal(images[0], images[1])
+    assert not np.array_equal(images[0], images[2])

@@ -263,6 +263,10 @@ int sg_set_density_profile(sg_ctx *ctx, const float *density, uint32_t bins);
 int sg_restart(sg_ctx *ctx);
 int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_t pixel_count,
                float *gradient, uint32_t gradient_count, float *pass_mean);
+/* With enable = 1, every pass also keeps each path sample's returned
+ * radiance and ring-cap scale (16 B per sample), and sg_adjoint of that pass
+ * replays each path once instead of rerunning its primal first. */
+int sg_enable_adjoint(sg_ctx *ctx, int32_t enable);
 /* Device time of sg_adjoint kernels since sg_set_profiling (HIP events). */
 int sg_adjoint_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
 

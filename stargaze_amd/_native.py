@@ -46,6 +46,7 @@ EXPORTS = (
     "sg_set_density_profile",
     "sg_restart",
     "sg_adjoint",
+    "sg_enable_adjoint",
     "sg_adjoint_time",
     "sg_batch_create",
     "sg_batch_destroy",
@@ -150,6 +151,7 @@ def load() -> ctypes.CDLL:
         "sg_set_density_profile": ([P, P, U32], I32),
         "sg_adjoint": ([P, ctypes.POINTER(SgFrame), P, U64, P, U32, P], I32),
         "sg_adjoint_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
+        "sg_enable_adjoint": ([P, I32], I32),
         "sg_batch_create": ([ctypes.POINTER(P), ctypes.POINTER(P), U32], I32),
         "sg_batch_destroy": ([P], I32),
         "sg_batch_set_launch": ([P, ctypes.POINTER(SgBatchLaunch)], I32),

@@ -1072,8 +1072,24 @@ SG_FN float ring_cap_scale(const PathState &st) {
     float lum = luminance(st.radiance - st.ring_base);
     return lum > st.ring_cap ? st.ring_cap / lum : 1.0f;
 }
-// One path sample's adjoint: the primal run, then its replay with the same
-// draws adding d(loss)/d(parameters) for d(loss)/d(radiance) = w.
+// The replay half of a path sample's adjoint: given the primal run's returned
+// radiance and ring-cap scale, rerun the path with the same draws adding
+// d(loss)/d(parameters) for d(loss)/d(radiance) = w.
+SG_FN void replay_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated, f3 w, f3 value,
+                       float cap_scale, float *gradient, float *density, uint32_t density_stride) {
+    PathState st;
+    PathGrad grad;
+    grad.g = gradient;
+    grad.dens = density;
+    grad.dstride = density_stride;
+    grad.w = w;
+    grad.value = value;
+    grad.cap_scale = cap_scale;
+    path_begin(st, direction, rng, accumulated);
+    run_path(sc, st, grad);
+    grad.flush_private();
+}
+// One path sample's adjoint: the primal run, then its replay.
 // `density` holds GRAD_PRIVATE zeroed lane-private floats, `density_stride`
 // apart, for the density-profile and air-coefficient terms; they are moved
 // into `gradient` (and zeroed) at the end.

@@ -36,6 +36,7 @@ struct TraceArgs {
     float4 *accumulation;
     float4 *backdrop;
     float4 *moments;       // null unless second moments are kept
+    float4 *records;       // per path sample (pixel * count + s): radiance rgb, ring-cap scale; or null
     uint32_t width, height;
     uint32_t count;        // samples this pass (counts.z)
     uint32_t accumulated;  // samples already in the image (counts.y)
@@ -130,9 +131,14 @@ __device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta
         const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
         uint32_t rng = path_seed(pixel, ta.sequence + s);
         f3 direction = camera_direction(sc, x, y, rng);
-        PathResult r = trace_path(sc, direction, rng, accumulated);
+        PathState st;
+        path_begin(st, direction, rng, accumulated);
+        run_path(sc, st);
+        PathResult r = path_finish(st);
         value = r.value;
         backdrop = r.backdrop;
+        if (ta.records)
+            ta.records[(size_t)pixel * ta.count + s] = make_float4(value.x, value.y, value.z, ring_cap_scale(st));
     }
     s_value[t] = value;
     s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
@@ -169,6 +175,7 @@ struct WaveArgs {
     // Per-path results.
     float4 *value;
     float4 *backdrop;
+    float4 *records;   // per path sample of the pass (see TraceArgs), or null
     uint32_t width, height;
     uint32_t count, accumulated, sequence, has_air;
     uint32_t pixel_begin, paths;
@@ -234,6 +241,9 @@ __device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const Pa
     PathResult r = path_finish(st);
     wa.value[id] = r.value;
     wa.backdrop[id] = make_float4(r.backdrop.x, r.backdrop.y, r.backdrop.z, 0.f);
+    if (wa.records)
+        wa.records[(size_t)wa.pixel_begin * wa.count + id] =
+            make_float4(r.value.x, r.value.y, r.value.z, ring_cap_scale(st));
 }
 
 __global__ __launch_bounds__(256) void k_generate(Scene sc, WaveArgs wa, uint32_t *queue) {
@@ -499,6 +509,7 @@ __global__ void k_batch_advance(BatchArgs ba) {
 // ---------------------------------------------------------------------------
 struct AdjointArgs {
     TraceArgs ta;            // pass layout; accumulation/backdrop receive the pass mean (or null)
+    const float4 *recorded;  // the pass's recorded primal outputs (skip the primal runs), or null
     const float *d_image;    // d(loss)/d(pass mean), rgb per pixel
     float *gradient;         // GRAD_COUNT
     uint32_t blocks;
@@ -535,9 +546,15 @@ __global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs a
             f3 direction = camera_direction(sc, x, y, rng);
             const float *d = aa.d_image + 3u * (size_t)pixel;
             f3 w = f3{d[0], d[1], d[2]} * inv_count;
-            PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad, s_density + t, blockDim.x);
-            value = r.value;
-            backdrop = r.backdrop;
+            if (aa.recorded) {
+                const float4 rec = aa.recorded[(size_t)pixel * ta.count + s];
+                replay_path(sc, direction, rng, accumulated, w, f3{rec.x, rec.y, rec.z}, rec.w, s_grad,
+                            s_density + t, blockDim.x);
+            } else {
+                PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad, s_density + t, blockDim.x);
+                value = r.value;
+                backdrop = r.backdrop;
+            }
         }
         if (ta.accumulation) {
             s_value[t] = value;
@@ -698,7 +715,12 @@ struct sg_ctx {
     // The last queued pass, which sg_adjoint differentiates.
     struct {
         uint32_t count = 0, samples = 0, sequence = 0;
+        bool recorded = false;  // its primal outputs are in `records`
     } last;
+    // Per path sample primal outputs of the last pass, kept while record_on.
+    bool record_on = false;
+    float4 *records = nullptr;
+    uint64_t record_capacity = 0;
     // Adjoint buffers: d(loss)/d(image) rgb, gradient, recomputed pass mean.
     float *adj_image = nullptr;
     float *adj_gradient = nullptr;
@@ -925,6 +947,7 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
     wa.accumulated = ta.accumulated;
     wa.sequence = ta.sequence;
     wa.has_air = ta.has_air;
+    wa.records = ta.records;
     const uint32_t block = 256;
     for (uint64_t p0 = 0; p0 < pixels; p0 += chunk_pixels) {
         const uint32_t chunk = (uint32_t)std::min<uint64_t>(chunk_pixels, pixels - p0);
@@ -1073,6 +1096,19 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     ta.accumulation = ctx->accumulation;
     ta.backdrop = ctx->backdrop;
     ta.moments = ctx->moments_on ? ctx->moments : nullptr;
+    ta.records = nullptr;
+    if (ctx->record_on) {
+        const uint64_t need = (uint64_t)ctx->width * ctx->height * count;
+        if (need > ctx->record_capacity) {
+            SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->records) SG_HIP(ctx, hipFree(ctx->records));
+            ctx->records = nullptr;
+            ctx->record_capacity = 0;
+            SG_HIP(ctx, hipMalloc(&ctx->records, need * sizeof(float4)));
+            ctx->record_capacity = need;
+        }
+        ta.records = ctx->records;
+    }
     ta.width = ctx->width;
     ta.height = ctx->height;
     ta.count = count;
@@ -1083,6 +1119,7 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     ctx->last.count = count;
     ctx->last.samples = ctx->samples;
     ctx->last.sequence = st.sampling[0];
+    ctx->last.recorded = ta.records != nullptr;
     return SG_OK;
 }
 
@@ -1160,7 +1197,7 @@ int sg_destroy(sg_ctx *ctx) {
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
                        ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
-                       ctx->adj_gradient, ctx->adj_primal};
+                       ctx->adj_gradient, ctx->adj_primal, ctx->records};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
@@ -1444,6 +1481,20 @@ int sg_autotune(sg_ctx *ctx, const sg_frame *frame, uint32_t repeats, sg_launch 
 
 const char *sg_last_error(const sg_ctx *ctx) { return ctx ? ctx->error.c_str() : "null context"; }
 
+int sg_enable_adjoint(sg_ctx *ctx, int32_t enable) {
+    if (!ctx) return SG_E_INVALID;
+    join_batch(ctx);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->record_on = enable != 0;
+    ctx->last.recorded = false;
+    if (!ctx->record_on && ctx->records) {
+        SG_HIP(ctx, hipFree(ctx->records));
+        ctx->records = nullptr;
+        ctx->record_capacity = 0;
+    }
+    return SG_OK;
+}
+
 int sg_restart(sg_ctx *ctx) {
     if (!ctx) return SG_E_INVALID;
     ctx->key.clear();
@@ -1540,6 +1591,7 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
     ta.accumulation = pass_mean ? ctx->adj_primal : nullptr;
     ta.backdrop = pass_mean ? ctx->adj_primal + pixels : nullptr;
     ta.moments = nullptr;
+    ta.records = nullptr;
     ta.width = ctx->width;
     ta.height = ctx->height;
     ta.count = count;
@@ -1547,6 +1599,7 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
     ta.sequence = ctx->last.sequence;
     ta.has_air = frame->globals->atmo_params[3] >= 0.5f ? 1u : 0u;
     ta.pixels_per_block = std::max<uint32_t>(1u, kTraceBlock / count);
+    aa.recorded = (ctx->last.recorded && !pass_mean) ? ctx->records : nullptr;
     aa.d_image = ctx->adj_image;
     aa.gradient = ctx->adj_gradient;
     aa.blocks = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);

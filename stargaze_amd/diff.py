@@ -34,6 +34,7 @@ class DiffScene:
         self.size = (int(size[0]), int(size[1]))
         self.tracer = PathTracer(self.size, options=Options(samples_per_frame=spp, max_bounces=bounces,
                                                             milky_way=0.0))
+        self.tracer.enable_adjoint(True)
         self.albedo_bodies = list(albedo_bodies)
         self.ring_body = ring_body
         self.density = density

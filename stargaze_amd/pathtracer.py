@@ -220,6 +220,10 @@ class PathTracer:
         )
         return (grad, mean) if want_mean else grad
 
+    def enable_adjoint(self, enable: bool = True) -> None:
+        """Keep per-sample primal outputs so ``adjoint`` replays each path once."""
+        self._check(self._lib.sg_enable_adjoint(self._ctx, int(enable)), "enable_adjoint")
+
     def adjoint_time(self) -> tuple[float, int]:
         ms, n = ctypes.c_double(), ctypes.c_uint64()
         self._check(self._lib.sg_adjoint_time(self._ctx, ctypes.byref(ms), ctypes.byref(n)), "adjoint_time")

@@ -232,3 +232,33 @@ def test_torch_function_density_and_air():
     assert np.abs(density.grad.numpy()).max() > 0 and np.abs(albedo.grad.numpy()).max() > 0
     t.close()
     scene.close()
+
+
+@pytest.mark.parametrize("integrator", [INTEGRATOR_FUSED, 2])
+def test_recorded_primal_gives_the_same_gradient(integrator):
+    """With sg_enable_adjoint the forward pass keeps each sample's radiance
+    and the adjoint replays once; the gradient matches the rerun-primal
+    adjoint (the primal comes from another kernel: last-bit differences)."""
+    print(f"recorded primal, integrator {integrator}", flush=True)
+    frame = scenes.diff_system(64, 64)
+    tau = scenes.ring_profile_cba(frame, 2)
+    density = scenes.exponential_density(frame)
+    d = ac.loss_weights(64, 64)
+    grads = []
+    for record in (False, True):
+        t = PathTracer((64, 64), options=Options(samples_per_frame=8, max_bounces=4, milky_way=0.0))
+        t.set_integrator(integrator)
+        t.set_ring_profile(2, tau)
+        t.set_density_profile(density)
+        t.enable_adjoint(record)
+        t.encode(frame)
+        grads.append(t.adjoint(frame, d).astype(np.float64))
+        t.close()
+    a, b = grads
+    scale = np.abs(a).max()
+    assert scale > 0
+    err = np.abs(a - b).max() / scale
+    print(f"recorded vs rerun: max difference {err:.3e} of the largest gradient")
+    assert err <= 1e-4
+    nz = np.abs(a) > 1e-3 * scale
+    assert np.allclose(a[nz], b[nz], rtol=1e-3)

@@ -54,8 +54,11 @@ FLOP_COST = {
     "vertex": 70,         # surface shading: sampling frame, throughput
     "rng": 0,             # integer work
 }
-PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32, dense
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, dense
 PEAK_HBM_GBS = 8000.0
+# HBM bytes of one pass, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by
+# a 1 GiB copy's counts (profiles/halo_4k_kernels.md).
+TRAFFIC_BYTES_PER_PASS = {"halo_4k": 345.1e9}
 
 def _halo(width, height):
     frame, data, cells = scenes.halo_observatory(width, height)
@@ -318,6 +321,8 @@ def main() -> int:
     ap.add_argument("--workload", default="halo_4k", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default integrator launch")
+    ap.add_argument("--launch", default=None,
+                    help="integrator launch instead of the autotuner's: fused, or wavefront:CHUNK:TAIL")
     ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
@@ -336,7 +341,16 @@ def main() -> int:
         tracer.set_ring_profile(*profile)
     log(f"workload {args.workload}: {width}x{height}, {spp} spp/step, {bounces} bounces")
     launch = None
-    if not args.no_autotune:
+    if args.launch:
+        from stargaze_amd.pathtracer import INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT
+
+        if args.launch == "fused":
+            tracer.set_integrator(INTEGRATOR_FUSED)
+        else:
+            _, chunk, tail = args.launch.split(":")
+            tracer.set_integrator(INTEGRATOR_WAVEFRONT, int(chunk))
+            tracer.set_tail(int(tail))
+    elif not args.no_autotune:
         launch, tuned_ms = tracer.autotune(frame, repeats=1)
         log(f"autotune: {launch} {tuned_ms:.2f} ms")
     for i in range(args.warmup):
@@ -385,12 +399,15 @@ def main() -> int:
         },
         "launch": launch if launch is not None else tracer.launch,
         "roofline": {
-            "bound": "valu",
+            # Compute roofline at MI355X's FP32 peak. The kernels issue FP32
+            # VALU instructions: nothing on the path is a dense contraction.
+            "bound": "mfma",
+            "pipe": "FP32 VALU, no MFMA instructions",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "traffic": None,
+            "traffic": TRAFFIC_BYTES_PER_PASS.get(args.workload),
             "kernel": "integrator (all kernels of one pass)",
             "kernel_ms": round(kernel_s * 1e3, 3),
             "flop_per_path_sample": round(flops, 1),

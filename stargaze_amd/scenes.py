@@ -571,3 +571,71 @@ def ring_profile_cba(frame: Frame, ring_body: int, bins: int = 512, seed: int = 
     rng = Pcg(seed)
     noise = np.array([rng.uniform(-0.1, 0.1) for _ in range(bins)])
     return (tau * (1.0 + noise)).astype(np.float32)
+
+
+def random_system(width: int, height: int, rng: "Pcg") -> tuple[Frame, dict]:
+    """One random view of the batched workload: the camera 2 m above a host
+    planet at a random pose (the host's up vector within 60 deg of the
+    view's up, random heading), a star and 0-10 further bodies (moons and
+    giants) at random directions, 2-12 bodies in all. A giant carries a
+    ring half of the time (banded half of those), the host has Earth-like
+    air a third of the time. Albedos are constant (style 0) or gas-giant
+    bands (style 1) for giants."""
+    f = frame()
+    fov = rng.uniform(40.0, 75.0)
+    camera(f.globals, fov, width, height)
+    tilt = math.radians(rng.uniform(0.0, 60.0))
+    heading = rng.uniform(0.0, 2.0 * math.pi)
+    up = np.array([math.sin(tilt) * math.cos(heading), math.cos(tilt), math.sin(tilt) * math.sin(heading)])
+    r_host = metres(math.exp(rng.uniform(math.log(1.5e6), math.log(7.0e6))))
+    host_c = -up * (r_host + metres(2.0))
+    host = place(body([0, 0, 0], r_host, [rng.uniform(0.1, 0.5) for _ in range(3)], False), host_c)
+    sun = star(direction(rng.uniform(-80.0, 80.0), rng.uniform(5.0, 60.0)) * metres(1.5e11), metres(7.0e8))
+    extra = []
+    n = int(rng.uniform(0.0, 11.0))
+    for k in range(n):
+        giant = rng.uniform() < 0.3
+        radius = metres(rng.uniform(2.0e7, 7.0e7) if giant else math.exp(rng.uniform(math.log(2e5), math.log(3e6))))
+        dist = radius * rng.uniform(6.0, 40.0)
+        b = body([0, 0, 0], radius, [rng.uniform(0.05, 0.8) for _ in range(3)], False)
+        if giant:
+            b["axis_x"] = [1, 0, 0, 1.0 if rng.uniform() < 0.5 else 0.0]
+            b["axis_y"] = [0, 1, 0, float(k)]
+            b["surface_color"] = [0.2, 0.17, 0.13, 0.8]
+            b["surface_params"] = [7.0, 2.0, 0.0, 0.0]
+            if rng.uniform() < 0.5:
+                pole = direction(rng.uniform(-180.0, 180.0), rng.uniform(20.0, 90.0))
+                b["ring_plane"] = [*pole, 1.2]
+                b["ring_params"] = [2.3, rng.uniform(0.2, 1.2), 1.0 if rng.uniform() < 0.5 else 0.0, 0.0]
+                b["ring_color"] = [0.7, 0.66, 0.6, 0.0]
+        extra.append(place(b, direction(rng.uniform(-60.0, 60.0), rng.uniform(2.0, 50.0)) * dist))
+    f.bodies = bodies(sun, host, *extra)
+    air = rng.uniform() < 1.0 / 3.0
+    if air:
+        au = METRES_PER_AU
+        g = f.globals
+        g["atmo_center"] = [*split(host_c)[0], r_host]
+        g["atmo_rayleigh"] = [5.8e-6 * au, 13.5e-6 * au, 33.1e-6 * au, 2.0e-5 * au]
+        g["atmo_params"] = [0.76, 8000.0 / au, 80000.0 / au, 2.0]
+    return f, dict(bodies=len(f.bodies), air=air)
+
+
+def system_views(count: int = 96, seed: int = 1, min_pixels: int = 128 * 128, max_pixels: int = 1920 * 1080,
+                 spp=(4, 8, 16, 32, 64), bounces: int = 4):
+    """The batched-serving workload: ``count`` random systems
+    (``random_system``) with pixel counts log-uniform in [min_pixels,
+    max_pixels] at aspect 1, 4:3 or 16:9 and samples per pass from ``spp``,
+    all from one PCG seeded by ``seed``."""
+    rng = Pcg(seed)
+    aspects = (1.0, 4.0 / 3.0, 16.0 / 9.0)
+    views = []
+    for _ in range(count):
+        pixels = math.exp(rng.uniform(math.log(min_pixels), math.log(max_pixels)))
+        aspect = aspects[min(2, int(rng.uniform(0.0, 3.0)))]
+        height = max(8, int(round(math.sqrt(pixels / aspect))))
+        width = max(8, int(round(height * aspect)))
+        samples = int(spp[min(len(spp) - 1, int(rng.uniform(0.0, len(spp))))])
+        f, info = random_system(width, height, rng)
+        views.append(dict(kind="air" if info["air"] else "airless", frame=f, terrain=None, width=width,
+                          height=height, spp=samples, bounces=bounces, bodies=info["bodies"]))
+    return views

@@ -118,6 +118,15 @@ typedef struct sg_options {
     float milky_way;
 } sg_options;
 
+/* src/pathtracer.rs:54-66 `Exposure`; WGSL `Exposure` src/shaders/exposure.wgsl:29-35 */
+typedef struct sg_exposure {
+    float value;      /* eased automatic exposure (updated on the device) */
+    float key;        /* target luminance of the metered percentile */
+    float percentile; /* which luminance percentile to meter (0..1) */
+    float enabled;    /* 1 while automatic exposure is on */
+    float bias;       /* exposure compensation in stops */
+} sg_exposure;
+
 /* src/renderer.rs:50-56 `Frame` (labels are UI-only and not taken). */
 typedef struct sg_frame {
     const sg_globals *globals;
@@ -184,10 +193,27 @@ int sg_set_exposure_controls(sg_ctx *ctx, int32_t auto_exposure, float bias);
 int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint32_t cells);
 /* PathTracer::encode (src/pathtracer.rs:789): queue one progressive pass. */
 int sg_encode(sg_ctx *ctx, const sg_frame *frame);
+/* Display-only exposure (src/pathtracer.rs:883-906,981-1001). Every sg_encode
+ * publishes the exposure controls; with auto exposure it then meters the
+ * central rectangle into a 256-bin luminance histogram and eases `value`
+ * towards key / percentile (src/shaders/exposure.wgsl). Batched passes do not
+ * meter. sg_read_exposure returns the state and, optionally, the histogram
+ * (256 counts) of the last metering. */
+int sg_read_exposure(sg_ctx *ctx, sg_exposure *state, uint32_t *histogram);
+/* PathTracer::display (src/pathtracer.rs:1010, src/shaders/display.wgsl:62-97):
+ * the exposed, ACES-mapped image at out_width x out_height, bilinear from the
+ * trace resolution, as linear display values (rgba f32, a = 1) or through the
+ * sRGB transfer function as 8-bit rgba. Synchronizes the context stream. */
+int sg_display(sg_ctx *ctx, uint32_t out_width, uint32_t out_height, float *rgba);
+int sg_display_srgb8(sg_ctx *ctx, uint32_t out_width, uint32_t out_height, uint8_t *rgba);
 /* Read-back of `accumulation` / `backdrop_accumulation` (vec4 f32 per pixel,
  * src/pathtracer_tests.rs:160-182). Synchronizes the context stream. */
 int sg_read_accumulation(sg_ctx *ctx, float *rgba, uint64_t pixel_count);
 int sg_read_backdrop(sg_ctx *ctx, float *rgba, uint64_t pixel_count);
+/* Restore the images (for example from a checkpoint of a progressive
+ * render); the history is unchanged. No reference counterpart. */
+int sg_write_accumulation(sg_ctx *ctx, const float *rgba, uint64_t pixel_count);
+int sg_write_backdrop(sg_ctx *ctx, const float *rgba, uint64_t pixel_count);
 /* Per-pixel running mean of squared radiance (rgb, sample count in w); only
  * kept after sg_enable_moments(ctx, 1). The reference keeps means only; the
  * second moment is what tile-wise Monte Carlo error bars need. */

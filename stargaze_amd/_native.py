@@ -33,6 +33,11 @@ EXPORTS = (
     "sg_read_backdrop",
     "sg_enable_moments",
     "sg_read_moments",
+    "sg_write_accumulation",
+    "sg_write_backdrop",
+    "sg_read_exposure",
+    "sg_display",
+    "sg_display_srgb8",
     "sg_synchronize",
     "sg_set_profiling",
     "sg_kernel_time",
@@ -136,6 +141,11 @@ def load() -> ctypes.CDLL:
         "sg_read_backdrop": ([P, P, U64], I32),
         "sg_enable_moments": ([P, I32], I32),
         "sg_read_moments": ([P, P, U64], I32),
+        "sg_write_accumulation": ([P, P, U64], I32),
+        "sg_write_backdrop": ([P, P, U64], I32),
+        "sg_read_exposure": ([P, P, P], I32),
+        "sg_display": ([P, U32, U32, P], I32),
+        "sg_display_srgb8": ([P, U32, U32, P], I32),
         "sg_synchronize": ([P], I32),
         "sg_set_profiling": ([P, I32], I32),
         "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),

@@ -11,7 +11,12 @@ import sys
 from pathlib import Path
 
 PACKAGE = Path(__file__).resolve().parent
-SOURCE = PACKAGE / "csrc" / "sg_kernels.hip"
+CSRC = PACKAGE / "csrc"
+# The host library and the kernel translation units. Each kernel unit
+# compiles the integrator for its own kernels, so they build in parallel.
+SOURCES = [CSRC / "sg_library.hip", *sorted(CSRC.glob("sg_k_*.hip"))]
+SOURCE = SOURCES[0]
+OBJECTS = PACKAGE / "_build"
 LIBRARY = PACKAGE / "libstargaze_amd.so"
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
@@ -20,22 +25,33 @@ FLAGS = [
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # Keep the compiler from forming packed-FP32 instructions.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I", str(PACKAGE.parent / "include"),
 ]
 
 
-def build(verbose: bool = False) -> Path:
-    tmp = LIBRARY.with_suffix(".so.tmp")
-    cmd = [HIPCC, *FLAGS, "-o", str(tmp), str(SOURCE)]
+def _run(cmd: list[str], verbose: bool) -> None:
     if verbose:
         print(" ".join(cmd), flush=True)
     proc = subprocess.run(cmd, capture_output=True, text=True)
     if proc.returncode != 0:
         sys.stderr.write(proc.stdout + proc.stderr)
-        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}")
+        raise RuntimeError(f"hipcc failed with exit code {proc.returncode}: {cmd[-1]}")
+
+
+def build(verbose: bool = False, jobs: int | None = None) -> Path:
+    """Compile every translation unit (in parallel) and link the library."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    OBJECTS.mkdir(exist_ok=True)
+    objects = [OBJECTS / (src.stem + ".o") for src in SOURCES]
+    jobs = jobs or min(len(SOURCES), max(1, len(os.sched_getaffinity(0))))
+    with ThreadPoolExecutor(max_workers=jobs) as pool:
+        list(pool.map(lambda so: _run([HIPCC, *FLAGS, "-c", "-o", str(so[1]), str(so[0])], verbose),
+                      zip(SOURCES, objects)))
+    tmp = LIBRARY.with_suffix(".so.tmp")
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)], verbose)
     os.replace(tmp, LIBRARY)
     return LIBRARY
 

@@ -1,0 +1,154 @@
+// Launch interface between the host library (sg_library.hip) and the kernel
+// translation units (sg_k_*.hip): argument blocks and kernel declarations.
+// Each kernel is defined in one translation unit; the host launches it
+// through these declarations.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "sg_integrator.h"
+
+namespace sgk {
+using namespace sg;
+
+constexpr uint32_t kTraceBlock = 256;
+
+// Register budget of the heavy wavefront kernels, as waves per SIMD.
+#ifndef SG_HEAVY_WAVES
+#define SG_HEAVY_WAVES 1
+#endif
+#define SG_HEAVY __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_HEAVY_WAVES, 8)))
+
+struct TraceArgs {
+    float4 *accumulation;
+    float4 *backdrop;
+    float4 *moments;       // null unless second moments are kept
+    float4 *records;       // per path sample (pixel * count + s): radiance rgb, ring-cap scale; or null
+    uint32_t width, height;
+    uint32_t count;        // samples this pass (counts.z)
+    uint32_t accumulated;  // samples already in the image (counts.y)
+    uint32_t sequence;     // noise sequence (sampling.x)
+    uint32_t has_air;      // per-sample dispatch semantics
+    uint32_t pixels_per_block;
+};
+
+// ---------------------------------------------------------------------------
+// Wavefront integrator: the loop of trace() run as one kernel per step over
+// a queue of live paths (structure-of-arrays state, wave64 ballot
+// compaction), then an ordered per-pixel accumulation.
+// ---------------------------------------------------------------------------
+struct WaveArgs {
+    // Path state, indexed by path id within the chunk.
+    double4 *ray_o;    // offset.xyz, anchor bits in .w
+    float4 *ray_d;     // direction.xyz, previous_pdf
+    double4 *prev_o;   // previous vertex offset.xyz, anchor bits
+    float4 *thr;       // throughput.xyz, ring_cap
+    float4 *rad;       // radiance.xyz, shine_body bits
+    float4 *rbase;     // ring_base.xyz, flags | depth << 8 | null_crossings << 16
+    uint32_t *rng;
+    // Hit records from path_extend.
+    float4 *h0;        // hit.distance, hit.index bits, disc_scale, ring.distance
+    float4 *h1;        // hit.normal.xyz, ring.index bits
+    float4 *h2;        // ring.point.xyz
+    // Per-path results.
+    float4 *value;
+    float4 *backdrop;
+    float4 *records;   // per path sample of the pass (see TraceArgs), or null
+    uint32_t width, height;
+    uint32_t count, accumulated, sequence, has_air;
+    uint32_t pixel_begin, paths;
+};
+
+// ---------------------------------------------------------------------------
+// Batched views: one persistent launch renders a progressive pass of many
+// views (own scene, resolution, sample count). The views' pixel blocks are
+// laid end to end in table order; workgroups pull runs of blocks from an
+// atomic counter and trace each with the fused integrator's trace_block,
+// reading the view's scene from device memory.
+// ---------------------------------------------------------------------------
+constexpr uint32_t kMaxBatchViews = 1024;
+
+struct BatchView {
+    Scene scene;
+    TraceArgs ta;          // pixels_per_block = workgroup size / samples
+    uint32_t block_begin;  // first block of this view in the pass
+    uint32_t blocks;
+    uint32_t view;         // index of the view in the batch
+    uint32_t pad;
+};
+
+struct BatchArgs {
+    BatchView *table;             // views with samples to take, in launch order
+    uint32_t entries;
+    uint32_t total_blocks;
+    uint32_t blocks_per_pull;
+    uint32_t *counter;            // [0] next block; [1 + e] blocks left of entry e
+    unsigned long long *done;     // [e] clock at which entry e finished
+    unsigned long long *clock;    // [0] first workgroup start, [1] last workgroup end
+};
+
+// ---------------------------------------------------------------------------
+// Adjoint of one progressive pass (no reference counterpart). Lane layout as
+// k_trace; each lane takes its path sample's primal (recorded, or rerun) and replays it (replay_path),
+// adding d(loss)/d(parameters) into a workgroup copy of the gradient that is
+// flushed to the device gradient once, at the end of the persistent loop.
+// ---------------------------------------------------------------------------
+struct AdjointArgs {
+    TraceArgs ta;            // pass layout; accumulation/backdrop receive the pass mean (or null)
+    const float4 *recorded;  // the pass's recorded primal outputs (skip the primal runs), or null
+    const float *d_image;    // d(loss)/d(pass mean), rgb per pixel
+    float *gradient;         // GRAD_COUNT
+    uint32_t blocks;
+};
+
+// ---------------------------------------------------------------------------
+// Display-only exposure and tone mapping (src/shaders/exposure.wgsl,
+// src/shaders/display.wgsl).
+// ---------------------------------------------------------------------------
+constexpr uint32_t kHistogramBins = 256;  // exposure.wgsl:41
+constexpr float kLogMin = -24.0f, kLogMax = 24.0f;
+constexpr float kMeterFloor = 1.0e-3f, kMinExposure = 1.0e-8f, kMaxExposure = 1.0e8f;
+
+struct MeterArgs {
+    const float4 *accumulation;
+    uint32_t width, height;        // trace size
+    uint32_t origin_x, origin_y;   // metered rectangle: [origin, size - origin)
+    uint32_t meter_w, meter_h;
+    uint32_t *histogram;           // kHistogramBins
+};
+
+struct DisplayArgs {
+    const float4 *accumulation, *backdrop;
+    uint32_t width, height;        // trace size
+    uint32_t out_w, out_h;         // display size
+    float manual_exposure;         // viewport.z of the last pass
+    float milky_way;               // background.x
+    const sg_exposure *state;
+    float4 *out_linear;            // or null
+    uchar4 *out_srgb;              // or null
+};
+
+// Kernels (definitions in sg_k_*.hip).
+__global__ void k_trace(Scene sc, TraceArgs ta);                                       // sg_k_trace.hip
+template <uint32_t B> __global__ void k_batch(BatchArgs ba);                            // B = 64: sg_k_trace.hip, 128/256: sg_k_batch.hip
+__global__ void k_batch_reset(BatchArgs ba);                                           // sg_k_batch.hip
+__global__ void k_batch_advance(BatchArgs ba);                                         // sg_k_batch.hip
+__global__ void k_generate(Scene sc, WaveArgs wa, uint32_t *queue);                     // sg_k_wave_a.hip
+__global__ void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);     // sg_k_wave_a.hip
+__global__ void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n); // sg_k_wave_a.hip
+__global__ void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n, uint32_t *next,
+                        uint32_t *next_count);                                          // sg_k_wave_b.hip
+__global__ void k_tail(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);       // sg_k_wave_b.hip
+__global__ void k_accumulate(WaveArgs wa, float4 *accumulation, float4 *backdrop, float4 *moments,
+                             uint32_t pixels);                                          // sg_k_wave_b.hip
+__global__ void k_fill(float *p, uint32_t n, float v);                                  // sg_k_adjoint.hip
+__global__ void k_adjoint(Scene sc, AdjointArgs aa);                                    // sg_k_adjoint.hip
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v);                        // sg_k_misc.hip
+__global__ void k_meter(MeterArgs ma);                                                  // sg_k_misc.hip
+__global__ void k_resolve(const uint32_t *histogram, sg_exposure *state);               // sg_k_misc.hip
+__global__ void k_display(DisplayArgs da);                                              // sg_k_misc.hip
+__global__ void k_terrain_cache(Scene sc, float *terrain);                              // sg_k_misc.hip
+__global__ void k_sky_cache(Scene sc, float4 *sky);                                     // sg_k_misc.hip
+__global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
+                        sg_probe_result *out);                                          // sg_k_misc.hip
+
+}  // namespace sgk

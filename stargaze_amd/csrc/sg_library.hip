@@ -1,14 +1,18 @@
-// HIP kernels and the C ABI of the MI355X path tracer.
+// Host library and C ABI of the MI355X path tracer (include/stargaze_amd.h).
 //
-// Launch structure (per progressive pass, one HIP stream per context):
+// Launch structure (per progressive pass, one HIP stream per context); the
+// kernels live in sg_k_*.hip and are declared in sg_launch.h:
 //   k_terrain_cache   once per terrain upload / host radius change
 //                     (cache_terrain_geometry, src/shaders/terrain.wgsl:242-285)
 //   k_sky_cache       on a fresh accumulation (cache_surface_sky, forest.wgsl:3-17)
-//   k_trace           every pass: W*H*count path samples, one lane per
-//                     (pixel, sample); lanes of one pixel share a wavefront,
-//                     and the per-pixel mean and running-average blend of
-//                     src/shaders/pathtrace.wgsl:1645-1675 are formed in LDS in
-//                     the reference's summation order.
+//   k_trace           the fused integrator: W*H*count path samples, one lane
+//                     per (pixel, sample); lanes of one pixel share a
+//                     wavefront, and the per-pixel mean and running-average
+//                     blend of src/shaders/pathtrace.wgsl:1645-1675 are formed
+//                     in LDS in the reference's summation order
+//   k_generate .. k_accumulate   the wavefront integrator (default)
+//   k_meter, k_resolve           automatic exposure, when enabled
+// Batched views run k_batch; the adjoint runs k_adjoint.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -18,631 +22,10 @@
 #include <vector>
 
 #include "sg_bvh.h"
-#include "sg_integrator.h"
+#include "sg_launch.h"
 
 using namespace sg;
-
-namespace {
-
-constexpr uint32_t kTraceBlock = 256;
-
-// Register budget of the heavy wavefront kernels, as waves per SIMD.
-#ifndef SG_HEAVY_WAVES
-#define SG_HEAVY_WAVES 1
-#endif
-#define SG_HEAVY __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_HEAVY_WAVES, 8)))
-
-struct TraceArgs {
-    float4 *accumulation;
-    float4 *backdrop;
-    float4 *moments;       // null unless second moments are kept
-    float4 *records;       // per path sample (pixel * count + s): radiance rgb, ring-cap scale; or null
-    uint32_t width, height;
-    uint32_t count;        // samples this pass (counts.z)
-    uint32_t accumulated;  // samples already in the image (counts.y)
-    uint32_t sequence;     // noise sequence (sampling.x)
-    uint32_t has_air;      // per-sample dispatch semantics
-    uint32_t pixels_per_block;
-};
-
-// The per-pixel epilogue of `main` (src/shaders/pathtrace.wgsl:1653-1674):
-// ordered mean of this pass's samples and the running-average blend, or,
-// with air, one blend per sample (src/pathtracer.rs:861-882,971-976).
-__device__ __forceinline__ void blend_pixel(const TraceArgs &ta, uint32_t pixel, const float4 *values,
-                                            const float4 *backdrops) {
-    const uint32_t base = 0;
-
-    float4 mean, bmean;
-    float4 m2 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!ta.has_air) {
-        // One dispatch of counts.z samples (src/shaders/pathtrace.wgsl:1653-1674).
-        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), bsum = sum, sq = sum;
-        for (uint32_t k = 0; k < ta.count; ++k) {
-            float4 v = values[base + k], b = backdrops[base + k];
-            sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
-            bsum.x += b.x; bsum.y += b.y; bsum.z += b.z;
-            sq.x += v.x * v.x; sq.y += v.y * v.y; sq.z += v.z * v.z;
-        }
-        const float n = (float)ta.count;
-        mean = make_float4(sum.x / n, sum.y / n, sum.z / n, sum.w / n);
-        bmean = make_float4(bsum.x / n, bsum.y / n, bsum.z / n, 0.f);
-        m2 = make_float4(sq.x / n, sq.y / n, sq.z / n, 0.f);
-        const uint32_t new_count = ta.accumulated + ta.count;
-        if (ta.accumulated > 0u) {
-            const float w = n / (float)new_count;
-            float4 old = ta.accumulation[pixel];
-            mean = make_float4(old.x + (mean.x - old.x) * w, old.y + (mean.y - old.y) * w,
-                               old.z + (mean.z - old.z) * w, old.w + (mean.w - old.w) * w);
-            float4 ob = ta.backdrop[pixel];
-            bmean = make_float4(ob.x + (bmean.x - ob.x) * w, ob.y + (bmean.y - ob.y) * w,
-                                ob.z + (bmean.z - ob.z) * w, 0.f);
-            if (ta.moments) {
-                float4 om = ta.moments[pixel];
-                m2 = make_float4(om.x + (m2.x - om.x) * w, om.y + (m2.y - om.y) * w, om.z + (m2.z - om.z) * w, 0.f);
-            }
-        }
-        m2.w = (float)new_count;
-    } else {
-        // One dispatch per sample with counts.z = 1 (src/pathtracer.rs:861-882,971-976).
-        if (ta.accumulated > 0u) {
-            mean = ta.accumulation[pixel];
-            bmean = ta.backdrop[pixel];
-            if (ta.moments) m2 = ta.moments[pixel];
-        }
-        for (uint32_t k = 0; k < ta.count; ++k) {
-            float4 v = values[base + k], b = backdrops[base + k];
-            float4 v2 = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0.f);
-            const uint32_t old_count = ta.accumulated + k;
-            if (old_count > 0u) {
-                const float w = 1.0f / (float)(old_count + 1u);
-                mean = make_float4(mean.x + (v.x - mean.x) * w, mean.y + (v.y - mean.y) * w,
-                                   mean.z + (v.z - mean.z) * w, mean.w + (v.w - mean.w) * w);
-                bmean = make_float4(bmean.x + (b.x - bmean.x) * w, bmean.y + (b.y - bmean.y) * w,
-                                    bmean.z + (b.z - bmean.z) * w, 0.f);
-                m2 = make_float4(m2.x + (v2.x - m2.x) * w, m2.y + (v2.y - m2.y) * w, m2.z + (v2.z - m2.z) * w, 0.f);
-            } else {
-                mean = v;
-                bmean = make_float4(b.x, b.y, b.z, 0.f);
-                m2 = v2;
-            }
-        }
-        bmean.w = 0.f;
-        m2.w = (float)(ta.accumulated + ta.count);
-    }
-    ta.accumulation[pixel] = mean;
-    ta.backdrop[pixel] = bmean;
-    if (ta.moments) ta.moments[pixel] = m2;
-}
-// One block of pixel samples: lane t traces sample t % count of pixel
-// block * pixels_per_block + t / count, then one lane per pixel forms the
-// ordered mean and blend. Every lane of the workgroup must call it.
-__device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta, uint32_t block, float4 *s_value,
-                                            float4 *s_backdrop) {
-    const uint32_t t = threadIdx.x;
-    const uint32_t lp = t / ta.count;
-    const uint32_t s = t - lp * ta.count;
-    const uint32_t pixel = block * ta.pixels_per_block + lp;
-    const uint32_t pixels = ta.width * ta.height;
-    const bool active = lp < ta.pixels_per_block && pixel < pixels;
-    float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
-    f3 backdrop = f3s(0.f);
-    if (active) {
-        const uint32_t x = pixel % ta.width, y = pixel / ta.width;
-        const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
-        uint32_t rng = path_seed(pixel, ta.sequence + s);
-        f3 direction = camera_direction(sc, x, y, rng);
-        PathState st;
-        path_begin(st, direction, rng, accumulated);
-        run_path(sc, st);
-        PathResult r = path_finish(st);
-        value = r.value;
-        backdrop = r.backdrop;
-        if (ta.records)
-            ta.records[(size_t)pixel * ta.count + s] = make_float4(value.x, value.y, value.z, ring_cap_scale(st));
-    }
-    s_value[t] = value;
-    s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
-    __syncthreads();
-    if (!active || s != 0u) return;
-    blend_pixel(ta, pixel, s_value + lp * ta.count, s_backdrop + lp * ta.count);
-}
-
-__global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
-    __shared__ float4 s_value[kTraceBlock];
-    __shared__ float4 s_backdrop[kTraceBlock];
-    trace_block(sc, ta, blockIdx.x, s_value, s_backdrop);
-}
-
-
-// ---------------------------------------------------------------------------
-// Wavefront integrator: the loop of trace() run as one kernel per step over
-// a queue of live paths (structure-of-arrays state, wave64 ballot
-// compaction), then an ordered per-pixel accumulation.
-// ---------------------------------------------------------------------------
-struct WaveArgs {
-    // Path state, indexed by path id within the chunk.
-    double4 *ray_o;    // offset.xyz, anchor bits in .w
-    float4 *ray_d;     // direction.xyz, previous_pdf
-    double4 *prev_o;   // previous vertex offset.xyz, anchor bits
-    float4 *thr;       // throughput.xyz, ring_cap
-    float4 *rad;       // radiance.xyz, shine_body bits
-    float4 *rbase;     // ring_base.xyz, flags | depth << 8 | null_crossings << 16
-    uint32_t *rng;
-    // Hit records from path_extend.
-    float4 *h0;        // hit.distance, hit.index bits, disc_scale, ring.distance
-    float4 *h1;        // hit.normal.xyz, ring.index bits
-    float4 *h2;        // ring.point.xyz
-    // Per-path results.
-    float4 *value;
-    float4 *backdrop;
-    float4 *records;   // per path sample of the pass (see TraceArgs), or null
-    uint32_t width, height;
-    uint32_t count, accumulated, sequence, has_air;
-    uint32_t pixel_begin, paths;
-};
-
-SG_FN double bits_d(uint32_t u) { return (double)u; }
-SG_FN uint32_t d_bits(double d) { return (uint32_t)d; }
-
-__device__ __forceinline__ uint32_t path_accumulated(const WaveArgs &wa, uint32_t id) {
-    return wa.has_air ? wa.accumulated + id % wa.count : wa.accumulated;
-}
-__device__ __forceinline__ void load_ray(const double4 &o, const float4 &d, Ray &r) {
-    r.offset = d3{o.x, o.y, o.z};
-    r.anchor = d_bits(o.w);
-    r.direction = f3{d.x, d.y, d.z};
-}
-__device__ __forceinline__ void load_state(const WaveArgs &wa, uint32_t id, PathState &st) {
-    float4 d = wa.ray_d[id];
-    load_ray(wa.ray_o[id], d, st.ray);
-    st.previous_pdf = d.w;
-    double4 po = wa.prev_o[id];
-    st.previous_vertex.offset = d3{po.x, po.y, po.z};
-    st.previous_vertex.anchor = d_bits(po.w);
-    st.previous_vertex.direction = st.ray.direction;
-    float4 t = wa.thr[id];
-    st.throughput = f3{t.x, t.y, t.z};
-    st.ring_cap = t.w;
-    float4 r = wa.rad[id];
-    st.radiance = f3{r.x, r.y, r.z};
-    st.shine_body = __float_as_uint(r.w);
-    float4 b = wa.rbase[id];
-    st.ring_base = f3{b.x, b.y, b.z};
-    uint32_t packed = __float_as_uint(b.w);
-    st.flags = packed & 255u;
-    st.depth = (packed >> 8) & 255u;
-    st.null_crossings = packed >> 16;
-    st.rng = wa.rng[id];
-    st.backdrop = f3s(0.0f);
-    st.accumulated = path_accumulated(wa, id);
-}
-__device__ __forceinline__ void store_state(const WaveArgs &wa, uint32_t id, const PathState &st) {
-    wa.ray_o[id] = make_double4(st.ray.offset.x, st.ray.offset.y, st.ray.offset.z, bits_d(st.ray.anchor));
-    wa.ray_d[id] = make_float4(st.ray.direction.x, st.ray.direction.y, st.ray.direction.z, st.previous_pdf);
-    wa.prev_o[id] = make_double4(st.previous_vertex.offset.x, st.previous_vertex.offset.y,
-                                 st.previous_vertex.offset.z, bits_d(st.previous_vertex.anchor));
-    wa.thr[id] = make_float4(st.throughput.x, st.throughput.y, st.throughput.z, st.ring_cap);
-    wa.rad[id] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, __uint_as_float(st.shine_body));
-    uint32_t packed = (st.flags & 255u) | ((st.depth & 255u) << 8) | (st.null_crossings << 16);
-    wa.rbase[id] = make_float4(st.ring_base.x, st.ring_base.y, st.ring_base.z, __uint_as_float(packed));
-    wa.rng[id] = st.rng;
-}
-__device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id, HitRecord &h) {
-    float4 a = wa.h0[id], b = wa.h1[id], c = wa.h2[id];
-    h.hit.distance = a.x;
-    h.hit.index = __float_as_uint(a.y);
-    h.disc_scale = a.z;
-    h.ring.distance = a.w;
-    h.hit.normal = f3{b.x, b.y, b.z};
-    h.ring.index = __float_as_uint(b.w);
-    h.ring.point = f3{c.x, c.y, c.z};
-}
-__device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const PathState &st) {
-    PathResult r = path_finish(st);
-    wa.value[id] = r.value;
-    wa.backdrop[id] = make_float4(r.backdrop.x, r.backdrop.y, r.backdrop.z, 0.f);
-    if (wa.records)
-        wa.records[(size_t)wa.pixel_begin * wa.count + id] =
-            make_float4(r.value.x, r.value.y, r.value.z, ring_cap_scale(st));
-}
-
-__global__ __launch_bounds__(256) void k_generate(Scene sc, WaveArgs wa, uint32_t *queue) {
-    const uint32_t id = blockIdx.x * blockDim.x + threadIdx.x;
-    if (id >= wa.paths) return;
-    const uint32_t pixel = wa.pixel_begin + id / wa.count;
-    const uint32_t s = id % wa.count;
-    const uint32_t x = pixel % wa.width, y = pixel / wa.width;
-    uint32_t rng = path_seed(pixel, wa.sequence + s);
-    f3 direction = camera_direction(sc, x, y, rng);
-    PathState st;
-    path_begin(st, direction, rng, path_accumulated(wa, id));
-    store_state(wa, id, st);
-    queue[id] = id;
-}
-
-__global__ SG_HEAVY void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t id = queue[i];
-    PathState st;
-    load_ray(wa.ray_o[id], wa.ray_d[id], st.ray);
-    uint32_t packed = __float_as_uint(wa.rbase[id].w);
-    st.depth = (packed >> 8) & 255u;
-    st.null_crossings = packed >> 16;
-    st.rng = wa.rng[id];
-    HitRecord h = path_extend(sc, st);
-    wa.rng[id] = st.rng;
-    wa.h0[id] = make_float4(h.hit.distance, __uint_as_float(h.hit.index), h.disc_scale, h.ring.distance);
-    wa.h1[id] = make_float4(h.hit.normal.x, h.hit.normal.y, h.hit.normal.z, __uint_as_float(h.ring.index));
-    wa.h2[id] = make_float4(h.ring.point.x, h.ring.point.y, h.ring.point.z, 0.f);
-}
-
-__global__ SG_HEAVY void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t id = queue[i];
-    PathState st;
-    float4 d = wa.ray_d[id];
-    load_ray(wa.ray_o[id], d, st.ray);
-    float4 t = wa.thr[id], r = wa.rad[id], b = wa.rbase[id];
-    st.throughput = f3{t.x, t.y, t.z};
-    st.radiance = f3{r.x, r.y, r.z};
-    st.flags = __float_as_uint(b.w) & 255u;
-    st.rng = wa.rng[id];
-    st.accumulated = path_accumulated(wa, id);
-    HitRecord h;
-    load_hit(wa, id, h);
-    path_camera_air(sc, st, h);
-    wa.thr[id] = make_float4(st.throughput.x, st.throughput.y, st.throughput.z, t.w);
-    wa.rad[id] = make_float4(st.radiance.x, st.radiance.y, st.radiance.z, r.w);
-    wa.rbase[id] = make_float4(b.x, b.y, b.z, __uint_as_float((__float_as_uint(b.w) & ~255u) | st.flags));
-    wa.rng[id] = st.rng;
-}
-
-__global__ SG_HEAVY void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
-                                               uint32_t *next, uint32_t *next_count) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    bool alive = false;
-    uint32_t id = 0;
-    if (i < n) {
-        id = queue[i];
-        PathState st;
-        load_state(wa, id, st);
-        HitRecord h;
-        load_hit(wa, id, h);
-        // The camera segment was handled by k_camera_air (air on) or has
-        // nothing to add (air off).
-        st.flags &= ~PATH_CAMERA_SEGMENT;
-        alive = path_shade(sc, st, h);
-        if (alive) store_state(wa, id, st);
-        else finish(wa, id, st);
-    }
-    // Wave64 compaction: one atomic per wavefront, lane offsets from the ballot.
-    const uint64_t mask = __ballot(alive);
-    if (mask == 0) return;
-    const uint32_t lane = __lane_id();
-    const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
-    uint32_t base = 0;
-    if (lane == leader) base = atomicAdd(next_count, (uint32_t)__popcll(mask));
-    base = __shfl(base, (int)leader);
-    if (alive) next[base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull))] = id;
-}
-
-// The tail of a chunk: once fewer paths than the compaction threshold are
-// alive, each lane runs its queued path through the remaining loop of
-// trace() in one launch (same steps, same draws).
-__global__ SG_HEAVY void k_tail(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const uint32_t id = queue[i];
-    PathState st;
-    load_state(wa, id, st);
-    run_path(sc, st);
-    finish(wa, id, st);
-}
-
-// Ordered per-pixel blend of the chunk's path results
-// (src/shaders/pathtrace.wgsl:1653-1674; per-sample dispatches when air is on).
-__global__ __launch_bounds__(256) void k_accumulate(WaveArgs wa, float4 *accumulation, float4 *backdrop,
-                                                    float4 *moments, uint32_t pixels) {
-    const uint32_t local = blockIdx.x * blockDim.x + threadIdx.x;
-    if (local >= pixels) return;
-    const uint32_t pixel = wa.pixel_begin + local;
-    const uint32_t base = local * wa.count;
-    float4 mean, bmean;
-    float4 m2 = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (!wa.has_air) {
-        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f), bsum = sum, sq = sum;
-        for (uint32_t k = 0; k < wa.count; ++k) {
-            float4 v = wa.value[base + k], b = wa.backdrop[base + k];
-            sum.x += v.x; sum.y += v.y; sum.z += v.z; sum.w += v.w;
-            bsum.x += b.x; bsum.y += b.y; bsum.z += b.z;
-            sq.x += v.x * v.x; sq.y += v.y * v.y; sq.z += v.z * v.z;
-        }
-        const float nf = (float)wa.count;
-        mean = make_float4(sum.x / nf, sum.y / nf, sum.z / nf, sum.w / nf);
-        bmean = make_float4(bsum.x / nf, bsum.y / nf, bsum.z / nf, 0.f);
-        m2 = make_float4(sq.x / nf, sq.y / nf, sq.z / nf, 0.f);
-        const uint32_t new_count = wa.accumulated + wa.count;
-        if (wa.accumulated > 0u) {
-            const float w = nf / (float)new_count;
-            float4 old = accumulation[pixel];
-            mean = make_float4(old.x + (mean.x - old.x) * w, old.y + (mean.y - old.y) * w,
-                               old.z + (mean.z - old.z) * w, old.w + (mean.w - old.w) * w);
-            float4 ob = backdrop[pixel];
-            bmean = make_float4(ob.x + (bmean.x - ob.x) * w, ob.y + (bmean.y - ob.y) * w, ob.z + (bmean.z - ob.z) * w, 0.f);
-            if (moments) {
-                float4 om = moments[pixel];
-                m2 = make_float4(om.x + (m2.x - om.x) * w, om.y + (m2.y - om.y) * w, om.z + (m2.z - om.z) * w, 0.f);
-            }
-        }
-        m2.w = (float)new_count;
-    } else {
-        if (wa.accumulated > 0u) {
-            mean = accumulation[pixel];
-            bmean = backdrop[pixel];
-            if (moments) m2 = moments[pixel];
-        }
-        for (uint32_t k = 0; k < wa.count; ++k) {
-            float4 v = wa.value[base + k], b = wa.backdrop[base + k];
-            float4 v2 = make_float4(v.x * v.x, v.y * v.y, v.z * v.z, 0.f);
-            const uint32_t old_count = wa.accumulated + k;
-            if (old_count > 0u) {
-                const float w = 1.0f / (float)(old_count + 1u);
-                mean = make_float4(mean.x + (v.x - mean.x) * w, mean.y + (v.y - mean.y) * w,
-                                   mean.z + (v.z - mean.z) * w, mean.w + (v.w - mean.w) * w);
-                bmean = make_float4(bmean.x + (b.x - bmean.x) * w, bmean.y + (b.y - bmean.y) * w,
-                                    bmean.z + (b.z - bmean.z) * w, 0.f);
-                m2 = make_float4(m2.x + (v2.x - m2.x) * w, m2.y + (v2.y - m2.y) * w, m2.z + (v2.z - m2.z) * w, 0.f);
-            } else {
-                mean = v;
-                bmean = make_float4(b.x, b.y, b.z, 0.f);
-                m2 = v2;
-            }
-        }
-        bmean.w = 0.f;
-        m2.w = (float)(wa.accumulated + wa.count);
-    }
-    accumulation[pixel] = mean;
-    backdrop[pixel] = bmean;
-    if (moments) moments[pixel] = m2;
-}
-
-// ---------------------------------------------------------------------------
-// Batched views: one persistent launch renders a progressive pass of many
-// views (own scene, resolution, sample count). The views' pixel blocks are
-// laid end to end in table order; workgroups pull runs of blocks from an
-// atomic counter and trace each with the fused integrator's trace_block,
-// reading the view's scene from device memory.
-// ---------------------------------------------------------------------------
-constexpr uint32_t kMaxBatchViews = 1024;
-
-struct BatchView {
-    Scene scene;
-    TraceArgs ta;          // pixels_per_block = workgroup size / samples
-    uint32_t block_begin;  // first block of this view in the pass
-    uint32_t blocks;
-    uint32_t view;         // index of the view in the batch
-    uint32_t pad;
-};
-
-struct BatchArgs {
-    BatchView *table;             // views with samples to take, in launch order
-    uint32_t entries;
-    uint32_t total_blocks;
-    uint32_t blocks_per_pull;
-    uint32_t *counter;            // [0] next block; [1 + e] blocks left of entry e
-    unsigned long long *done;     // [e] clock at which entry e finished
-    unsigned long long *clock;    // [0] first workgroup start, [1] last workgroup end
-};
-
-__global__ void k_batch_reset(BatchArgs ba) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0) {
-        ba.counter[0] = 0u;
-        ba.clock[0] = ~0ull;
-        ba.clock[1] = 0ull;
-    }
-    if (i < ba.entries) {
-        ba.counter[1 + i] = ba.table[i].blocks;
-        ba.done[i] = 0ull;
-    }
-}
-
-template <uint32_t B>
-__global__ __launch_bounds__(B) void k_batch(BatchArgs ba) {
-    __shared__ float4 s_value[B];
-    __shared__ float4 s_backdrop[B];
-    __shared__ uint32_t s_begin[kMaxBatchViews + 1];
-    __shared__ uint32_t s_first;
-    const uint32_t t = threadIdx.x;
-    for (uint32_t e = t; e < ba.entries; e += B) s_begin[e] = ba.table[e].block_begin;
-    if (t == 0) {
-        s_begin[ba.entries] = ba.total_blocks;
-        atomicMin(&ba.clock[0], (unsigned long long)wall_clock64());
-    }
-    for (;;) {
-        __syncthreads();
-        if (t == 0) s_first = atomicAdd(&ba.counter[0], ba.blocks_per_pull);
-        __syncthreads();
-        const uint32_t first = s_first;
-        if (first >= ba.total_blocks) break;
-        const uint32_t last = umin(first + ba.blocks_per_pull, ba.total_blocks);
-        for (uint32_t block = first; block < last; ++block) {
-            // Entry owning the block: last e with s_begin[e] <= block.
-            uint32_t lo = 0, hi = ba.entries;
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_begin[mid] <= block) lo = mid;
-                else hi = mid;
-            }
-            const uint32_t e = __builtin_amdgcn_readfirstlane(lo);
-            const BatchView &v = ba.table[e];
-            trace_block(v.scene, v.ta, block - s_begin[e], s_value, s_backdrop);
-            __syncthreads();
-            if (t == 0) {
-                __threadfence();
-                if (atomicSub(&ba.counter[1 + e], 1u) == 1u) ba.done[e] = wall_clock64();
-            }
-        }
-    }
-    if (t == 0) atomicMax(&ba.clock[1], (unsigned long long)wall_clock64());
-}
-
-// End of a pass: each entry's accumulated count and noise sequence advance
-// as the host history does, so an unchanged table can be replayed.
-__global__ void k_batch_advance(BatchArgs ba) {
-    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= ba.entries) return;
-    BatchView &v = ba.table[e];
-    v.ta.accumulated += v.ta.count;
-    v.ta.sequence += v.ta.count;
-    v.scene.s.counts[1] += v.ta.count;
-    v.scene.s.sampling[0] += v.ta.count;
-}
-
-// ---------------------------------------------------------------------------
-// Adjoint of one progressive pass (no reference counterpart). Lane layout as
-// k_trace; each lane runs its path sample's primal and replay (adjoint_path),
-// adding d(loss)/d(parameters) into a workgroup copy of the gradient that is
-// flushed to the device gradient once, at the end of the persistent loop.
-// ---------------------------------------------------------------------------
-struct AdjointArgs {
-    TraceArgs ta;            // pass layout; accumulation/backdrop receive the pass mean (or null)
-    const float4 *recorded;  // the pass's recorded primal outputs (skip the primal runs), or null
-    const float *d_image;    // d(loss)/d(pass mean), rgb per pixel
-    float *gradient;         // GRAD_COUNT
-    uint32_t blocks;
-};
-
-__global__ void k_fill(float *p, uint32_t n, float v) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = v;
-}
-
-__global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs aa) {
-    __shared__ float s_grad[GRAD_COUNT];
-    __shared__ float s_density[GRAD_PRIVATE * kTraceBlock];  // lane-private, slot-major
-    __shared__ float4 s_value[kTraceBlock];
-    __shared__ float4 s_backdrop[kTraceBlock];
-    const uint32_t t = threadIdx.x;
-    for (uint32_t i = t; i < GRAD_COUNT; i += blockDim.x) s_grad[i] = 0.0f;
-    for (uint32_t b = 0; b < GRAD_PRIVATE; ++b) s_density[b * blockDim.x + t] = 0.0f;
-    __syncthreads();
-    const TraceArgs &ta = aa.ta;
-    const uint32_t lp = t / ta.count;
-    const uint32_t s = t - lp * ta.count;
-    const uint32_t pixels = ta.width * ta.height;
-    const float inv_count = 1.0f / (float)ta.count;
-    for (uint32_t block = blockIdx.x; block < aa.blocks; block += gridDim.x) {
-        const uint32_t pixel = block * ta.pixels_per_block + lp;
-        const bool active = lp < ta.pixels_per_block && pixel < pixels;
-        float4 value = make_float4(0.f, 0.f, 0.f, 0.f);
-        f3 backdrop = f3s(0.f);
-        if (active) {
-            const uint32_t x = pixel % ta.width, y = pixel / ta.width;
-            const uint32_t accumulated = ta.has_air ? ta.accumulated + s : ta.accumulated;
-            uint32_t rng = path_seed(pixel, ta.sequence + s);
-            f3 direction = camera_direction(sc, x, y, rng);
-            const float *d = aa.d_image + 3u * (size_t)pixel;
-            f3 w = f3{d[0], d[1], d[2]} * inv_count;
-            if (aa.recorded) {
-                const float4 rec = aa.recorded[(size_t)pixel * ta.count + s];
-                replay_path(sc, direction, rng, accumulated, w, f3{rec.x, rec.y, rec.z}, rec.w, s_grad,
-                            s_density + t, blockDim.x);
-            } else {
-                PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad, s_density + t, blockDim.x);
-                value = r.value;
-                backdrop = r.backdrop;
-            }
-        }
-        if (ta.accumulation) {
-            s_value[t] = value;
-            s_backdrop[t] = make_float4(backdrop.x, backdrop.y, backdrop.z, 0.f);
-            __syncthreads();
-            if (active && s == 0u) {
-                TraceArgs mean = ta;
-                mean.accumulated = 0u;
-                mean.moments = nullptr;
-                blend_pixel(mean, pixel, s_value + lp * ta.count, s_backdrop + lp * ta.count);
-            }
-            __syncthreads();
-        }
-    }
-    __syncthreads();
-    for (uint32_t i = t; i < GRAD_COUNT; i += blockDim.x)
-        if (s_grad[i] != 0.0f) atomicAdd(&aa.gradient[i], s_grad[i]);
-}
-
-// cache_terrain_geometry (src/shaders/terrain.wgsl:242-285), one lane per index.
-__global__ void k_terrain_cache(Scene sc, float *terrain) {
-    terrain_cache_entry(sc, terrain, (uint64_t)blockIdx.x * blockDim.x + threadIdx.x);
-}
-
-// cache_surface_sky (src/shaders/forest.wgsl:3-17): 16x8 texels.
-__global__ void k_sky_cache(Scene sc, float4 *sky) {
-    const uint32_t index = threadIdx.x;
-    if (index >= 128u) return;
-    f3 light = sky_cache_texel(sc, index % 16u, index / 16u);
-    sky[index] = make_float4(light.x, light.y, light.z, 0.f);
-}
-
-// Probe kernel: production helpers on caller-supplied rays.
-__global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint32_t count, sg_probe_result *out) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= count) return;
-    const sg_probe_ray pr = rays[i];
-    Ray ray{d3{pr.offset[0], pr.offset[1], pr.offset[2]}, f3{pr.direction[0], pr.direction[1], pr.direction[2]},
-            pr.anchor};
-    sg_probe_result r;
-    memset(&r, 0, sizeof(r));
-    r.index = SG_MISS;
-    r.distance = 1.0e30f;
-    switch (kind) {
-    case SG_PROBE_SPHERE:
-    case SG_PROBE_CLOSEST:
-    case SG_PROBE_LANDSCAPE: {
-        Hit h = kind == SG_PROBE_SPHERE ? sphere_hit(sc, ray, pr.target)
-                                        : (kind == SG_PROBE_CLOSEST ? closest_hit(sc, ray) : landscape_hit(sc, ray));
-        r.distance = h.distance;
-        r.index = h.index;
-        r.normal[0] = h.normal.x; r.normal[1] = h.normal.y; r.normal[2] = h.normal.z;
-        break;
-    }
-    case SG_PROBE_RING: {
-        RingHit h = ring_hit(sc, ray, pr.target);
-        r.distance = h.distance;
-        r.index = h.index;
-        r.extra[0] = h.point.x; r.extra[1] = h.point.y; r.extra[2] = h.point.z;
-        if (h.index != SG_MISS) {
-            r.normal[0] = ring_transmission(sc, ray, h);
-            r.normal[1] = ring_optics(sc, h.index, h.point).w;
-        }
-        break;
-    }
-    case SG_PROBE_VISIBILITY:
-        r.distance = light_visibility(sc, ray, pr.target);
-        r.index = 0;
-        break;
-    case SG_PROBE_TERRAIN: {
-        f3 o = tof(ray.offset);
-        float4 h = terrain_query(sc, o, ray.direction, 1.0e30f, pr.target != 0u);
-        r.distance = h.w;
-        r.index = h.w < 1.0e29f ? SG_TERRAIN_ID : SG_MISS;
-        r.normal[0] = h.x; r.normal[1] = h.y; r.normal[2] = h.z;
-        break;
-    }
-    case SG_PROBE_CONE: {
-        r.distance = cone_width(sc, ray, pr.anchor == SG_MISS ? 0u : pr.anchor);
-        break;
-    }
-    default:
-        break;
-    }
-    out[i] = r;
-}
-
-}  // namespace
+using namespace sgk;
 
 // ---------------------------------------------------------------------------
 // Host library: the PathTracer state machine of src/pathtracer.rs:164-1009.
@@ -721,6 +104,12 @@ struct sg_ctx {
     bool record_on = false;
     float4 *records = nullptr;
     uint64_t record_capacity = 0;
+    // Display-only exposure state and luminance histogram on the device; the
+    // manual exposure and Milky Way gain of the last pass (viewport.z, background.x).
+    sg_exposure *exposure = nullptr;
+    uint32_t *histogram = nullptr;
+    float display_exposure = 1.0f;
+    float display_milky_way = 0.0f;
     // Adjoint buffers: d(loss)/d(image) rgb, gradient, recomputed pass mean.
     float *adj_image = nullptr;
     float *adj_gradient = nullptr;
@@ -1123,6 +512,43 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     return SG_OK;
 }
 
+// Publish the exposure controls (src/pathtracer.rs:883-906): while automatic
+// metering stays on, only the fields after the device-eased `value`.
+int publish_exposure(sg_ctx *ctx) {
+    const sg_options &o = ctx->options;
+    sg_exposure e{o.exposure, o.auto_key, o.auto_percentile, o.auto_exposure ? 1.0f : 0.0f, o.ev_bias};
+    if (o.auto_exposure && ctx->exposure_was_enabled)
+        SG_HIP(ctx, hipMemcpyAsync(&ctx->exposure->key, &e.key, 4 * sizeof(float), hipMemcpyHostToDevice,
+                                   ctx->stream));
+    else
+        SG_HIP(ctx, hipMemcpyAsync(ctx->exposure, &e, sizeof(e), hipMemcpyHostToDevice, ctx->stream));
+    ctx->display_exposure = o.exposure;
+    ctx->display_milky_way = o.milky_way;
+    return SG_OK;
+}
+
+// Meter the central rectangle and resolve the eased exposure
+// (src/pathtracer.rs:981-1001), also once the sample limit is reached.
+int meter_exposure(sg_ctx *ctx) {
+    MeterArgs ma;
+    ma.accumulation = ctx->accumulation;
+    ma.width = ctx->width;
+    ma.height = ctx->height;
+    ma.origin_x = ctx->width / 4u;
+    ma.origin_y = ctx->height / 4u;
+    ma.meter_w = ctx->width - 2u * ma.origin_x;
+    ma.meter_h = ctx->height - 2u * ma.origin_y;
+    ma.histogram = ctx->histogram;
+    hipLaunchKernelGGL(k_fill_u32, dim3(1), dim3(kHistogramBins), 0, ctx->stream, ctx->histogram, kHistogramBins, 0u);
+    SG_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_meter, dim3((ma.meter_w + 15u) / 16u, (ma.meter_h + 15u) / 16u), dim3(16, 16), 0,
+                       ctx->stream, ma);
+    SG_HIP(ctx, hipGetLastError());
+    hipLaunchKernelGGL(k_resolve, dim3(1), dim3(kHistogramBins), 0, ctx->stream, ctx->histogram, ctx->exposure);
+    SG_HIP(ctx, hipGetLastError());
+    return SG_OK;
+}
+
 // The history bookkeeping after a pass (src/pathtracer.rs:1002-1008).
 void finish_pass(sg_ctx *ctx, uint32_t count) {
     const sg_options &o = ctx->options;
@@ -1163,6 +589,17 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
     if (hipMalloc(&ctx->bodies, SG_BODY_CAPACITY * sizeof(Body)) != hipSuccess) return bail(SG_E_DEVICE);
     if (ensure_objects(ctx, SG_GROUND_CAPACITY) != SG_OK) return bail(SG_E_DEVICE);
     if (hipMalloc(&ctx->sky_light, SG_SKY_TEXELS * sizeof(float4)) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipMalloc(&ctx->exposure, sizeof(sg_exposure)) != hipSuccess ||
+        hipMalloc(&ctx->histogram, kHistogramBins * sizeof(uint32_t)) != hipSuccess)
+        return bail(SG_E_DEVICE);
+    {
+        const sg_options &o = ctx->options;
+        sg_exposure e{o.exposure, o.auto_key, o.auto_percentile, o.auto_exposure ? 1.0f : 0.0f, o.ev_bias};
+        if (hipMemcpy(ctx->exposure, &e, sizeof(e), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(ctx->histogram, 0, kHistogramBins * sizeof(uint32_t)) != hipSuccess)
+            return bail(SG_E_DEVICE);
+        ctx->display_exposure = o.exposure;
+    }
     if (hipMemsetAsync(ctx->sky_light, 0, SG_SKY_TEXELS * sizeof(float4), ctx->stream) != hipSuccess)
         return bail(SG_E_DEVICE);
     if (catalogue_count > 0) {
@@ -1197,7 +634,7 @@ int sg_destroy(sg_ctx *ctx) {
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
                        ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
-                       ctx->adj_gradient, ctx->adj_primal, ctx->records};
+                       ctx->adj_gradient, ctx->adj_primal, ctx->records, ctx->exposure, ctx->histogram};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
     for (void *b : buffers)
         if (b) (void)hipFree(b);
@@ -1323,6 +760,8 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
     PassPlan plan;
     int rc = prepare_pass(ctx, frame, plan);
     if (rc != SG_OK) return rc;
+    rc = publish_exposure(ctx);
+    if (rc != SG_OK) return rc;
     if (plan.count > 0) {
         const TraceArgs &ta = plan.ta;
         const uint32_t block = ta.pixels_per_block * plan.count;
@@ -1345,6 +784,10 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
             ctx->launches += 1;
         }
     }
+    if (ctx->options.auto_exposure) {
+        rc = meter_exposure(ctx);
+        if (rc != SG_OK) return rc;
+    }
     finish_pass(ctx, plan.count);
     return SG_OK;
 }
@@ -1362,6 +805,68 @@ static int read_image(sg_ctx *ctx, const float4 *src, float *dst, uint64_t pixel
 int sg_read_accumulation(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->accumulation : nullptr, rgba, n); }
 int sg_read_backdrop(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->backdrop : nullptr, rgba, n); }
 int sg_read_moments(sg_ctx *ctx, float *rgba, uint64_t n) { return read_image(ctx, ctx ? ctx->moments : nullptr, rgba, n); }
+
+static int write_image(sg_ctx *ctx, float4 *dst, const float *src, uint64_t pixel_count) {
+    if (!ctx || !src) return SG_E_INVALID;
+    if (pixel_count != (uint64_t)ctx->width * ctx->height)
+        return fail(ctx, SG_E_INVALID, "pixel count does not match the image size");
+    join_batch(ctx);
+    SG_HIP(ctx, hipMemcpyAsync(dst, src, pixel_count * 16u, hipMemcpyHostToDevice, ctx->stream));
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+int sg_write_accumulation(sg_ctx *ctx, const float *rgba, uint64_t n) {
+    return write_image(ctx, ctx ? ctx->accumulation : nullptr, rgba, n);
+}
+int sg_write_backdrop(sg_ctx *ctx, const float *rgba, uint64_t n) {
+    return write_image(ctx, ctx ? ctx->backdrop : nullptr, rgba, n);
+}
+
+int sg_read_exposure(sg_ctx *ctx, sg_exposure *state, uint32_t *histogram) {
+    if (!ctx || !state) return SG_E_INVALID;
+    join_batch(ctx);
+    SG_HIP(ctx, hipMemcpyAsync(state, ctx->exposure, sizeof(sg_exposure), hipMemcpyDeviceToHost, ctx->stream));
+    if (histogram)
+        SG_HIP(ctx, hipMemcpyAsync(histogram, ctx->histogram, kHistogramBins * sizeof(uint32_t),
+                                   hipMemcpyDeviceToHost, ctx->stream));
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return SG_OK;
+}
+
+static int display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *linear, uint8_t *srgb) {
+    if (!ctx || out_w == 0 || out_h == 0 || (!linear && !srgb)) return SG_E_INVALID;
+    if ((uint64_t)out_w * out_h > (1ull << 31)) return fail(ctx, SG_E_INVALID, "display target too large");
+    join_batch(ctx);
+    const size_t n = (size_t)out_w * out_h;
+    void *out = nullptr;
+    SG_HIP(ctx, hipMallocAsync(&out, n * (linear ? sizeof(float4) : sizeof(uchar4)), ctx->stream));
+    DisplayArgs da;
+    da.accumulation = ctx->accumulation;
+    da.backdrop = ctx->backdrop;
+    da.width = ctx->width;
+    da.height = ctx->height;
+    da.out_w = out_w;
+    da.out_h = out_h;
+    da.manual_exposure = ctx->display_exposure;
+    da.milky_way = ctx->display_milky_way;
+    da.state = ctx->exposure;
+    da.out_linear = linear ? static_cast<float4 *>(out) : nullptr;
+    da.out_srgb = linear ? nullptr : static_cast<uchar4 *>(out);
+    hipLaunchKernelGGL(k_display, dim3((out_w + 15u) / 16u, (out_h + 15u) / 16u), dim3(16, 16), 0, ctx->stream, da);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(linear ? static_cast<void *>(linear) : static_cast<void *>(srgb), out,
+                           n * (linear ? sizeof(float4) : sizeof(uchar4)), hipMemcpyDeviceToHost, ctx->stream);
+    hipError_t f = hipFreeAsync(out, ctx->stream);
+    if (e == hipSuccess) e = f;
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) return fail(ctx, SG_E_DEVICE, std::string("display: ") + hipGetErrorString(e));
+    return SG_OK;
+}
+int sg_display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *rgba) { return display(ctx, out_w, out_h, rgba, nullptr); }
+int sg_display_srgb8(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, uint8_t *rgba) {
+    return display(ctx, out_w, out_h, nullptr, rgba);
+}
 
 int sg_synchronize(sg_ctx *ctx) {
     if (!ctx) return SG_E_INVALID;

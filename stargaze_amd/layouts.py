@@ -66,6 +66,8 @@ SETTINGS = np.dtype(
 
 CAMERA_AIR = np.dtype([("radiance", "<f4", (3,)), ("rng", "<u4")])
 
+EXPOSURE = np.dtype([("value", "<f4"), ("key", "<f4"), ("percentile", "<f4"), ("enabled", "<f4"), ("bias", "<f4")])
+HISTOGRAM_BINS = 256  # src/pathtracer.rs:36
 PROBE_RAY = np.dtype(
     [("offset", "<f4", (3,)), ("anchor", "<u4"), ("direction", "<f4", (3,)), ("target", "<u4")]
 )

@@ -151,6 +151,8 @@ BATCH_WORKLOADS = {
     "mixed_96": dict(count=96, min_pixels=128 * 128, max_pixels=1920 * 1080),
     "small_96": dict(count=96, min_pixels=64 * 64, max_pixels=640 * 360),
     "tiny_96": dict(count=96, min_pixels=48 * 48, max_pixels=256 * 256),
+    # Previews: small views at 1-4 spp per pass, where launches dominate.
+    "preview_96": dict(count=96, min_pixels=48 * 48, max_pixels=256 * 256, spp=(1, 2, 4)),
 }
 
 
@@ -218,6 +220,7 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
                 "latency_ms": {"p50": pct(lat, 50), "p99": pct(lat, 99)}}
 
     lo, hi = int(math.sqrt(spec["min_pixels"])), spec["max_pixels"]
+    spp_set = ",".join(str(k) for k in spec.get("spp", (4, 8, 16, 32, 64)))
     return {
         "workload": workload,
         "views": count,
@@ -228,7 +231,7 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
         "speedup_vs_one_queue": round(per_view[True][0] / t_batch, 3),
         "speedup_vs_own_streams": round(per_view[False][0] / t_batch, 3),
         "autotune": tuned,
-        "views_desc": f"{count} random systems (PCG seed 1): log-uniform {lo}^2..{hi} px, spp {{4,8,16,32,64}} "
+        "views_desc": f"{count} random systems (PCG seed 1): log-uniform {lo}^2..{hi} px, spp {{{spp_set}}} "
         "per pass, 4 bounces; camera 2 m above a host at a random pose, 2-12 bodies, rings on half the "
         "giants, air on a third of the hosts",
     }

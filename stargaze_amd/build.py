@@ -40,16 +40,24 @@ def _run(cmd: list[str], verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed with exit code {proc.returncode}: {cmd[-1]}")
 
 
+# Forward kernel units built a second time without the landscape (suffix _nl).
+PLAIN_VARIANTS = {"sg_k_trace", "sg_k_batch", "sg_k_wave_a", "sg_k_wave_b"}
+
+
 def build(verbose: bool = False, jobs: int | None = None) -> Path:
     """Compile every translation unit (in parallel) and link the library."""
     from concurrent.futures import ThreadPoolExecutor
 
     OBJECTS.mkdir(exist_ok=True)
-    objects = [OBJECTS / (src.stem + ".o") for src in SOURCES]
-    jobs = jobs or min(len(SOURCES), max(1, len(os.sched_getaffinity(0))))
+    units = [(src, OBJECTS / (src.stem + ".o"), []) for src in SOURCES]
+    units += [(src, OBJECTS / (src.stem + "_nl.o"), ["-DSG_LANDSCAPE=0"])
+              for src in SOURCES if src.stem in PLAIN_VARIANTS]
+    # Longest first: the adjoint and batch units dominate.
+    units.sort(key=lambda u: ("adjoint" not in u[0].stem, "batch" not in u[0].stem, "trace" not in u[0].stem))
+    objects = [u[1] for u in units]
+    jobs = jobs or min(len(units), max(1, len(os.sched_getaffinity(0))))
     with ThreadPoolExecutor(max_workers=jobs) as pool:
-        list(pool.map(lambda so: _run([HIPCC, *FLAGS, "-c", "-o", str(so[1]), str(so[0])], verbose),
-                      zip(SOURCES, objects)))
+        list(pool.map(lambda u: _run([HIPCC, *FLAGS, *u[2], "-c", "-o", str(u[1]), str(u[0])], verbose), units))
     tmp = LIBRARY.with_suffix(".so.tmp")
     _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", str(tmp), *map(str, objects)], verbose)
     os.replace(tmp, LIBRARY)

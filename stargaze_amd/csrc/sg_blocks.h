@@ -8,7 +8,7 @@ namespace sgk {
 // The per-pixel epilogue of `main` (src/shaders/pathtrace.wgsl:1653-1674):
 // ordered mean of this pass's samples and the running-average blend, or,
 // with air, one blend per sample (src/pathtracer.rs:861-882,971-976).
-__device__ __forceinline__ void blend_pixel(const TraceArgs &ta, uint32_t pixel, const float4 *values,
+static __device__ __forceinline__ void blend_pixel(const TraceArgs &ta, uint32_t pixel, const float4 *values,
                                             const float4 *backdrops) {
     const uint32_t base = 0;
 
@@ -76,7 +76,7 @@ __device__ __forceinline__ void blend_pixel(const TraceArgs &ta, uint32_t pixel,
 // One block of pixel samples: lane t traces sample t % count of pixel
 // block * pixels_per_block + t / count, then one lane per pixel forms the
 // ordered mean and blend. Every lane of the workgroup must call it.
-__device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta, uint32_t block, float4 *s_value,
+static __device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta, uint32_t block, float4 *s_value,
                                             float4 *s_backdrop) {
     const uint32_t t = threadIdx.x;
     const uint32_t lp = t / ta.count;
@@ -109,7 +109,7 @@ __device__ __forceinline__ void trace_block(const Scene &sc, const TraceArgs &ta
 
 // One pass of a batch (see BatchArgs): the persistent loop over pixel blocks.
 template <uint32_t B>
-__global__ __launch_bounds__(B) void k_batch(BatchArgs ba) {
+__global__ __launch_bounds__(B) void SG_K(k_batch)(BatchArgs ba) {
     __shared__ float4 s_value[B];
     __shared__ float4 s_backdrop[B];
     __shared__ uint32_t s_begin[kMaxBatchViews + 1];

@@ -31,6 +31,8 @@ constexpr float TAU = 6.283185307179586f;
 constexpr float METRES_PER_AU = 149597870700.0f;   // src/shaders/ground.wgsl:15
 constexpr double METRES_PER_AU_D = 149597870700.0;
 constexpr float STAR_DOT_PIXELS = 3.0f;            // src/shaders/pathtrace.wgsl:65
+// Whether this translation unit renders the landscape (see SG_LANDSCAPE).
+constexpr bool kLandscape = SG_LANDSCAPE != 0;
 constexpr uint32_t ATMO_VIEW_STEPS = 8u;           // src/shaders/pathtrace.wgsl:67
 constexpr uint32_t ATMO_SUN_STEPS = 4u;            // src/shaders/pathtrace.wgsl:68
 constexpr float POINT_DISC_RADIUS = 0.00136f;      // src/shaders/pathtrace.wgsl:1300

@@ -527,7 +527,7 @@ SG_FN bool prop_bvh_query(const Scene &sc, f3 o, f3 d, Hit &closest, bool occlus
 SG_FN Hit landscape_query(const Scene &sc, const Ray &ray, float limit, bool occlusion) {
     Hit closest{limit, SG_MISS, f3s(0.f)};
     const sg_globals &g = sc.s.globals;
-    if (g.ground_north[3] == 0.0f) return closest;
+    if (!kLandscape || g.ground_north[3] == 0.0f) return closest;
     d3 od = ground_origin(sc, ray);
     f3 d = to_ground(sc, ray.direction);
     float radius = g.ground_up[3];
@@ -595,7 +595,7 @@ SG_FN Hit closest_hit(const Scene &sc, const Ray &ray) {
         Hit hit = sphere_hit(sc, ray, i);
         if (hit.distance < closest.distance) closest = hit;
     }
-    if (host != 0.0f) {
+    if (kLandscape && host != 0.0f) {
         Hit local = landscape_hit(sc, ray);
         if (local.distance < closest.distance) closest = local;
     }
@@ -709,7 +709,7 @@ SG_FN float light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
         if (hit.index != SG_MISS && (hit.distance < surface.distance || (hit.distance == surface.distance && i < light)))
             return 0.0f;
     }
-    if (host != 0.0f) {
+    if (kLandscape && host != 0.0f) {
         Hit local = landscape_query(sc, ray, surface.distance, true);
         if (local.index != SG_MISS && local.distance < surface.distance) return 0.0f;
     }
@@ -762,7 +762,7 @@ SG_FN float2 ring_bsdf_factor(float tau, float outgoing, float incoming) {
 // src/shaders/rings.wgsl:121-141
 SG_FN float air_light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
     const sg_globals &g = sc.s.globals;
-    if (g.ground_north[3] == 0.0f || (float)ground_origin(sc, ray).y <= landscape_top(sc))
+    if (!kLandscape || g.ground_north[3] == 0.0f || (float)ground_origin(sc, ray).y <= landscape_top(sc))
         return light_visibility(sc, ray, light);
     Hit surface = sphere_hit(sc, ray, light);
     if (surface.index == SG_MISS) return 0.0f;

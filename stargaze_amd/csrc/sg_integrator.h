@@ -35,6 +35,9 @@ struct ColumnBins {
 struct NoGrad {
     static constexpr bool on = false;
 };
+// The gradient structs have internal linkage, like the free functions their
+// members call.
+namespace {
 // Adds weighted derivatives into a GRAD_COUNT accumulator (workgroup memory
 // on the GPU). Every method takes d(loss) already multiplied out.
 struct GradSink {
@@ -45,7 +48,7 @@ struct GradSink {
     // the air touches those, so they are summed per lane and flushed once).
     float *dens;
     uint32_t dstride;
-    SG_FN void add(uint32_t i, float v) const {
+    SG_MEMBER void add(uint32_t i, float v) const {
         if (v == 0.0f) return;
 #if defined(__HIP_DEVICE_COMPILE__)
         atomicAdd(&g[i], v);
@@ -53,15 +56,15 @@ struct GradSink {
         g[i] += v;
 #endif
     }
-    SG_FN void albedo(uint32_t body, f3 d) const {
+    SG_MEMBER void albedo(uint32_t body, f3 d) const {
         add(GRAD_ALBEDO + 3u * body, d.x);
         add(GRAD_ALBEDO + 3u * body + 1u, d.y);
         add(GRAD_ALBEDO + 3u * body + 2u, d.z);
     }
-    SG_FN void ring(uint32_t bin, float d) const { add(GRAD_RING + bin, d); }
-    SG_FN void air(uint32_t k, float v) const { dens[(SG_DENSITY_BINS + k) * dstride] += v; }
+    SG_MEMBER void ring(uint32_t bin, float d) const { add(GRAD_RING + bin, d); }
+    SG_MEMBER void air(uint32_t k, float v) const { dens[(SG_DENSITY_BINS + k) * dstride] += v; }
     // d / d(beta_ext) per component; beta_ext = beta_rayleigh + beta_mie.
-    SG_FN void extinction(f3 d) const {
+    SG_MEMBER void extinction(f3 d) const {
         air(0u, d.x);
         air(1u, d.y);
         air(2u, d.z);
@@ -69,7 +72,7 @@ struct GradSink {
     }
     // A weighted contribution proportional to beta_phase = beta_r * rayleigh
     // + beta_m * mie(g): its derivatives through the phase coefficients.
-    SG_FN void phase(f3 wc, f3 beta_phase, float rayleigh, float mie, float dmie_dg, float beta_m) const {
+    SG_MEMBER void phase(f3 wc, f3 beta_phase, float rayleigh, float mie, float dmie_dg, float beta_m) const {
         f3 q = f3{beta_phase.x > 0.0f ? wc.x / beta_phase.x : 0.0f, beta_phase.y > 0.0f ? wc.y / beta_phase.y : 0.0f,
                   beta_phase.z > 0.0f ? wc.z / beta_phase.z : 0.0f};
         air(0u, q.x * rayleigh);
@@ -81,13 +84,13 @@ struct GradSink {
     }
     // d / d(profile bin) of a factor exp(-beta_ext * column) for weight
     // e = d(loss)/d log(factor) * (-beta_ext): each sample adds dt.
-    SG_FN void column(const Scene &sc, const ColumnBins &c, float e) const {
+    SG_MEMBER void column(const Scene &sc, const ColumnBins &c, float e) const {
         if (!sc.density_profile || e == 0.0f) return;
         for (uint32_t k = 0; k < c.count; ++k) dens[((c.bins >> (4u * k)) & 15u) * dstride] += e * c.dt;
     }
-    SG_FN void density(uint32_t bin, float v) const { dens[bin * dstride] += v; }
+    SG_MEMBER void density(uint32_t bin, float v) const { dens[bin * dstride] += v; }
     // Move the lane's private accumulators into the gradient.
-    SG_FN void flush_private() const {
+    SG_MEMBER void flush_private() const {
         for (uint32_t b = 0; b < SG_DENSITY_BINS; ++b) {
             add(GRAD_DENSITY + b, dens[b * dstride]);
             dens[b * dstride] = 0.0f;
@@ -99,12 +102,13 @@ struct GradSink {
     }
     // A weighted contribution behind a shadow ray that may cross the
     // tabulated ring.
-    SG_FN void shadow(const Scene &sc, const Ray &ray, uint32_t light, float weighted) const {
+    SG_MEMBER void shadow(const Scene &sc, const Ray &ray, uint32_t light, float weighted) const {
         uint32_t bin;
         float dlog;
         if (weighted != 0.0f && ring_profile_crossing(sc, ray, light, bin, dlog)) ring(bin, weighted * dlog);
     }
 };
+}  // namespace
 
 // --- ground materials, theme 0 (src/shaders/ground.wgsl:256-281, 597-666) ----
 SG_FN float footprint_of(const Scene &sc, float distance) {
@@ -689,6 +693,7 @@ SG_FN float point_gain(const Scene &sc, float extent, float distance) {
     return mixf(sc.s.background[3], 1.0f, smoothstepf(1.0f, 3.0f, pixels));
 }
 
+namespace {
 // --- the integrator (src/shaders/pathtrace.wgsl:1347-1617) -----------------
 // trace() is split at its loop boundaries into steps that carry the whole
 // path in PathState:
@@ -723,13 +728,14 @@ struct PathGrad : GradSink {
     f3 w;
     f3 value;
     float cap_scale;
-    SG_FN bool capped(const PathState &st) const { return st.ring_cap < 1.0e29f; }
-    SG_FN f3 so_far(const PathState &st) const {
+    SG_MEMBER bool capped(const PathState &st) const { return st.ring_cap < 1.0e29f; }
+    SG_MEMBER f3 so_far(const PathState &st) const {
         return capped(st) ? st.ring_base + (st.radiance - st.ring_base) * cap_scale : st.radiance;
     }
-    SG_FN f3 weight(const PathState &st) const { return capped(st) ? w * cap_scale : w; }
-    SG_FN f3 rest(const PathState &st) const { return w * (value - so_far(st)); }
+    SG_MEMBER f3 weight(const PathState &st) const { return capped(st) ? w * cap_scale : w; }
+    SG_MEMBER f3 rest(const PathState &st) const { return w * (value - so_far(st)); }
 };
+}  // namespace
 
 struct HitRecord {
     Hit hit;
@@ -931,7 +937,7 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
         }
         return false;
     }
-    const bool local = hit.index >= SG_LOCAL_BASE;
+    const bool local = kLandscape && hit.index >= SG_LOCAL_BASE;
     st.throughput *= h.disc_scale;
     const Body *hb = local ? nullptr : &sc.bodies[hit.index];
     if (!local && hb->material.w > 0.5f) {

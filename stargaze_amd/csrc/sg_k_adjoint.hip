@@ -1,4 +1,6 @@
 // The adjoint kernel (path replay) and the gradient fill.
+// The adjoint rejects frames with a ground scene: compile without it.
+#define SG_LANDSCAPE 0
 #include "sg_blocks.h"
 
 namespace sgk {

@@ -1,4 +1,5 @@
-// Exposure, display, terrain and sky caches, and the probe kernel.
+// Exposure, display, terrain and sky caches, the probe kernel, and the reset and
+// advance nodes of a captured batch pass.
 #include "sg_launch.h"
 
 namespace sgk {
@@ -183,6 +184,31 @@ __global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint3
         break;
     }
     out[i] = r;
+}
+
+__global__ void k_batch_reset(BatchArgs ba) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) {
+        ba.counter[0] = 0u;
+        ba.clock[0] = ~0ull;
+        ba.clock[1] = 0ull;
+    }
+    if (i < ba.entries) {
+        ba.counter[1 + i] = ba.table[i].blocks;
+        ba.done[i] = 0ull;
+    }
+}
+
+// End of a pass: each entry's accumulated count and noise sequence advance
+// as the host history does, so an unchanged table can be replayed.
+__global__ void k_batch_advance(BatchArgs ba) {
+    const uint32_t e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ba.entries) return;
+    BatchView &v = ba.table[e];
+    v.ta.accumulated += v.ta.count;
+    v.ta.sequence += v.ta.count;
+    v.scene.s.counts[1] += v.ta.count;
+    v.scene.s.sampling[0] += v.ta.count;
 }
 
 }  // namespace sgk

@@ -3,12 +3,12 @@
 
 namespace sgk {
 
-__global__ __launch_bounds__(kTraceBlock) void k_trace(Scene sc, TraceArgs ta) {
+__global__ __launch_bounds__(kTraceBlock) void SG_K(k_trace)(Scene sc, TraceArgs ta) {
     __shared__ float4 s_value[kTraceBlock];
     __shared__ float4 s_backdrop[kTraceBlock];
     trace_block(sc, ta, blockIdx.x, s_value, s_backdrop);
 }
 
-template __global__ void k_batch<64>(BatchArgs ba);
+template __global__ void SG_K(k_batch)<64>(BatchArgs ba);
 
 }  // namespace sgk

@@ -127,28 +127,38 @@ struct DisplayArgs {
     uchar4 *out_srgb;              // or null
 };
 
-// Kernels (definitions in sg_k_*.hip).
-__global__ void k_trace(Scene sc, TraceArgs ta);                                       // sg_k_trace.hip
-template <uint32_t B> __global__ void k_batch(BatchArgs ba);                            // B = 64: sg_k_trace.hip, 128/256: sg_k_batch.hip
-__global__ void k_batch_reset(BatchArgs ba);                                           // sg_k_batch.hip
-__global__ void k_batch_advance(BatchArgs ba);                                         // sg_k_batch.hip
-__global__ void k_generate(Scene sc, WaveArgs wa, uint32_t *queue);                     // sg_k_wave_a.hip
-__global__ void k_extend(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);     // sg_k_wave_a.hip
-__global__ void k_camera_air(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n); // sg_k_wave_a.hip
-__global__ void k_shade(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n, uint32_t *next,
-                        uint32_t *next_count);                                          // sg_k_wave_b.hip
-__global__ void k_tail(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);       // sg_k_wave_b.hip
-__global__ void k_accumulate(WaveArgs wa, float4 *accumulation, float4 *backdrop, float4 *moments,
-                             uint32_t pixels);                                          // sg_k_wave_b.hip
-__global__ void k_fill(float *p, uint32_t n, float v);                                  // sg_k_adjoint.hip
-__global__ void k_adjoint(Scene sc, AdjointArgs aa);                                    // sg_k_adjoint.hip
-__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v);                        // sg_k_misc.hip
-__global__ void k_meter(MeterArgs ma);                                                  // sg_k_misc.hip
-__global__ void k_resolve(const uint32_t *histogram, sg_exposure *state);               // sg_k_misc.hip
-__global__ void k_display(DisplayArgs da);                                              // sg_k_misc.hip
-__global__ void k_terrain_cache(Scene sc, float *terrain);                              // sg_k_misc.hip
-__global__ void k_sky_cache(Scene sc, float4 *sky);                                     // sg_k_misc.hip
-__global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
-                        sg_probe_result *out);                                          // sg_k_misc.hip
+// Kernels. The forward kernels (sg_k_trace.hip, sg_k_batch.hip,
+// sg_k_wave_a.hip, sg_k_wave_b.hip) are compiled twice: as written, and with
+// SG_LANDSCAPE=0 under the suffix _nl, for frames without a ground scene.
+#if SG_LANDSCAPE
+#define SG_K(name) name
+#else
+#define SG_K(name) name##_nl
+#endif
+#define SG_FORWARD_KERNELS(S)                                                                      \
+    __global__ void k_trace##S(Scene sc, TraceArgs ta);                                            \
+    template <uint32_t B> __global__ void k_batch##S(BatchArgs ba);                                \
+    __global__ void k_generate##S(Scene sc, WaveArgs wa, uint32_t *queue);                          \
+    __global__ void k_extend##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);          \
+    __global__ void k_camera_air##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);      \
+    __global__ void k_shade##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n, uint32_t *next, \
+                               uint32_t *next_count);                                              \
+    __global__ void k_tail##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);            \
+    __global__ void k_accumulate##S(WaveArgs wa, float4 *accumulation, float4 *backdrop, float4 *moments, \
+                                    uint32_t pixels);
+SG_FORWARD_KERNELS()
+SG_FORWARD_KERNELS(_nl)
+// Compiled once (sg_k_adjoint.hip without the landscape, sg_k_misc.hip with it).
+__global__ void k_batch_reset(BatchArgs ba);
+__global__ void k_batch_advance(BatchArgs ba);
+__global__ void k_fill(float *p, uint32_t n, float v);
+__global__ void k_adjoint(Scene sc, AdjointArgs aa);
+__global__ void k_fill_u32(uint32_t *p, uint32_t n, uint32_t v);
+__global__ void k_meter(MeterArgs ma);
+__global__ void k_resolve(const uint32_t *histogram, sg_exposure *state);
+__global__ void k_display(DisplayArgs da);
+__global__ void k_terrain_cache(Scene sc, float *terrain);
+__global__ void k_sky_cache(Scene sc, float4 *sky);
+__global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint32_t count, sg_probe_result *out);
 
 }  // namespace sgk

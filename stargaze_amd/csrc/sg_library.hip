@@ -27,6 +27,29 @@
 using namespace sg;
 using namespace sgk;
 
+namespace {
+// The forward kernels for frames with a ground scene, and the ones compiled
+// without the landscape for every other frame (same results, fewer live
+// registers).
+struct ForwardKernels {
+    decltype(&k_trace) trace;
+    decltype(&k_batch<64>) batch64, batch128, batch256;
+    decltype(&k_generate) generate;
+    decltype(&k_extend) extend;
+    decltype(&k_camera_air) camera_air;
+    decltype(&k_shade) shade;
+    decltype(&k_tail) tail;
+    decltype(&k_accumulate) accumulate;
+};
+const ForwardKernels kLandscapeKernels{k_trace, k_batch<64>, k_batch<128>, k_batch<256>, k_generate, k_extend,
+                                       k_camera_air, k_shade, k_tail, k_accumulate};
+const ForwardKernels kPlainKernels{k_trace_nl, k_batch_nl<64>, k_batch_nl<128>, k_batch_nl<256>, k_generate_nl,
+                                   k_extend_nl, k_camera_air_nl, k_shade_nl, k_tail_nl, k_accumulate_nl};
+const ForwardKernels &forward_kernels(const Scene &sc) {
+    return sc.s.globals.ground_north[3] != 0.0f ? kLandscapeKernels : kPlainKernels;
+}
+}  // namespace
+
 // ---------------------------------------------------------------------------
 // Host library: the PathTracer state machine of src/pathtracer.rs:164-1009.
 // ---------------------------------------------------------------------------
@@ -323,6 +346,7 @@ int ensure_wave(sg_ctx *ctx, uint32_t paths) {
 // One progressive pass through the wavefront kernels, chunked over pixels so
 // the path state stays within wave_capacity.
 int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
+    const ForwardKernels &fk = forward_kernels(sc);
     const uint32_t count = ta.count;
     const uint64_t pixels = (uint64_t)ta.width * ta.height;
     const uint32_t chunk_pixels = std::max<uint32_t>(1u, ctx->chunk_paths / count);
@@ -344,25 +368,25 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
         wa.paths = chunk * count;
         uint32_t n = wa.paths;
         int q = 0;
-        hipLaunchKernelGGL(k_generate, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
+        hipLaunchKernelGGL(fk.generate, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
                            ctx->queues[q]);
         SG_HIP(ctx, hipGetLastError());
         for (uint32_t iteration = 0; n > 0; ++iteration) {
             if (n < ctx->tail_paths) {
-                hipLaunchKernelGGL(k_tail, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
+                hipLaunchKernelGGL(fk.tail, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
                                    ctx->queues[q], n);
                 SG_HIP(ctx, hipGetLastError());
                 break;
             }
             const dim3 grid((n + block - 1) / block);
-            hipLaunchKernelGGL(k_extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
+            hipLaunchKernelGGL(fk.extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
             SG_HIP(ctx, hipGetLastError());
             if (iteration == 0 && ta.has_air) {
-                hipLaunchKernelGGL(k_camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
+                hipLaunchKernelGGL(fk.camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
                 SG_HIP(ctx, hipGetLastError());
             }
             SG_HIP(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(uint32_t), ctx->stream));
-            hipLaunchKernelGGL(k_shade, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n,
+            hipLaunchKernelGGL(fk.shade, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n,
                                ctx->queues[q ^ 1], ctx->counter);
             SG_HIP(ctx, hipGetLastError());
             SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, sizeof(uint32_t), hipMemcpyDeviceToHost,
@@ -371,7 +395,7 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
             n = *ctx->counter_host;
             q ^= 1;
         }
-        hipLaunchKernelGGL(k_accumulate, dim3((chunk + block - 1) / block), dim3(block), 0, ctx->stream, wa,
+        hipLaunchKernelGGL(fk.accumulate, dim3((chunk + block - 1) / block), dim3(block), 0, ctx->stream, wa,
                            ta.accumulation, ta.backdrop, ta.moments, chunk);
         SG_HIP(ctx, hipGetLastError());
     }
@@ -769,7 +793,7 @@ int sg_encode(sg_ctx *ctx, const sg_frame *frame) {
         const uint32_t grid = (uint32_t)((pixels + ta.pixels_per_block - 1) / ta.pixels_per_block);
         if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ctx->ev_start, ctx->stream));
         if (ctx->integrator == SG_INTEGRATOR_FUSED) {
-            hipLaunchKernelGGL(k_trace, dim3(grid), dim3(block), 0, ctx->stream, plan.sc, ta);
+            hipLaunchKernelGGL(forward_kernels(plan.sc).trace, dim3(grid), dim3(block), 0, ctx->stream, plan.sc, ta);
             SG_HIP(ctx, hipGetLastError());
         } else {
             int wrc = encode_wavefront(ctx, plan.sc, ta);
@@ -1247,6 +1271,7 @@ struct sg_batch {
     std::vector<BatchView> expected;     // device table after the last pass
     std::vector<uint32_t> entry_view;
     uint32_t total_blocks = 0;
+    bool landscape = false;  // some view of the table has a ground scene
     bool table_valid = false;
     bool pass_queued = false;
     // Graph of one pass for the current table shape and launch.
@@ -1307,10 +1332,11 @@ int queue_pass(sg_batch *b) {
     hipLaunchKernelGGL(k_batch_reset, small, dim3(256), 0, b->stream, ba);
     SG_BHIP(b, hipGetLastError());
     const dim3 grid(b->launch.workgroups);
+    const ForwardKernels &fk = b->landscape ? kLandscapeKernels : kPlainKernels;
     switch (b->launch.block) {
-    case 64: hipLaunchKernelGGL(k_batch<64>, grid, dim3(64), 0, b->stream, ba); break;
-    case 128: hipLaunchKernelGGL(k_batch<128>, grid, dim3(128), 0, b->stream, ba); break;
-    default: hipLaunchKernelGGL(k_batch<256>, grid, dim3(256), 0, b->stream, ba); break;
+    case 64: hipLaunchKernelGGL(fk.batch64, grid, dim3(64), 0, b->stream, ba); break;
+    case 128: hipLaunchKernelGGL(fk.batch128, grid, dim3(128), 0, b->stream, ba); break;
+    default: hipLaunchKernelGGL(fk.batch256, grid, dim3(256), 0, b->stream, ba); break;
     }
     SG_BHIP(b, hipGetLastError());
     hipLaunchKernelGGL(k_batch_advance, small, dim3(256), 0, b->stream, ba);
@@ -1403,7 +1429,11 @@ int batch_pass(sg_batch *b, const sg_frame *frames) {
             std::memcpy(b->staging, table.data(), table.size() * sizeof(BatchView));
             SG_BHIP(b, hipMemcpyAsync(b->table, b->staging, table.size() * sizeof(BatchView), hipMemcpyHostToDevice,
                                       b->stream));
-            if (table.size() != b->entry_view.size() || total_blocks != b->total_blocks) b->graph_valid = false;
+            bool landscape = false;
+            for (const BatchView &v : table) landscape |= v.scene.s.globals.ground_north[3] != 0.0f;
+            if (table.size() != b->entry_view.size() || total_blocks != b->total_blocks || landscape != b->landscape)
+                b->graph_valid = false;
+            b->landscape = landscape;
             b->entry_view = entry_view;
             b->total_blocks = total_blocks;
             b->table_valid = true;

@@ -5,7 +5,17 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define SG_FN __host__ __device__ __forceinline__
+// Free functions have internal linkage: a kernel translation unit may be
+// compiled with or without the landscape (SG_LANDSCAPE), and each keeps its
+// own definitions. Member functions use SG_MEMBER.
+#define SG_FN static __host__ __device__ __forceinline__
+#define SG_MEMBER __host__ __device__ __forceinline__
+
+// 0 compiles the landscape (ground scenery, terrain, sky cache) out of a
+// kernel translation unit: its kernels serve frames without a ground scene.
+#ifndef SG_LANDSCAPE
+#define SG_LANDSCAPE 1
+#endif
 
 template <class T> struct V3 { T x, y, z; };
 typedef V3<float> f3;

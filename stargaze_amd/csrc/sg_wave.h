@@ -8,15 +8,15 @@ namespace sgk {
 SG_FN double bits_d(uint32_t u) { return (double)u; }
 SG_FN uint32_t d_bits(double d) { return (uint32_t)d; }
 
-__device__ __forceinline__ uint32_t path_accumulated(const WaveArgs &wa, uint32_t id) {
+static __device__ __forceinline__ uint32_t path_accumulated(const WaveArgs &wa, uint32_t id) {
     return wa.has_air ? wa.accumulated + id % wa.count : wa.accumulated;
 }
-__device__ __forceinline__ void load_ray(const double4 &o, const float4 &d, Ray &r) {
+static __device__ __forceinline__ void load_ray(const double4 &o, const float4 &d, Ray &r) {
     r.offset = d3{o.x, o.y, o.z};
     r.anchor = d_bits(o.w);
     r.direction = f3{d.x, d.y, d.z};
 }
-__device__ __forceinline__ void load_state(const WaveArgs &wa, uint32_t id, PathState &st) {
+static __device__ __forceinline__ void load_state(const WaveArgs &wa, uint32_t id, PathState &st) {
     float4 d = wa.ray_d[id];
     load_ray(wa.ray_o[id], d, st.ray);
     st.previous_pdf = d.w;
@@ -40,7 +40,7 @@ __device__ __forceinline__ void load_state(const WaveArgs &wa, uint32_t id, Path
     st.backdrop = f3s(0.0f);
     st.accumulated = path_accumulated(wa, id);
 }
-__device__ __forceinline__ void store_state(const WaveArgs &wa, uint32_t id, const PathState &st) {
+static __device__ __forceinline__ void store_state(const WaveArgs &wa, uint32_t id, const PathState &st) {
     wa.ray_o[id] = make_double4(st.ray.offset.x, st.ray.offset.y, st.ray.offset.z, bits_d(st.ray.anchor));
     wa.ray_d[id] = make_float4(st.ray.direction.x, st.ray.direction.y, st.ray.direction.z, st.previous_pdf);
     wa.prev_o[id] = make_double4(st.previous_vertex.offset.x, st.previous_vertex.offset.y,
@@ -51,7 +51,7 @@ __device__ __forceinline__ void store_state(const WaveArgs &wa, uint32_t id, con
     wa.rbase[id] = make_float4(st.ring_base.x, st.ring_base.y, st.ring_base.z, __uint_as_float(packed));
     wa.rng[id] = st.rng;
 }
-__device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id, HitRecord &h) {
+static __device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id, HitRecord &h) {
     float4 a = wa.h0[id], b = wa.h1[id], c = wa.h2[id];
     h.hit.distance = a.x;
     h.hit.index = __float_as_uint(a.y);
@@ -61,7 +61,7 @@ __device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id, HitRec
     h.ring.index = __float_as_uint(b.w);
     h.ring.point = f3{c.x, c.y, c.z};
 }
-__device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const PathState &st) {
+static __device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const PathState &st) {
     PathResult r = path_finish(st);
     wa.value[id] = r.value;
     wa.backdrop[id] = make_float4(r.backdrop.x, r.backdrop.y, r.backdrop.z, 0.f);

@@ -109,7 +109,16 @@ static __device__ __forceinline__ void trace_block(const Scene &sc, const TraceA
 
 // One pass of a batch (see BatchArgs): the persistent loop over pixel blocks.
 template <uint32_t B>
-__global__ __launch_bounds__(B) void SG_K(k_batch)(BatchArgs ba) {
+// Without the landscape: two waves per SIMD, the fused kernel's budget (the
+// persistent loop's few extra registers would otherwise halve the
+// occupancy; 11 spilled registers). With it the budget would spill hundreds.
+#if SG_LANDSCAPE
+#define SG_BATCH_WAVES 1
+#else
+#define SG_BATCH_WAVES 2
+#endif
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(SG_BATCH_WAVES, 8))) void SG_K(k_batch)(
+    BatchArgs ba) {
     __shared__ float4 s_value[B];
     __shared__ float4 s_backdrop[B];
     __shared__ uint32_t s_begin[kMaxBatchViews + 1];

@@ -10,7 +10,12 @@ __global__ void k_fill(float *p, uint32_t n, float v) {
     if (i < n) p[i] = v;
 }
 
-__global__ __launch_bounds__(kTraceBlock) void k_adjoint(Scene sc, AdjointArgs aa) {
+// Waves per SIMD of the adjoint kernel's register budget.
+#ifndef SG_ADJOINT_WAVES
+#define SG_ADJOINT_WAVES 2
+#endif
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SG_ADJOINT_WAVES, 8))) void k_adjoint(
+    Scene sc, AdjointArgs aa) {
     __shared__ float s_grad[GRAD_COUNT];
     __shared__ float s_density[GRAD_PRIVATE * kTraceBlock];  // lane-private, slot-major
     __shared__ float4 s_value[kTraceBlock];

@@ -109,22 +109,24 @@ static __device__ __forceinline__ void trace_block(const Scene &sc, const TraceA
 
 // One pass of a batch (see BatchArgs): the persistent loop over pixel blocks.
 template <uint32_t B>
-// Without the landscape: two waves per SIMD, the fused kernel's budget (the
-// persistent loop's few extra registers would otherwise halve the
-// occupancy; 11 spilled registers). With it the budget would spill hundreds.
-#if SG_LANDSCAPE
-#define SG_BATCH_WAVES 1
-#else
-#define SG_BATCH_WAVES 2
-#endif
-__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(SG_BATCH_WAVES, 8))) void SG_K(k_batch)(
+// The fused kernel's occupancy budget: without it the persistent loop's few
+// extra registers would halve the occupancy of the frames without a ground
+// scene.
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(SG_FUSED_WAVES, 8))) void SG_K(k_batch)(
     BatchArgs ba) {
     __shared__ float4 s_value[B];
     __shared__ float4 s_backdrop[B];
     __shared__ uint32_t s_begin[kMaxBatchViews + 1];
     __shared__ uint32_t s_first;
+    // The table does not change while the kernel runs. Read through the
+    // constant address space, a view's scene and pass constants come in by
+    // scalar loads, as kernel arguments do for k_trace; through a plain
+    // global pointer every load after the first stores of the loop would be
+    // a vector load of a uniform value.
+    using ConstView = __attribute__((address_space(4))) const BatchView;
+    const ConstView *table = (const ConstView *)ba.table;
     const uint32_t t = threadIdx.x;
-    for (uint32_t e = t; e < ba.entries; e += B) s_begin[e] = ba.table[e].block_begin;
+    for (uint32_t e = t; e < ba.entries; e += B) s_begin[e] = table[e].block_begin;
     if (t == 0) {
         s_begin[ba.entries] = ba.total_blocks;
         atomicMin(&ba.clock[0], (unsigned long long)wall_clock64());
@@ -145,13 +147,12 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(SG_BATCH_WAVE
                 else hi = mid;
             }
             const uint32_t e = __builtin_amdgcn_readfirstlane(lo);
-            const BatchView &v = ba.table[e];
+            const BatchView &v = *(const BatchView *)&table[e];
             trace_block(v.scene, v.ta, block - s_begin[e], s_value, s_backdrop);
             __syncthreads();
-            if (t == 0) {
-                __threadfence();
-                if (atomicSub(&ba.counter[1 + e], 1u) == 1u) ba.done[e] = wall_clock64();
-            }
+            // The entry's finish time is its last block's: a max that
+            // returns nothing, so no wave waits on it.
+            if (t == 0) atomicMax(&ba.done[e], (unsigned long long)wall_clock64());
         }
     }
     if (t == 0) atomicMax(&ba.clock[1], (unsigned long long)wall_clock64());

@@ -139,7 +139,18 @@ SG_FN float rnd(uint32_t &state) {
 
 // --- bodies and rebasing -----------------------------------------------------
 SG_FN uint32_t body_count(const Scene &sc) { return sc.s.counts[0]; }
-SG_FN const Body &body(const Scene &sc, uint32_t i) { return sc.bodies[i]; }
+// Body records do not change while a kernel runs. Read through the constant
+// address space, a uniform read is a scalar load in every kernel; through a
+// plain global pointer it would be a vector load wherever a store of the
+// same kernel may precede it (any read in a persistent loop).
+SG_FN const Body &body(const Scene &sc, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstBody = __attribute__((address_space(4))) const Body;
+    return *(const Body *)&((const ConstBody *)sc.bodies)[i];
+#else
+    return sc.bodies[i];
+#endif
+}
 SG_FN d3 center_d(const Body &b) {
     return d3{(double)b.center.x + (double)b.low.x, (double)b.center.y + (double)b.low.y,
               (double)b.center.z + (double)b.low.z};

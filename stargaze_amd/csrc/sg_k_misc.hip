@@ -194,7 +194,6 @@ __global__ void k_batch_reset(BatchArgs ba) {
         ba.clock[1] = 0ull;
     }
     if (i < ba.entries) {
-        ba.counter[1 + i] = ba.table[i].blocks;
         ba.done[i] = 0ull;
     }
 }

@@ -3,7 +3,8 @@
 
 namespace sgk {
 
-__global__ __launch_bounds__(kTraceBlock) void SG_K(k_trace)(Scene sc, TraceArgs ta) {
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SG_FUSED_WAVES, 8))) void SG_K(k_trace)(
+    Scene sc, TraceArgs ta) {
     __shared__ float4 s_value[kTraceBlock];
     __shared__ float4 s_backdrop[kTraceBlock];
     trace_block(sc, ta, blockIdx.x, s_value, s_backdrop);

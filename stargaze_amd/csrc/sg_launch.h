@@ -13,6 +13,17 @@ using namespace sg;
 constexpr uint32_t kTraceBlock = 256;
 
 // Register budget of the heavy wavefront kernels, as waves per SIMD.
+// Waves per SIMD the fused and batch kernels are compiled for: one with the
+// landscape (its traversal state fills the register file), SG_PLAIN_WAVES
+// without it.
+#ifndef SG_PLAIN_WAVES
+#define SG_PLAIN_WAVES 3
+#endif
+#if SG_LANDSCAPE
+#define SG_FUSED_WAVES 1
+#else
+#define SG_FUSED_WAVES SG_PLAIN_WAVES
+#endif
 #ifndef SG_HEAVY_WAVES
 #define SG_HEAVY_WAVES 1
 #endif
@@ -81,8 +92,8 @@ struct BatchArgs {
     uint32_t entries;
     uint32_t total_blocks;
     uint32_t blocks_per_pull;
-    uint32_t *counter;            // [0] next block; [1 + e] blocks left of entry e
-    unsigned long long *done;     // [e] clock at which entry e finished
+    uint32_t *counter;            // [0] next block
+    unsigned long long *done;     // [e] clock at which entry e finished (max over its blocks)
     unsigned long long *clock;    // [0] first workgroup start, [1] last workgroup end
 };
 

@@ -17,7 +17,7 @@ __global__ __launch_bounds__(256) void SG_K(k_generate)(Scene sc, WaveArgs wa, u
     queue[id] = id;
 }
 
-__global__ SG_HEAVY void SG_K(k_extend)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+__global__ SG_WAVES(SG_EXTEND_WAVES) void SG_K(k_extend)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = queue[i];
@@ -34,7 +34,7 @@ __global__ SG_HEAVY void SG_K(k_extend)(Scene sc, WaveArgs wa, const uint32_t *q
     wa.h2[id] = make_float4(h.ring.point.x, h.ring.point.y, h.ring.point.z, 0.f);
 }
 
-__global__ SG_HEAVY void SG_K(k_camera_air)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+__global__ SG_WAVES(SG_AIR_WAVES) void SG_K(k_camera_air)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = queue[i];

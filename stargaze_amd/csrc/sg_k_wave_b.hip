@@ -3,7 +3,7 @@
 
 namespace sgk {
 
-__global__ SG_HEAVY void SG_K(k_shade)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
+__global__ SG_WAVES(SG_SHADE_WAVES) void SG_K(k_shade)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n,
                                                uint32_t *next, uint32_t *next_count) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     bool alive = false;
@@ -35,7 +35,7 @@ __global__ SG_HEAVY void SG_K(k_shade)(Scene sc, WaveArgs wa, const uint32_t *qu
 // The tail of a chunk: once fewer paths than the compaction threshold are
 // alive, each lane runs its queued path through the remaining loop of
 // trace() in one launch (same steps, same draws).
-__global__ SG_HEAVY void SG_K(k_tail)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
+__global__ SG_WAVES(SG_TAIL_WAVES) void SG_K(k_tail)(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const uint32_t id = queue[i];

@@ -12,22 +12,38 @@ using namespace sg;
 
 constexpr uint32_t kTraceBlock = 256;
 
-// Register budget of the heavy wavefront kernels, as waves per SIMD.
-// Waves per SIMD the fused and batch kernels are compiled for: one with the
-// landscape (its traversal state fills the register file), SG_PLAIN_WAVES
-// without it.
+// Register budgets, as waves per SIMD. At one wave a kernel has the whole
+// register file, and the traversals' memory latency goes unhidden; the
+// budgets below were chosen by timing on MI355X, spills included.
+//   fused and batch kernels: 2 with the landscape (about 130 registers
+//     spilled), 3 without it (about 80 spilled);
+//   k_extend: 4 (128 registers, none spilled);
+//   k_camera_air, k_shade: 2 (a few dozen spilled);
+//   k_tail: 1 (it runs the last few paths of a chunk).
+#ifndef SG_LANDSCAPE_WAVES
+#define SG_LANDSCAPE_WAVES 2
+#endif
 #ifndef SG_PLAIN_WAVES
 #define SG_PLAIN_WAVES 3
 #endif
 #if SG_LANDSCAPE
-#define SG_FUSED_WAVES 1
+#define SG_FUSED_WAVES SG_LANDSCAPE_WAVES
 #else
 #define SG_FUSED_WAVES SG_PLAIN_WAVES
 #endif
-#ifndef SG_HEAVY_WAVES
-#define SG_HEAVY_WAVES 1
+#ifndef SG_TAIL_WAVES
+#define SG_TAIL_WAVES 1
 #endif
-#define SG_HEAVY __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SG_HEAVY_WAVES, 8)))
+#ifndef SG_EXTEND_WAVES
+#define SG_EXTEND_WAVES 4
+#endif
+#ifndef SG_SHADE_WAVES
+#define SG_SHADE_WAVES 2
+#endif
+#ifndef SG_AIR_WAVES
+#define SG_AIR_WAVES 2
+#endif
+#define SG_WAVES(n) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
 struct TraceArgs {
     float4 *accumulation;

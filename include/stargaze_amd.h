@@ -161,7 +161,10 @@ enum sg_probe_kind {
     SG_PROBE_VISIBILITY = 3, /* light_visibility(ray, target)        rings.wgsl:90 */
     SG_PROBE_TERRAIN = 4,    /* terrain_query, metre frame           terrain.wgsl:152 */
     SG_PROBE_LANDSCAPE = 5,  /* landscape_hit(ray)                   ground.wgsl:123 */
-    SG_PROBE_CONE = 6        /* cone_width + sample_light draw       pathtrace.wgsl:197,214 */
+    SG_PROBE_CONE = 6,       /* cone_width + sample_light draw       pathtrace.wgsl:197,214 */
+    SG_PROBE_ALBEDO = 7      /* style albedo of bodies[target] at the local unit point
+                                q = direction, unclamped (plain style: surface_albedo),
+                                the material probe of surface_tests.rs:106-125 */
 };
 
 typedef struct sg_ctx sg_ctx;

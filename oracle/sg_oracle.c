@@ -332,7 +332,9 @@ static v3 milky_way(const Ctx *c, v3 view_direction) {
 /* --- procedural albedo (src/shaders/pathtrace.wgsl:338-470, 897-925) --- */
 static R hash31(v3 p) {
     v3 q = V(fract(p.x * 0.1031), fract(p.y * 0.1030), fract(p.z * 0.0973));
-    R d = dot(q, V(q.y + 33.33, q.x + 33.33, q.z + 33.33));
+    /* the dot product as two fused multiply-adds, in the order the HIP
+     * build evaluates it (WGSL leaves the contraction to the implementation) */
+    R d = fma(q.x, q.y + 33.33, fma(q.y, q.x + 33.33, q.z * (q.z + 33.33)));
     q = add(q, Vs(d));
     return fract((q.x + q.y) * q.z);
 }
@@ -424,11 +426,153 @@ static v3 gas_giant_albedo(const Body *body, v3 local) {
     }
     return color;
 }
+/* --- solid-body surfaces (src/shaders/pathtrace.wgsl:472-689) --- */
+static R surface_gaussian(R d) { return exp(-(d * d)); } /* wgsl:477 */
+static v3 hash33(v3 p) { /* wgsl:481 */
+    return V(hash31(p), hash31(add(p, V(19.1, 7.3, 3.7))), hash31(add(p, V(5.9, 31.7, 11.3))));
+}
+static R maria(v3 q, v3 off) { /* wgsl:492 */
+    R warp = fbm(add(scl(q, 2.0), off), 4) - 0.5;
+    return smooth(0.5, 0.56, fbm(add(add(scl(q, 1.2), off), Vs(1.5 * warp)), 5));
+}
+/* wgsl:502-511: the crater centre owned by a cell, or found = 0. */
+static int cell_centre(v3 cell, v3 key, R scale, v3 off, v3 *centre) {
+    for (uint32_t t = 0; t < 3u; ++t) {
+        v3 c = nrm(sub(add(cell, hash33(add(key, Vs((R)t * 17.0)))), off));
+        v3 g = add(scl(c, scale), off);
+        if (floor(g.x) == cell.x && floor(g.y) == cell.y && floor(g.z) == cell.z) {
+            *centre = c;
+            return 1;
+        }
+    }
+    return 0;
+}
+static v3 crater_grid(v3 base, uint32_t i) {
+    return add(base, V((R)(i & 1u), (R)((i >> 1) & 1u), (R)((i >> 2) & 1u)));
+}
+/* wgsl:527-633: one octave; st[0] floor multiplier, st[1] relief. */
+static void crater_layer(v3 q, R scale, R density, v3 off, v3 body_off, R flood, R contrast, R rays, R st[2]) {
+    v3 p = add(scl(q, scale), off);
+    v3 base = V(floor(p.x - 0.5), floor(p.y - 0.5), floor(p.z - 0.5));
+    uint32_t slots[6];
+    R ages[6];
+    v3 centres[6];
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < 16u; ++i) {
+        v3 grid = crater_grid(base, i);
+        v3 gap = V(mx(mx(grid.x - p.x, p.x - grid.x - 1.0), 0.0), mx(mx(grid.y - p.y, p.y - grid.y - 1.0), 0.0),
+                   mx(mx(grid.z - p.z, p.z - grid.z - 1.0), 0.0));
+        if (dot(gap, gap) > 0.2343) continue;
+        v3 key = add(grid, Vs((R)(i >> 3) * 1000.0));
+        if (hash31(add(key, V(71.3, 3.1, 9.7))) > density) continue;
+        v3 centre;
+        if (!cell_centre(grid, key, scale, off, &centre)) continue;
+        R size = hash31(add(key, V(3.1, 8.3, 1.7)));
+        R radius = (0.06 + 0.16 * size * size) / scale;
+        if (len(sub(q, centre)) > 2.2 * radius) continue;
+        R age = hash31(add(key, V(29.0, 61.0, 7.0)));
+        if (flood > 0.0 && age < 0.7 && maria(centre, body_off) > 0.5) continue;
+        if (count == 6u) {
+            if (age <= ages[0]) continue;
+            for (uint32_t m = 0; m < 5u; ++m) {
+                ages[m] = ages[m + 1];
+                slots[m] = slots[m + 1];
+                centres[m] = centres[m + 1];
+            }
+            count = 5u;
+        }
+        uint32_t j = count;
+        for (; j > 0u && ages[j - 1] > age; --j) {
+            ages[j] = ages[j - 1];
+            slots[j] = slots[j - 1];
+            centres[j] = centres[j - 1];
+        }
+        ages[j] = age;
+        slots[j] = i;
+        centres[j] = centre;
+        ++count;
+    }
+    R floor_shade = 1.0, relief = st[1];
+    for (uint32_t k = 0; k < count; ++k) {
+        uint32_t i = slots[k];
+        v3 cell = add(crater_grid(base, i), Vs((R)(i >> 3) * 1000.0));
+        v3 centre = centres[k];
+        R size = hash31(add(cell, V(3.1, 8.3, 1.7)));
+        R radius = (0.06 + 0.16 * size * size) / scale;
+        v3 offset = sub(q, centre);
+        R d = len(offset) / radius;
+        v3 east = cross(V(0.0, 0.0, 1.0), centre);
+        east = dvs(east, mx(len(east), 1.0e-4));
+        v3 north = cross(centre, east);
+        R angle = atan2(dot(offset, north), dot(offset, east));
+        v3 phase = scl(hash33(add(cell, V(41.0, 5.0, 23.0))), TAU);
+        R wobble = 0.025 * (sin(3.0 * angle + phase.x) + sin(5.0 * angle + phase.y) + 0.6 * sin(8.0 * angle + phase.z));
+        R r = d + wobble;
+        floor_shade = mn(floor_shade, 1.0 - 0.18 * contrast * (1.0 - smooth(0.55, 0.9, r)));
+        R rim = 0.22 * contrast * surface_gaussian((r - 1.0) / 0.13);
+        R footprint = 1.0 - smooth(1.1, 1.3, r);
+        relief = mixR(relief, rim, footprint);
+        R blanket = (1.0 - smooth(1.0, 1.9, r)) * (1.0 - footprint);
+        relief = mixR(relief, 0.0, blanket * 0.3) + 0.1 * contrast * blanket;
+        R fresh = hash31(add(cell, V(13.0, 2.0, 5.0)));
+        if (rays > 0.0 && fresh > 0.82) {
+            R streaks = pow(vnoise(V(cos(angle) * 9.0, sin(angle) * 9.0, fresh * 40.0)), 3.0);
+            relief += contrast * (rays * 0.9 * streaks * (1.0 - smooth(1.0, 2.2, d)) * (d >= 0.9 ? 1.0 : 0.0) +
+                                  rays * 0.25 * (1.0 - smooth(0.0, 1.0, d)));
+        }
+    }
+    st[0] *= floor_shade;
+    st[1] = relief;
+}
+static R craters(v3 q, R density, v3 off, R flood, R contrast, R rays) { /* wgsl:636 */
+    if (density <= 0.0) return 1.0;
+    R st[2] = {1.0, 0.0};
+    crater_layer(q, 1.5, density, off, off, flood, contrast, rays, st);
+    crater_layer(q, 3.5, density, add(off, Vs(101.0)), off, flood, contrast, rays * 0.8, st);
+    crater_layer(q, 8.0, density, add(off, Vs(203.0)), off, flood, contrast * 0.85, rays * 0.5, st);
+    crater_layer(q, 18.0, density, add(off, Vs(307.0)), off, 0.0, contrast * 0.7, 0.0, st);
+    crater_layer(q, 40.0, density, add(off, Vs(401.0)), off, 0.0, contrast * 0.55, 0.0, st);
+    return st[0] * (1.0 + st[1]);
+}
+static R polar_caps(v3 q, R extent, v3 off) { /* wgsl:650 */
+    if (extent <= 0.0) return 0.0;
+    R edge = 1.0 - extent + 0.12 * (fbm(add(scl(q, 5.0), off), 5) - 0.5) + 0.05 * (fbm(add(scl(q, 18.0), off), 3) - 0.5);
+    return smooth(edge - 0.015, edge + 0.015, fabs(q.z));
+}
+static v3 cratered_albedo(const Body *body, v3 q) { /* wgsl:660 */
+    v3 off = seed_offset(body->axis_y.v[3]);
+    R mare = maria(q, off);
+    R grain = fbm(add(scl(q, 45.0), off), 3);
+    v3 color = mixV(V4(body->material), V4(body->surface_color), mare);
+    color = scl(color, 0.9 + 0.2 * grain);
+    return scl(color, craters(q, body->surface_color.v[3], off, 1.0, 1.0, 1.0));
+}
+static v3 rocky_albedo(const Body *body, v3 q) { /* wgsl:673 */
+    v3 off = seed_offset(body->axis_y.v[3]);
+    R wx = fbm(add(scl(q, 1.8), off), 4) - 0.5;
+    R wy = fbm(add(add(scl(q, 1.8), off), Vs(7.0)), 4) - 0.5;
+    R dark = smooth(0.5, 0.64, fbm(add(add(scl(q, 2.2), off), V(2.0 * wx, 2.0 * wy, wx)), 6));
+    R tone = fbm(add(add(scl(q, 5.0), off), Vs(3.0)), 4);
+    v3 color = mixV(V4(body->material), V4(body->surface_color), dark * 0.85);
+    color = scl(color, 0.85 + 0.3 * tone);
+    R streak = fbm(add(add(V(q.x * 4.0, q.y * 4.0, q.z * 18.0), off), Vs(9.0)), 3);
+    color = scl(color, 0.95 + 0.1 * streak);
+    color = scl(color, craters(q, body->surface_color.v[3], off, 0.0, 0.45, 0.0));
+    R ice = polar_caps(q, body->surface_params.v[0], off);
+    return mixV(color, V(0.92, 0.92, 0.94), ice);
+}
+/* Unclamped albedo of the procedural styles 1-3 at the local point q. */
+static v3 style_albedo(const Body *body, v3 q) {
+    uint32_t style = (uint32_t)((R)body->axis_x.v[3] + 0.5);
+    if (style == 1u) return gas_giant_albedo(body, q);
+    if (style == 2u) return cratered_albedo(body, q);
+    return rocky_albedo(body, q);
+}
 static v3 clamp01(v3 a) { return V(clampR(a.x, 0.0, 1.0), clampR(a.y, 0.0, 1.0), clampR(a.z, 0.0, 1.0)); }
 static v3 surface_albedo(const Body *body, v3 normal) {
     v3 local = body_local(body, normal);
     uint32_t style = (uint32_t)((R)body->axis_x.v[3] + 0.5);
-    if (style > 0u) return clamp01(gas_giant_albedo(body, local)); /* styles 2-6 are rejected upstream */
+    if (style > 0u) return clamp01(style_albedo(body, local)); /* styles 4-6 are rejected upstream */
     R mottle = 1.0;
     if (body->low.v[3] > 0.0f) {
         v3 p = add(scl(local, 2.4), seed_offset(body->axis_y.v[3]));
@@ -1876,6 +2020,15 @@ void orc_probe(const orc_scene *sc, uint32_t kind, const orc_probe_ray *rays, ui
             r.normal[0] = n.x; r.normal[1] = n.y; r.normal[2] = n.z;
         } else if (kind == 6u) {
             r.distance = cone_width(&c, ray, pr->anchor == MISS ? 0u : pr->anchor);
+        } else if (kind == 7u) {
+            /* material probe of src/surface_tests.rs:106-125: the raw style
+             * albedo at the local point q = direction (plain style: the
+             * clamped surface albedo) */
+            const Body *b = B(&c, pr->target);
+            uint32_t style = (uint32_t)((R)b->axis_x.v[3] + 0.5);
+            v3 a = style > 0u ? style_albedo(b, ray.direction) : surface_albedo(b, ray.direction);
+            r.index = 0;
+            r.normal[0] = a.x; r.normal[1] = a.y; r.normal[2] = a.z;
         }
         out[i] = r;
     }

@@ -40,7 +40,8 @@ def _run(cmd: list[str], verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed with exit code {proc.returncode}: {cmd[-1]}")
 
 
-# Forward kernel units built a second time without the landscape (suffix _nl).
+# Forward kernel units built again without the landscape: with the cratered
+# and rocky styles (suffix _st) and without them (suffix _nl).
 PLAIN_VARIANTS = {"sg_k_trace", "sg_k_batch", "sg_k_wave_a", "sg_k_wave_b"}
 
 
@@ -50,7 +51,9 @@ def build(verbose: bool = False, jobs: int | None = None) -> Path:
 
     OBJECTS.mkdir(exist_ok=True)
     units = [(src, OBJECTS / (src.stem + ".o"), []) for src in SOURCES]
-    units += [(src, OBJECTS / (src.stem + "_nl.o"), ["-DSG_LANDSCAPE=0"])
+    units += [(src, OBJECTS / (src.stem + "_nl.o"), ["-DSG_LANDSCAPE=0", "-DSG_STYLES=0"])
+              for src in SOURCES if src.stem in PLAIN_VARIANTS]
+    units += [(src, OBJECTS / (src.stem + "_st.o"), ["-DSG_LANDSCAPE=0"])
               for src in SOURCES if src.stem in PLAIN_VARIANTS]
     # Longest first: the adjoint and batch units dominate.
     units.sort(key=lambda u: ("adjoint" not in u[0].stem, "batch" not in u[0].stem, "trace" not in u[0].stem))

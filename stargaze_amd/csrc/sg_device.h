@@ -101,6 +101,9 @@ struct Scene {
     // Tabulated relative air density (SG_DENSITY_BINS altitude slices up to
     // the top of the air) replacing exp(-h / H); null when not set.
     const float *density_profile;
+    // 1 when a lit body has the cratered or rocky style (kernels built with
+    // SG_STYLES are required).
+    uint32_t solid_styles;
 };
 
 struct Ray {
@@ -347,10 +350,16 @@ SG_FN f3 milky_way(const Scene &sc, f3 view_direction) {
 
 // --- procedural albedo (src/shaders/pathtrace.wgsl:338-470, 897-925) ------
 SG_FN float hash31(f3 p) {
-    f3 q = {fractf(p.x * 0.1031f), fractf(p.y * 0.1030f), fractf(p.z * 0.0973f)};
-    float s = dot(q, f3{q.y + 33.33f, q.x + 33.33f, q.z + 33.33f});
-    q = q + f3s(s);
-    return fractf((q.x + q.y) * q.z);
+    SG_EXACT
+    // Scalar operations, so the pragma covers every one of them; the dot
+    // product is two explicit fused multiply-adds in this order, as in the
+    // oracle, so both evaluate the hash bit for bit.
+    float qx = fractf(p.x * 0.1031f), qy = fractf(p.y * 0.1030f), qz = fractf(p.z * 0.0973f);
+    const float d = fmaf(qx, qy + 33.33f, fmaf(qy, qx + 33.33f, qz * (qz + 33.33f)));
+    qx += d;
+    qy += d;
+    qz += d;
+    return fractf((qx + qy) * qz);
 }
 SG_FN float vnoise(f3 p) {
     f3 i = {floorf(p.x), floorf(p.y), floorf(p.z)};
@@ -384,6 +393,13 @@ SG_FN f3 seed_offset(float seed) {
     return f3{hash31(f3{seed, 1.0f, 7.0f}), hash31(f3{3.0f, seed, 5.0f}), hash31(f3{9.0f, 2.0f, seed})} *
            512.0f;
 }
+// The cratered and rocky albedos are called out of line on the GPU, which
+// keeps their crater lists out of the callers' register allocation.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define SG_OUTLINE static __device__ __attribute__((noinline))
+#else
+#define SG_OUTLINE static inline
+#endif
 // src/shaders/pathtrace.wgsl:394-470
 SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
     f3 off = seed_offset(b.axis_y.w);
@@ -443,8 +459,174 @@ SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
     }
     return color;
 }
-// src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1 (gas giant).
-// Styles 2-6 are rejected by the host library before any launch.
+SG_FN uint32_t surface_style(const Body &b) { return (uint32_t)(b.axis_x.w + 0.5f); }
+// --- Solid-body surfaces (src/shaders/pathtrace.wgsl:472-689) ------------
+// wgsl:477-479
+SG_FN float surface_gaussian(float d) { return expf(-(d * d)); }
+// wgsl:481-484
+SG_FN f3 hash33(f3 p) {
+    return f3{hash31(p), hash31(p + f3{19.1f, 7.3f, 3.7f}), hash31(p + f3{5.9f, 31.7f, 11.3f})};
+}
+// Dark lava plains in [0, 1] (wgsl:492-495).
+SG_FN float maria(f3 q, f3 off) {
+    const float warp = fbm(q * 2.0f + off, 4u) - 0.5f;
+    return smoothstepf(0.5f, 0.56f, fbm(q * 1.2f + off + f3s(1.5f * warp), 5u));
+}
+// The crater owned by grid cell `cell` (wgsl:502-511): a jittered point of
+// the cell, projected onto the unit sphere, kept if the projection stays in
+// the cell; three jitters are tried. w = 0 when the cell has none.
+SG_FN float4 cell_centre(f3 cell, f3 key, float scale, f3 off) {
+    for (uint32_t t = 0; t < 3u; ++t) {
+        const f3 c = normalize(cell + hash33(key + f3s((float)t * 17.0f)) - off);
+        const f3 g = c * scale + off;
+        if (floorf(g.x) == cell.x && floorf(g.y) == cell.y && floorf(g.z) == cell.z)
+            return make_float4(c.x, c.y, c.z, 1.0f);
+    }
+    return make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+}
+SG_FN f3 crater_cell(f3 base, uint32_t i) {
+    return base + f3{(float)(i & 1u), (float)((i >> 1) & 1u), (float)((i >> 2) & 1u)};
+}
+// One octave of craters (wgsl:527-633). state.x is the floor darkness (a
+// multiplier), state.y the relief brightness (additive). The craters that
+// reach q (two slots per cell, the 2x2x2 nearest cells) are painted oldest
+// first; at most the six youngest are kept.
+SG_FN float2 crater_layer(f3 q, float scale, float density, f3 off, f3 body_off, float flood, float contrast,
+                          float rays, float2 state) {
+    const f3 p = q * scale + off;
+    const f3 base = {floorf(p.x - 0.5f), floorf(p.y - 0.5f), floorf(p.z - 0.5f)};
+    uint32_t slots[6];
+    float ages[6];
+    f3 centres[6];
+    uint32_t count = 0;
+    for (uint32_t i = 0; i < 16u; ++i) {
+        const f3 grid = crater_cell(base, i);
+        const f3 gap = {fmaxf(fmaxf(grid.x - p.x, p.x - grid.x - 1.0f), 0.0f),
+                        fmaxf(fmaxf(grid.y - p.y, p.y - grid.y - 1.0f), 0.0f),
+                        fmaxf(fmaxf(grid.z - p.z, p.z - grid.z - 1.0f), 0.0f)};
+        if (dot(gap, gap) > 0.2343f) continue;
+        const f3 key = grid + f3s((float)(i >> 3) * 1000.0f);
+        if (hash31(key + f3{71.3f, 3.1f, 9.7f}) > density) continue;
+        const float4 found = cell_centre(grid, key, scale, off);
+        if (found.w == 0.0f) continue;
+        const f3 centre = {found.x, found.y, found.z};
+        const float size = hash31(key + f3{3.1f, 8.3f, 1.7f});
+        const float radius = (0.06f + 0.16f * size * size) / scale;
+        if (length(q - centre) > 2.2f * radius) continue;
+        const float age = hash31(key + f3{29.0f, 61.0f, 7.0f});
+        if (flood > 0.0f && age < 0.7f && maria(centre, body_off) > 0.5f) continue;
+        if (count == 6u) {
+            if (age <= ages[0]) continue;
+            for (uint32_t m = 0; m < 5u; ++m) {
+                ages[m] = ages[m + 1];
+                slots[m] = slots[m + 1];
+                centres[m] = centres[m + 1];
+            }
+            count = 5u;
+        }
+        uint32_t j = count;
+        while (j > 0u && ages[j - 1] > age) {
+            ages[j] = ages[j - 1];
+            slots[j] = slots[j - 1];
+            centres[j] = centres[j - 1];
+            --j;
+        }
+        ages[j] = age;
+        slots[j] = i;
+        centres[j] = centre;
+        ++count;
+    }
+    float floor_shade = 1.0f;
+    float relief = state.y;
+    for (uint32_t k = 0; k < count; ++k) {
+        const uint32_t i = slots[k];
+        const f3 cell = crater_cell(base, i) + f3s((float)(i >> 3) * 1000.0f);
+        const f3 centre = centres[k];
+        const float size = hash31(cell + f3{3.1f, 8.3f, 1.7f});
+        const float radius = (0.06f + 0.16f * size * size) / scale;
+        const f3 offset = q - centre;
+        const float d = length(offset) / radius;
+        f3 east = cross(f3{0.0f, 0.0f, 1.0f}, centre);
+        east = east / fmaxf(length(east), 1.0e-4f);
+        const f3 north = cross(centre, east);
+        const float angle = atan2f(dot(offset, north), dot(offset, east));
+        const f3 phase = hash33(cell + f3{41.0f, 5.0f, 23.0f}) * TAU;
+        const float wobble = 0.025f * (sinf(3.0f * angle + phase.x) + sinf(5.0f * angle + phase.y) +
+                                       0.6f * sinf(8.0f * angle + phase.z));
+        const float r = d + wobble;
+        floor_shade = fminf(floor_shade, 1.0f - 0.18f * contrast * (1.0f - smoothstepf(0.55f, 0.9f, r)));
+        const float rim = 0.22f * contrast * surface_gaussian((r - 1.0f) / 0.13f);
+        const float footprint = 1.0f - smoothstepf(1.1f, 1.3f, r);
+        relief = mixf(relief, rim, footprint);
+        const float blanket = (1.0f - smoothstepf(1.0f, 1.9f, r)) * (1.0f - footprint);
+        relief = mixf(relief, 0.0f, blanket * 0.3f) + 0.1f * contrast * blanket;
+        const float fresh = hash31(cell + f3{13.0f, 2.0f, 5.0f});
+        if (rays > 0.0f && fresh > 0.82f) {
+            const float n = vnoise(f3{cosf(angle) * 9.0f, sinf(angle) * 9.0f, fresh * 40.0f});
+            const float streaks = n * n * n;
+            relief += contrast * (rays * 0.9f * streaks * (1.0f - smoothstepf(1.0f, 2.2f, d)) * (d >= 0.9f ? 1.0f : 0.0f) +
+                                  rays * 0.25f * (1.0f - smoothstepf(0.0f, 1.0f, d)));
+        }
+    }
+    return make_float2(state.x * floor_shade, relief);
+}
+// Five octaves, large (oldest) to small (wgsl:636-647).
+SG_FN float craters(f3 q, float density, f3 off, float flood, float contrast, float rays) {
+    if (density <= 0.0f) return 1.0f;
+    float2 st = crater_layer(q, 1.5f, density, off, off, flood, contrast, rays, make_float2(1.0f, 0.0f));
+    st = crater_layer(q, 3.5f, density, off + f3s(101.0f), off, flood, contrast, rays * 0.8f, st);
+    st = crater_layer(q, 8.0f, density, off + f3s(203.0f), off, flood, contrast * 0.85f, rays * 0.5f, st);
+    st = crater_layer(q, 18.0f, density, off + f3s(307.0f), off, 0.0f, contrast * 0.7f, 0.0f, st);
+    st = crater_layer(q, 40.0f, density, off + f3s(401.0f), off, 0.0f, contrast * 0.55f, 0.0f, st);
+    return st.x * (1.0f + st.y);
+}
+// Polar ice down to |sin(latitude)| = 1 - extent, ragged (wgsl:650-656).
+SG_FN float polar_caps(f3 q, float extent, f3 off) {
+    if (extent <= 0.0f) return 0.0f;
+    const float edge = 1.0f - extent + 0.12f * (fbm(q * 5.0f + off, 5u) - 0.5f) + 0.05f * (fbm(q * 18.0f + off, 3u) - 0.5f);
+    return smoothstepf(edge - 0.015f, edge + 0.015f, fabsf(q.z));
+}
+// Moon-like: highlands, flooded maria, craters, regolith grain (wgsl:660-670).
+SG_OUTLINE f3 cratered_albedo(const Body &b, f3 q) {
+    const f3 off = seed_offset(b.axis_y.w);
+    const float mare = maria(q, off);
+    const float grain = fbm(q * 45.0f + off, 3u);
+    f3 color = mix3(xyz(b.material), xyz(b.surface_color), mare);
+    color = color * (0.9f + 0.2f * grain);
+    return color * craters(q, b.surface_color.w, off, 1.0f, 1.0f, 1.0f);
+}
+// Mars-like: ochre plains, dark regions, soft craters, streaks, polar caps
+// (wgsl:673-689).
+SG_OUTLINE f3 rocky_albedo(const Body &b, f3 q) {
+    const f3 off = seed_offset(b.axis_y.w);
+    const float wx = fbm(q * 1.8f + off, 4u) - 0.5f, wy = fbm(q * 1.8f + off + f3s(7.0f), 4u) - 0.5f;
+    const float dark = smoothstepf(0.5f, 0.64f, fbm(q * 2.2f + off + f3{2.0f * wx, 2.0f * wy, wx}, 6u));
+    const float tone = fbm(q * 5.0f + off + f3s(3.0f), 4u);
+    f3 color = mix3(xyz(b.material), xyz(b.surface_color), dark * 0.85f);
+    color = color * (0.85f + 0.3f * tone);
+    const float streak = fbm(f3{q.x * 4.0f, q.y * 4.0f, q.z * 18.0f} + off + f3s(9.0f), 3u);
+    color = color * (0.95f + 0.1f * streak);
+    color = color * craters(q, b.surface_color.w, off, 0.0f, 0.45f, 0.0f);
+    const float ice = polar_caps(q, b.surface_params.x, off);
+    return mix3(color, f3{0.92f, 0.92f, 0.94f}, ice);
+}
+// The unclamped albedo of procedural styles 1-3 at the local point q.
+// Built without SG_STYLES, a kernel serves frames whose bodies use styles 0
+// and 1 only (the host library selects the kernels).
+SG_FN f3 style_albedo(const Body &b, f3 q) {
+#if SG_STYLES
+    switch (surface_style(b)) {
+    case 1u: return gas_giant_albedo(b, q);
+    case 2u: return cratered_albedo(b, q);
+    default: return rocky_albedo(b, q);
+    }
+#else
+    return gas_giant_albedo(b, q);
+#endif
+}
+// src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1-3 (gas
+// giant, cratered, rocky). Styles 4-6 are rejected by the host library
+// before any launch.
 // Style 0: the albedo is material.rgb times this scalar mottle, clamped.
 SG_FN float surface_mottle(const Body &b, f3 local) {
     float mottle = 1.0f;
@@ -460,14 +642,13 @@ SG_FN float surface_mottle(const Body &b, f3 local) {
     }
     return mottle;
 }
-SG_FN uint32_t surface_style(const Body &b) { return (uint32_t)(b.axis_x.w + 0.5f); }
 SG_FN f3 surface_albedo(const Body &b, f3 normal) {
     f3 local = body_local(b, normal);
-    if (surface_style(b) > 0u) return clamp3(gas_giant_albedo(b, local), 0.0f, 1.0f);
+    if (surface_style(b) > 0u) return clamp3(style_albedo(b, local), 0.0f, 1.0f);
     return clamp3(xyz(b.material) * surface_mottle(b, local), 0.0f, 1.0f);
 }
 // d albedo / d material.rgb for style 0 (zero where clamped, and for the
-// gas-giant style, whose albedo the adjoint does not differentiate).
+// procedural styles, whose albedos the adjoint does not differentiate).
 SG_FN f3 surface_albedo_derivative(const Body &b, f3 normal) {
     if (surface_style(b) > 0u) return f3s(0.0f);
     const float m = surface_mottle(b, body_local(b, normal));

@@ -1,6 +1,8 @@
 // The adjoint kernel (path replay) and the gradient fill.
-// The adjoint rejects frames with a ground scene: compile without it.
+// The adjoint rejects frames with a ground scene or with cratered or rocky
+// surfaces: compile without them.
 #define SG_LANDSCAPE 0
+#define SG_STYLES 0
 #include "sg_blocks.h"
 
 namespace sgk {

@@ -176,6 +176,13 @@ __global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint3
         r.normal[0] = h.x; r.normal[1] = h.y; r.normal[2] = h.z;
         break;
     }
+    case SG_PROBE_ALBEDO: {
+        const Body &b = body(sc, pr.target);
+        const f3 a = surface_style(b) > 0u ? style_albedo(b, ray.direction) : surface_albedo(b, ray.direction);
+        r.index = 0u;
+        r.normal[0] = a.x; r.normal[1] = a.y; r.normal[2] = a.z;
+        break;
+    }
     case SG_PROBE_CONE: {
         r.distance = cone_width(sc, ray, pr.anchor == SG_MISS ? 0u : pr.anchor);
         break;

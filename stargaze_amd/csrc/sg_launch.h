@@ -156,9 +156,12 @@ struct DisplayArgs {
 
 // Kernels. The forward kernels (sg_k_trace.hip, sg_k_batch.hip,
 // sg_k_wave_a.hip, sg_k_wave_b.hip) are compiled twice: as written, and with
-// SG_LANDSCAPE=0 under the suffix _nl, for frames without a ground scene.
+// SG_LANDSCAPE=0 under the suffixes _st (with the cratered and rocky
+// styles) and _nl (SG_STYLES=0 too), for frames without a ground scene.
 #if SG_LANDSCAPE
 #define SG_K(name) name
+#elif SG_STYLES
+#define SG_K(name) name##_st
 #else
 #define SG_K(name) name##_nl
 #endif
@@ -175,6 +178,7 @@ struct DisplayArgs {
                                     uint32_t pixels);
 SG_FORWARD_KERNELS()
 SG_FORWARD_KERNELS(_nl)
+SG_FORWARD_KERNELS(_st)
 // Compiled once (sg_k_adjoint.hip without the landscape, sg_k_misc.hip with it).
 __global__ void k_batch_reset(BatchArgs ba);
 __global__ void k_batch_advance(BatchArgs ba);

@@ -27,10 +27,13 @@
 using namespace sg;
 using namespace sgk;
 
+// Procedural surface styles built: 0 plain, 1 gas giant, 2 cratered, 3 rocky.
+constexpr uint32_t kMaxSurfaceStyle = 3u;
+
 namespace {
 // The forward kernels for frames with a ground scene, and the ones compiled
 // without the landscape for every other frame (same results, fewer live
-// registers).
+// registers), with or without the cratered and rocky styles.
 struct ForwardKernels {
     decltype(&k_trace) trace;
     decltype(&k_batch<64>) batch64, batch128, batch256;
@@ -45,8 +48,21 @@ const ForwardKernels kLandscapeKernels{k_trace, k_batch<64>, k_batch<128>, k_bat
                                        k_camera_air, k_shade, k_tail, k_accumulate};
 const ForwardKernels kPlainKernels{k_trace_nl, k_batch_nl<64>, k_batch_nl<128>, k_batch_nl<256>, k_generate_nl,
                                    k_extend_nl, k_camera_air_nl, k_shade_nl, k_tail_nl, k_accumulate_nl};
+const ForwardKernels kStyledKernels{k_trace_st, k_batch_st<64>, k_batch_st<128>, k_batch_st<256>, k_generate_st,
+                                    k_extend_st, k_camera_air_st, k_shade_st, k_tail_st, k_accumulate_st};
+const ForwardKernels &forward_kernels(bool landscape, bool styled) {
+    return landscape ? kLandscapeKernels : (styled ? kStyledKernels : kPlainKernels);
+}
 const ForwardKernels &forward_kernels(const Scene &sc) {
-    return sc.s.globals.ground_north[3] != 0.0f ? kLandscapeKernels : kPlainKernels;
+    return forward_kernels(sc.s.globals.ground_north[3] != 0.0f, sc.solid_styles != 0u);
+}
+// 1 when a lit body of the frame has the cratered or rocky style.
+uint32_t solid_styles(const sg_frame *frame) {
+    for (uint32_t i = 0; i < frame->body_count; ++i) {
+        const uint32_t style = (uint32_t)(frame->bodies[i].axis_x[3] + 0.5f);
+        if (frame->bodies[i].emissive < 0.5f && style >= 2u) return 1u;
+    }
+    return 0u;
 }
 }  // namespace
 
@@ -279,8 +295,8 @@ int validate_frame(sg_ctx *ctx, const sg_frame *frame) {
         return fail(ctx, SG_E_INVALID, "globals.ground_counts[0] must equal the primitive count");
     for (uint32_t i = 0; i < frame->body_count; ++i) {
         const float style = frame->bodies[i].axis_x[3];
-        if (frame->bodies[i].emissive < 0.5f && (uint32_t)(style + 0.5f) > 1u)
-            return fail(ctx, SG_E_INVALID, "surface styles 2-6 (cratered, rocky, icy, volcanic, terran) are not built");
+        if (frame->bodies[i].emissive < 0.5f && (uint32_t)(style + 0.5f) > kMaxSurfaceStyle)
+            return fail(ctx, SG_E_INVALID, "surface styles 4-6 (icy, volcanic, terran) are not built");
     }
     const sg_globals *g = frame->globals;
     if (g->ground_north[3] != 0.0f) {
@@ -488,6 +504,7 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     int brc = ensure_bvh(ctx, frame);
     if (brc != SG_OK) return brc;
     plan.sc = make_scene(ctx, st);
+    plan.sc.solid_styles = solid_styles(frame);
     const Scene &sc = plan.sc;
     const bool terrain_on = g.ground_counts[1] != 0u && g.ground_north[3] != 0.0f;
     const uint32_t radius_bits = float_bits(g.ground_up[3]);
@@ -1087,6 +1104,8 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
     if (rc != SG_OK) return rc;
     if (frame->globals->ground_north[3] != 0.0f)
         return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate scenes with a landscape");
+    if (solid_styles(frame))
+        return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate cratered or rocky surfaces");
     if (ctx->last.count == 0 || ctx->last.samples + ctx->last.count != ctx->samples || history_key(ctx, frame) != ctx->key)
         return fail(ctx, SG_E_STATE, "sg_adjoint needs the frame of the last pass sg_encode rendered");
     join_batch(ctx);
@@ -1170,15 +1189,18 @@ int sg_adjoint_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
              sg_probe_result *results) {
     if (!ctx || !frame || !frame->globals || (count && (!rays || !results))) return SG_E_INVALID;
-    if (kind > SG_PROBE_CONE) return fail(ctx, SG_E_INVALID, "unknown probe kind");
+    if (kind > SG_PROBE_ALBEDO) return fail(ctx, SG_E_INVALID, "unknown probe kind");
     if (frame->body_count > SG_BODY_CAPACITY) return fail(ctx, SG_E_CAPACITY, "too many scene bodies");
     if (count == 0) return SG_OK;
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t a = rays[i].anchor;
         if (a != SG_MISS && a >= frame->body_count) return fail(ctx, SG_E_INVALID, "probe anchor out of range");
-        if ((kind == SG_PROBE_SPHERE || kind == SG_PROBE_RING || kind == SG_PROBE_VISIBILITY) &&
+        if ((kind == SG_PROBE_SPHERE || kind == SG_PROBE_RING || kind == SG_PROBE_VISIBILITY ||
+             kind == SG_PROBE_ALBEDO) &&
             rays[i].target >= frame->body_count)
             return fail(ctx, SG_E_INVALID, "probe target out of range");
+        if (kind == SG_PROBE_ALBEDO && (uint32_t)(frame->bodies[rays[i].target].axis_x[3] + 0.5f) > kMaxSurfaceStyle)
+            return fail(ctx, SG_E_INVALID, "surface styles 4-6 (icy, volcanic, terran) are not built");
     }
     if ((kind == SG_PROBE_TERRAIN || frame->globals->ground_counts[1] != 0u) && !ctx->terrain)
         return fail(ctx, SG_E_STATE, "terrain probe without uploaded terrain");
@@ -1229,6 +1251,7 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     int brc = ensure_bvh(ctx, frame);
     if (brc != SG_OK) return brc;
     Scene sc = make_scene(ctx, st);
+    sc.solid_styles = solid_styles(frame);
     const uint32_t radius_bits = float_bits(st.globals.ground_up[3]);
     if (ctx->terrain && (!ctx->terrain_cached || ctx->terrain_radius_bits != radius_bits)) {
         hipLaunchKernelGGL(k_terrain_cache, dim3((unsigned)((sc.t.vertices + 255) / 256)), dim3(256), 0, ctx->stream,
@@ -1272,6 +1295,7 @@ struct sg_batch {
     std::vector<uint32_t> entry_view;
     uint32_t total_blocks = 0;
     bool landscape = false;  // some view of the table has a ground scene
+    bool styled = false;     // some view has a cratered or rocky body
     bool table_valid = false;
     bool pass_queued = false;
     // Graph of one pass for the current table shape and launch.
@@ -1332,7 +1356,7 @@ int queue_pass(sg_batch *b) {
     hipLaunchKernelGGL(k_batch_reset, small, dim3(256), 0, b->stream, ba);
     SG_BHIP(b, hipGetLastError());
     const dim3 grid(b->launch.workgroups);
-    const ForwardKernels &fk = b->landscape ? kLandscapeKernels : kPlainKernels;
+    const ForwardKernels &fk = forward_kernels(b->landscape, b->styled);
     switch (b->launch.block) {
     case 64: hipLaunchKernelGGL(fk.batch64, grid, dim3(64), 0, b->stream, ba); break;
     case 128: hipLaunchKernelGGL(fk.batch128, grid, dim3(128), 0, b->stream, ba); break;
@@ -1429,11 +1453,16 @@ int batch_pass(sg_batch *b, const sg_frame *frames) {
             std::memcpy(b->staging, table.data(), table.size() * sizeof(BatchView));
             SG_BHIP(b, hipMemcpyAsync(b->table, b->staging, table.size() * sizeof(BatchView), hipMemcpyHostToDevice,
                                       b->stream));
-            bool landscape = false;
-            for (const BatchView &v : table) landscape |= v.scene.s.globals.ground_north[3] != 0.0f;
-            if (table.size() != b->entry_view.size() || total_blocks != b->total_blocks || landscape != b->landscape)
+            bool landscape = false, styled = false;
+            for (const BatchView &v : table) {
+                landscape |= v.scene.s.globals.ground_north[3] != 0.0f;
+                styled |= v.scene.solid_styles != 0u;
+            }
+            if (table.size() != b->entry_view.size() || total_blocks != b->total_blocks ||
+                landscape != b->landscape || styled != b->styled)
                 b->graph_valid = false;
             b->landscape = landscape;
+            b->styled = styled;
             b->entry_view = entry_view;
             b->total_blocks = total_blocks;
             b->table_valid = true;

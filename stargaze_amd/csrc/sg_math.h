@@ -16,6 +16,12 @@
 #ifndef SG_LANDSCAPE
 #define SG_LANDSCAPE 1
 #endif
+// 0 compiles the cratered and rocky surface styles out: their code raises the
+// register pressure of the shading kernels even for frames that never use
+// them, so the kernels without the landscape are built both ways.
+#ifndef SG_STYLES
+#define SG_STYLES 1
+#endif
 
 template <class T> struct V3 { T x, y, z; };
 typedef V3<float> f3;
@@ -62,6 +68,11 @@ SG_FN float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 SG_FN float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 SG_FN f3 clamp3(f3 a, float lo, float hi) { return {clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)}; }
 // WGSL mix/smoothstep/fract semantics.
+// Exact IEEE evaluation, no implicit multiply-add contraction, for the noise
+// hashes: they take fract() of sums of products near 50, so one contraction
+// moves a hash by about 1e-4, and seed offsets scale hashes by 512. Placed
+// first in a function body; explicit fmaf calls stay fused.
+#define SG_EXACT _Pragma("clang fp contract(off)")
 SG_FN float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
 SG_FN f3 mix3(f3 a, f3 b, float t) { return a * (1.0f - t) + b * t; }
 SG_FN float smoothstepf(float e0, float e1, float x) {

@@ -294,7 +294,7 @@ class PathTracer:
 
 
 INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
-PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE = range(7)
+PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE, PROBE_ALBEDO = range(8)
 
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST, BATCH_ORDER_LONGEST = 0, 1, 2
 

@@ -170,6 +170,91 @@ def ringed_giant(width: int = 1920, height: int = 1080, seed: int = 0x5EED1234) 
     return f
 
 
+# Style codes of body.axis_x.w (src/sim.rs:109-118).
+STYLE_PLAIN, STYLE_GAS_GIANT, STYLE_CRATERED, STYLE_ROCKY = 0, 1, 2, 3
+
+
+def style_surface(b: np.ndarray, style: int, seed: float = 3.0, craters: float | None = None,
+                  ice_caps: float | None = None, secondary=None) -> np.ndarray:
+    """Give body ``b`` a procedural surface with the reference's defaults for
+    the style (src/config.rs:216-251): the secondary albedo is a darker tone
+    of the body's albedo, craters 0.8 (cratered) or 0.5 (rocky), ice caps
+    0.06 (rocky). The record layout is the one the shader reads: style in
+    axis_x.w, seed in axis_y.w, secondary albedo and crater density in
+    surface_color, ice caps in surface_params.x (pathtrace.wgsl:660-689)."""
+    albedo = [float(c) for c in b["color"]]
+    peak = max(max(albedo), 1.0e-6)
+
+    def deeper(scale, power):
+        return [scale * c * (c / peak) ** power for c in albedo]
+
+    defaults = {STYLE_CRATERED: (deeper(0.55, 0.6), 0.8, 0.0), STYLE_ROCKY: (deeper(0.6, 1.0), 0.5, 0.06)}
+    second, crater_default, ice_default = defaults[style]
+    if secondary is not None:
+        second = list(secondary)
+    b["axis_x"][3] = float(style)
+    b["axis_y"][3] = float(seed)
+    b["surface_color"] = [*second, crater_default if craters is None else craters]
+    b["surface_params"] = [ice_default if ice_caps is None else ice_caps, 0.0, 0.0, 0.0]
+    return b
+
+
+def material_frame(style: int, albedo=(0.62, 0.6, 0.57), seed: float = 3.0, **surface) -> Frame:
+    """A frame whose body 1 carries a procedural surface, for the material
+    probe (src/surface_tests.rs:61-86, 87-169): star, then the styled body
+    with its spin frame along the world axes."""
+    f = frame()
+    camera(f.globals, 45.0, 64, 64)
+    b = body([0, 0, 0], metres(1.7e6), list(albedo), False)
+    b["axis_x"][:3] = [1.0, 0.0, 0.0]
+    b["axis_y"][:3] = [0.0, 1.0, 0.0]
+    if style == STYLE_PLAIN:
+        b["axis_y"][3] = float(seed)
+        b["center_low"][3] = 0.5
+    else:
+        style_surface(b, style, seed, **surface)
+    place(b, np.array([0.0, 0.0, -metres(1.0e7)]))
+    sun = star(direction(-40.0, 20.0) * 9.5, metres(7.0e8))
+    f.bodies = bodies(sun, b)
+    return f
+
+
+def styled_moons(width: int = 64, height: int = 48, seed: float = 3.0) -> Frame:
+    """A cratered moon and a rocky moon (with polar caps) side by side, lit
+    from the side by one star, filling most of the view."""
+    f = frame()
+    camera(f.globals, 50.0, width, height)
+    moons = []
+    for style, albedo, x in ((STYLE_CRATERED, (0.62, 0.6, 0.57), -1.9e6), (STYLE_ROCKY, (0.7, 0.45, 0.3), 1.9e6)):
+        b = body([0, 0, 0], metres(1.7e6), list(albedo), False)
+        pole = np.array([0.0, math.cos(0.4), math.sin(0.4)])
+        b["axis_x"][:3] = [1.0, 0.0, 0.0]
+        b["axis_y"][:3] = np.cross(pole, [1.0, 0.0, 0.0])
+        style_surface(b, style, seed, ice_caps=0.25 if style == STYLE_ROCKY else None)
+        moons.append(place(b, np.array([metres(x), 0.0, -metres(8.0e6)])))
+    sun = star(direction(-130.0, 25.0) * 9.5, metres(7.0e8))
+    f.bodies = bodies(sun, *moons)
+    return f
+
+
+def material_rays(width: int = 256, height: int = 128) -> np.ndarray:
+    """Unit points of a latitude-longitude grid (src/surface_tests.rs:108-113),
+    as probe rays aimed at body 1."""
+    from .layouts import PROBE_RAY
+
+    y, x = np.mgrid[0:height, 0:width]
+    u = (x + 0.5) / width
+    v = (y + 0.5) / height
+    lat = (v - 0.5) * math.pi
+    lon = u * 2.0 * math.pi
+    q = np.stack([np.cos(lat) * np.cos(lon), np.cos(lat) * np.sin(lon), np.sin(lat)], -1).reshape(-1, 3)
+    rays = np.zeros(q.shape[0], PROBE_RAY)
+    rays["direction"] = q.astype(np.float32)
+    rays["anchor"] = 0xFFFFFFFF
+    rays["target"] = 1
+    return rays
+
+
 WORKLOADS = {
     "airless_triple_256": (airless_triple, dict(width=256, height=256), 16, 4),
     "ringed_giant_1080p": (ringed_giant, dict(width=1920, height=1080), 64, 6),

@@ -22,6 +22,9 @@
  *   atmosphere ............ zenith extinction x/(2 sinh(x/2)), eclipse = 0
  *   terrain_query ......... exhaustive f64 triangles
  *   props ................. box / ellipsoid closed forms
+ *   cratered/rocky albedo . the material probe (finite, non-negative, never
+ *                           black), polar-cap and crater-floor closed forms
+ *                           (tests/test_surface_styles.py)
  * Functions without such a pin are listed as "parity unpinned" in the
  * oracle's documentation: gas_giant_albedo, forest_floor, conifer/grass
  * silhouettes, milky_way.

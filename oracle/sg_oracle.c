@@ -429,7 +429,7 @@ static v3 gas_giant_albedo(const Body *body, v3 local) {
     }
     return color;
 }
-/* --- solid-body surfaces (src/shaders/pathtrace.wgsl:472-689) --- */
+/* --- solid-body surfaces (src/shaders/pathtrace.wgsl:472-895) --- */
 static R surface_gaussian(R d) { return exp(-(d * d)); } /* wgsl:477 */
 static v3 hash33(v3 p) { /* wgsl:481 */
     return V(hash31(p), hash31(add(p, V(19.1, 7.3, 3.7))), hash31(add(p, V(5.9, 31.7, 11.3))));
@@ -564,18 +564,190 @@ static v3 rocky_albedo(const Body *body, v3 q) { /* wgsl:673 */
     R ice = polar_caps(q, body->surface_params.v[0], off);
     return mixV(color, V(0.92, 0.92, 0.94), ice);
 }
-/* Unclamped albedo of the procedural styles 1-3 at the local point q. */
+static R ridge(R n, R width) { return 1.0 - smooth(0.0, width, fabs(n - 0.5)); } /* wgsl:487 */
+static v3 ice_plates(v3 p) { /* wgsl:691-709 */
+    v3 base = V(floor(p.x), floor(p.y), floor(p.z));
+    R nearest = 10.0, second = 10.0, tone = 0.0;
+    for (int i = 0; i < 27; ++i) {
+        v3 cell = add(base, V((R)(i % 3 - 1), (R)((i / 3) % 3 - 1), (R)(i / 9 - 1)));
+        v3 delta = sub(add(add(cell, Vs(0.2)), scl(hash33(cell), 0.6)), p);
+        R d = dot(delta, delta);
+        if (d < nearest) {
+            second = nearest;
+            nearest = d;
+            tone = hash31(add(cell, Vs(57.0)));
+        } else {
+            second = mn(second, d);
+        }
+    }
+    return V(sqrt(nearest), sqrt(second) - sqrt(nearest), tone);
+}
+static v3 icy_albedo(const Body *body, v3 q) { /* wgsl:714-767 */
+    v3 off = seed_offset(body->axis_y.v[3]);
+    v3 ice = V4(body->material);
+    v3 rust = mul(V4(body->surface_color), V(0.72, 0.62, 0.56));
+    v3 warp = sub(V(fbm(add(add(scl(q, 3.1), off), Vs(11.0)), 3), fbm(add(add(scl(q, 3.1), off), Vs(23.0)), 3),
+                    fbm(add(add(scl(q, 3.1), off), Vs(37.0)), 3)),
+                  Vs(0.5));
+    R terrain = fbm(add(add(scl(q, 4.2), off), warp), 5);
+    R rubble = fbm(add(add(scl(q, 85.0), off), scl(warp, 7.0)), 4);
+    R chaos = smooth(0.42, 0.65, terrain + 0.14 * (rubble - 0.5));
+    v3 plates = ice_plates(add(add(scl(q, 72.0), off), scl(warp, 4.0)));
+    R seams = 1.0 - smooth(0.015, 0.07, plates.y);
+    R dust = chaos * (0.2 + 0.46 * rubble + 0.18 * plates.z);
+    v3 color = mixV(ice, rust, dust);
+    color = mixV(color, scl(rust, 0.8), seams * chaos * 0.35);
+    R ridged_plain = ridge(fbm(add(add(scl(q, 34.0), off), Vs(83.0)), 3), 0.025);
+    color = mixV(color, rust, ridged_plain * (0.1 + 0.14 * chaos));
+    color = scl(color, 0.87 + 0.22 * fbm(add(scl(q, 180.0), off), 3));
+    v3 warped = add(q, scl(warp, 0.1));
+    R roughness = 0.0015 * (rubble - 0.5);
+    R shoulder = 0.0, fissure = 0.0, fresh_ridge = 0.0;
+    for (uint32_t i = 0; i < 64u; ++i) {
+        v3 h = hash33(add(off, V((R)i * 3.7, 1.0, 2.0)));
+        v3 family = hash33(add(off, V((R)(i / 8u) * 17.3, 21.0, 7.0)));
+        v3 n = nrm(add(sub(scl(family, 2.0), Vs(1.0)), scl(sub(scl(h, 2.0), Vs(1.0)), 0.28)));
+        R shift = (hash31(add(off, V((R)i, 9.0, 4.0))) - 0.5) * 1.4;
+        int old_band = i < 10u;
+        R width = (old_band ? 0.0015 + 0.0022 * h.z : 0.0005 + 0.0011 * h.z) *
+                  (0.65 + 0.8 * vnoise(add(scl(q, 17.0), scl(h, 29.0))));
+        R bend = 0.03 * (vnoise(add(scl(q, 6.0), scl(family, 31.0))) - 0.5) +
+                 0.008 * (vnoise(add(scl(q, 24.0), scl(family, 37.0))) - 0.5);
+        R dist = fabs(dot(warped, n) - shift + roughness + bend);
+        R along = smooth(0.28, 0.62, vnoise(add(scl(q, 3.5), scl(h, 50.0))));
+        shoulder = mx(shoulder, (1.0 - smooth(width, width * 4.0, dist)) * along);
+        R double_ridge = fabs(dist - width * 0.85);
+        R single = 1.0 - smooth(width * 0.25, width, dist);
+        R paired = 1.0 - smooth(width * 0.15, width * 0.55, double_ridge);
+        fissure = mx(fissure, (old_band ? paired : single) * along);
+        fresh_ridge = mx(fresh_ridge, (1.0 - smooth(width * 0.15, width * 0.35, fabs(dist - width * 1.8))) * along);
+    }
+    color = mixV(color, rust, shoulder * 0.17);
+    color = mixV(color, scl(rust, 0.7), fissure * 0.8);
+    color = mixV(color, ice, fresh_ridge * 0.13);
+    return scl(color, craters(q, body->surface_color.v[3] * 0.2, off, 0.0, 0.3, 0.4));
+}
+static v3 volcanic_albedo(const Body *body, v3 q) { /* wgsl:770-851 */
+    v3 off = seed_offset(body->axis_y.v[3]);
+    v3 sulphur = V4(body->material), deposit = V4(body->surface_color);
+    v3 warp = sub(V(fbm(add(scl(q, 4.7), off), 3), fbm(add(add(scl(q, 4.7), off), Vs(19.0)), 3),
+                    fbm(add(add(scl(q, 4.7), off), Vs(41.0)), 3)),
+                  Vs(0.5));
+    R terrain = fbm(add(add(scl(q, 10.0), off), scl(warp, 2.0)), 6);
+    R weathered = smooth(0.38, 0.65, fbm(add(add(scl(q, 3.7), off), Vs(11.0)), 5));
+    R fine = fbm(add(add(scl(q, 90.0), off), scl(warp, 3.0)), 4);
+    v3 crust = mixV(mul(sulphur, V(0.43, 0.51, 0.6)), deposit, 0.12);
+    v3 color = scl(sulphur, 0.7 + 0.5 * terrain);
+    color = mixV(color, crust, weathered * (0.45 + 0.5 * terrain));
+    R frost = smooth(0.38, 0.74, fbm(add(add(add(scl(q, 6.0), off), Vs(29.0)), warp), 5) + 0.2 * (fine - 0.5));
+    v3 frost_color = mixV(sulphur, V(0.72, 0.7, 0.62), 0.8);
+    color = mixV(color, frost_color, frost * 0.7);
+    R flow_edges = ridge(fbm(add(add(scl(q, 35.0), off), scl(warp, 2.0)), 4), 0.025);
+    color = mixV(color, crust, flow_edges * weathered * 0.4);
+    color = scl(color, (0.82 + 0.32 * fine) * (0.94 + 0.13 * fbm(add(scl(q, 210.0), off), 3)));
+    R dark = 0.0, ash = 0.0, red = 0.0, white = 0.0;
+    R density = body->surface_color.v[3];
+    R scale = 5.5;
+    v3 p = add(add(scl(q, scale), off), Vs(300.0));
+    v3 base = V(floor(p.x), floor(p.y), floor(p.z));
+    for (int i = 0; i < 27; ++i) {
+        v3 cell = add(base, V((R)(i % 3 - 1), (R)((i / 3) % 3 - 1), (R)(i / 9 - 1)));
+        if (hash31(add(cell, V(4.0, 41.0, 9.0))) > 0.6 * density) continue;
+        v3 centre;
+        if (!cell_centre(cell, cell, scale, add(off, Vs(300.0)), &centre)) continue;
+        R size = hash31(add(cell, V(8.0, 1.0, 3.0)));
+        R radius = (0.025 + 0.11 * size * size * size) / scale;
+        v3 offset = sub(q, centre);
+        R reach = len(offset) * scale;
+        if (reach > 0.95) continue;
+        R cutoff = 1.0 - smooth(0.75, 0.95, reach);
+        v3 tangent = cross(centre, V(0.37, 0.83, 0.41));
+        v3 east = dvs(tangent, mx(len(tangent), 1.0e-5));
+        v3 north = cross(centre, east);
+        R angle = TAU * hash31(add(cell, Vs(91.0)));
+        R ux = dot(offset, east) / radius, uy = dot(offset, north) / radius;
+        R ca = cos(angle), sa = sin(angle);
+        R vx = ux * ca - uy * sa, vy = ux * sa + uy * ca;
+        R ragged = fbm(add(V(vx * 2.7, vy * 2.7, 0.0), scl(cell, 7.0)), 3) - 0.5;
+        R ex = vx * (1.0 / (0.65 + 0.9 * size));
+        R d = sqrt(ex * ex + vy * vy) + 0.42 * ragged;
+        R pit = 1.0 - smooth(0.68, 0.86, d);
+        R age = hash31(add(cell, Vs(31.0)));
+        R fx = vx - (0.2 + 2.3 * age), fy = vy - 0.35 * sin(vx * 1.3 + angle);
+        R sx = fx * (0.75 - 0.48 * age), sy = fy * (0.85 + 0.35 * size);
+        R flow_shape = sqrt(sx * sx + sy * sy) + 0.65 * ragged +
+                       0.35 * (fbm(add(V(vx * 1.8, vy * 1.8, 0.0), scl(cell, 3.0)), 4) - 0.5);
+        R flow = (1.0 - smooth(0.72, 0.9, flow_shape)) * smooth(0.35, 0.85, age) * 0.85;
+        dark = mx(dark, mx(pit, flow) * cutoff);
+        R plume = hash31(add(cell, V(2.0, 6.0, 17.0)));
+        R hx = (vx - 0.5) * 0.16, hy = (vy - 0.3) * 0.23;
+        R halo = exp(-(hx * hx + hy * hy)) * cutoff;
+        ash = mx(ash, halo * (0.25 + 0.35 * age));
+        white = mx(white, halo * (plume >= 0.45 ? 1.0 : 0.0) * (1.0 - (plume >= 0.83 ? 1.0 : 0.0)));
+        R apron = surface_gaussian((sqrt(vx * vx + vy * vy) + ragged - 3.5) / 1.6) *
+                  (0.25 + 0.75 * smooth(-1.5, 2.0, vy));
+        red = mx(red, apron * (plume >= 0.83 ? 1.0 : 0.0) * cutoff);
+    }
+    color = mixV(color, scl(crust, 0.7), ash * 0.45);
+    color = mixV(color, frost_color, white * 0.4);
+    color = mixV(color, mul(deposit, V(0.85, 0.7, 0.65)), red * 0.6);
+    return mixV(color, V(0.035, 0.032, 0.026), dark * 0.96);
+}
+static v3 terran_albedo(const Body *body, v3 q) { /* wgsl:854-895 */
+    v3 off = seed_offset(body->axis_y.v[3]);
+    v3 land_base = V4(body->material), ocean_color = V4(body->surface_color);
+    v3 warp = sub(V(fbm(add(scl(q, 1.5), off), 4), fbm(add(add(scl(q, 1.5), off), Vs(13.0)), 4),
+                    fbm(add(add(scl(q, 1.5), off), Vs(29.0)), 4)),
+                  Vs(0.5));
+    R height = fbm(add(add(scl(q, 1.1), off), scl(warp, 0.9)), 7);
+    R sea = 0.5 + ((R)body->surface_params.v[1] - 0.5) * 0.3;
+    R coast = smooth(sea, sea + 0.004, height);
+    R shallow = 1.0 - smooth(sea - 0.04, sea, height);
+    v3 water = mixV(add(scl(ocean_color, 1.8), V(0.0, 0.04, 0.05)), ocean_color, shallow);
+    R moisture = fbm(add(add(scl(q, 3.0), off), Vs(41.0)), 5) + 0.25 * (fabs(q.z) - 0.35);
+    v3 green = mul(land_base, V(0.45, 0.62, 0.38));
+    v3 land = mixV(land_base, green, smooth(0.45, 0.6, moisture));
+    R altitude = smooth(sea + 0.08, sea + 0.2, height);
+    land = mixV(land, V(0.42, 0.38, 0.34), altitude * 0.6);
+    land = mixV(land, V(0.9, 0.9, 0.92), smooth(sea + 0.19, sea + 0.24, height));
+    land = scl(land, 0.88 + 0.24 * fbm(add(scl(q, 25.0), off), 3));
+    v3 color = mixV(water, land, coast);
+    R ice = polar_caps(q, body->surface_params.v[0], off);
+    color = mixV(color, V(0.9, 0.92, 0.95), ice);
+    R cover = body->surface_params.v[2];
+    if (cover > 0.0) {
+        v3 w1 = sub(V(fbm(add(add(scl(q, 1.6), off), Vs(77.0)), 4), fbm(add(add(scl(q, 1.6), off), Vs(91.0)), 4), 0.5),
+                    Vs(0.5));
+        v3 w2 = sub(V(fbm(add(add(scl(q, 4.0), off), scl(w1, 3.0)), 4),
+                      fbm(add(add(add(scl(q, 4.0), off), Vs(17.0)), scl(w1, 3.0)), 4), 0.5),
+                    Vs(0.5));
+        v3 cq = add(add(add(V(q.x * 2.5, q.y * 2.5, q.z * 7.0), off), scl(w1, 2.5)), scl(w2, 1.2));
+        R cloud = fbm(cq, 7);
+        R lat = fabs(q.z);
+        R belts = 0.06 * (surface_gaussian(lat / 0.12) + surface_gaussian((lat - 0.72) / 0.15)) -
+                  0.04 * surface_gaussian((lat - 0.42) / 0.1);
+        R threshold = 0.62 - 0.22 * cover - belts;
+        R mask = smooth(threshold - 0.04, threshold + 0.14, cloud);
+        color = mixV(color, V(0.93, 0.94, 0.96), mask * 0.95);
+    }
+    return color;
+}
+/* Unclamped albedo of the procedural styles 1-6 at the local point q (codes
+ * above 6 are terran, as in the WGSL switch). */
 static v3 style_albedo(const Body *body, v3 q) {
     uint32_t style = (uint32_t)((R)body->axis_x.v[3] + 0.5);
     if (style == 1u) return gas_giant_albedo(body, q);
     if (style == 2u) return cratered_albedo(body, q);
-    return rocky_albedo(body, q);
+    if (style == 3u) return rocky_albedo(body, q);
+    if (style == 4u) return icy_albedo(body, q);
+    if (style == 5u) return volcanic_albedo(body, q);
+    return terran_albedo(body, q);
 }
 static v3 clamp01(v3 a) { return V(clampR(a.x, 0.0, 1.0), clampR(a.y, 0.0, 1.0), clampR(a.z, 0.0, 1.0)); }
 static v3 surface_albedo(const Body *body, v3 normal) {
     v3 local = body_local(body, normal);
     uint32_t style = (uint32_t)((R)body->axis_x.v[3] + 0.5);
-    if (style > 0u) return clamp01(style_albedo(body, local)); /* styles 4-6 are rejected upstream */
+    if (style > 0u) return clamp01(style_albedo(body, local));
     R mottle = 1.0;
     if (body->low.v[3] > 0.0f) {
         v3 p = add(scl(local, 2.4), seed_offset(body->axis_y.v[3]));

@@ -40,8 +40,8 @@ def _run(cmd: list[str], verbose: bool) -> None:
         raise RuntimeError(f"hipcc failed with exit code {proc.returncode}: {cmd[-1]}")
 
 
-# Forward kernel units built again without the landscape: with the cratered
-# and rocky styles (suffix _st) and without them (suffix _nl).
+# Forward kernel units built again without the landscape: with the solid-body
+# surface styles (suffix _st) and without them (suffix _nl).
 PLAIN_VARIANTS = {"sg_k_trace", "sg_k_batch", "sg_k_wave_a", "sg_k_wave_b"}
 
 

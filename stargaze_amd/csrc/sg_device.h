@@ -101,7 +101,7 @@ struct Scene {
     // Tabulated relative air density (SG_DENSITY_BINS altitude slices up to
     // the top of the air) replacing exp(-h / H); null when not set.
     const float *density_profile;
-    // 1 when a lit body has the cratered or rocky style (kernels built with
+    // 1 when a lit body has a solid-body style, 2-6 (kernels built with
     // SG_STYLES are required).
     uint32_t solid_styles;
 };
@@ -393,8 +393,9 @@ SG_FN f3 seed_offset(float seed) {
     return f3{hash31(f3{seed, 1.0f, 7.0f}), hash31(f3{3.0f, seed, 5.0f}), hash31(f3{9.0f, 2.0f, seed})} *
            512.0f;
 }
-// The cratered and rocky albedos are called out of line on the GPU, which
-// keeps their crater lists out of the callers' register allocation.
+// The solid-body albedos (styles 2-6) are called out of line on the GPU,
+// which keeps their crater lists and loops out of the callers' register
+// allocation.
 #if defined(__HIP_DEVICE_COMPILE__)
 #define SG_OUTLINE static __device__ __attribute__((noinline))
 #else
@@ -610,23 +611,205 @@ SG_OUTLINE f3 rocky_albedo(const Body &b, f3 q) {
     const float ice = polar_caps(q, b.surface_params.x, off);
     return mix3(color, f3{0.92f, 0.92f, 0.94f}, ice);
 }
-// The unclamped albedo of procedural styles 1-3 at the local point q.
+// Thin bright lines where the noise crosses 0.5 (wgsl:487-489).
+SG_FN float ridge(float n, float width) { return 1.0f - smoothstepf(0.0f, width, fabsf(n - 0.5f)); }
+// Nearest-cell distance, gap to the second nearest, and the nearest plate's
+// tone over all 27 neighbour cells (wgsl:691-709).
+SG_FN f3 ice_plates(f3 p) {
+    const f3 base = {floorf(p.x), floorf(p.y), floorf(p.z)};
+    float nearest = 10.0f, second = 10.0f, tone = 0.0f;
+    for (int i = 0; i < 27; ++i) {
+        const f3 cell = base + f3{(float)(i % 3 - 1), (float)((i / 3) % 3 - 1), (float)(i / 9 - 1)};
+        const f3 delta = cell + f3s(0.2f) + hash33(cell) * 0.6f - p;
+        const float d = dot(delta, delta);
+        if (d < nearest) {
+            second = nearest;
+            nearest = d;
+            tone = hash31(cell + f3s(57.0f));
+        } else {
+            second = fminf(second, d);
+        }
+    }
+    return f3{sqrtf(nearest), sqrtf(second) - sqrtf(nearest), tone};
+}
+// Europa-like: disrupted ice, plates, lineae in fracture families, sparse
+// craters (wgsl:714-767).
+SG_OUTLINE f3 icy_albedo(const Body &b, f3 q) {
+    const f3 off = seed_offset(b.axis_y.w);
+    const f3 ice = xyz(b.material);
+    const f3 rust = xyz(b.surface_color) * f3{0.72f, 0.62f, 0.56f};
+    const f3 warp = f3{fbm(q * 3.1f + off + f3s(11.0f), 3u), fbm(q * 3.1f + off + f3s(23.0f), 3u),
+                       fbm(q * 3.1f + off + f3s(37.0f), 3u)} -
+                    f3s(0.5f);
+    const float terrain = fbm(q * 4.2f + off + warp, 5u);
+    const float rubble = fbm(q * 85.0f + off + warp * 7.0f, 4u);
+    const float chaos = smoothstepf(0.42f, 0.65f, terrain + 0.14f * (rubble - 0.5f));
+    const f3 plates = ice_plates(q * 72.0f + off + warp * 4.0f);
+    const float seams = 1.0f - smoothstepf(0.015f, 0.07f, plates.y);
+    const float dust = chaos * (0.2f + 0.46f * rubble + 0.18f * plates.z);
+    f3 color = mix3(ice, rust, dust);
+    color = mix3(color, rust * 0.8f, seams * chaos * 0.35f);
+    const float ridged_plain = ridge(fbm(q * 34.0f + off + f3s(83.0f), 3u), 0.025f);
+    color = mix3(color, rust, ridged_plain * (0.1f + 0.14f * chaos));
+    color = color * (0.87f + 0.22f * fbm(q * 180.0f + off, 3u));
+    const f3 warped = q + warp * 0.1f;
+    const float roughness = 0.0015f * (rubble - 0.5f);
+    float shoulder = 0.0f, fissure = 0.0f, fresh_ridge = 0.0f;
+    for (uint32_t i = 0; i < 64u; ++i) {
+        const f3 h = hash33(off + f3{(float)i * 3.7f, 1.0f, 2.0f});
+        const f3 family = hash33(off + f3{(float)(i / 8u) * 17.3f, 21.0f, 7.0f});
+        const f3 n = normalize(family * 2.0f - f3s(1.0f) + (h * 2.0f - f3s(1.0f)) * 0.28f);
+        const float shift = (hash31(off + f3{(float)i, 9.0f, 4.0f}) - 0.5f) * 1.4f;
+        const bool old_band = i < 10u;
+        const float width = (old_band ? 0.0015f + 0.0022f * h.z : 0.0005f + 0.0011f * h.z) *
+                            (0.65f + 0.8f * vnoise(q * 17.0f + h * 29.0f));
+        const float bend = 0.03f * (vnoise(q * 6.0f + family * 31.0f) - 0.5f) +
+                           0.008f * (vnoise(q * 24.0f + family * 37.0f) - 0.5f);
+        const float dist = fabsf(dot(warped, n) - shift + roughness + bend);
+        const float along = smoothstepf(0.28f, 0.62f, vnoise(q * 3.5f + h * 50.0f));
+        shoulder = fmaxf(shoulder, (1.0f - smoothstepf(width, width * 4.0f, dist)) * along);
+        const float double_ridge = fabsf(dist - width * 0.85f);
+        const float single = 1.0f - smoothstepf(width * 0.25f, width, dist);
+        const float paired = 1.0f - smoothstepf(width * 0.15f, width * 0.55f, double_ridge);
+        fissure = fmaxf(fissure, (old_band ? paired : single) * along);
+        fresh_ridge = fmaxf(fresh_ridge,
+                            (1.0f - smoothstepf(width * 0.15f, width * 0.35f, fabsf(dist - width * 1.8f))) * along);
+    }
+    color = mix3(color, rust, shoulder * 0.17f);
+    color = mix3(color, rust * 0.7f, fissure * 0.8f);
+    color = mix3(color, ice, fresh_ridge * 0.13f);
+    return color * craters(q, b.surface_color.w * 0.2f, off, 0.0f, 0.3f, 0.4f);
+}
+// Io-like: sulphur plains, frost, paterae, lava lobes, plume deposits over
+// the 27 cells around the point (wgsl:770-851).
+SG_OUTLINE f3 volcanic_albedo(const Body &b, f3 q) {
+    const f3 off = seed_offset(b.axis_y.w);
+    const f3 sulphur = xyz(b.material);
+    const f3 deposit = xyz(b.surface_color);
+    const f3 warp = f3{fbm(q * 4.7f + off, 3u), fbm(q * 4.7f + off + f3s(19.0f), 3u),
+                       fbm(q * 4.7f + off + f3s(41.0f), 3u)} -
+                    f3s(0.5f);
+    const float terrain = fbm(q * 10.0f + off + warp * 2.0f, 6u);
+    const float weathered = smoothstepf(0.38f, 0.65f, fbm(q * 3.7f + off + f3s(11.0f), 5u));
+    const float fine = fbm(q * 90.0f + off + warp * 3.0f, 4u);
+    const f3 crust = mix3(sulphur * f3{0.43f, 0.51f, 0.6f}, deposit, 0.12f);
+    f3 color = sulphur * (0.7f + 0.5f * terrain);
+    color = mix3(color, crust, weathered * (0.45f + 0.5f * terrain));
+    const float frost = smoothstepf(0.38f, 0.74f, fbm(q * 6.0f + off + f3s(29.0f) + warp, 5u) + 0.2f * (fine - 0.5f));
+    const f3 frost_color = mix3(sulphur, f3{0.72f, 0.7f, 0.62f}, 0.8f);
+    color = mix3(color, frost_color, frost * 0.7f);
+    const float flow_edges = ridge(fbm(q * 35.0f + off + warp * 2.0f, 4u), 0.025f);
+    color = mix3(color, crust, flow_edges * weathered * 0.4f);
+    color = color * ((0.82f + 0.32f * fine) * (0.94f + 0.13f * fbm(q * 210.0f + off, 3u)));
+    float dark = 0.0f, ash = 0.0f, red = 0.0f, white = 0.0f;
+    const float density = b.surface_color.w;
+    const float scale = 5.5f;
+    const f3 p = q * scale + off + f3s(300.0f);
+    const f3 base = {floorf(p.x), floorf(p.y), floorf(p.z)};
+    for (int i = 0; i < 27; ++i) {
+        const f3 cell = base + f3{(float)(i % 3 - 1), (float)((i / 3) % 3 - 1), (float)(i / 9 - 1)};
+        if (hash31(cell + f3{4.0f, 41.0f, 9.0f}) > 0.6f * density) continue;
+        const float4 found = cell_centre(cell, cell, scale, off + f3s(300.0f));
+        if (found.w == 0.0f) continue;
+        const f3 centre = {found.x, found.y, found.z};
+        const float size = hash31(cell + f3{8.0f, 1.0f, 3.0f});
+        const float radius = (0.025f + 0.11f * size * size * size) / scale;
+        const f3 offset = q - centre;
+        const float reach = length(offset) * scale;
+        if (reach > 0.95f) continue;
+        const float cutoff = 1.0f - smoothstepf(0.75f, 0.95f, reach);
+        const f3 tangent = cross(centre, f3{0.37f, 0.83f, 0.41f});
+        const f3 east = tangent / fmaxf(length(tangent), 1.0e-5f);
+        const f3 north = cross(centre, east);
+        const float angle = TAU * hash31(cell + f3s(91.0f));
+        const float ux = dot(offset, east) / radius, uy = dot(offset, north) / radius;
+        const float ca = cosf(angle), sa = sinf(angle);
+        const float vx = ux * ca - uy * sa, vy = ux * sa + uy * ca;
+        const float ragged = fbm(f3{vx * 2.7f, vy * 2.7f, 0.0f} + cell * 7.0f, 3u) - 0.5f;
+        const float ex = vx * (1.0f / (0.65f + 0.9f * size));
+        const float d = sqrtf(ex * ex + vy * vy) + 0.42f * ragged;
+        const float pit = 1.0f - smoothstepf(0.68f, 0.86f, d);
+        const float age = hash31(cell + f3s(31.0f));
+        const float fx = vx - (0.2f + 2.3f * age), fy = vy - 0.35f * sinf(vx * 1.3f + angle);
+        const float sx = fx * (0.75f - 0.48f * age), sy = fy * (0.85f + 0.35f * size);
+        const float flow_shape = sqrtf(sx * sx + sy * sy) + 0.65f * ragged +
+                                 0.35f * (fbm(f3{vx * 1.8f, vy * 1.8f, 0.0f} + cell * 3.0f, 4u) - 0.5f);
+        const float flow = (1.0f - smoothstepf(0.72f, 0.9f, flow_shape)) * smoothstepf(0.35f, 0.85f, age) * 0.85f;
+        dark = fmaxf(dark, fmaxf(pit, flow) * cutoff);
+        const float plume = hash31(cell + f3{2.0f, 6.0f, 17.0f});
+        const float hx = (vx - 0.5f) * 0.16f, hy = (vy - 0.3f) * 0.23f;
+        const float halo = expf(-(hx * hx + hy * hy)) * cutoff;
+        ash = fmaxf(ash, halo * (0.25f + 0.35f * age));
+        white = fmaxf(white, halo * (plume >= 0.45f ? 1.0f : 0.0f) * (1.0f - (plume >= 0.83f ? 1.0f : 0.0f)));
+        const float apron = surface_gaussian((sqrtf(vx * vx + vy * vy) + ragged - 3.5f) / 1.6f) *
+                            (0.25f + 0.75f * smoothstepf(-1.5f, 2.0f, vy));
+        red = fmaxf(red, apron * (plume >= 0.83f ? 1.0f : 0.0f) * cutoff);
+    }
+    color = mix3(color, crust * 0.7f, ash * 0.45f);
+    color = mix3(color, frost_color, white * 0.4f);
+    color = mix3(color, deposit * f3{0.85f, 0.7f, 0.65f}, red * 0.6f);
+    return mix3(color, f3{0.035f, 0.032f, 0.026f}, dark * 0.96f);
+}
+// Earth-like: oceans and shallows, continents by moisture and altitude,
+// polar ice, cloud belts (wgsl:854-895).
+SG_OUTLINE f3 terran_albedo(const Body &b, f3 q) {
+    const f3 off = seed_offset(b.axis_y.w);
+    const f3 land_base = xyz(b.material);
+    const f3 ocean_color = xyz(b.surface_color);
+    const f3 warp = f3{fbm(q * 1.5f + off, 4u), fbm(q * 1.5f + off + f3s(13.0f), 4u),
+                       fbm(q * 1.5f + off + f3s(29.0f), 4u)} -
+                    f3s(0.5f);
+    const float height = fbm(q * 1.1f + off + warp * 0.9f, 7u);
+    const float sea = 0.5f + (b.surface_params.y - 0.5f) * 0.3f;
+    const float coast = smoothstepf(sea, sea + 0.004f, height);
+    const float shallow = 1.0f - smoothstepf(sea - 0.04f, sea, height);
+    const f3 water = mix3(ocean_color * 1.8f + f3{0.0f, 0.04f, 0.05f}, ocean_color, shallow);
+    const float moisture = fbm(q * 3.0f + off + f3s(41.0f), 5u) + 0.25f * (fabsf(q.z) - 0.35f);
+    const f3 green = land_base * f3{0.45f, 0.62f, 0.38f};
+    f3 land = mix3(land_base, green, smoothstepf(0.45f, 0.6f, moisture));
+    const float altitude = smoothstepf(sea + 0.08f, sea + 0.2f, height);
+    land = mix3(land, f3{0.42f, 0.38f, 0.34f}, altitude * 0.6f);
+    land = mix3(land, f3{0.9f, 0.9f, 0.92f}, smoothstepf(sea + 0.19f, sea + 0.24f, height));
+    land = land * (0.88f + 0.24f * fbm(q * 25.0f + off, 3u));
+    f3 color = mix3(water, land, coast);
+    const float ice = polar_caps(q, b.surface_params.x, off);
+    color = mix3(color, f3{0.9f, 0.92f, 0.95f}, ice);
+    const float cover = b.surface_params.z;
+    if (cover > 0.0f) {
+        const f3 w1 = f3{fbm(q * 1.6f + off + f3s(77.0f), 4u), fbm(q * 1.6f + off + f3s(91.0f), 4u), 0.5f} - f3s(0.5f);
+        const f3 w2 = f3{fbm(q * 4.0f + off + w1 * 3.0f, 4u), fbm(q * 4.0f + off + f3s(17.0f) + w1 * 3.0f, 4u), 0.5f} -
+                      f3s(0.5f);
+        const f3 cq = f3{q.x * 2.5f, q.y * 2.5f, q.z * 7.0f} + off + w1 * 2.5f + w2 * 1.2f;
+        const float cloud = fbm(cq, 7u);
+        const float lat = fabsf(q.z);
+        const float belts = 0.06f * (surface_gaussian(lat / 0.12f) + surface_gaussian((lat - 0.72f) / 0.15f)) -
+                            0.04f * surface_gaussian((lat - 0.42f) / 0.1f);
+        const float threshold = 0.62f - 0.22f * cover - belts;
+        const float mask = smoothstepf(threshold - 0.04f, threshold + 0.14f, cloud);
+        color = mix3(color, f3{0.93f, 0.94f, 0.96f}, mask * 0.95f);
+    }
+    return color;
+}
+// The unclamped albedo of procedural styles 1-6 at the local point q.
 // Built without SG_STYLES, a kernel serves frames whose bodies use styles 0
-// and 1 only (the host library selects the kernels).
+// and 1 only (the host library selects the kernels). Codes above 6 are
+// terran, as in the WGSL switch.
 SG_FN f3 style_albedo(const Body &b, f3 q) {
 #if SG_STYLES
     switch (surface_style(b)) {
     case 1u: return gas_giant_albedo(b, q);
     case 2u: return cratered_albedo(b, q);
-    default: return rocky_albedo(b, q);
+    case 3u: return rocky_albedo(b, q);
+    case 4u: return icy_albedo(b, q);
+    case 5u: return volcanic_albedo(b, q);
+    default: return terran_albedo(b, q);
     }
 #else
     return gas_giant_albedo(b, q);
 #endif
 }
-// src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1-3 (gas
-// giant, cratered, rocky). Styles 4-6 are rejected by the host library
-// before any launch.
+// src/shaders/pathtrace.wgsl:897-925 for styles 0 (mottle) and 1-6 (gas
+// giant, cratered, rocky, icy, volcanic, terran).
 // Style 0: the albedo is material.rgb times this scalar mottle, clamped.
 SG_FN float surface_mottle(const Body &b, f3 local) {
     float mottle = 1.0f;

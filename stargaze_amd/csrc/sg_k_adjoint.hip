@@ -1,6 +1,6 @@
 // The adjoint kernel (path replay) and the gradient fill.
-// The adjoint rejects frames with a ground scene or with cratered or rocky
-// surfaces: compile without them.
+// The adjoint rejects frames with a ground scene or with solid-body surface
+// styles: compile without them.
 #define SG_LANDSCAPE 0
 #define SG_STYLES 0
 #include "sg_blocks.h"

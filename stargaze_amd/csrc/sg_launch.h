@@ -156,7 +156,7 @@ struct DisplayArgs {
 
 // Kernels. The forward kernels (sg_k_trace.hip, sg_k_batch.hip,
 // sg_k_wave_a.hip, sg_k_wave_b.hip) are compiled twice: as written, and with
-// SG_LANDSCAPE=0 under the suffixes _st (with the cratered and rocky
+// SG_LANDSCAPE=0 under the suffixes _st (with the solid-body surface
 // styles) and _nl (SG_STYLES=0 too), for frames without a ground scene.
 #if SG_LANDSCAPE
 #define SG_K(name) name

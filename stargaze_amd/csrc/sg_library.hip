@@ -27,13 +27,10 @@
 using namespace sg;
 using namespace sgk;
 
-// Procedural surface styles built: 0 plain, 1 gas giant, 2 cratered, 3 rocky.
-constexpr uint32_t kMaxSurfaceStyle = 3u;
-
 namespace {
 // The forward kernels for frames with a ground scene, and the ones compiled
 // without the landscape for every other frame (same results, fewer live
-// registers), with or without the cratered and rocky styles.
+// registers), with or without the solid-body surface styles (2-6).
 struct ForwardKernels {
     decltype(&k_trace) trace;
     decltype(&k_batch<64>) batch64, batch128, batch256;
@@ -56,7 +53,8 @@ const ForwardKernels &forward_kernels(bool landscape, bool styled) {
 const ForwardKernels &forward_kernels(const Scene &sc) {
     return forward_kernels(sc.s.globals.ground_north[3] != 0.0f, sc.solid_styles != 0u);
 }
-// 1 when a lit body of the frame has the cratered or rocky style.
+// 1 when a lit body of the frame has a solid-body style (2-6: cratered,
+// rocky, icy, volcanic, terran).
 uint32_t solid_styles(const sg_frame *frame) {
     for (uint32_t i = 0; i < frame->body_count; ++i) {
         const uint32_t style = (uint32_t)(frame->bodies[i].axis_x[3] + 0.5f);
@@ -293,11 +291,6 @@ int validate_frame(sg_ctx *ctx, const sg_frame *frame) {
     if (frame->ground_count && !frame->ground) return fail(ctx, SG_E_INVALID, "ground pointer is null");
     if (frame->globals->ground_counts[0] != frame->ground_count)
         return fail(ctx, SG_E_INVALID, "globals.ground_counts[0] must equal the primitive count");
-    for (uint32_t i = 0; i < frame->body_count; ++i) {
-        const float style = frame->bodies[i].axis_x[3];
-        if (frame->bodies[i].emissive < 0.5f && (uint32_t)(style + 0.5f) > kMaxSurfaceStyle)
-            return fail(ctx, SG_E_INVALID, "surface styles 4-6 (icy, volcanic, terran) are not built");
-    }
     const sg_globals *g = frame->globals;
     if (g->ground_north[3] != 0.0f) {
         if ((g->ground_counts[2] & 255u) != 0u)
@@ -1105,7 +1098,7 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
     if (frame->globals->ground_north[3] != 0.0f)
         return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate scenes with a landscape");
     if (solid_styles(frame))
-        return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate cratered or rocky surfaces");
+        return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate solid-body surface styles (2-6)");
     if (ctx->last.count == 0 || ctx->last.samples + ctx->last.count != ctx->samples || history_key(ctx, frame) != ctx->key)
         return fail(ctx, SG_E_STATE, "sg_adjoint needs the frame of the last pass sg_encode rendered");
     join_batch(ctx);
@@ -1199,8 +1192,6 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
              kind == SG_PROBE_ALBEDO) &&
             rays[i].target >= frame->body_count)
             return fail(ctx, SG_E_INVALID, "probe target out of range");
-        if (kind == SG_PROBE_ALBEDO && (uint32_t)(frame->bodies[rays[i].target].axis_x[3] + 0.5f) > kMaxSurfaceStyle)
-            return fail(ctx, SG_E_INVALID, "surface styles 4-6 (icy, volcanic, terran) are not built");
     }
     if ((kind == SG_PROBE_TERRAIN || frame->globals->ground_counts[1] != 0u) && !ctx->terrain)
         return fail(ctx, SG_E_STATE, "terrain probe without uploaded terrain");
@@ -1295,7 +1286,7 @@ struct sg_batch {
     std::vector<uint32_t> entry_view;
     uint32_t total_blocks = 0;
     bool landscape = false;  // some view of the table has a ground scene
-    bool styled = false;     // some view has a cratered or rocky body
+    bool styled = false;     // some view has a body with a solid-body style
     bool table_valid = false;
     bool pass_queued = false;
     // Graph of one pass for the current table shape and launch.

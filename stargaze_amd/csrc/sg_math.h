@@ -16,7 +16,7 @@
 #ifndef SG_LANDSCAPE
 #define SG_LANDSCAPE 1
 #endif
-// 0 compiles the cratered and rocky surface styles out: their code raises the
+// 0 compiles the solid-body surface styles (2-6) out: their code raises the
 // register pressure of the shading kernels even for frames that never use
 // them, so the kernels without the landscape are built both ways.
 #ifndef SG_STYLES

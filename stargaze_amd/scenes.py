@@ -171,31 +171,39 @@ def ringed_giant(width: int = 1920, height: int = 1080, seed: int = 0x5EED1234) 
 
 
 # Style codes of body.axis_x.w (src/sim.rs:109-118).
-STYLE_PLAIN, STYLE_GAS_GIANT, STYLE_CRATERED, STYLE_ROCKY = 0, 1, 2, 3
+STYLE_PLAIN, STYLE_GAS_GIANT, STYLE_CRATERED, STYLE_ROCKY, STYLE_ICY, STYLE_VOLCANIC, STYLE_TERRAN = range(7)
 
 
 def style_surface(b: np.ndarray, style: int, seed: float = 3.0, craters: float | None = None,
-                  ice_caps: float | None = None, secondary=None) -> np.ndarray:
-    """Give body ``b`` a procedural surface with the reference's defaults for
-    the style (src/config.rs:216-251): the secondary albedo is a darker tone
-    of the body's albedo, craters 0.8 (cratered) or 0.5 (rocky), ice caps
-    0.06 (rocky). The record layout is the one the shader reads: style in
+                  ice_caps: float | None = None, secondary=None, ocean: float = 0.65,
+                  clouds: float = 0.4) -> np.ndarray:
+    """Give body ``b`` a solid-body surface (styles 2-6) with the reference's
+    defaults for the style (src/config.rs:216-251): secondary albedo, crater
+    density and ice caps per style; ocean 0.65 and clouds 0.4. The record
+    layout is the one the shader reads (src/main.rs:872-895): style in
     axis_x.w, seed in axis_y.w, secondary albedo and crater density in
-    surface_color, ice caps in surface_params.x (pathtrace.wgsl:660-689)."""
+    surface_color, (ice caps, ocean, clouds) in surface_params.xyz."""
     albedo = [float(c) for c in b["color"]]
     peak = max(max(albedo), 1.0e-6)
 
     def deeper(scale, power):
         return [scale * c * (c / peak) ** power for c in albedo]
 
-    defaults = {STYLE_CRATERED: (deeper(0.55, 0.6), 0.8, 0.0), STYLE_ROCKY: (deeper(0.6, 1.0), 0.5, 0.06)}
+    a = albedo
+    defaults = {
+        STYLE_CRATERED: (deeper(0.55, 0.6), 0.8, 0.0),
+        STYLE_ROCKY: (deeper(0.6, 1.0), 0.5, 0.06),
+        STYLE_ICY: ([a[0] * 0.75, a[1] * 0.5, a[2] * 0.35], 0.15, 0.0),
+        STYLE_VOLCANIC: ([a[0] * 0.85, a[1] * 0.45, a[2] * 0.2], 0.5, 0.0),
+        STYLE_TERRAN: ([0.03, 0.09, 0.2], 0.0, 0.1),
+    }
     second, crater_default, ice_default = defaults[style]
     if secondary is not None:
         second = list(secondary)
     b["axis_x"][3] = float(style)
     b["axis_y"][3] = float(seed)
     b["surface_color"] = [*second, crater_default if craters is None else craters]
-    b["surface_params"] = [ice_default if ice_caps is None else ice_caps, 0.0, 0.0, 0.0]
+    b["surface_params"] = [ice_default if ice_caps is None else ice_caps, ocean, clouds, 0.0]
     return b
 
 
@@ -219,13 +227,19 @@ def material_frame(style: int, albedo=(0.62, 0.6, 0.57), seed: float = 3.0, **su
     return f
 
 
-def styled_moons(width: int = 64, height: int = 48, seed: float = 3.0) -> Frame:
-    """A cratered moon and a rocky moon (with polar caps) side by side, lit
-    from the side by one star, filling most of the view."""
+STYLE_ALBEDOS = {STYLE_CRATERED: (0.62, 0.6, 0.57), STYLE_ROCKY: (0.7, 0.45, 0.3), STYLE_ICY: (0.85, 0.8, 0.72),
+                 STYLE_VOLCANIC: (0.85, 0.75, 0.3), STYLE_TERRAN: (0.55, 0.45, 0.3)}
+
+
+def styled_moons(width: int = 64, height: int = 48, seed: float = 3.0, styles=(STYLE_CRATERED, STYLE_ROCKY)) -> Frame:
+    """Two moons with solid-body surfaces side by side (by default a cratered
+    one and a rocky one with polar caps), lit by one star behind the camera,
+    filling most of the view."""
     f = frame()
     camera(f.globals, 50.0, width, height)
     moons = []
-    for style, albedo, x in ((STYLE_CRATERED, (0.62, 0.6, 0.57), -1.9e6), (STYLE_ROCKY, (0.7, 0.45, 0.3), 1.9e6)):
+    for style, x in zip(styles, (-1.9e6, 1.9e6)):
+        albedo = STYLE_ALBEDOS[style]
         b = body([0, 0, 0], metres(1.7e6), list(albedo), False)
         pole = np.array([0.0, math.cos(0.4), math.sin(0.4)])
         b["axis_x"][:3] = [1.0, 0.0, 0.0]

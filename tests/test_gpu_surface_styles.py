@@ -1,5 +1,5 @@
-"""Procedural surface styles 0-3 on MI355X against the FP32 oracle: the
-material probe (src/surface_tests.rs:87-169) and a rendered scene."""
+"""Procedural surface styles 0-6 on MI355X against the FP32 oracle: the
+material probe (src/surface_tests.rs:87-169) and rendered scenes."""
 import numpy as np
 import pytest
 
@@ -11,7 +11,7 @@ from stargaze_amd.pathtracer import PROBE_ALBEDO, Options, PathTracer
 
 pytestmark = pytest.mark.gpu
 
-STYLES = (scenes.STYLE_PLAIN, scenes.STYLE_GAS_GIANT, scenes.STYLE_CRATERED, scenes.STYLE_ROCKY)
+STYLES = tuple(range(7))  # plain, gas giant, cratered, rocky, icy, volcanic, terran
 
 
 def style_frame(style):
@@ -27,9 +27,10 @@ def style_frame(style):
 @pytest.mark.parametrize("style", STYLES)
 def test_material_probe_matches_oracle(style):
     """Raw style albedo over the reference's 256x128 latitude-longitude grid.
-    The noise hashes take fract() of products near 50, so a rounding
-    difference (the GPU contracts multiply-adds) moves a hash across a cell or
-    crater decision at a few points; everywhere else the two agree to 1e-3."""
+    Both sides evaluate the noise hash bit for bit; elsewhere the GPU
+    contracts multiply-adds, which moves a point across a cell or crater
+    decision at a few points. Everywhere else the two agree to 1e-3, except on
+    the edges of the icy lineae (see below)."""
     print(f"style {style}", flush=True)
     frame = style_frame(style)
     rays = scenes.material_rays(256, 128)
@@ -40,14 +41,27 @@ def test_material_probe_matches_oracle(style):
     diff = np.abs(gpu.astype(np.float64) - ref.astype(np.float64)).max(axis=1)
     close = np.mean(diff < 1e-3)
     print(f"style {style}: points within 1e-3 {close:.4f}, median diff {np.median(diff):.2e}")
-    assert close >= 0.98, close
+    if style == scenes.STYLE_ICY:
+        # The icy lineae are 5e-4 to 4e-3 wide and their smoothstep edges
+        # have slopes near 3e3, so a last-bit difference in a fracture's
+        # distance moves a point on an edge by up to the lineae's contrast.
+        # Off the edges the two agree; over the sphere the mean agrees.
+        mean_diff = np.abs(gpu.astype(np.float64).mean(0) - ref.astype(np.float64).mean(0)).max()
+        print(f"icy: within 0.1 {np.mean(diff < 0.1):.4f}, mean colour diff {mean_diff:.2e}")
+        assert close >= 0.85, close
+        assert np.mean(diff < 0.1) >= 0.97
+        assert mean_diff < 2e-3
+    else:
+        assert close >= 0.98, close
     assert np.median(diff) < 1e-4
 
 
-def test_styled_moons_match_oracle_per_tile():
-    """A cratered and a rocky moon rendered by both: per-tile 3-sigma test."""
+@pytest.mark.parametrize("styles", [(2, 3), (4, 5), (6, 2)], ids=["cratered-rocky", "icy-volcanic", "terran-cratered"])
+def test_styled_moons_match_oracle_per_tile(styles):
+    """Two styled moons rendered by both: per-tile 3-sigma test."""
+    print(f"styled moons {styles}", flush=True)
     size = (64, 48)
-    frame = scenes.styled_moons(*size)
+    frame = scenes.styled_moons(*size, styles=styles)
     opts = Options(samples_per_frame=16, max_bounces=4, milky_way=0.0)
     gpu = PathTracer(size, options=opts)
     gpu.enable_moments(True)
@@ -62,24 +76,23 @@ def test_styled_moons_match_oracle_per_tile():
     z = parity.tile_z(ga, gm, oa, om)
     over, worst = parity.assert_tiles(z)
     agree = parity.pixel_agreement(ga, oa)
-    print(f"styled moons: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
+    print(f"styled moons {styles}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
+    assert agree >= 0.9, agree
 
 
-def test_unbuilt_styles_are_rejected():
-    """Styles 4-6 (icy, volcanic, terran) are refused with SG_E_INVALID by
-    encode and by the material probe, before any launch."""
-    frame = scenes.material_frame(scenes.STYLE_ROCKY)
-    frame.bodies[1]["axis_x"][3] = 4.0
-    tracer = PathTracer((8, 8), options=Options(milky_way=0.0))
-    with pytest.raises(NativeError, match="not built"):
-        tracer.encode(frame)
-    with pytest.raises(NativeError, match="not built"):
-        tracer.probe(frame, PROBE_ALBEDO, scenes.material_rays(8, 4))
+def test_style_codes_above_six_are_terran():
+    """The WGSL switch sends every code above 5 to terran_albedo."""
+    rays = scenes.material_rays(64, 32)
+    frames = [scenes.material_frame(scenes.STYLE_TERRAN) for _ in range(2)]
+    frames[1].bodies[1]["axis_x"][3] = 9.0
+    t = PathTracer((4, 4), options=Options(milky_way=0.0))
+    a, b = (t.probe(f, PROBE_ALBEDO, rays)["normal"] for f in frames)
+    assert np.array_equal(a, b)
 
 
 def test_styled_kernels_agree_across_integrators_and_batch():
-    """Frames with cratered or rocky bodies take the kernels built with those
-    styles, in every form: fused, wavefront and batched passes give the same
+    """Frames with solid-body styles take the kernels built with them, in
+    every form: fused, wavefront and batched passes give the same
     image to 1e-5 relative (the kernels differ in the compiler's fused
     multiply-add choices). A batch with one styled view runs all its views
     through the styled batch kernel."""
@@ -112,11 +125,11 @@ def test_styled_kernels_agree_across_integrators_and_batch():
 
 
 def test_adjoint_rejects_solid_styles():
-    """The adjoint does not differentiate the cratered or rocky albedos."""
+    """The adjoint does not differentiate the solid-body albedos (styles 2-6)."""
     size = (16, 12)
     frame = scenes.styled_moons(*size)
     t = PathTracer(size, options=Options(samples_per_frame=4, max_bounces=2, milky_way=0.0))
     t.enable_adjoint(True)
     t.encode(frame)
-    with pytest.raises(NativeError, match="cratered or rocky"):
+    with pytest.raises(NativeError, match="solid-body surface styles"):
         t.adjoint(frame, np.ones((size[1], size[0], 3), np.float32))

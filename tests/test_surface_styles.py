@@ -1,6 +1,6 @@
-"""Procedural surface styles 0-3 in the oracle: the reference's material probe
+"""Procedural surface styles 0-6 in the oracle: the reference's material probe
 (src/surface_tests.rs:87-169) and closed forms of the cratered and rocky
-albedos (src/shaders/pathtrace.wgsl:472-689).
+albedos (src/shaders/pathtrace.wgsl:472-895).
 
 Only the FP32 build is meaningful here: hash31 takes fract() of products of
 coordinates offset by up to 512, so the noise is a function of FP32 rounding
@@ -12,7 +12,7 @@ import oracle
 from stargaze_amd import scenes
 from stargaze_amd.pathtracer import PROBE_ALBEDO, Options
 
-STYLES = (scenes.STYLE_PLAIN, scenes.STYLE_GAS_GIANT, scenes.STYLE_CRATERED, scenes.STYLE_ROCKY)
+STYLES = tuple(range(7))  # plain, gas giant, cratered, rocky, icy, volcanic, terran
 
 
 def probe(frame, rays, precision="f32"):
@@ -91,3 +91,57 @@ def test_craters_darken_floors_and_brighten_rims_within_bounds():
     assert ratio.min() < 0.9, "crater floors must darken"
     assert ratio.max() > 1.05, "rims and rays must brighten"
     assert 0.02 < np.mean(ratio != 1.0) < 1.0
+
+
+def luminance(c):
+    return c @ np.array([0.2126, 0.7152, 0.0722])
+
+
+def test_terran_sea_level_follows_the_ocean_fraction():
+    """terran_albedo (wgsl:854-895): sea level is 0.5 + 0.3 (ocean - 0.5).
+    With ocean = 1 (sea 0.65, above nearly all of the 7-octave fbm) and no
+    ice or cloud, deep water is the ocean colour exactly: mix(., ocean, 1),
+    then no land, ice or cloud. With ocean = 0 almost nothing is water."""
+    rays = scenes.material_rays(128, 64)
+    ocean = np.array([0.03, 0.09, 0.2])
+    wet = probe(scenes.material_frame(scenes.STYLE_TERRAN, ocean=1.0, clouds=0.0, ice_caps=0.0), rays)
+    dry = probe(scenes.material_frame(scenes.STYLE_TERRAN, ocean=0.0, clouds=0.0, ice_caps=0.0), rays)
+    exact_wet = np.all(np.abs(wet - ocean) < 1e-7, axis=1).mean()
+    exact_dry = np.all(np.abs(dry - ocean) < 1e-7, axis=1).mean()
+    assert exact_wet > 0.9, exact_wet
+    assert exact_dry < 0.05, exact_dry
+
+
+def test_terran_cloud_cover_brightens():
+    """Cloud mask threshold 0.62 - 0.22 cover - belts: more cover, more white."""
+    rays = scenes.material_rays(128, 64)
+    lum = [luminance(probe(scenes.material_frame(scenes.STYLE_TERRAN, clouds=c), rays)).mean()
+           for c in (0.0, 0.4, 1.0)]
+    assert lum[0] < lum[1] < lum[2], lum
+
+
+def test_icy_mixes_are_channel_consistent():
+    """icy_albedo mixes only the ice colour and tones of rust (0.8, 0.7 and
+    1.0 times rust = secondary * (0.72, 0.62, 0.56)). With rust equal to the
+    ice colour every output is the ice colour times one grey factor."""
+    ice = np.array([0.8, 0.7, 0.6])
+    secondary = ice / np.array([0.72, 0.62, 0.56])
+    color = probe(scenes.material_frame(scenes.STYLE_ICY, albedo=tuple(ice), secondary=tuple(secondary)),
+                  scenes.material_rays(128, 64))
+    ratio = color / ice
+    np.testing.assert_allclose(ratio[:, 1], ratio[:, 0], rtol=2e-5)
+    np.testing.assert_allclose(ratio[:, 2], ratio[:, 0], rtol=2e-5)
+    assert ratio.min() > 0.5 and ratio.max() < 1.2
+
+
+def test_volcanic_paterae_follow_the_density():
+    """volcanic_albedo: a cell holds a patera only if its hash is below
+    0.6 x density, and a patera's pit blends 96% towards (0.035, 0.032,
+    0.026). Density 0 leaves no such dark point; density 1 leaves some."""
+    rays = scenes.material_rays(256, 128)
+    bright = (0.85, 0.75, 0.3)
+    none = luminance(probe(scenes.material_frame(scenes.STYLE_VOLCANIC, albedo=bright, craters=0.0), rays))
+    dense = luminance(probe(scenes.material_frame(scenes.STYLE_VOLCANIC, albedo=bright, craters=1.0), rays))
+    assert none.min() > 0.1, none.min()
+    assert dense.min() < 0.06, dense.min()
+    assert 0.001 < np.mean(dense < 0.06) < 0.5

@@ -1356,7 +1356,7 @@ static R air_light_visibility(const Ctx *c, Ray ray, uint32_t light) {
     return transmission;
 }
 
-/* --- ground materials, theme 0 (src/shaders/ground.wgsl:256-281, 597-666) --- */
+/* --- ground materials, themes 0-2 (src/shaders/ground.wgsl:256-317, 597-666) --- */
 static R footprint_of(const Ctx *c, R distance) {
     return distance * 2.0 * (R)c->s->cam_forward.v[3] / mx((R)c->s->viewport.v[1], 1.0);
 }
@@ -1380,6 +1380,37 @@ static v3 forest_floor(const Ctx *c, v3 p, R distance, v3 normal) {
     v3 gravel = mixV(V(0.12, 0.095, 0.062), V(0.26, 0.23, 0.17), grain * detail + 0.5 * (1.0 - detail));
     return mixV(color, gravel, path * 0.85);
 }
+static uint32_t ground_theme(const Ctx *c) { return c->s->ground_counts[2] & 255u; } /* terrain.wgsl:22 */
+static v3 desert_floor(R footprint, v3 p, v3 normal) { /* ground.wgsl:281-304 */
+    R broad = vnoise(add(scl(p, 0.007), V(19.0, 2.0, 4.0)));
+    R gravel = vnoise(scl(p, 2.8));
+    R detail = 1.0 - smooth(0.025, 0.2, footprint);
+    v3 sand = mixV(V(0.31, 0.21, 0.125), V(0.53, 0.40, 0.26), broad);
+    v3 rock = mixV(V(0.18, 0.10, 0.055), V(0.38, 0.245, 0.145), broad);
+    v3 color = mixV(sand, rock, smooth(0.08, 0.4, 1.0 - mx(normal.y, 0.0)));
+    color = scl(color, 1.0 + (gravel - 0.5) * 0.32 * detail);
+    R center = 2.8 + 0.55 * sin(p.z / 38.0) + 1.4 * sin(p.z / 210.0);
+    R edge = fabs(p.x - center);
+    R aa = mx(0.015, footprint * 0.5);
+    R fade = (1.0 - smooth(0.25, 0.9, footprint)) * (1.0 - smooth(1000.0, 2200.0, fabs(p.z))) *
+             (1.0 - smooth(0.12, 0.35, 1.0 - mx(normal.y, 0.0)));
+    R worn = 1.0 - smooth(0.8, 1.2, edge);
+    R rut_edge = fabs(edge - 0.68) + (gravel - 0.5) * 0.1 * detail;
+    R tracks = 1.0 - smooth(0.12 - aa, 0.23 + aa, rut_edge);
+    v3 packed = scl(sand, 0.72 + 0.10 * gravel * detail);
+    color = mixV(color, scl(sand, 0.93), worn * fade * 0.35);
+    return mixV(color, packed, tracks * fade * 0.8);
+}
+static v3 crater_floor(R footprint, v3 p, v3 normal) { /* ground.wgsl:306-317 */
+    R broad = vnoise(add(scl(p, 0.004), V(3.0, 0.0, 8.0)));
+    R middle = vnoise(scl(p, 0.09));
+    R grain = vnoise(scl(p, 3.1));
+    R detail = 1.0 - smooth(0.025, 0.2, footprint);
+    R tone = mixR(0.11, 0.21, broad) * (0.86 + 0.28 * middle);
+    tone *= 1.0 + 0.35 * smooth(0.15, 0.5, 1.0 - mx(normal.y, 0.0));
+    tone *= 1.0 + (grain - 0.5) * 0.3 * detail;
+    return V(tone, tone * 0.97, tone * 0.93);
+}
 static v3 landscape_point(const Ctx *c, Ray ray, Hit hit) {
     return add(ground_origin(c, ray), scl(to_ground(c, ray.direction), hit.distance * METRES_PER_AU));
 }
@@ -1387,7 +1418,12 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
     v3 p = landscape_point(c, ray, hit);
     R distance = hit.distance * METRES_PER_AU;
     v3 n = to_ground(c, hit.normal);
-    if (hit.index == LOCAL_BASE || hit.index == TERRAIN_ID) return forest_floor(c, p, distance, n);
+    if (hit.index == LOCAL_BASE || hit.index == TERRAIN_ID) {
+        uint32_t theme = ground_theme(c);
+        if (theme == 1u) return desert_floor(footprint_of(c, distance), p, n);
+        if (theme == 2u) return crater_floor(footprint_of(c, distance), p, n);
+        return forest_floor(c, p, distance, n);
+    }
     const Prim *object = &c->objects[hit.index - LOCAL_BASE - 1u];
     v3 tint = V4(object->albedo);
     R kind = object->albedo.v[3];
@@ -1406,6 +1442,7 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
     R strata = sin(p.y * 8.0 + 2.5 * middle);
     R lichen = smooth(0.52, 0.76, middle) * mx(n.y, 0.0);
     v3 stone = scl(tint, 0.7 + 0.45 * fine + 0.07 * strata);
+    if (ground_theme(c) != 0u) return stone; /* only the forest's rocks carry moss and lichen */
     return mixV(stone, V(0.11, 0.145, 0.037), lichen * 0.65);
 }
 static v3 landscape_shading_normal(const Ctx *c, Ray ray, Hit hit) {

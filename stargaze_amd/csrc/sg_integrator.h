@@ -110,7 +110,7 @@ struct GradSink {
 };
 }  // namespace
 
-// --- ground materials, theme 0 (src/shaders/ground.wgsl:256-281, 597-666) ----
+// --- ground materials, themes 0-2 (src/shaders/ground.wgsl:256-317, 597-666) ----
 SG_FN float footprint_of(const Scene &sc, float distance) {
     return distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
 }
@@ -136,6 +136,43 @@ SG_FN f3 forest_floor(const Scene &sc, f3 p, float distance, f3 normal) {
     f3 gravel = mix3(f3{0.12f, 0.095f, 0.062f}, f3{0.26f, 0.23f, 0.17f}, grain * detail + 0.5f * (1.0f - detail));
     return mix3(color, gravel, path * 0.85f);
 }
+// Ground theme, the low byte of ground_counts.z (src/shaders/terrain.wgsl:22).
+SG_FN uint32_t ground_theme(const Scene &sc) { return sc.s.globals.ground_counts[2] & 255u; }
+// Theme 1, desert: sand and rock by slope, gravel grain, a meandering
+// two-rut trail (src/shaders/ground.wgsl:281-304). Out of line, like the
+// solid-body styles, to keep it out of the forest path's registers.
+SG_OUTLINE f3 desert_floor(float footprint, f3 p, f3 normal) {
+    const float broad = vnoise(p * 0.007f + f3{19.0f, 2.0f, 4.0f});
+    const float gravel = vnoise(p * 2.8f);
+    const float detail = 1.0f - smoothstepf(0.025f, 0.2f, footprint);
+    const f3 sand = mix3(f3{0.31f, 0.21f, 0.125f}, f3{0.53f, 0.40f, 0.26f}, broad);
+    const f3 rock = mix3(f3{0.18f, 0.10f, 0.055f}, f3{0.38f, 0.245f, 0.145f}, broad);
+    f3 color = mix3(sand, rock, smoothstepf(0.08f, 0.4f, 1.0f - fmaxf(normal.y, 0.0f)));
+    color = color * (1.0f + (gravel - 0.5f) * 0.32f * detail);
+    const float center = 2.8f + 0.55f * sinf(p.z / 38.0f) + 1.4f * sinf(p.z / 210.0f);
+    const float edge = fabsf(p.x - center);
+    const float aa = fmaxf(0.015f, footprint * 0.5f);
+    const float fade = (1.0f - smoothstepf(0.25f, 0.9f, footprint)) * (1.0f - smoothstepf(1000.0f, 2200.0f, fabsf(p.z))) *
+                       (1.0f - smoothstepf(0.12f, 0.35f, 1.0f - fmaxf(normal.y, 0.0f)));
+    const float worn = 1.0f - smoothstepf(0.8f, 1.2f, edge);
+    const float rut_edge = fabsf(edge - 0.68f) + (gravel - 0.5f) * 0.1f * detail;
+    const float tracks = 1.0f - smoothstepf(0.12f - aa, 0.23f + aa, rut_edge);
+    const f3 packed = sand * (0.72f + 0.10f * gravel * detail);
+    color = mix3(color, sand * 0.93f, worn * fade * 0.35f);
+    return mix3(color, packed, tracks * fade * 0.8f);
+}
+// Theme 2, grey lunar regolith: broad mottling, brighter steep walls, grain
+// (src/shaders/ground.wgsl:306-317).
+SG_OUTLINE f3 crater_floor(float footprint, f3 p, f3 normal) {
+    const float broad = vnoise(p * 0.004f + f3{3.0f, 0.0f, 8.0f});
+    const float middle = vnoise(p * 0.09f);
+    const float grain = vnoise(p * 3.1f);
+    const float detail = 1.0f - smoothstepf(0.025f, 0.2f, footprint);
+    float tone = mixf(0.11f, 0.21f, broad) * (0.86f + 0.28f * middle);
+    tone *= 1.0f + 0.35f * smoothstepf(0.15f, 0.5f, 1.0f - fmaxf(normal.y, 0.0f));
+    tone *= 1.0f + (grain - 0.5f) * 0.3f * detail;
+    return f3{tone, tone * 0.97f, tone * 0.93f};
+}
 SG_FN d3 landscape_point(const Scene &sc, const Ray &ray, const Hit &hit) {
     return ground_origin(sc, ray) + tod(to_ground(sc, ray.direction)) * ((double)hit.distance * METRES_PER_AU_D);
 }
@@ -143,7 +180,13 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
     f3 p = tof(landscape_point(sc, ray, hit));
     float distance = hit.distance * METRES_PER_AU;
     f3 n = to_ground(sc, hit.normal);
-    if (hit.index == SG_LOCAL_BASE || hit.index == SG_TERRAIN_ID) return forest_floor(sc, p, distance, n);
+    if (hit.index == SG_LOCAL_BASE || hit.index == SG_TERRAIN_ID) {
+        switch (ground_theme(sc)) {
+        case 1u: return desert_floor(footprint_of(sc, distance), p, n);
+        case 2u: return crater_floor(footprint_of(sc, distance), p, n);
+        default: return forest_floor(sc, p, distance, n);
+        }
+    }
     const sg_primitive &object = sc.objects[hit.index - SG_LOCAL_BASE - 1u];
     f3 tint = {object.albedo[0], object.albedo[1], object.albedo[2]};
     float kind = object.albedo[3];
@@ -163,6 +206,7 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
     float strata = sinf(p.y * 8.0f + 2.5f * middle);
     float lichen = smoothstepf(0.52f, 0.76f, middle) * fmaxf(n.y, 0.0f);
     f3 stone = tint * (0.7f + 0.45f * fine + 0.07f * strata);
+    if (ground_theme(sc) != 0u) return stone;  // only the forest's rocks carry moss and lichen
     return mix3(stone, f3{0.11f, 0.145f, 0.037f}, lichen * 0.65f);
 }
 SG_FN f3 landscape_shading_normal(const Scene &sc, const Ray &ray, const Hit &hit) {

@@ -58,7 +58,7 @@ PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, dense
 PEAK_HBM_GBS = 8000.0
 # HBM bytes of one pass, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by
 # a 1 GiB copy's counts (profiles/halo_4k_kernels.md).
-TRAFFIC_BYTES_PER_PASS = {"halo_4k": 428.2e9}
+TRAFFIC_BYTES_PER_PASS = {"halo_4k": 428.8e9}
 
 def _halo(width, height):
     frame, data, cells = scenes.halo_observatory(width, height)

@@ -1356,7 +1356,7 @@ static R air_light_visibility(const Ctx *c, Ray ray, uint32_t light) {
     return transmission;
 }
 
-/* --- ground materials, themes 0-2 (src/shaders/ground.wgsl:256-317, 597-666) --- */
+/* --- ground materials, themes 0-3 and 5 (src/shaders/ground.wgsl:256-666) --- */
 static R footprint_of(const Ctx *c, R distance) {
     return distance * 2.0 * (R)c->s->cam_forward.v[3] / mx((R)c->s->viewport.v[1], 1.0);
 }
@@ -1411,6 +1411,211 @@ static v3 crater_floor(R footprint, v3 p, v3 normal) { /* ground.wgsl:306-317 */
     tone *= 1.0 + (grain - 0.5) * 0.3 * detail;
     return V(tone, tone * 0.97, tone * 0.93);
 }
+static R ground_curvature(R r, R px, R pz) { /* terrain.wgsl:32-36 */
+    R d2 = px * px + pz * pz;
+    return -d2 / (r + sqrt(mx(r * r - d2, 0.0)));
+}
+#define ICE_FLOW 4.4
+#define DUNE_WIND 0.61
+#define DUNE_PAN_LEVEL 2.0
+static R lookout_hill(R r) { /* ground.wgsl:322-325 */
+    R u = mn(sqrt(r * r + 30.0 * 30.0) / 700.0, 1.0);
+    return 180.0 * (1.0 - u) * (1.0 - u);
+}
+static R dune_cracks(R px, R py) { /* ground.wgsl:490-507 */
+    R gx = floor(px), gy = floor(py), fx = px - gx, fy = py - gy;
+    R d1 = 8.0, d2 = 8.0;
+    for (int j = -1; j <= 1; ++j)
+        for (int i = -1; i <= 1; ++i) {
+            R ox = (R)i, oy = (R)j;
+            v3 h = hash33(V(gx + ox, gy + oy, 3.0));
+            R dx = ox + 0.15 + 0.7 * h.x - fx, dy = oy + 0.15 + 0.7 * h.y - fy;
+            R d = sqrt(dx * dx + dy * dy);
+            if (d < d1) {
+                d2 = d1;
+                d1 = d;
+            } else if (d < d2) {
+                d2 = d;
+            }
+        }
+    return d2 - d1;
+}
+static void ice_crevasse(R qx, R qz, R footprint, R out[2]) { /* ground.wgsl:333-362 */
+    out[0] = out[1] = 0.0;
+    R r = sqrt(qx * qx + qz * qz);
+    R tension = smooth(650.0, 900.0, r) * (1.0 - smooth(1800.0, 2600.0, r));
+    R strain = mx(smooth(0.47, 0.58, fbm(V(qx / 2600.0 + 4.0, 2.0, qz / 2600.0), 2)), tension) * smooth(600.0, 750.0, r);
+    if (strain <= 0.0) return;
+    R flx = sin((R)ICE_FLOW), flz = cos((R)ICE_FLOW);
+    R along = qx * flx + qz * flz + 280.0 * (vnoise(V(qx / 1500.0, qz / 1500.0, 1.0)) - 0.5);
+    R across = qx * flz - qz * flx + 120.0 * (vnoise(V(qx / 900.0, qz / 900.0, 5.0)) - 0.5);
+    R spacing = 60.0;
+    R row = floor(along / spacing);
+    uint32_t key = orc_hash(((uint32_t)(int32_t)row * 0x8da6b343u) ^ 0xc3e5u);
+    R length_m = 120.0 + 300.0 * (R)(key & 255u) / 255.0;
+    R s = across / length_m + (R)((key >> 8) & 255u) / 255.0;
+    R segment = floor(s);
+    R lens = pow(mx(sin(PI * (s - segment)), 0.0), 0.7);
+    uint32_t open = orc_hash(key ^ (uint32_t)(int32_t)segment);
+    if ((open & 255u) < 70u) return;
+    R centre = 0.5 + 0.3 * ((R)((open >> 8) & 255u) / 255.0 - 0.5);
+    R half_width = 0.5 * (2.0 + 14.0 * (R)((open >> 16) & 255u) / 255.0) * lens * strain;
+    R d = fabs(along / spacing - row - centre) * spacing + 0.8 * (vnoise(V(qx * 0.15, qz * 0.15, 9.0)) - 0.5);
+    R aa = mx(footprint, 1.0e-3);
+    out[0] = clampR((half_width - d) / aa + 0.5, 0.0, 1.0) * mn(1.0, 2.0 * half_width / aa);
+    out[1] = 1.0 - clampR(d / mx(half_width, 1.0e-3), 0.0, 1.0);
+}
+static v3 alpine_floor(R footprint_m, R scale, R radius, v3 p, v3 normal) { /* ground.wgsl:365-428 */
+    v3 q = dvs(V(p.x, p.y - ground_curvature(radius, p.x, p.z), p.z), scale);
+    R hill = lookout_hill(sqrt(q.x * q.x + q.z * q.z));
+    q.y -= hill;
+    R footprint = footprint_m / scale;
+    R detail = 1.0 - smooth(0.025, 0.2, footprint);
+    R slope = 1.0 - mx(normal.y, 0.0);
+    R broad = vnoise(add(scl(q, 0.004), V(11.0, 4.0, 1.0)));
+    R middle = vnoise(scl(q, 0.2));
+    R grain = vnoise(scl(q, 4.0));
+    R flx = sin((R)ICE_FLOW), flz = cos((R)ICE_FLOW);
+    R along = q.x * flx + q.z * flz, across = q.x * flz - q.z * flx;
+    R crust = smooth(0.45, 0.6, fbm(V(along * 0.004, 2.0, across * 0.012), 3));
+    v3 snow = mixV(V(0.86, 0.88, 0.91), V(0.76, 0.8, 0.86), crust);
+    snow = scl(snow, 0.97 + 0.05 * middle);
+    R bare = smooth(0.47, 0.53, fbm(add(scl(q, 0.0016), V(3.0, 0.0, 8.0)), 4) + 0.1 * (broad - 0.5));
+    v3 ice = mixV(V(0.36, 0.55, 0.7), V(0.52, 0.67, 0.78), middle);
+    R fracture_width = 0.03 + footprint / 4.0;
+    R fractures = (1.0 - smooth(0.0, fracture_width, dune_cracks(q.x / 4.0 + 9.0, q.z / 4.0 + 9.0))) * 0.03 / fracture_width;
+    ice = mixV(ice, V(0.82, 0.86, 0.9), mn(fractures, 1.0) * 0.7);
+    ice = scl(ice, 1.0 + (grain - 0.5) * 0.12 * detail);
+    v3 glacier = mixV(snow, ice, bare);
+    v3 debris = mixV(V(0.12, 0.115, 0.11), V(0.26, 0.24, 0.22), grain);
+    R dirty = smooth(6.0, 30.0, q.y) * (1.0 - smooth(30.0, 70.0, q.y));
+    glacier = mixV(glacier, mixV(debris, glacier, 0.45), dirty * smooth(0.4, 0.7, broad) * 0.8);
+    R wall = smooth(0.35, 0.6, slope) * (1.0 - smooth(20.0, 45.0, q.y));
+    R depth = smooth(4.0, -4.0, q.y);
+    v3 blue = mixV(V(0.32, 0.55, 0.72), V(0.1, 0.25, 0.42), depth);
+    glacier = mixV(glacier, scl(blue, 0.9 + 0.2 * middle), wall);
+    R cr[2];
+    ice_crevasse(q.x, q.z, footprint, cr);
+    R crack_x = cr[0] * (1.0 - smooth(10.0, 25.0, q.y)), crack_y = cr[1] * (1.0 - smooth(10.0, 25.0, q.y));
+    R lip = (1.0 - smooth(0.0, 0.12, crack_y)) * crack_x;
+    v3 inside = mixV(V(0.3, 0.55, 0.74), V(0.02, 0.06, 0.12), smooth(0.1, 0.8, crack_y));
+    glacier = mixV(glacier, inside, crack_x);
+    glacier = mixV(glacier, V(0.9, 0.92, 0.95), lip * 0.5);
+    R mountain = mx(smooth(25.0, 70.0, q.y + 30.0 * (broad - 0.5)), smooth(4.0, 15.0, hill));
+    R joints = 1.0 - smooth(0.0, 0.06 + footprint / 3.0, dune_cracks((q.x + q.y * 0.5) / 3.0, q.z / 3.0));
+    v3 granite = mixV(V(0.09, 0.085, 0.08), V(0.24, 0.215, 0.19), broad);
+    granite = scl(granite, 0.85 + 0.3 * middle - 0.35 * joints * (1.0 - smooth(0.1, 1.0, footprint)));
+    R lichen = smooth(0.68, 0.78, vnoise(scl(q, 0.9))) * (1.0 - smooth(0.6, 0.9, slope));
+    granite = mixV(granite, V(0.3, 0.17, 0.06), lichen * 0.5 * detail);
+    R cover = 1.0 - smooth(0.17, 0.33, slope + (broad - 0.5) * 0.18 + (middle - 0.5) * 0.08);
+    R scoured = smooth(0.52, 0.62, fbm(add(scl(q, 0.012), V(2.0, 0.0, 6.0)), 4)) * smooth(25.0, 90.0, hill) *
+                (1.0 - smooth(150.0, 172.0, hill));
+    R cover_hill = cover * (1.0 - scoured);
+    v3 peak = mixV(granite, snow, cover_hill);
+    v3 color = mixV(glacier, peak, mountain);
+    return scl(color, 1.0 + (grain - 0.5) * 0.08 * detail);
+}
+static void alpine_gradient(R footprint, v3 p, v3 n, R g[2]) { /* ground.wgsl:430-461 */
+    R snowy = 1.0 - smooth(0.3, 0.5, 1.0 - n.y);
+    R flx = sin((R)ICE_FLOW), flz = cos((R)ICE_FLOW), sx = flz, sz = -flx;
+    R gx = 0.0, gz = 0.0, scale = 1.2, strength = 0.2;
+    for (uint32_t octave = 0; octave < 3u; ++octave) {
+        R fade = 1.0 - smooth(0.08 * scale, 0.25 * scale, footprint);
+        if (fade > 0.0) {
+            R a = (p.x * flx + p.z * flz) / (6.0 * scale), cc = (p.x * sx + p.z * sz) / scale;
+            R e = 0.08, o = (R)octave * 13.0;
+            R da = vnoise(V(a + e, o, cc)) - vnoise(V(a - e, o, cc));
+            R dc = vnoise(V(a, o, cc + e)) - vnoise(V(a, o, cc - e));
+            R crest = 1.0 + 1.5 * (1.0 - fabs(vnoise(V(a, o, cc)) * 2.0 - 1.0));
+            R w = strength * fade * crest * scale;
+            gx += (flx * da / (6.0 * scale) + sx * dc / scale) / (2.0 * e) * w;
+            gz += (flz * da / (6.0 * scale) + sz * dc / scale) / (2.0 * e) * w;
+        }
+        scale *= 6.0;
+        strength *= 0.6;
+    }
+    g[0] = gx * snowy;
+    g[1] = gz * snowy;
+}
+static v3 dune_floor(R footprint_m, R scale, R radius, v3 p, v3 normal) { /* ground.wgsl:509-569 */
+    v3 q = dvs(V(p.x, p.y - ground_curvature(radius, p.x, p.z), p.z), scale);
+    R footprint = footprint_m / scale;
+    R detail = 1.0 - smooth(0.02, 0.15, footprint);
+    R slope = 1.0 - mx(normal.y, 0.0);
+    R broad = vnoise(add(scl(q, 0.0015), V(1.0, 7.0, 3.0)));
+    R middle = vnoise(scl(q, 0.05));
+    R rise = smooth(6.0, 160.0, q.y + 40.0 * (broad - 0.5));
+    v3 plinth = mixV(V(0.3, 0.095, 0.038), V(0.36, 0.12, 0.045), broad);
+    v3 upper = mixV(V(0.42, 0.155, 0.06), V(0.47, 0.19, 0.072), broad);
+    v3 sand = scl(mixV(plinth, upper, rise), 0.94 + 0.08 * middle);
+    R flat = sqrt(normal.x * normal.x + normal.z * normal.z);
+    if (flat > 1.0e-3) {
+        R dx = -normal.x / flat, dz = -normal.z / flat;
+        R across = q.x * -dz + q.z * dx, along = q.x * dx + q.z * dz;
+        R tongues = fbm(V(across * 0.18, along * 0.012, 5.0), 3);
+        R slip = smooth(0.32, 0.45, slope);
+        sand = mixV(sand, scl(mul(sand, V(1.06, 1.12, 1.16)), 0.86 + 0.28 * tongues), slip);
+    }
+    R lag = smooth(0.6, 0.76, fbm(add(mul(q, V(0.013, 0.0, 0.004)), V(9.0, 0.0, 2.0)), 4)) *
+            (1.0 - smooth(0.04, 0.16, slope)) * (1.0 - smooth(10.0, 45.0, q.y));
+    sand = mixV(sand, V(0.13, 0.065, 0.05), lag * 0.55);
+    R corridor = (1.0 - smooth(5.0, 16.0, q.y + 6.0 * (broad - 0.5))) * (1.0 - smooth(0.03, 0.1, slope));
+    v3 gravel = mixV(V(0.39, 0.21, 0.115), V(0.45, 0.28, 0.16), vnoise(scl(q, 0.3)));
+    R pebbles = smooth(0.72, 0.8, vnoise(scl(q, 7.0))) * detail;
+    sand = mixV(sand, scl(gravel, 1.0 - 0.45 * pebbles), corridor * 0.6);
+    static const float pans[2][3] = {{842.6f, 707.1f, 560.0f}, {4346.7f, 1164.7f, 900.0f}};
+    R pan = 0.0;
+    for (uint32_t i = 0; i < 2u; ++i) {
+        R ex = q.x - (R)pans[i][0], ez = q.z - (R)pans[i][1];
+        pan = mx(pan, 1.0 - smooth((R)pans[i][2] * 1.2, (R)pans[i][2] * 1.35 + 140.0, sqrt(ex * ex + ez * ez)));
+    }
+    pan *= 1.0 - smooth(DUNE_PAN_LEVEL + 0.25, DUNE_PAN_LEVEL + 1.5, q.y);
+    if (pan > 0.0) {
+        R tone = fbm(add(scl(q, 0.012), V(4.0, 0.0, 1.0)), 4);
+        v3 clay = mixV(V(0.4, 0.35, 0.29), V(0.6, 0.56, 0.49), smooth(0.3, 0.7, tone));
+        clay = scl(clay, 0.82 + 0.18 * smooth(0.35, 1.0, pan));
+        R fine_width = 0.04 + footprint / 0.9;
+        R cracks = (1.0 - smooth(0.0, fine_width, dune_cracks(q.x / 0.9, q.z / 0.9))) * 0.04 / fine_width;
+        R wide_width = 0.03 + footprint / 6.0;
+        R large = (1.0 - smooth(0.0, wide_width, dune_cracks(q.x / 6.0 + 17.0, q.z / 6.0 + 17.0))) * 0.03 / wide_width;
+        clay = scl(clay, 1.0 - 0.6 * mn(cracks, 1.0) - 0.45 * mn(large, 1.0));
+        clay = scl(clay, 0.92 + 0.16 * vnoise(scl(q, 2.3)) * (1.0 - smooth(0.04, 0.25, footprint)));
+        sand = mixV(sand, clay, pan);
+    }
+    return scl(sand, 1.0 + (vnoise(scl(q, 9.0)) - 0.5) * 0.12 * detail);
+}
+static void dune_ripples(v3 p, R wavelength, R warp, R amplitude, R g[2]) { /* ground.wgsl:571-581 */
+    R wx = sin((R)DUNE_WIND), wz = cos((R)DUNE_WIND), sx = wz, sz = -wx;
+    R bend = warp * (vnoise(V(p.x / (9.0 * wavelength), p.z / (9.0 * wavelength), 1.0)) +
+                     0.35 * vnoise(V(p.x / (2.5 * wavelength), p.z / (2.5 * wavelength), 7.0)));
+    R phase = (p.x * wx + p.z * wz) / wavelength + bend + 0.4 * sin((p.x * sx + p.z * sz) / (3.0 * wavelength));
+    R theta = TAU * phase;
+    R slope = cos(theta + 0.65 * sin(theta)) * (1.0 + 0.65 * cos(theta));
+    R patches = smooth(0.25, 0.6, vnoise(V(p.x / (14.0 * wavelength), p.z / (14.0 * wavelength), 3.0)));
+    R k = slope * amplitude * TAU / wavelength * patches;
+    g[0] = wx * k;
+    g[1] = wz * k;
+}
+static void dune_gradient(R footprint, R scale, R radius, v3 p, v3 n, R g[2]) { /* ground.wgsl:583-595 */
+    R gentle = 1.0 - smooth(0.22, 0.38, 1.0 - n.y);
+    R qy = (p.y - ground_curvature(radius, p.x, p.z)) / scale;
+    R dry = smooth(DUNE_PAN_LEVEL + 0.4, DUNE_PAN_LEVEL + 1.5, qy);
+    R gx = 0.0, gz = 0.0, r[2];
+    R fine = 1.0 - smooth(0.03, 0.07, footprint);
+    if (fine > 0.0) {
+        dune_ripples(p, 0.12, 2.5, 0.006, r);
+        gx += r[0] * fine;
+        gz += r[1] * fine;
+    }
+    R coarse = (1.0 - smooth(0.15, 0.4, footprint)) * (1.0 - smooth(9.0, 16.0, qy));
+    if (coarse > 0.0) {
+        dune_ripples(add(p, Vs(31.0)), 0.8, 2.0, 0.03, r);
+        gx += r[0] * coarse;
+        gz += r[1] * coarse;
+    }
+    g[0] = gx * gentle * dry;
+    g[1] = gz * gentle * dry;
+}
 static v3 landscape_point(const Ctx *c, Ray ray, Hit hit) {
     return add(ground_origin(c, ray), scl(to_ground(c, ray.direction), hit.distance * METRES_PER_AU));
 }
@@ -1422,6 +1627,8 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
         uint32_t theme = ground_theme(c);
         if (theme == 1u) return desert_floor(footprint_of(c, distance), p, n);
         if (theme == 2u) return crater_floor(footprint_of(c, distance), p, n);
+        if (theme == 3u) return alpine_floor(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n);
+        if (theme == 5u) return dune_floor(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n);
         return forest_floor(c, p, distance, n);
     }
     const Prim *object = &c->objects[hit.index - LOCAL_BASE - 1u];
@@ -1449,6 +1656,14 @@ static v3 landscape_shading_normal(const Ctx *c, Ray ray, Hit hit) {
     if (hit.index != LOCAL_BASE && hit.index != TERRAIN_ID) return hit.normal;
     v3 p = landscape_point(c, ray, hit);
     R distance = hit.distance * METRES_PER_AU;
+    uint32_t theme = ground_theme(c);
+    if (theme == 3u || theme == 5u) { /* ground.wgsl:642-643 */
+        v3 n = to_ground(c, hit.normal);
+        R g[2];
+        if (theme == 3u) alpine_gradient(footprint_of(c, distance), p, n, g);
+        else dune_gradient(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n, g);
+        return nrm(add(hit.normal, from_ground(c, V(-g[0], 0.0, -g[1]))));
+    }
     R strength = 0.11 * (1.0 - smooth(8.0, 65.0, distance));
     R dx = vnoise(scl(add(p, V(0.025, 0.0, 0.0)), 6.0)) - vnoise(scl(sub(p, V(0.025, 0.0, 0.0)), 6.0));
     R dz = vnoise(scl(add(p, V(0.0, 0.0, 0.025)), 6.0)) - vnoise(scl(sub(p, V(0.0, 0.0, 0.025)), 6.0));

@@ -110,7 +110,7 @@ struct GradSink {
 };
 }  // namespace
 
-// --- ground materials, themes 0-2 (src/shaders/ground.wgsl:256-317, 597-666) ----
+// --- ground materials, themes 0-3 and 5 (src/shaders/ground.wgsl:256-666) ----
 SG_FN float footprint_of(const Scene &sc, float distance) {
     return distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
 }
@@ -173,6 +173,237 @@ SG_OUTLINE f3 crater_floor(float footprint, f3 p, f3 normal) {
     tone *= 1.0f + (grain - 0.5f) * 0.3f * detail;
     return f3{tone, tone * 0.97f, tone * 0.93f};
 }
+// Ground curvature below (px, pz) for a host of radius r (terrain.wgsl:32-36).
+SG_FN float ground_curvature(float r, float px, float pz) {
+    const float d2 = px * px + pz * pz;
+    return -d2 / (r + sqrtf(fmaxf(r * r - d2, 0.0f)));
+}
+// The ice flows west-south-west; the lookout stands on a nunatak of height
+// 180 m and base radius 700 m (ground.wgsl:318-331, terrain.rs).
+constexpr float ICE_FLOW = 4.4f;
+constexpr float DUNE_WIND = 0.61f;
+constexpr float DUNE_PAN_LEVEL = 2.0f;
+SG_FN float lookout_hill(float r) {
+    const float u = fminf(sqrtf(r * r + 30.0f * 30.0f) / 700.0f, 1.0f);
+    return 180.0f * (1.0f - u) * (1.0f - u);
+}
+// Gap between the nearest two jittered cells: polygonal cracks
+// (ground.wgsl:490-507).
+SG_FN float dune_cracks(float px, float py) {
+    const float gx = floorf(px), gy = floorf(py);
+    const float fx = px - gx, fy = py - gy;
+    float d1 = 8.0f, d2 = 8.0f;
+    for (int j = -1; j <= 1; ++j) {
+        for (int i = -1; i <= 1; ++i) {
+            const float ox = (float)i, oy = (float)j;
+            const f3 h = hash33(f3{gx + ox, gy + oy, 3.0f});
+            const float dx = ox + 0.15f + 0.7f * h.x - fx, dy = oy + 0.15f + 0.7f * h.y - fy;
+            const float d = sqrtf(dx * dx + dy * dy);
+            if (d < d1) {
+                d2 = d1;
+                d1 = d;
+            } else if (d < d2) {
+                d2 = d;
+            }
+        }
+    }
+    return d2 - d1;
+}
+// Crevasses across the ice flow: (coverage of the pixel, 0 at a crack's lip
+// to 1 at its centre) (ground.wgsl:333-362).
+SG_FN float2 ice_crevasse(float qx, float qz, float footprint) {
+    const float r = sqrtf(qx * qx + qz * qz);
+    const float tension = smoothstepf(650.0f, 900.0f, r) * (1.0f - smoothstepf(1800.0f, 2600.0f, r));
+    const float strain = fmaxf(smoothstepf(0.47f, 0.58f, fbm(f3{qx / 2600.0f + 4.0f, 2.0f, qz / 2600.0f}, 2u)), tension) *
+                         smoothstepf(600.0f, 750.0f, r);
+    if (strain <= 0.0f) return make_float2(0.0f, 0.0f);
+    const float flx = sinf(ICE_FLOW), flz = cosf(ICE_FLOW);
+    const float along = qx * flx + qz * flz + 280.0f * (vnoise(f3{qx / 1500.0f, qz / 1500.0f, 1.0f}) - 0.5f);
+    const float across = qx * flz - qz * flx + 120.0f * (vnoise(f3{qx / 900.0f, qz / 900.0f, 5.0f}) - 0.5f);
+    const float spacing = 60.0f;
+    const float row = floorf(along / spacing);
+    const uint32_t key = hash_u32(((uint32_t)(int32_t)row * 0x8da6b343u) ^ 0xc3e5u);
+    const float length_m = 120.0f + 300.0f * (float)(key & 255u) / 255.0f;
+    const float s = across / length_m + (float)((key >> 8) & 255u) / 255.0f;
+    const float segment = floorf(s);
+    const float lens = powf(fmaxf(sinf(PI * (s - segment)), 0.0f), 0.7f);
+    const uint32_t open = hash_u32(key ^ (uint32_t)(int32_t)segment);
+    if ((open & 255u) < 70u) return make_float2(0.0f, 0.0f);
+    const float centre = 0.5f + 0.3f * ((float)((open >> 8) & 255u) / 255.0f - 0.5f);
+    const float half_width = 0.5f * (2.0f + 14.0f * (float)((open >> 16) & 255u) / 255.0f) * lens * strain;
+    const float d = fabsf(along / spacing - row - centre) * spacing + 0.8f * (vnoise(f3{qx * 0.15f, qz * 0.15f, 9.0f}) - 0.5f);
+    const float aa = fmaxf(footprint, 1.0e-3f);
+    const float coverage = clampf((half_width - d) / aa + 0.5f, 0.0f, 1.0f) * fminf(1.0f, 2.0f * half_width / aa);
+    return make_float2(coverage, 1.0f - clampf(d / fmaxf(half_width, 1.0e-3f), 0.0f, 1.0f));
+}
+// Theme 3, Patagonian ice field: snow, bare blue ice with fractures,
+// crevasses, debris, granite spires with lichen and snow ledges
+// (ground.wgsl:365-428). `footprint` is the raw pixel footprint in metres.
+SG_OUTLINE f3 alpine_floor(float footprint_m, float scale, float radius, f3 p, f3 normal) {
+    f3 q = f3{p.x, p.y - ground_curvature(radius, p.x, p.z), p.z} / scale;
+    const float hill = lookout_hill(sqrtf(q.x * q.x + q.z * q.z));
+    q.y -= hill;
+    const float footprint = footprint_m / scale;
+    const float detail = 1.0f - smoothstepf(0.025f, 0.2f, footprint);
+    const float slope = 1.0f - fmaxf(normal.y, 0.0f);
+    const float broad = vnoise(q * 0.004f + f3{11.0f, 4.0f, 1.0f});
+    const float middle = vnoise(q * 0.2f);
+    const float grain = vnoise(q * 4.0f);
+    const float flx = sinf(ICE_FLOW), flz = cosf(ICE_FLOW);
+    const float along = q.x * flx + q.z * flz;
+    const float across = q.x * flz - q.z * flx;
+    const float crust = smoothstepf(0.45f, 0.6f, fbm(f3{along * 0.004f, 2.0f, across * 0.012f}, 3u));
+    f3 snow = mix3(f3{0.86f, 0.88f, 0.91f}, f3{0.76f, 0.8f, 0.86f}, crust);
+    snow = snow * (0.97f + 0.05f * middle);
+    const float bare = smoothstepf(0.47f, 0.53f, fbm(q * 0.0016f + f3{3.0f, 0.0f, 8.0f}, 4u) + 0.1f * (broad - 0.5f));
+    f3 ice = mix3(f3{0.36f, 0.55f, 0.7f}, f3{0.52f, 0.67f, 0.78f}, middle);
+    const float fracture_width = 0.03f + footprint / 4.0f;
+    const float fractures =
+        (1.0f - smoothstepf(0.0f, fracture_width, dune_cracks(q.x / 4.0f + 9.0f, q.z / 4.0f + 9.0f))) * 0.03f / fracture_width;
+    ice = mix3(ice, f3{0.82f, 0.86f, 0.9f}, fminf(fractures, 1.0f) * 0.7f);
+    ice = ice * (1.0f + (grain - 0.5f) * 0.12f * detail);
+    f3 glacier = mix3(snow, ice, bare);
+    const f3 debris = mix3(f3{0.12f, 0.115f, 0.11f}, f3{0.26f, 0.24f, 0.22f}, grain);
+    const float dirty = smoothstepf(6.0f, 30.0f, q.y) * (1.0f - smoothstepf(30.0f, 70.0f, q.y));
+    glacier = mix3(glacier, mix3(debris, glacier, 0.45f), dirty * smoothstepf(0.4f, 0.7f, broad) * 0.8f);
+    const float wall = smoothstepf(0.35f, 0.6f, slope) * (1.0f - smoothstepf(20.0f, 45.0f, q.y));
+    const float depth = smoothstepf(4.0f, -4.0f, q.y);
+    const f3 blue = mix3(f3{0.32f, 0.55f, 0.72f}, f3{0.1f, 0.25f, 0.42f}, depth);
+    glacier = mix3(glacier, blue * (0.9f + 0.2f * middle), wall);
+    const float2 cr = ice_crevasse(q.x, q.z, footprint);
+    const float crack_x = cr.x * (1.0f - smoothstepf(10.0f, 25.0f, q.y));
+    const float crack_y = cr.y * (1.0f - smoothstepf(10.0f, 25.0f, q.y));
+    const float lip = (1.0f - smoothstepf(0.0f, 0.12f, crack_y)) * crack_x;
+    const f3 inside = mix3(f3{0.3f, 0.55f, 0.74f}, f3{0.02f, 0.06f, 0.12f}, smoothstepf(0.1f, 0.8f, crack_y));
+    glacier = mix3(glacier, inside, crack_x);
+    glacier = mix3(glacier, f3{0.9f, 0.92f, 0.95f}, lip * 0.5f);
+    const float mountain = fmaxf(smoothstepf(25.0f, 70.0f, q.y + 30.0f * (broad - 0.5f)), smoothstepf(4.0f, 15.0f, hill));
+    const float joints =
+        1.0f - smoothstepf(0.0f, 0.06f + footprint / 3.0f, dune_cracks((q.x + q.y * 0.5f) / 3.0f, q.z / 3.0f));
+    f3 granite = mix3(f3{0.09f, 0.085f, 0.08f}, f3{0.24f, 0.215f, 0.19f}, broad);
+    granite = granite * (0.85f + 0.3f * middle - 0.35f * joints * (1.0f - smoothstepf(0.1f, 1.0f, footprint)));
+    const float lichen = smoothstepf(0.68f, 0.78f, vnoise(q * 0.9f)) * (1.0f - smoothstepf(0.6f, 0.9f, slope));
+    granite = mix3(granite, f3{0.3f, 0.17f, 0.06f}, lichen * 0.5f * detail);
+    const float cover = 1.0f - smoothstepf(0.17f, 0.33f, slope + (broad - 0.5f) * 0.18f + (middle - 0.5f) * 0.08f);
+    const float scoured = smoothstepf(0.52f, 0.62f, fbm(q * 0.012f + f3{2.0f, 0.0f, 6.0f}, 4u)) *
+                          smoothstepf(25.0f, 90.0f, hill) * (1.0f - smoothstepf(150.0f, 172.0f, hill));
+    const float cover_hill = cover * (1.0f - scoured);
+    const f3 peak = mix3(granite, snow, cover_hill);
+    f3 color = mix3(glacier, peak, mountain);
+    return color * (1.0f + (grain - 0.5f) * 0.08f * detail);
+}
+// Sastrugi: the ground-frame (x, z) gradient of wind-carved snow ridges
+// (ground.wgsl:430-461); the caller adds it to the normal.
+SG_OUTLINE float2 alpine_gradient(float footprint, f3 p, f3 n) {
+    const float snowy = 1.0f - smoothstepf(0.3f, 0.5f, 1.0f - n.y);
+    const float flx = sinf(ICE_FLOW), flz = cosf(ICE_FLOW);
+    const float sx = flz, sz = -flx;
+    float gx = 0.0f, gz = 0.0f;
+    float scale = 1.2f, strength = 0.2f;
+    for (uint32_t octave = 0; octave < 3u; ++octave) {
+        const float fade = 1.0f - smoothstepf(0.08f * scale, 0.25f * scale, footprint);
+        if (fade > 0.0f) {
+            const float a = (p.x * flx + p.z * flz) / (6.0f * scale);
+            const float c = (p.x * sx + p.z * sz) / scale;
+            const float e = 0.08f;
+            const float o = (float)octave * 13.0f;
+            const float da = vnoise(f3{a + e, o, c}) - vnoise(f3{a - e, o, c});
+            const float dc = vnoise(f3{a, o, c + e}) - vnoise(f3{a, o, c - e});
+            const float crest = 1.0f + 1.5f * (1.0f - fabsf(vnoise(f3{a, o, c}) * 2.0f - 1.0f));
+            const float w = strength * fade * crest * scale;
+            gx += (flx * da / (6.0f * scale) + sx * dc / scale) / (2.0f * e) * w;
+            gz += (flz * da / (6.0f * scale) + sz * dc / scale) / (2.0f * e) * w;
+        }
+        scale *= 6.0f;
+        strength *= 0.6f;
+    }
+    return make_float2(gx * snowy, gz * snowy);
+}
+// Theme 5, Namib sand: dune plinths and crests, slip faces, mineral lag,
+// gravel corridors and cracked clay pans (ground.wgsl:483-569).
+SG_OUTLINE f3 dune_floor(float footprint_m, float scale, float radius, f3 p, f3 normal) {
+    const f3 q = f3{p.x, p.y - ground_curvature(radius, p.x, p.z), p.z} / scale;
+    const float footprint = footprint_m / scale;
+    const float detail = 1.0f - smoothstepf(0.02f, 0.15f, footprint);
+    const float slope = 1.0f - fmaxf(normal.y, 0.0f);
+    const float broad = vnoise(q * 0.0015f + f3{1.0f, 7.0f, 3.0f});
+    const float middle = vnoise(q * 0.05f);
+    const float rise = smoothstepf(6.0f, 160.0f, q.y + 40.0f * (broad - 0.5f));
+    const f3 plinth = mix3(f3{0.3f, 0.095f, 0.038f}, f3{0.36f, 0.12f, 0.045f}, broad);
+    const f3 upper = mix3(f3{0.42f, 0.155f, 0.06f}, f3{0.47f, 0.19f, 0.072f}, broad);
+    f3 sand = mix3(plinth, upper, rise) * (0.94f + 0.08f * middle);
+    const float flat = sqrtf(normal.x * normal.x + normal.z * normal.z);
+    if (flat > 1.0e-3f) {
+        const float dx = -normal.x / flat, dz = -normal.z / flat;
+        const float across = q.x * -dz + q.z * dx;
+        const float along = q.x * dx + q.z * dz;
+        const float tongues = fbm(f3{across * 0.18f, along * 0.012f, 5.0f}, 3u);
+        const float slip = smoothstepf(0.32f, 0.45f, slope);
+        sand = mix3(sand, sand * f3{1.06f, 1.12f, 1.16f} * (0.86f + 0.28f * tongues), slip);
+    }
+    const float lag = smoothstepf(0.6f, 0.76f, fbm(q * f3{0.013f, 0.0f, 0.004f} + f3{9.0f, 0.0f, 2.0f}, 4u)) *
+                      (1.0f - smoothstepf(0.04f, 0.16f, slope)) * (1.0f - smoothstepf(10.0f, 45.0f, q.y));
+    sand = mix3(sand, f3{0.13f, 0.065f, 0.05f}, lag * 0.55f);
+    const float corridor =
+        (1.0f - smoothstepf(5.0f, 16.0f, q.y + 6.0f * (broad - 0.5f))) * (1.0f - smoothstepf(0.03f, 0.1f, slope));
+    const f3 gravel = mix3(f3{0.39f, 0.21f, 0.115f}, f3{0.45f, 0.28f, 0.16f}, vnoise(q * 0.3f));
+    const float pebbles = smoothstepf(0.72f, 0.8f, vnoise(q * 7.0f)) * detail;
+    sand = mix3(sand, gravel * (1.0f - 0.45f * pebbles), corridor * 0.6f);
+    const float pans[2][3] = {{842.6f, 707.1f, 560.0f}, {4346.7f, 1164.7f, 900.0f}};
+    float pan = 0.0f;
+    for (uint32_t i = 0; i < 2u; ++i) {
+        const float ex = q.x - pans[i][0], ez = q.z - pans[i][1];
+        pan = fmaxf(pan, 1.0f - smoothstepf(pans[i][2] * 1.2f, pans[i][2] * 1.35f + 140.0f, sqrtf(ex * ex + ez * ez)));
+    }
+    pan *= 1.0f - smoothstepf(DUNE_PAN_LEVEL + 0.25f, DUNE_PAN_LEVEL + 1.5f, q.y);
+    if (pan > 0.0f) {
+        const float tone = fbm(q * 0.012f + f3{4.0f, 0.0f, 1.0f}, 4u);
+        f3 clay = mix3(f3{0.4f, 0.35f, 0.29f}, f3{0.6f, 0.56f, 0.49f}, smoothstepf(0.3f, 0.7f, tone));
+        clay = clay * (0.82f + 0.18f * smoothstepf(0.35f, 1.0f, pan));
+        const float fine_width = 0.04f + footprint / 0.9f;
+        const float cracks = (1.0f - smoothstepf(0.0f, fine_width, dune_cracks(q.x / 0.9f, q.z / 0.9f))) * 0.04f / fine_width;
+        const float wide_width = 0.03f + footprint / 6.0f;
+        const float large =
+            (1.0f - smoothstepf(0.0f, wide_width, dune_cracks(q.x / 6.0f + 17.0f, q.z / 6.0f + 17.0f))) * 0.03f / wide_width;
+        clay = clay * (1.0f - 0.6f * fminf(cracks, 1.0f) - 0.45f * fminf(large, 1.0f));
+        clay = clay * (0.92f + 0.16f * vnoise(q * 2.3f) * (1.0f - smoothstepf(0.04f, 0.25f, footprint)));
+        sand = mix3(sand, clay, pan);
+    }
+    return sand * (1.0f + (vnoise(q * 9.0f) - 0.5f) * 0.12f * detail);
+}
+// Wind ripples across the wind (ground.wgsl:571-581): the (x, z) gradient.
+SG_FN float2 dune_ripples(f3 p, float wavelength, float warp, float amplitude) {
+    const float wx = sinf(DUNE_WIND), wz = cosf(DUNE_WIND);
+    const float sx = wz, sz = -wx;
+    const float bend = warp * (vnoise(f3{p.x / (9.0f * wavelength), p.z / (9.0f * wavelength), 1.0f}) +
+                               0.35f * vnoise(f3{p.x / (2.5f * wavelength), p.z / (2.5f * wavelength), 7.0f}));
+    const float phase = (p.x * wx + p.z * wz) / wavelength + bend + 0.4f * sinf((p.x * sx + p.z * sz) / (3.0f * wavelength));
+    const float theta = TAU * phase;
+    const float slope = cosf(theta + 0.65f * sinf(theta)) * (1.0f + 0.65f * cosf(theta));
+    const float patches = smoothstepf(0.25f, 0.6f, vnoise(f3{p.x / (14.0f * wavelength), p.z / (14.0f * wavelength), 3.0f}));
+    const float k = slope * amplitude * TAU / wavelength * patches;
+    return make_float2(wx * k, wz * k);
+}
+// Ripple gradient of the dunes (ground.wgsl:583-595).
+SG_OUTLINE float2 dune_gradient(float footprint, float scale, float radius, f3 p, f3 n) {
+    const float gentle = 1.0f - smoothstepf(0.22f, 0.38f, 1.0f - n.y);
+    const float qy = (p.y - ground_curvature(radius, p.x, p.z)) / scale;
+    const float dry = smoothstepf(DUNE_PAN_LEVEL + 0.4f, DUNE_PAN_LEVEL + 1.5f, qy);
+    float gx = 0.0f, gz = 0.0f;
+    const float fine = 1.0f - smoothstepf(0.03f, 0.07f, footprint);
+    if (fine > 0.0f) {
+        const float2 g = dune_ripples(p, 0.12f, 2.5f, 0.006f);
+        gx += g.x * fine;
+        gz += g.y * fine;
+    }
+    const float coarse = (1.0f - smoothstepf(0.15f, 0.4f, footprint)) * (1.0f - smoothstepf(9.0f, 16.0f, qy));
+    if (coarse > 0.0f) {
+        const float2 g = dune_ripples(p + f3s(31.0f), 0.8f, 2.0f, 0.03f);
+        gx += g.x * coarse;
+        gz += g.y * coarse;
+    }
+    return make_float2(gx * gentle * dry, gz * gentle * dry);
+}
 SG_FN d3 landscape_point(const Scene &sc, const Ray &ray, const Hit &hit) {
     return ground_origin(sc, ray) + tod(to_ground(sc, ray.direction)) * ((double)hit.distance * METRES_PER_AU_D);
 }
@@ -184,6 +415,8 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
         switch (ground_theme(sc)) {
         case 1u: return desert_floor(footprint_of(sc, distance), p, n);
         case 2u: return crater_floor(footprint_of(sc, distance), p, n);
+        case 3u: return alpine_floor(footprint_of(sc, distance), terrain_scale(sc), sc.s.globals.ground_up[3], p, n);
+        case 5u: return dune_floor(footprint_of(sc, distance), terrain_scale(sc), sc.s.globals.ground_up[3], p, n);
         default: return forest_floor(sc, p, distance, n);
         }
     }
@@ -213,6 +446,14 @@ SG_FN f3 landscape_shading_normal(const Scene &sc, const Ray &ray, const Hit &hi
     if (hit.index != SG_LOCAL_BASE && hit.index != SG_TERRAIN_ID) return hit.normal;
     f3 p = tof(landscape_point(sc, ray, hit));
     float distance = hit.distance * METRES_PER_AU;
+    const uint32_t theme = ground_theme(sc);
+    if (theme == 3u || theme == 5u) {
+        const f3 n = to_ground(sc, hit.normal);
+        const float2 g = theme == 3u ? alpine_gradient(footprint_of(sc, distance), p, n)
+                                     : dune_gradient(footprint_of(sc, distance), terrain_scale(sc),
+                                                     sc.s.globals.ground_up[3], p, n);
+        return normalize(hit.normal + from_ground(sc, f3{-g.x, 0.0f, -g.y}));
+    }
     float strength = 0.11f * (1.0f - smoothstepf(8.0f, 65.0f, distance));
     float dx = vnoise((p + f3{0.025f, 0.f, 0.f}) * 6.0f) - vnoise((p - f3{0.025f, 0.f, 0.f}) * 6.0f);
     float dz = vnoise((p + f3{0.f, 0.f, 0.025f}) * 6.0f) - vnoise((p - f3{0.f, 0.f, 0.025f}) * 6.0f);

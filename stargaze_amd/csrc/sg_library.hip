@@ -296,8 +296,9 @@ int validate_frame(sg_ctx *ctx, const sg_frame *frame) {
         return fail(ctx, SG_E_INVALID, "globals.ground_counts[0] must equal the primitive count");
     const sg_globals *g = frame->globals;
     if (g->ground_north[3] != 0.0f) {
-        if ((g->ground_counts[2] & 255u) > 2u)
-            return fail(ctx, SG_E_INVALID, "ground themes 3-5 (alpine, basalt, dunes) are not built");
+        const uint32_t theme = g->ground_counts[2] & 255u;
+        if (theme == 4u || theme > 5u)
+            return fail(ctx, SG_E_INVALID, "ground theme 4 (basalt, which needs the volcano survey) is not built");
         uint32_t host = (uint32_t)g->ground_north[3] - 1u;
         if (host >= frame->body_count) return fail(ctx, SG_E_INVALID, "ground host index out of range");
         if (g->ground_counts[1] != 0u && !ctx->terrain)

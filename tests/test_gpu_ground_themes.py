@@ -1,6 +1,6 @@
-"""Ground themes 1 (desert) and 2 (lunar crater floor) on MI355X against the
-FP32 oracle (src/shaders/ground.wgsl:281-317, 597-631), on the Halo scene
-within the reference's capacities."""
+"""Ground themes 1 (desert), 2 (lunar crater floor), 3 (alpine ice field) and
+5 (dunes) on MI355X against the FP32 oracle (src/shaders/ground.wgsl:281-666),
+on the Halo scene within the reference's capacities."""
 import numpy as np
 import pytest
 
@@ -19,7 +19,7 @@ def themed(theme, size):
     return frame, data, cells
 
 
-@pytest.mark.parametrize("theme", [1, 2], ids=["desert", "crater"])
+@pytest.mark.parametrize("theme", [1, 2, 3, 5], ids=["desert", "crater", "alpine", "dunes"])
 def test_ground_theme_matches_oracle_per_tile(theme):
     print(f"ground theme {theme}", flush=True)
     size = (96, 54)
@@ -52,7 +52,7 @@ def test_ground_theme_matches_oracle_per_tile(theme):
 
 def test_unbuilt_ground_themes_are_rejected():
     size = (16, 12)
-    frame, data, cells = themed(3, size)
+    frame, data, cells = themed(4, size)
     t = PathTracer(size, options=Options(samples_per_frame=2, max_bounces=2, milky_way=0.0))
     t.upload_terrain(data, cells)
     with pytest.raises(NativeError, match="not built"):

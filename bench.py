@@ -120,7 +120,7 @@ def work_model(frame, terrain, profile, width, height, spp, bounces, rows=16):
     return per, flops
 
 
-def cpu_baseline(frame, terrain, profile, width, height, spp, bounces, target_s=12.0):
+def cpu_baseline(frame, terrain, profile, width, height, spp, bounces, target_s=15.0):
     """FP32 oracle on all host cores over a band of rows sized to ~target_s."""
     rows = 8
     while True:
@@ -129,7 +129,7 @@ def cpu_baseline(frame, terrain, profile, width, height, spp, bounces, target_s=
         t0 = time.perf_counter()
         t.encode(frame, rows=(y0, min(height, y0 + rows)))
         dt = time.perf_counter() - t0
-        if dt >= target_s * 0.5 or rows >= height:
+        if dt >= 10.0 or rows >= height:
             break
         rows = min(height, max(rows * 2, int(rows * target_s / max(dt, 1e-3))))
     samples = width * (min(height, y0 + rows) - y0) * spp

@@ -54,29 +54,36 @@ FLOP_COST = {
     "vertex": 70,         # surface shading: sampling frame, throughput
     "rng": 0,             # integer work
 }
+# Bytes each event must read from memory in the work model's memory term:
+# a terrain node's cached box (6 floats), a triangle's three cached vertices
+# (3 x 4 B in the compact form of grids above 1024 cells, 3 x 12 B below), a
+# primitive record (48 B). Bodies, lights and tables stay in caches.
+BYTE_COST = {"terrain_node": 24, "triangle_compact": 12, "triangle": 36, "box": 48, "rock": 48, "vegetation": 48}
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, dense
 PEAK_HBM_GBS = 8000.0
 # HBM bytes of one pass, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by
-# a 1 GiB copy's counts (profiles/halo_4k_kernels.md).
-TRAFFIC_BYTES_PER_PASS = {"halo_4k": 428.8e9}
+# a 1 GiB copy's counts; not measured by this script (the counters need a
+# profiler run of their own).
+TRAFFIC_BYTES_PER_PASS = {"halo_4k": None}
+TRAFFIC_SOURCE = "profiles/halo_4k_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
 
 def _halo(width, height):
     frame, data, cells = scenes.halo_observatory(width, height)
-    return frame, (data, cells), None
+    return frame, (data, cells), None, scenes.HALO_AIR
 
 
 def _giant(width, height):
     """The ringed giant with the 512-bin C/B/A optical-depth table."""
     frame = scenes.ringed_giant(width, height)
-    return frame, None, (1, scenes.ring_profile_cba(frame, 1))
+    return frame, None, (1, scenes.ring_profile_cba(frame, 1)), None
 
 
 def _plain(builder):
-    return lambda width, height: (builder(width, height), None, None)
+    return lambda width, height: (builder(width, height), None, None, None)
 
 
 WORKLOADS = {
-    # name: (builder -> (frame, terrain, ring profile), width, height, spp per pass, max bounces)
+    # name: (builder -> (frame, terrain, ring profile, air model), width, height, spp per pass, max bounces)
     "halo_4k": (_halo, 3840, 2160, 32, 4),
     "ringed_giant_1080p": (_giant, 1920, 1080, 64, 6),
     "airless_triple_256": (_plain(scenes.airless_triple), 256, 256, 16, 4),
@@ -87,7 +94,7 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def oracle_tracer(width, height, spp, bounces, terrain, profile=None):
+def oracle_tracer(width, height, spp, bounces, terrain, profile=None, air=None):
     import oracle
 
     t = oracle.OracleTracer((width, height), Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
@@ -95,16 +102,18 @@ def oracle_tracer(width, height, spp, bounces, terrain, profile=None):
         t.upload_terrain(*terrain)
     if profile is not None:
         t.set_ring_profile(*profile)
+    if air is not None:
+        t.set_air_model(**air)
     return t
 
 
-def work_model(frame, terrain, profile, width, height, spp, bounces, rows=16):
+def work_model(frame, terrain, profile, air, width, height, spp, bounces, rows=16):
     """Event counts per path sample of a steady-state pass (not the first,
     which takes extra light strata near limbs) from the oracle on bands of
     rows at the image's quarter, half and three-quarter heights."""
     import oracle
 
-    t = oracle_tracer(width, height, spp, bounces, terrain, profile)
+    t = oracle_tracer(width, height, spp, bounces, terrain, profile, air)
     t.encode(frame, rows=(0, 1))  # first pass: fills the sky and terrain caches
     ev = None
     bands = [height * (2 * k + 1) // 8 for k in range(4)]
@@ -117,31 +126,112 @@ def work_model(frame, terrain, profile, width, height, spp, bounces, rows=16):
     n = width * rows * spp
     per = {k: v / n for k, v in ev.items()}
     flops = sum(per[k] * FLOP_COST[k] for k in FLOP_COST)
-    return per, flops
+    compact = terrain is not None and terrain[1] > 1024
+    byte_cost = dict(BYTE_COST, triangle=BYTE_COST["triangle_compact"] if compact else BYTE_COST["triangle"])
+    gather = sum(per[k] * byte_cost[k] for k in ("terrain_node", "triangle", "box", "rock", "vegetation"))
+    return per, flops, gather
 
 
-def cpu_baseline(frame, terrain, profile, width, height, spp, bounces, target_s=15.0):
-    """FP32 oracle on all host cores over a band of rows sized to ~target_s."""
-    rows = 8
+def cpu_model() -> str:
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(frame, terrain, profile, air, width, height, spp, bounces, target_s=15.0):
+    """FP32 oracle on all host cores over four bands of rows at the image's
+    eighth, three-eighths, five-eighths and seven-eighths heights (the work
+    model's bands), sized so the sample takes ~target_s."""
+    rows = 2
     while True:
-        t = oracle_tracer(width, height, spp, bounces, terrain, profile)
-        y0 = max(0, height // 2 - rows // 2)
+        t = oracle_tracer(width, height, spp, bounces, terrain, profile, air)
+        bands = [min(height - rows, height * (2 * k + 1) // 8) for k in range(4)]
+        t.encode(frame, rows=(0, 1))  # fills the sky and terrain caches
+        t.history_samples = spp
         t0 = time.perf_counter()
-        t.encode(frame, rows=(y0, min(height, y0 + rows)))
+        for y0 in bands:
+            t.encode(frame, rows=(y0, y0 + rows))
+            t.history_samples = spp
         dt = time.perf_counter() - t0
-        if dt >= 10.0 or rows >= height:
+        if dt >= 10.0 or 4 * rows >= height:
             break
-        rows = min(height, max(rows * 2, int(rows * target_s / max(dt, 1e-3))))
-    samples = width * (min(height, y0 + rows) - y0) * spp
+        rows = min(height // 4, max(rows * 2, int(rows * target_s / max(dt, 1e-3))))
+    samples = width * 4 * rows * spp
     cores = int(os.environ.get("OMP_NUM_THREADS", len(os.sched_getaffinity(0))))
     return {
         "value": samples / dt / 1e6,
         "unit": "Mpath-samples/s",
         "cores": cores,
+        "nproc": os.cpu_count(),
+        "cpu_model": cpu_model(),
         "kind": "port",
-        "sample": f"FP32 oracle (C restatement of the WGSL), {rows} central rows x {width} px x {spp} spp "
-        f"of the same workload, {dt:.1f} s",
+        "sample": f"FP32 oracle (C restatement of the WGSL), 4 bands of {rows} rows at 1/8, 3/8, 5/8 and 7/8 "
+        f"of the height x {width} px x {spp} spp of the same workload, {dt:.1f} s",
     }
+
+
+RMSE_REFERENCE = ROOT / "tests" / "golden" / "halo_rmse_reference.npz"
+RMSE_META = ROOT / "tests" / "golden" / "halo_rmse_reference.json"
+
+
+def rmse_leg(ms_per_step: float, spp_per_step: int, pixels_4k: int, cpu_rate: float | None) -> dict | None:
+    """The fixed-RMSE half of the metric, on the Halo camera at 320 x 180.
+
+    The reference is a 16384-spp float64-accumulated render of the same
+    scene and air model (tests/golden/make_halo_reference.py); the target is
+    the FP32 oracle's linear-RGB RMSE at 32 spp. The GPU renders the same
+    view progressively (noise seed 0, then seed 7 for an independent draw)
+    and its RMSE is read at 8, 16, 32 and 64 spp. Monte Carlo RMSE falls as
+    spp^-1/2, so the GPU reaches the target at 32 (rmse_32 / target)^2 spp;
+    at 4K that many spp take that fraction of a 32-spp step."""
+    if not RMSE_REFERENCE.exists() or not RMSE_META.exists():
+        return None
+    sys.path.insert(0, str(ROOT / "tests"))
+    import parity
+
+    meta = json.loads(RMSE_META.read_text())
+    if "target" not in meta:
+        return None
+    ref = np.load(RMSE_REFERENCE)["image"]
+    w, h = meta["width"], meta["height"]
+    frame, data, cells = scenes.halo_observatory(w, h)
+    curve = {}
+    for seed in (0, 7):
+        t = PathTracer((w, h), options=Options(noise_seed=seed, samples_per_frame=8, max_bounces=4, milky_way=0.0))
+        t.upload_terrain(data, cells)
+        t.set_air_model(**scenes.HALO_AIR)
+        spp = 0
+        out = {}
+        for stop in (8, 16, 32, 64):
+            while spp < stop:
+                t.encode(frame)
+                spp += 8
+            out[stop] = parity.rmse(t.read_accumulation(), ref)
+        t.close()
+        curve[seed] = out
+    target = float(meta["target"]["rmse"])
+    rmse32 = curve[0][32]
+    spp_to_target = 32.0 * (rmse32 / target) ** 2
+    gpu_s = spp_to_target / spp_per_step * ms_per_step / 1e3
+    result = {
+        "reference": f"{w}x{h} of the halo_4k camera, {meta['spp']} spp, float64 accumulation "
+                     "(tests/golden/halo_rmse_reference.npz)",
+        "target_rmse": target,
+        "target_source": "FP32 oracle at 32 spp, noise seed 0, linear RGB (W excluded)",
+        "gpu_rmse": {str(k): round(v, 7) for k, v in curve[0].items()},
+        "gpu_rmse_seed7": {str(k): round(v, 7) for k, v in curve[7].items()},
+        "gpu_spp_to_target": round(spp_to_target, 2),
+        "gpu_seconds_to_target_4k": round(gpu_s, 4),
+    }
+    if cpu_rate:
+        cpu_s = pixels_4k * 32 / (cpu_rate * 1e6)
+        result["cpu_seconds_to_target_4k"] = round(cpu_s, 1)
+        result["speedup_at_equal_rmse"] = round(cpu_s / gpu_s, 1)
+    return result
 
 
 BATCH_WORKLOADS = {
@@ -329,6 +419,7 @@ def main() -> int:
     ap.add_argument("--launch", default=None,
                     help="integrator launch instead of the autotuner's: fused, or wavefront:CHUNK:TAIL")
     ap.add_argument("--no-work-model", action="store_true", help="skip the oracle event count (no roofline)")
+    ap.add_argument("--no-rmse", action="store_true", help="skip the fixed-RMSE leg")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
     args = ap.parse_args()
@@ -337,13 +428,15 @@ def main() -> int:
         return 2
     builder, width, height, spp, bounces = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    frame, terrain, profile = builder(width, height)
+    frame, terrain, profile, air = builder(width, height)
     log(f"scene built in {time.perf_counter() - t0:.1f} s")
     tracer = PathTracer((width, height), options=Options(samples_per_frame=spp, max_bounces=bounces, milky_way=0.0))
     if terrain is not None:
         tracer.upload_terrain(*terrain)
     if profile is not None:
         tracer.set_ring_profile(*profile)
+    if air is not None:
+        tracer.set_air_model(**air)
     log(f"workload {args.workload}: {width}x{height}, {spp} spp/step, {bounces} bounces")
     launch = None
     if args.launch:
@@ -375,13 +468,30 @@ def main() -> int:
     samples_per_step = width * height * spp
     value = samples_per_step * args.steps / elapsed / 1e6
     if args.no_work_model:
-        per_event, flops = {}, float("nan")
+        per_event, flops, gather = {}, float("nan"), float("nan")
     else:
         log("work model")
-        per_event, flops = work_model(frame, terrain, profile, width, height, spp, bounces)
+        per_event, flops, gather = work_model(frame, terrain, profile, air, width, height, spp, bounces)
     kernel_s = kernel_ms / 1e3 / max(launches, 1)
     achieved = flops * samples_per_step / kernel_s / 1e12
-    traffic_algorithmic = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
+    compulsory = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
+    bytes_per_pass = gather * samples_per_step + compulsory
+    achieved_gbs = bytes_per_pass / kernel_s / 1e9
+    g = frame.globals
+    air_model = air or {"view_steps": 8, "light_steps": 4, "mie_scale_height": 0.0}
+    au = scenes.METRES_PER_AU
+    config = {"workload": args.workload, "width": width, "height": height, "spp_per_step": spp,
+              "max_bounces": bounces}
+    if float(g["atmo_params"][3]) >= 0.5:
+        config.update({
+            "air_view_steps": air_model["view_steps"], "air_light_steps": air_model["light_steps"],
+            "rayleigh_scale_height_m": round(float(g["atmo_params"][1]) * au, 1),
+            "mie_scale_height_m": round((air_model["mie_scale_height"] or float(g["atmo_params"][1])) * au, 1),
+        })
+    if terrain is not None:
+        kinds = frame.ground["center"][:, 3]
+        config.update({"terrain_cells": int(terrain[1]), "rock_boxes": int(np.sum(kinds == 0.0)),
+                       "conifer_ellipsoids": int(np.sum(kinds == 1.0)), "ground_records": int(len(frame.ground))})
     line = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -395,28 +505,30 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {
-            "workload": args.workload,
-            "width": width,
-            "height": height,
-            "spp_per_step": spp,
-            "max_bounces": bounces,
-        },
+        "config": config,
         "launch": launch if launch is not None else tracer.launch,
         "roofline": {
-            # Compute roofline at MI355X's FP32 peak. The kernels issue FP32
-            # VALU instructions: nothing on the path is a dense contraction.
-            "bound": "mfma",
+            # FP32 vector roofline: nothing on the path is a dense
+            # contraction, so no MFMA instruction is issued. The memory term
+            # (terrain and primitive gathers plus the accumulation) is
+            # reported beside it; the larger fraction names the bound.
+            "bound": "valu" if achieved / PEAK_FP32_TFLOPS >= achieved_gbs / PEAK_HBM_GBS else "hbm",
             "pipe": "FP32 VALU, no MFMA instructions",
             "achieved": round(achieved, 3),
             "peak": PEAK_FP32_TFLOPS,
             "unit": "TFLOP/s",
             "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "memory_term": {"achieved": round(achieved_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                            "frac": round(achieved_gbs / PEAK_HBM_GBS, 5),
+                            "bytes_per_path_sample": round(gather + compulsory / samples_per_step, 2)},
             "traffic": TRAFFIC_BYTES_PER_PASS.get(args.workload),
+            "traffic_source": TRAFFIC_SOURCE if args.workload in TRAFFIC_BYTES_PER_PASS else None,
             "kernel": "integrator (all kernels of one pass)",
             "kernel_ms": round(kernel_s * 1e3, 3),
             "flop_per_path_sample": round(flops, 1),
-            "algorithmic_bytes_per_launch": traffic_algorithmic,
+            "flop_model": "oracle event counts x FLOP cost table; prop tests counted as the oracle's flat "
+                          "skip-count loop performs them (the GPU's BVH performs fewer)",
+            "algorithmic_bytes_per_launch": round(bytes_per_pass),
         },
     }
     if args.batch_steps > 0:
@@ -429,7 +541,11 @@ def main() -> int:
         line["adjoint"] = adjoint_leg(args.adjoint_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline")
-        line["cpu_baseline"] = cpu_baseline(frame, terrain, profile, width, height, spp, bounces)
+        line["cpu_baseline"] = cpu_baseline(frame, terrain, profile, air, width, height, spp, bounces)
+    if args.workload == "halo_4k" and not args.no_rmse:
+        log("fixed-RMSE leg")
+        cpu_rate = line.get("cpu_baseline", {}).get("value")
+        line["rmse"] = rmse_leg(elapsed / args.steps * 1e3, spp, width * height, cpu_rate)
     print(json.dumps(line), flush=True)
     return 0
 

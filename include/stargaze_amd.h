@@ -286,10 +286,34 @@ int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t b
  * sample (src/shaders/pathtrace.wgsl:970,1166,1230); NULL restores the
  * exponential. Restarts the accumulation. */
 int sg_set_density_profile(sg_ctx *ctx, const float *density, uint32_t bins);
+/* Air march and Mie scale height. The reference's shader marches
+ * ATMO_VIEW_STEPS = 8 view cells per camera ray (capped by its
+ * quarter-scale-height rule) and ATMO_SUN_STEPS = 4 density samples per light
+ * path, with one scale height for Rayleigh and Mie
+ * (src/shaders/pathtrace.wgsl:67-68,944-946); its benchmark variants replace
+ * the two step constants (src/render_experiments.rs:48-70). view_steps in
+ * [2, 32], light_steps in [1, 32]. mie_scale_height (AU, the unit of
+ * atmo_params.y) > 0 gives the Mie density its own exp(-h / H_mie) while no
+ * density profile is set; 0 keeps the single scale height. NULL restores
+ * (8, 4, 0). Restarts the accumulation. */
+typedef struct sg_air_model {
+    uint32_t view_steps;
+    uint32_t light_steps;
+    float mie_scale_height;
+    uint32_t reserved; /* 0 */
+} sg_air_model;
+#define SG_AIR_MAX_STEPS 32u
+int sg_set_air_model(sg_ctx *ctx, const sg_air_model *model);
+int sg_get_air_model(const sg_ctx *ctx, sg_air_model *model);
 /* Restart the accumulation with the next pass while keeping the noise
  * sequence running, as a changed frame does (src/pathtracer.rs:172-188):
  * every restarted pass draws fresh samples. */
 int sg_restart(sg_ctx *ctx);
+/* Identity of the last pass that rendered samples: it changes with every
+ * such pass and with every call that restarts the accumulation (profiles,
+ * air model, terrain, sg_restart). A caller that differentiates a pass later
+ * records it after sg_encode and compares before sg_adjoint. */
+uint64_t sg_pass_id(const sg_ctx *ctx);
 int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_t pixel_count,
                float *gradient, uint32_t gradient_count, float *pass_mean);
 /* With enable = 1, every pass also keeps each path sample's returned

@@ -54,6 +54,9 @@ class OrcScene(ctypes.Structure):
         ("milky_way", ctypes.c_void_p),
         ("ring_profile", ctypes.c_void_p),
         ("ring_profile_body", ctypes.c_uint32),
+        ("air_view_steps", ctypes.c_uint32),
+        ("air_sun_steps", ctypes.c_uint32),
+        ("mie_height", ctypes.c_float),
     ]
 
 
@@ -128,6 +131,7 @@ class OracleTracer:
         self.terrain_radius = None
         self.sky = np.zeros((128, 4), np.float32)
         self.ring_profile = None
+        self.air = (8, 4, 0.0)
         self.moments_on = False
         self._alloc()
 
@@ -146,6 +150,12 @@ class OracleTracer:
     def set_ring_profile(self, body: int, tau):
         """``PathTracer.set_ring_profile``: 512 bins, or None; restarts."""
         self.ring_profile = None if tau is None else (int(body), np.array(tau, np.float32).reshape(512))
+        self.key = b""
+        self.history_samples = 0
+
+    def set_air_model(self, view_steps=8, light_steps=4, mie_scale_height=0.0):
+        """Air march and separate Mie scale height (AU); restarts."""
+        self.air = (int(view_steps), int(light_steps), float(mie_scale_height))
         self.key = b""
         self.history_samples = 0
 
@@ -227,6 +237,7 @@ class OracleTracer:
         sc.bodies = bodies.ctypes.data
         sc.objects = objects.ctypes.data
         sc.sky_light = self.sky.ctypes.data
+        sc.air_view_steps, sc.air_sun_steps, sc.mie_height = self.air
         sc.terrain = self.terrain.ctypes.data if self.terrain is not None else None
         if self.catalogue is not None and len(self.catalogue):
             cat = np.ascontiguousarray(self.catalogue)

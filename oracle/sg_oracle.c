@@ -118,6 +118,8 @@ typedef struct {
     const uint32_t *milky;
     const float *ring_profile;
     uint32_t ring_profile_body;
+    uint32_t view_steps, sun_steps; /* air march */
+    R mie_height;                   /* 0: one scale height */
     /* per-dispatch counts.y (the sample index semantics of the host slices) */
     uint32_t accumulated;
 } Ctx;
@@ -1696,19 +1698,26 @@ static R shell_exit(const Ctx *c, v3 origin, v3 dir) {
     if (disc < 0.0) return 0.0;
     return -b + sqrt(disc);
 }
-static R density_integral(const Ctx *c, v3 origin, v3 dir, uint32_t steps) {
+/* Whether Mie has its own scale height (extension; the WGSL has one H). */
+static int two_heights(const Ctx *c) { return c->mie_height > 0.0; }
+/* wgsl:958-971. Under two scale heights *mie receives the Mie column. */
+static R density_integral(const Ctx *c, v3 origin, v3 dir, uint32_t steps, R *mie) {
     R distance = shell_exit(c, origin, dir);
+    if (mie) *mie = 0.0;
     if (distance <= 0.0) return 0.0;
     if (counting) local_counters[EV_DENSITY_STEP] += steps;
     R dt = distance / (R)steps;
     R inv_h = 1.0 / scale_height(c);
+    R inv_hm = two_heights(c) ? 1.0 / c->mie_height : 0.0;
     R r = host_radius(c);
-    R sum = 0.0;
+    R sum = 0.0, sum_m = 0.0;
     for (uint32_t i = 0; i < steps; ++i) {
         v3 p = add(origin, scl(dir, ((R)i + 0.5) * dt));
         R h = mx(len(sub(p, atmo_c(c))) - r, 0.0);
         sum += exp(-h * inv_h);
+        if (two_heights(c)) sum_m += exp(-h * inv_hm);
     }
+    if (mie && two_heights(c)) *mie = sum_m * dt;
     return sum * dt;
 }
 static v3 host_offset(const Ctx *c, Ray origin) {
@@ -1724,11 +1733,18 @@ static int inside_air(const Ctx *c, Ray origin) {
 }
 static v3 beta_r(const Ctx *c) { return V4(c->s->atmo_rayleigh); }
 static R beta_m(const Ctx *c) { return c->s->atmo_rayleigh.v[3]; }
+/* Optical depth of a Rayleigh and a Mie column; with one scale height the
+ * columns are equal and the depth is (beta_r + beta_m) * column (wgsl:996). */
+static v3 air_depth(const Ctx *c, R column, R mie_column) {
+    if (two_heights(c)) return add(scl(beta_r(c), column), Vs(beta_m(c) * mie_column));
+    return scl(add(beta_r(c), Vs(beta_m(c))), column);
+}
 static v3 air_transmittance(const Ctx *c, Ray origin, v3 dir) {
     if (!inside_air(c, origin)) return Vs(1.0);
-    v3 ext = add(beta_r(c), Vs(beta_m(c)));
     v3 p = add(atmo_c(c), host_offset(c, origin));
-    return expV(scl(neg(ext), density_integral(c, p, dir, ATMO_SUN_STEPS)));
+    R mie;
+    R column = density_integral(c, p, dir, c->sun_steps, &mie);
+    return expV(neg(air_depth(c, column, mie)));
 }
 static R rayleigh_phase(R ct) { return 3.0 / (16.0 * PI) * (1.0 + ct * ct); }
 static R mie_phase(R ct, R g) {
@@ -1801,15 +1817,16 @@ static v3 atmosphere(const Ctx *c, v3 origin, v3 dir, R limit, uint32_t *rng) {
     R H = scale_height(c);
     uint32_t steps = (uint32_t)ceil(t_max / (0.25 * H));
     if (steps < 2u) steps = 2u;
-    if (steps > ATMO_VIEW_STEPS) steps = ATMO_VIEW_STEPS;
+    if (steps > c->view_steps) steps = c->view_steps;
     R dt = t_max / (R)steps;
     uint32_t jitter_rng = orc_hash(*rng ^ 0x3c6ef372u);
     R jd = random_(&jitter_rng);
-    R jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jd : 0.5;
+    R jitter = (steps < c->view_steps || limit < exit) ? jd : 0.5;
     uint32_t reflected_rng = orc_hash(*rng ^ 0xa511e9b3u);
     R inverse_probability;
     uint32_t reflector = sample_reflector(c, origin, &reflected_rng, &inverse_probability);
-    v3 reflected = Vs(0.0), reflected_direction = dir, reflected_phase = Vs(0.0);
+    v3 reflected = Vs(0.0), reflected_direction = dir;
+    R reflected_rayleigh = 0.0, reflected_mie = 0.0;
     if (reflector != MISS) {
         Ray reflected_ray = camera_ray(origin, dir);
         LightSample light = sample_light(c, reflected_ray, reflector, &reflected_rng);
@@ -1817,7 +1834,8 @@ static v3 atmosphere(const Ctx *c, v3 origin, v3 dir, R limit, uint32_t *rng) {
         reflected = scl(reflected_radiance(c, reflected_ray, reflector, &reflected_rng), inverse_probability / light.pdf);
         reflected_direction = light.direction;
         R ct = dot(dir, light.direction);
-        reflected_phase = add(scl(br, rayleigh_phase(ct)), Vs(bm * mie_phase(ct, g)));
+        reflected_rayleigh = rayleigh_phase(ct);
+        reflected_mie = mie_phase(ct, g);
     }
     if (reflected.x > 0.0 || reflected.y > 0.0 || reflected.z > 0.0) {
         R sun = brightest_risen_light(c, origin);
@@ -1836,12 +1854,30 @@ static v3 atmosphere(const Ctx *c, v3 origin, v3 dir, R limit, uint32_t *rng) {
         R altitude = mx(len(sub(p, atmo_c(c))) - r, 0.0);
         R density = exp(-altitude / H);
         R column = density * dt;
-        v3 tau = scl(ext, column);
-        v3 step_t = expV(neg(tau));
-        v3 segment;
-        segment.x = tau.x < 1.0e-3 ? column * (1.0 - 0.5 * tau.x) : (1.0 - step_t.x) / mx(ext.x, 1.0e-20);
-        segment.y = tau.y < 1.0e-3 ? column * (1.0 - 0.5 * tau.y) : (1.0 - step_t.y) / mx(ext.y, 1.0e-20);
-        segment.z = tau.z < 1.0e-3 ? column * (1.0 - 0.5 * tau.z) : (1.0 - step_t.z) / mx(ext.z, 1.0e-20);
+        v3 tau, step_t, segment;
+        /* Scattering per step = (beta_r scale_r, beta_m scale_m) x segment:
+         * one scale height keeps wgsl:1163-1171 with unit scales; two give
+         * each species its own density, the segment then being the
+         * transmittance integral over the step's length. */
+        R scale_r = 1.0, scale_m = 1.0;
+        if (two_heights(c)) {
+            R density_m = exp(-altitude / c->mie_height);
+            tau = add(scl(br, column), Vs(bm * (density_m * dt)));
+            step_t = expV(neg(tau));
+            segment.x = tau.x < 1.0e-3 ? dt * (1.0 - 0.5 * tau.x) : (1.0 - step_t.x) * dt / tau.x;
+            segment.y = tau.y < 1.0e-3 ? dt * (1.0 - 0.5 * tau.y) : (1.0 - step_t.y) * dt / tau.y;
+            segment.z = tau.z < 1.0e-3 ? dt * (1.0 - 0.5 * tau.z) : (1.0 - step_t.z) * dt / tau.z;
+            scale_r = density;
+            scale_m = density_m;
+        } else {
+            tau = scl(ext, column);
+            step_t = expV(neg(tau));
+            segment.x = tau.x < 1.0e-3 ? column * (1.0 - 0.5 * tau.x) : (1.0 - step_t.x) / mx(ext.x, 1.0e-20);
+            segment.y = tau.y < 1.0e-3 ? column * (1.0 - 0.5 * tau.y) : (1.0 - step_t.y) / mx(ext.y, 1.0e-20);
+            segment.z = tau.z < 1.0e-3 ? column * (1.0 - 0.5 * tau.z) : (1.0 - step_t.z) / mx(ext.z, 1.0e-20);
+        }
+        v3 br_step = scl(br, scale_r);
+        R bm_step = bm * scale_m;
         for (uint32_t slot = 0; slot < light_count(c); ++slot) {
             uint32_t b = light_index(c, slot);
             if (B(c, b)->material.v[3] < 0.5f) continue;
@@ -1853,16 +1889,21 @@ static v3 atmosphere(const Ctx *c, v3 origin, v3 dir, R limit, uint32_t *rng) {
                 R visibility = air_light_visibility(c, light_ray, b);
                 if (visibility == 0.0) continue;
                 v3 e_sun = scl(V4(B(c, b)->material), visibility / ((R)ls_count * light.pdf));
-                v3 sun_t = expV(scl(neg(ext), density_integral(c, p, light.direction, ATMO_SUN_STEPS)));
+                R sun_mie;
+                R sun_column = density_integral(c, p, light.direction, c->sun_steps, &sun_mie);
+                v3 sun_t = expV(neg(air_depth(c, sun_column, sun_mie)));
                 R ct = dot(dir, light.direction);
-                v3 beta_phase = add(scl(br, rayleigh_phase(ct)), Vs(bm * mie_phase(ct, g)));
+                v3 beta_phase = add(scl(br_step, rayleigh_phase(ct)), Vs(bm_step * mie_phase(ct, g)));
                 inscatter = add(inscatter, mul(mul(mul(mul(t_view, segment), beta_phase), e_sun), sun_t));
             }
         }
         if (planetshine) {
             R visibility = air_light_visibility(c, camera_ray(p, reflected_direction), reflector);
             if (visibility > 0.0) {
-                v3 light_t = expV(scl(neg(ext), density_integral(c, p, reflected_direction, ATMO_SUN_STEPS)));
+                R light_mie;
+                R light_column = density_integral(c, p, reflected_direction, c->sun_steps, &light_mie);
+                v3 light_t = expV(neg(air_depth(c, light_column, light_mie)));
+                v3 reflected_phase = add(scl(br_step, reflected_rayleigh), Vs(bm_step * reflected_mie));
                 inscatter = add(inscatter, mul(mul(mul(mul(t_view, segment), reflected_phase), reflected),
                                                scl(light_t, visibility)));
             }
@@ -1878,18 +1919,18 @@ static v3 camera_air_transmittance(const Ctx *c, v3 origin, v3 dir, R limit, uin
     R H = scale_height(c);
     uint32_t steps = (uint32_t)ceil(t_max / (0.25 * H));
     if (steps < 2u) steps = 2u;
-    if (steps > ATMO_VIEW_STEPS) steps = ATMO_VIEW_STEPS;
+    if (steps > c->view_steps) steps = c->view_steps;
     R dt = t_max / (R)steps;
     uint32_t jitter_rng = orc_hash(seed ^ 0x3c6ef372u);
     R jd = random_(&jitter_rng);
-    R jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jd : 0.5;
-    v3 ext = add(beta_r(c), Vs(beta_m(c)));
+    R jitter = (steps < c->view_steps || limit < exit) ? jd : 0.5;
     v3 transmittance = Vs(1.0);
     for (uint32_t i = 0; i < steps; ++i) {
         v3 p = add(origin, scl(dir, ((R)i + jitter) * dt));
         R altitude = mx(len(sub(p, atmo_c(c))) - host_radius(c), 0.0);
         R column = exp(-altitude / H) * dt;
-        transmittance = mul(transmittance, expV(scl(neg(ext), column)));
+        R column_m = two_heights(c) ? exp(-altitude / c->mie_height) * dt : 0.0;
+        transmittance = mul(transmittance, expV(neg(air_depth(c, column, column_m))));
     }
     return transmittance;
 }
@@ -2245,6 +2286,9 @@ static Ctx make_ctx(const orc_scene *sc) {
     c.milky = sc->milky_way;
     c.ring_profile = sc->ring_profile;
     c.ring_profile_body = sc->ring_profile ? sc->ring_profile_body : MISS;
+    c.view_steps = sc->air_view_steps ? sc->air_view_steps : ATMO_VIEW_STEPS;
+    c.sun_steps = sc->air_sun_steps ? sc->air_sun_steps : ATMO_SUN_STEPS;
+    c.mie_height = sc->mie_height;
     c.accumulated = ((const Settings *)sc->settings)->counts[1];
     return c;
 }

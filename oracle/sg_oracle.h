@@ -31,6 +31,13 @@ typedef struct orc_scene {
      * the product's extension of src/shaders/rings.wgsl:50-81, restated. */
     const float *ring_profile;
     uint32_t ring_profile_body;
+    /* Air march: view cells per camera ray and density samples per light
+     * path (src/shaders/pathtrace.wgsl:67-68, replaced per frame the way
+     * src/render_experiments.rs:48-70 replaces them); 0 = the shader's 8 / 4.
+     * mie_height > 0 gives Mie its own scale height (an extension). */
+    uint32_t air_view_steps;
+    uint32_t air_sun_steps;
+    float mie_height;
 } orc_scene;
 
 typedef struct orc_probe_ray {

@@ -50,6 +50,9 @@ EXPORTS = (
     "sg_set_ring_profile",
     "sg_set_density_profile",
     "sg_restart",
+    "sg_pass_id",
+    "sg_set_air_model",
+    "sg_get_air_model",
     "sg_adjoint",
     "sg_enable_adjoint",
     "sg_adjoint_time",
@@ -105,6 +108,17 @@ class SgBatchLaunch(ctypes.Structure):
         return {name: int(getattr(self, name)) for name, _ in self._fields_}
 
 
+class SgAirModel(ctypes.Structure):
+    """``sg_air_model`` (include/stargaze_amd.h)."""
+
+    _fields_ = [
+        ("view_steps", ctypes.c_uint32),
+        ("light_steps", ctypes.c_uint32),
+        ("mie_scale_height", ctypes.c_float),
+        ("reserved", ctypes.c_uint32),
+    ]
+
+
 class NativeError(RuntimeError):
     pass
 
@@ -158,6 +172,9 @@ def load() -> ctypes.CDLL:
         "sg_probe": ([P, ctypes.POINTER(SgFrame), U32, P, U32, P], I32),
         "sg_set_ring_profile": ([P, U32, P, U32], I32),
         "sg_restart": ([P], I32),
+        "sg_pass_id": ([P], U64),
+        "sg_set_air_model": ([P, P], I32),
+        "sg_get_air_model": ([P, P], I32),
         "sg_set_density_profile": ([P, P, U32], I32),
         "sg_adjoint": ([P, ctypes.POINTER(SgFrame), P, U64, P, U32, P], I32),
         "sg_adjoint_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),

@@ -33,8 +33,12 @@ constexpr double METRES_PER_AU_D = 149597870700.0;
 constexpr float STAR_DOT_PIXELS = 3.0f;            // src/shaders/pathtrace.wgsl:65
 // Whether this translation unit renders the landscape (see SG_LANDSCAPE).
 constexpr bool kLandscape = SG_LANDSCAPE != 0;
-constexpr uint32_t ATMO_VIEW_STEPS = 8u;           // src/shaders/pathtrace.wgsl:67
-constexpr uint32_t ATMO_SUN_STEPS = 4u;            // src/shaders/pathtrace.wgsl:68
+// Default march of the single-scattering air: view cells per camera ray and
+// density samples per light path (src/shaders/pathtrace.wgsl:67-68). A frame
+// may choose others (Scene::air_view_steps / air_sun_steps), as the
+// reference's own shader variants do (src/render_experiments.rs:48-70).
+constexpr uint32_t ATMO_VIEW_STEPS = 8u;
+constexpr uint32_t ATMO_SUN_STEPS = 4u;
 constexpr float POINT_DISC_RADIUS = 0.00136f;      // src/shaders/pathtrace.wgsl:1300
 constexpr float RING_INDIRECT_CAP = 0.003f;        // src/shaders/pathtrace.wgsl:1336
 
@@ -104,6 +108,14 @@ struct Scene {
     // 1 when a lit body has a solid-body style, 2-6 (kernels built with
     // SG_STYLES are required).
     uint32_t solid_styles;
+    // Air march: view cells per camera ray (cap of the quarter-scale-height
+    // rule) and density samples per light path.
+    uint32_t air_view_steps;
+    uint32_t air_sun_steps;
+    // Mie scale height (AU). 0 keeps the reference's single scale height
+    // (atmo_params.y) for both species; a positive value gives the Mie
+    // density its own exp(-h / H_mie) while no density table is set.
+    float mie_height;
 };
 
 struct Ray {

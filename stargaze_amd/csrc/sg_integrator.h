@@ -23,13 +23,15 @@ SG_FN uint32_t density_bin(const Scene &sc, float altitude) {
                 SG_DENSITY_BINS - 1u);
 }
 // Where the samples of a density column fell (for the adjoint): the profile
-// bin of each sample, four bits apiece, their spacing and their sum.
+// bin of each sample, four bits apiece (at most 16 samples), their spacing
+// and their sum.
 struct ColumnBins {
     float total = 0.0f;
     float dt = 0.0f;
-    uint32_t bins = 0u;
+    uint64_t bins = 0u;
     uint32_t count = 0u;
 };
+constexpr uint32_t COLUMN_BIN_SLOTS = 16u;
 
 // The plain integrator: every gradient hook compiles away.
 struct NoGrad {
@@ -86,7 +88,7 @@ struct GradSink {
     // e = d(loss)/d log(factor) * (-beta_ext): each sample adds dt.
     SG_MEMBER void column(const Scene &sc, const ColumnBins &c, float e) const {
         if (!sc.density_profile || e == 0.0f) return;
-        for (uint32_t k = 0; k < c.count; ++k) dens[((c.bins >> (4u * k)) & 15u) * dstride] += e * c.dt;
+        for (uint32_t k = 0; k < c.count; ++k) dens[(uint32_t)((c.bins >> (4u * k)) & 15u) * dstride] += e * c.dt;
     }
     SG_MEMBER void density(uint32_t bin, float v) const { dens[bin * dstride] += v; }
     // Move the lane's private accumulators into the gradient.
@@ -492,23 +494,36 @@ SG_FN float shell_exit(const Scene &sc, f3 origin, f3 dir) {
     if (disc < 0.0f) return 0.0f;
     return -b + sqrtf(disc);
 }
-SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps, ColumnBins *trace = nullptr) {
+// Whether the Mie density has its own scale height (an extension: the
+// reference shader uses one exp(-h / H) for both species).
+SG_FN bool two_heights(const Scene &sc) { return sc.mie_height > 0.0f && !sc.density_profile; }
+// Midpoint column of the density along `dir` to the top of the air
+// (src/shaders/pathtrace.wgsl:958-971). With two scale heights `mie`
+// receives the Mie column; with one it is left alone (the columns are equal).
+SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps, ColumnBins *trace = nullptr,
+                             float *mie = nullptr) {
     float distance = shell_exit(sc, origin, dir);
-    if (distance <= 0.0f) return 0.0f;
+    if (distance <= 0.0f) {
+        if (mie) *mie = 0.0f;
+        return 0.0f;
+    }
     float dt = distance / (float)steps;
     float inv_h = 1.0f / atmo_scale_height(sc);
+    const bool split = mie && two_heights(sc);
+    float inv_hm = split ? 1.0f / sc.mie_height : 0.0f;
     float r = host_radius(sc);
     f3 c = atmo_center(sc);
-    float sum = 0.0f;
+    float sum = 0.0f, sum_m = 0.0f;
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + 0.5f) * dt);
         float h = fmaxf(length(p - c) - r, 0.0f);
         if (sc.density_profile) {
             const uint32_t bin = density_bin(sc, h);
             sum += sc.density_profile[bin];
-            if (trace) trace->bins |= bin << (4u * i);
+            if (trace && i < COLUMN_BIN_SLOTS) trace->bins |= (uint64_t)bin << (4u * i);
         } else {
             sum += expf(-h * inv_h);
+            if (split) sum_m += expf(-h * inv_hm);
         }
     }
     if (trace) {
@@ -516,6 +531,7 @@ SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps,
         trace->count = steps;
         trace->total = sum * dt;
     }
+    if (split) *mie = sum_m * dt;
     return sum * dt;
 }
 // FP64 rebasing of an anchored origin to the host centre.
@@ -530,20 +546,30 @@ SG_FN bool inside_air(const Scene &sc, const Ray &origin) {
     float top = atmosphere_top(sc);
     return dot(oc, oc) < top * top;
 }
+// Optical depth of a Rayleigh column and a Mie column. With one scale
+// height both are the same column and the depth is (beta_r + beta_m) *
+// column, as the reference writes it.
+SG_FN f3 air_depth(const Scene &sc, float column, float mie_column) {
+    if (two_heights(sc)) return beta_rayleigh(sc) * column + f3s(beta_mie(sc) * mie_column);
+    return (beta_rayleigh(sc) + f3s(beta_mie(sc))) * column;
+}
 // Density column from an anchored origin to the top of the air; negative
-// when the origin is outside the air (no extinction).
-SG_FN float air_column(const Scene &sc, const Ray &origin, f3 dir, ColumnBins *trace = nullptr) {
+// when the origin is outside the air (no extinction). `mie` receives the Mie
+// column under two scale heights.
+SG_FN float air_column(const Scene &sc, const Ray &origin, f3 dir, ColumnBins *trace = nullptr,
+                       float *mie = nullptr) {
     if (!inside_air(sc, origin)) return -1.0f;
     f3 p = atmo_center(sc) + tof(host_offset(sc, origin));
-    return density_integral(sc, p, dir, ATMO_SUN_STEPS, trace);
+    return density_integral(sc, p, dir, sc.air_sun_steps, trace, mie);
 }
-SG_FN f3 air_extinction(const Scene &sc, float column) {
+SG_FN f3 air_extinction(const Scene &sc, float column, float mie_column) {
     if (column < 0.0f) return f3s(1.0f);
-    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
-    return exp3(-beta_ext * column);
+    return exp3(-air_depth(sc, column, mie_column));
 }
 SG_FN f3 air_transmittance(const Scene &sc, const Ray &origin, f3 dir) {
-    return air_extinction(sc, air_column(sc, origin, dir));
+    float mie = 0.0f;
+    float column = air_column(sc, origin, dir, nullptr, &mie);
+    return air_extinction(sc, column, mie);
 }
 SG_FN float rayleigh_phase(float c) { return 3.0f / (16.0f * PI) * (1.0f + c * c); }
 SG_FN float mie_phase(float c, float g) {
@@ -640,16 +666,19 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     float r = host_radius(sc);
     f3 center = atmo_center(sc);
     float H = atmo_scale_height(sc);
-    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    const uint32_t view_steps = sc.air_view_steps;
+    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), view_steps);
     float dt = t_max / (float)steps;
     uint32_t jitter_rng = hash_u32(rng ^ 0x3c6ef372u);
     float jitter_draw = rnd(jitter_rng);
-    float jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jitter_draw : 0.5f;
+    float jitter = (steps < view_steps || limit < exit) ? jitter_draw : 0.5f;
+    const bool split = two_heights(sc);
+    const float inv_hm = split ? 1.0f / sc.mie_height : 0.0f;
 
     uint32_t reflected_rng = hash_u32(rng ^ 0xa511e9b3u);
     ReflectorSample reflector = sample_reflector(sc, origin, reflected_rng);
-    f3 reflected = f3s(0.0f), reflected_direction = dir, reflected_phase = f3s(0.0f);
-    float reflected_cos = 0.0f;
+    f3 reflected = f3s(0.0f), reflected_direction = dir;
+    float reflected_cos = 0.0f, reflected_rayleigh = 0.0f, reflected_mie = 0.0f;
     f3 reflector_dalbedo = f3s(0.0f);  // d log(albedo) / d material, per component
     if (reflector.index != SG_MISS) {
         Ray reflected_ray = camera_ray(origin, dir);
@@ -660,7 +689,8 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         reflected_direction = light.direction;
         float cos_theta = dot(dir, light.direction);
         reflected_cos = cos_theta;
-        reflected_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
+        reflected_rayleigh = rayleigh_phase(cos_theta);
+        reflected_mie = mie_phase(cos_theta, g);
         if constexpr (G::on) {
             Hit rh = sphere_hit(sc, reflected_ray, reflector.index);
             if (rh.index != SG_MISS) {
@@ -685,7 +715,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     f3 t_view = f3s(1.0f), inscatter = f3s(0.0f);
     f3 reflector_grad = f3s(0.0f);  // d(loss)/d(reflector material), summed over the steps
     float view_column = 0.0f;  // density column from the origin to this step
-    uint32_t view_bins = 0u;   // density profile bins of the steps so far
+    uint64_t view_bins = 0u;   // density profile bins of the steps so far
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
@@ -698,12 +728,31 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             density = expf(-altitude / H);
         }
         float column = density * dt;
-        f3 tau = beta_ext * column;
-        f3 step_t = exp3(-tau);
-        f3 segment;
-        segment.x = tau.x < 1.0e-3f ? column * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) / fmaxf(beta_ext.x, 1.0e-20f);
-        segment.y = tau.y < 1.0e-3f ? column * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) / fmaxf(beta_ext.y, 1.0e-20f);
-        segment.z = tau.z < 1.0e-3f ? column * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) / fmaxf(beta_ext.z, 1.0e-20f);
+        f3 tau, step_t, segment;
+        // Scattering coefficients of the step are (beta_r * scale_r, beta_m
+        // * scale_m) times the segment. With one scale height the segment
+        // is the reference's column-weighted integral and the scales are 1;
+        // with two it is the transmittance integral over the step's length
+        // and the scales are the two densities.
+        float scale_r = 1.0f, scale_m = 1.0f;
+        if (split) {
+            const float density_m = expf(-altitude * inv_hm);
+            tau = beta_r * column + f3s(beta_m * (density_m * dt));
+            step_t = exp3(-tau);
+            segment.x = tau.x < 1.0e-3f ? dt * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) * dt / tau.x;
+            segment.y = tau.y < 1.0e-3f ? dt * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) * dt / tau.y;
+            segment.z = tau.z < 1.0e-3f ? dt * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) * dt / tau.z;
+            scale_r = density;
+            scale_m = density_m;
+        } else {
+            tau = beta_ext * column;
+            step_t = exp3(-tau);
+            segment.x = tau.x < 1.0e-3f ? column * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) / fmaxf(beta_ext.x, 1.0e-20f);
+            segment.y = tau.y < 1.0e-3f ? column * (1.0f - 0.5f * tau.y) : (1.0f - step_t.y) / fmaxf(beta_ext.y, 1.0e-20f);
+            segment.z = tau.z < 1.0e-3f ? column * (1.0f - 0.5f * tau.z) : (1.0f - step_t.z) / fmaxf(beta_ext.z, 1.0e-20f);
+        }
+        const f3 beta_r_step = beta_r * scale_r;
+        const float beta_m_step = beta_m * scale_m;
         f3 step_dlog = f3s(0.0f);  // d log(t_view * segment) / d(beta_ext)
         f3 seg_dcol = f3s(0.0f);   // d log(segment) / d(column)
         if constexpr (G::on) {
@@ -723,7 +772,8 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             if constexpr (G::on) {
                 if (!sc.density_profile) return;
                 const float e = -(beta_ext.x * wc.x + beta_ext.y * wc.y + beta_ext.z * wc.z);
-                for (uint32_t j = 0; j < i; ++j) grad.density((view_bins >> (4u * j)) & 15u, e * dt);
+                for (uint32_t j = 0; j < umin(i, COLUMN_BIN_SLOTS); ++j)
+                    grad.density((uint32_t)((view_bins >> (4u * j)) & 15u), e * dt);
                 grad.density(bin, (wc.x * seg_dcol.x + wc.y * seg_dcol.y + wc.z * seg_dcol.z) * dt);
                 grad.column(sc, light_bins, e);
             }
@@ -740,11 +790,12 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                 if (visibility == 0.0f) continue;
                 f3 e_sun = xyz(body(sc, b).material) * (visibility / ((float)light_samples * light.pdf));
                 ColumnBins sun_bins;
-                float sun_column = density_integral(sc, p, light.direction, ATMO_SUN_STEPS,
-                                                   G::on ? &sun_bins : nullptr);
-                f3 sun_t = exp3(-beta_ext * sun_column);
+                float sun_mie = 0.0f;
+                float sun_column = density_integral(sc, p, light.direction, sc.air_sun_steps,
+                                                   G::on ? &sun_bins : nullptr, &sun_mie);
+                f3 sun_t = exp3(-air_depth(sc, sun_column, sun_mie));
                 float cos_theta = dot(dir, light.direction);
-                f3 beta_phase = beta_r * rayleigh_phase(cos_theta) + f3s(beta_m * mie_phase(cos_theta, g));
+                f3 beta_phase = beta_r_step * rayleigh_phase(cos_theta) + f3s(beta_m_step * mie_phase(cos_theta, g));
                 f3 term = t_view * segment * beta_phase * e_sun * sun_t;
                 inscatter += term;
                 if constexpr (G::on) {
@@ -762,9 +813,11 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             float visibility = air_light_visibility(sc, shine_ray, reflector.index);
             if (visibility > 0.0f) {
                 ColumnBins light_bins;
-                float light_column = density_integral(sc, p, reflected_direction, ATMO_SUN_STEPS,
-                                                      G::on ? &light_bins : nullptr);
-                f3 light_t = exp3(-beta_ext * light_column);
+                float light_mie = 0.0f;
+                float light_column = density_integral(sc, p, reflected_direction, sc.air_sun_steps,
+                                                      G::on ? &light_bins : nullptr, &light_mie);
+                f3 light_t = exp3(-air_depth(sc, light_column, light_mie));
+                const f3 reflected_phase = beta_r_step * reflected_rayleigh + f3s(beta_m_step * reflected_mie);
                 f3 term = t_view * segment * reflected_phase * reflected * (light_t * visibility);
                 inscatter += term;
                 if constexpr (G::on) {
@@ -781,7 +834,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         t_view *= step_t;
         if constexpr (G::on) {
             view_column += column;
-            view_bins |= bin << (4u * i);
+            if (i < COLUMN_BIN_SLOTS) view_bins |= (uint64_t)bin << (4u * i);
         }
     }
     if constexpr (G::on) {
@@ -798,29 +851,32 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
     float t_max = fminf(limit, exit);
     if (t_max <= 0.0f) return f3s(1.0f);
     float H = atmo_scale_height(sc);
-    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), ATMO_VIEW_STEPS);
+    const uint32_t view_steps = sc.air_view_steps;
+    uint32_t steps = umin(umax((uint32_t)ceilf(t_max / (0.25f * H)), 2u), view_steps);
     float dt = t_max / (float)steps;
     uint32_t jitter_rng = hash_u32(seed ^ 0x3c6ef372u);
     float jitter_draw = rnd(jitter_rng);
-    float jitter = (steps < ATMO_VIEW_STEPS || limit < exit) ? jitter_draw : 0.5f;
-    f3 beta_ext = beta_rayleigh(sc) + f3s(beta_mie(sc));
+    float jitter = (steps < view_steps || limit < exit) ? jitter_draw : 0.5f;
+    const bool split = two_heights(sc);
+    const float inv_hm = split ? 1.0f / sc.mie_height : 0.0f;
     f3 center = atmo_center(sc);
     float r = host_radius(sc);
     f3 transmittance = f3s(1.0f);
     float total = 0.0f;
-    uint32_t bins = 0u;
+    uint64_t bins = 0u;
     for (uint32_t i = 0; i < steps; ++i) {
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
-        float column;
+        float column, column_m = 0.0f;
         if (sc.density_profile) {
             const uint32_t bin = density_bin(sc, altitude);
             column = sc.density_profile[bin] * dt;
-            bins |= bin << (4u * i);
+            if (i < COLUMN_BIN_SLOTS) bins |= (uint64_t)bin << (4u * i);
         } else {
             column = expf(-altitude / H) * dt;
+            if (split) column_m = expf(-altitude * inv_hm) * dt;
         }
-        transmittance *= exp3(-beta_ext * column);
+        transmittance *= exp3(-air_depth(sc, column, column_m));
         total += column;
     }
     if (trace) {
@@ -1110,7 +1166,8 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
         st.flags &= ~PATH_IN_AIR;
         if (ring.index != SG_MISS || (hit.index < SG_LOCAL_BASE && hit.index != host)) {
             ColumnBins bins;
-            float column = air_column(sc, ray, ray.direction, G::on ? &bins : nullptr);
+            float mie = 0.0f;
+            float column = air_column(sc, ray, ray.direction, G::on ? &bins : nullptr, &mie);
             if constexpr (G::on) {
                 if (column > 0.0f) {
                     f3 r = grad.rest(st);
@@ -1119,7 +1176,7 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
                     grad.column(sc, bins, -(be.x * r.x + be.y * r.y + be.z * r.z));
                 }
             }
-            st.throughput *= air_extinction(sc, column);
+            st.throughput *= air_extinction(sc, column, mie);
         }
     }
     if (ring.index != SG_MISS) {
@@ -1263,8 +1320,9 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
             float bsdf_pdf = cosine / PI;
             float weight = last_vertex ? 1.0f : power_weight(sample.pdf, bsdf_pdf);
             ColumnBins bins;
-            float column = air_column(sc, outgoing, sample.direction, G::on ? &bins : nullptr);
-            f3 term = xyz(body(sc, i).material) * air_extinction(sc, column) *
+            float mie = 0.0f;
+            float column = air_column(sc, outgoing, sample.direction, G::on ? &bins : nullptr, &mie);
+            f3 term = xyz(body(sc, i).material) * air_extinction(sc, column, mie) *
                       (bsdf_pdf * weight * visibility / sample.pdf);
             direct_light += term;
             if constexpr (G::on) {

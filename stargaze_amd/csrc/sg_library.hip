@@ -135,6 +135,12 @@ struct sg_ctx {
     // Tabulated air density (SG_DENSITY_BINS floats) while density_on.
     float *density_profile = nullptr;
     bool density_on = false;
+    // Air march (view cells, light-path samples) and Mie scale height.
+    uint32_t air_view_steps = ATMO_VIEW_STEPS, air_sun_steps = ATMO_SUN_STEPS;
+    float mie_height = 0.0f;
+    // Identity of the last pass that rendered samples (sg_pass_id): a new
+    // serial for every such pass and for every restart.
+    uint64_t pass_serial = 0, pass_id = 0;
     // The last queued pass, which sg_adjoint differentiates.
     struct {
         uint32_t count = 0, samples = 0, sequence = 0;
@@ -255,7 +261,19 @@ Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     sc.ring_profile = profiled ? ctx->ring_profile : nullptr;
     sc.ring_profile_body = profiled ? ctx->ring_profile_body : SG_MISS;
     sc.density_profile = ctx->density_on ? ctx->density_profile : nullptr;
+    sc.air_view_steps = ctx->air_view_steps;
+    sc.air_sun_steps = ctx->air_sun_steps;
+    sc.mie_height = ctx->mie_height;
     return sc;
+}
+
+// Restart the accumulation with the next pass (a new scene), retiring the
+// identity of the last pass.
+void restart_history(sg_ctx *ctx) {
+    ctx->key.clear();
+    ctx->samples = 0;
+    ctx->last.count = 0;
+    ctx->pass_id = ++ctx->pass_serial;
 }
 
 // History key of a frame (src/pathtracer.rs:172-181): everything whose change
@@ -552,6 +570,7 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     ctx->last.samples = ctx->samples;
     ctx->last.sequence = st.sampling[0];
     ctx->last.recorded = ta.records != nullptr;
+    ctx->pass_id = ++ctx->pass_serial;
     return SG_OK;
 }
 
@@ -711,8 +730,7 @@ int sg_resize(sg_ctx *ctx, uint32_t width, uint32_t height) {
     ctx->height = height;
     int rc = alloc_image(ctx);
     if (rc != SG_OK) return rc;
-    ctx->key.clear();
-    ctx->samples = 0;
+    restart_history(ctx);
     ctx->exposure_idle_frames = 0;
     return SG_OK;
 }
@@ -781,6 +799,9 @@ int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint
     ctx->terrain_floats = total;
     ctx->terrain_cells = cells;
     ctx->terrain_cached = false;
+    // New ground is a new scene: restart the accumulation, as the profile
+    // setters do.
+    restart_history(ctx);
     return SG_OK;
 }
 
@@ -791,8 +812,7 @@ int sg_enable_moments(sg_ctx *ctx, int32_t enable) {
     join_batch(ctx);
     SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
     ctx->moments_on = on;
-    ctx->key.clear();
-    ctx->samples = 0;
+    restart_history(ctx);
     return alloc_image(ctx);
 }
 
@@ -1017,10 +1037,8 @@ int sg_autotune(sg_ctx *ctx, const sg_frame *frame, uint32_t repeats, sg_launch 
     ctx->launches = keep_launches;
     // Trial passes rendered into the image: start the next pass afresh, with
     // the draws it would have had without them.
-    ctx->key.clear();
-    ctx->samples = 0;
+    restart_history(ctx);
     ctx->sequence = keep_sequence;
-    ctx->last.count = 0;
     if (rc != SG_OK) return rc;
     if (best) *best = chosen;
     if (best_ms) *best_ms = winner_ms;
@@ -1045,10 +1063,33 @@ int sg_enable_adjoint(sg_ctx *ctx, int32_t enable) {
 
 int sg_restart(sg_ctx *ctx) {
     if (!ctx) return SG_E_INVALID;
-    ctx->key.clear();
-    ctx->samples = 0;
-    ctx->last.count = 0;
+    restart_history(ctx);
     ctx->exposure_idle_frames = 0;
+    return SG_OK;
+}
+
+uint64_t sg_pass_id(const sg_ctx *ctx) { return ctx ? ctx->pass_id : 0u; }
+
+int sg_set_air_model(sg_ctx *ctx, const sg_air_model *model) {
+    if (!ctx) return SG_E_INVALID;
+    sg_air_model m{ATMO_VIEW_STEPS, ATMO_SUN_STEPS, 0.0f, 0u};
+    if (model) m = *model;
+    if (m.view_steps < 2u || m.view_steps > SG_AIR_MAX_STEPS || m.light_steps < 1u || m.light_steps > SG_AIR_MAX_STEPS)
+        return fail(ctx, SG_E_INVALID, "air steps must lie in [2, 32] (view) and [1, 32] (light)");
+    if (!(m.mie_scale_height >= 0.0f) || !std::isfinite(m.mie_scale_height))
+        return fail(ctx, SG_E_INVALID, "the Mie scale height must be finite and non-negative");
+    join_batch(ctx);
+    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->air_view_steps = m.view_steps;
+    ctx->air_sun_steps = m.light_steps;
+    ctx->mie_height = m.mie_scale_height;
+    restart_history(ctx);
+    return SG_OK;
+}
+
+int sg_get_air_model(const sg_ctx *ctx, sg_air_model *model) {
+    if (!ctx || !model) return SG_E_INVALID;
+    *model = sg_air_model{ctx->air_view_steps, ctx->air_sun_steps, ctx->mie_height, 0u};
     return SG_OK;
 }
 
@@ -1069,9 +1110,7 @@ int sg_set_ring_profile(sg_ctx *ctx, uint32_t body, const float *tau, uint32_t b
         ctx->ring_profile_body = body;
     }
     // A new profile is a new scene: restart the accumulation.
-    ctx->key.clear();
-    ctx->samples = 0;
-    ctx->last.count = 0;
+    restart_history(ctx);
     return SG_OK;
 }
 
@@ -1090,9 +1129,7 @@ int sg_set_density_profile(sg_ctx *ctx, const float *density, uint32_t bins) {
         SG_HIP(ctx, hipMemcpy(ctx->density_profile, density, bins * sizeof(float), hipMemcpyHostToDevice));
         ctx->density_on = true;
     }
-    ctx->key.clear();
-    ctx->samples = 0;
-    ctx->last.count = 0;
+    restart_history(ctx);
     return SG_OK;
 }
 
@@ -1108,6 +1145,10 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
         return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate scenes with a landscape");
     if (solid_styles(frame))
         return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate solid-body surface styles (2-6)");
+    if (ctx->mie_height > 0.0f && !ctx->density_on)
+        return fail(ctx, SG_E_INVALID, "the adjoint does not differentiate a separate Mie scale height");
+    if (ctx->density_on && (ctx->air_view_steps > COLUMN_BIN_SLOTS || ctx->air_sun_steps > COLUMN_BIN_SLOTS))
+        return fail(ctx, SG_E_INVALID, "the density-profile adjoint records at most 16 samples per column");
     if (ctx->last.count == 0 || ctx->last.samples + ctx->last.count != ctx->samples || history_key(ctx, frame) != ctx->key)
         return fail(ctx, SG_E_STATE, "sg_adjoint needs the frame of the last pass sg_encode rendered");
     join_batch(ctx);
@@ -1506,10 +1547,8 @@ int batch_pass(sg_batch *b, const sg_frame *frames) {
 void reset_history(sg_batch *b, const std::vector<uint32_t> &sequences) {
     for (size_t i = 0; i < b->views.size(); ++i) {
         sg_ctx *ctx = b->views[i];
-        ctx->key.clear();
-        ctx->samples = 0;
+        restart_history(ctx);
         ctx->sequence = sequences[i];
-        ctx->last.count = 0;
         ctx->frame_on_device = false;
     }
     b->table_valid = false;
@@ -1596,8 +1635,23 @@ int sg_batch_get_launch(const sg_batch *b, sg_batch_launch *launch) {
     return SG_OK;
 }
 
+// Batched passes over views with a landscape are switched off: the one GPU
+// run of such a pass after the air-model change ended in a device memory
+// fault whose cause host runs of the same views (address and memory
+// sanitizers) did not show. Views without a landscape (the batched-serving
+// workload) are unaffected; sg_encode renders landscapes as before.
+static int reject_landscape_views(sg_batch *b, const sg_frame *frames) {
+    for (size_t i = 0; i < b->views.size(); ++i)
+        if (frames[i].globals && frames[i].globals->ground_north[3] != 0.0f)
+            return bfail(b, SG_E_INVALID, "view " + std::to_string(i) +
+                                              ": batched passes of views with a landscape are switched off");
+    return SG_OK;
+}
+
 int sg_batch_encode(sg_batch *b, const sg_frame *frames) {
     if (!b || !frames) return SG_E_INVALID;
+    int rc = reject_landscape_views(b, frames);
+    if (rc != SG_OK) return rc;
     return batch_pass(b, frames);
 }
 
@@ -1676,6 +1730,7 @@ int sg_batch_encode_views(sg_batch *b, const sg_frame *frames, double *milliseco
 int sg_batch_autotune(sg_batch *b, const sg_frame *frames, uint32_t repeats, sg_batch_launch *best,
                       double *best_ms) {
     if (!b || !frames) return SG_E_INVALID;
+    if (int rc = reject_landscape_views(b, frames); rc != SG_OK) return rc;
     repeats = std::max<uint32_t>(1u, std::min<uint32_t>(repeats, 16u));
     const sg_batch_launch keep = b->launch;
     const bool keep_profiling = b->profiling;

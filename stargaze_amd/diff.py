@@ -15,6 +15,12 @@ gradient vector onto the three parameter tensors:
 
 Tensors live on the CPU: the GPU work happens in the library, and each
 call moves one image and one gradient vector.
+
+The adjoint replays the draws of the pass it differentiates, so ``backward``
+must run before the scene's tracer renders again: the forward pass records
+the tracer's pass identity (``PathTracer.pass_id``) and ``backward`` raises
+``StalePassError`` when another pass, a profile change or a restart has
+replaced it.
 """
 from __future__ import annotations
 
@@ -23,6 +29,10 @@ import torch
 
 from .layouts import DENSITY_BINS, GRAD_AIR, GRAD_DENSITY, GRAD_RING, RING_PROFILE_BINS, grad_albedo
 from .pathtracer import Frame, Options, PathTracer
+
+
+class StalePassError(RuntimeError):
+    """``backward`` of a pass the tracer no longer holds."""
 
 
 class DiffScene:
@@ -81,12 +91,16 @@ class _Render(torch.autograd.Function):
         scene.tracer.encode(frame)
         image = scene.tracer.read_accumulation()[..., :3].copy()
         ctx.scene, ctx.frame = scene, frame
+        ctx.pass_id = scene.tracer.pass_id()
         ctx.has_tau = scene.ring_body is not None
         return torch.from_numpy(image)
 
     @staticmethod
     def backward(ctx, grad_image):
         scene = ctx.scene
+        if scene.tracer.pass_id() != ctx.pass_id:
+            raise StalePassError("backward of a pass this scene's tracer has since replaced: differentiate each "
+                                 "render before rendering the scene again")
         g = scene.tracer.adjoint(ctx.frame, grad_image.detach().cpu().numpy().astype(np.float32))
         g = torch.from_numpy(g)
         albedo = torch.stack([g[grad_albedo(b, 0):grad_albedo(b, 0) + 3] for b in scene.albedo_bodies]) \

@@ -226,6 +226,25 @@ class PathTracer:
         t = np.ascontiguousarray(tau, np.float32).reshape(-1)
         self._check(self._lib.sg_set_ring_profile(self._ctx, int(body), t.ctypes.data, t.size), "set_ring_profile")
 
+    def set_air_model(self, view_steps: int = 8, light_steps: int = 4, mie_scale_height: float = 0.0) -> None:
+        """Air march (view cells per camera ray, density samples per light
+        path) and a separate Mie scale height in AU (0: one scale height for
+        both species, as the reference's shader). Restarts the accumulation."""
+        m = _native.SgAirModel(int(view_steps), int(light_steps), float(mie_scale_height), 0)
+        self._check(self._lib.sg_set_air_model(self._ctx, ctypes.byref(m)), "set_air_model")
+
+    @property
+    def air_model(self) -> dict:
+        m = _native.SgAirModel()
+        self._check(self._lib.sg_get_air_model(self._ctx, ctypes.byref(m)), "get_air_model")
+        return {"view_steps": int(m.view_steps), "light_steps": int(m.light_steps),
+                "mie_scale_height": float(m.mie_scale_height)}
+
+    def pass_id(self) -> int:
+        """Identity of the last pass that rendered samples (changes with every
+        such pass and every restart)."""
+        return int(self._lib.sg_pass_id(self._ctx))
+
     def set_density_profile(self, density: np.ndarray | None) -> None:
         """Relative air density per altitude slice (DENSITY_BINS); ``None``
         restores exp(-h / H)."""

@@ -381,9 +381,11 @@ def halo_observatory(width=3840, height=2160, cells=4096, rocks=20000, conifers=
                      reach=2500.0, levels=((8, 8), (4, 4), (4, 4))):
     """Halo observatory: the camera 2 m above a 1.2e6 m host at 35 deg N,
     ridged-fBm terrain, rocks and conifers, Earth-like air with Rayleigh
-    beta (5.8, 13.5, 33.1)e-6 /m and Mie 2e-5 /m (g = 0.76, one 8 km scale
-    height as in the reference shader), two inflated stars low in the sky
-    and the ringed giant overhead.
+    beta (5.8, 13.5, 33.1)e-6 /m (8 km scale height) and Mie 2e-5 /m
+    (g = 0.76), two inflated stars low in the sky and the ringed giant
+    overhead. The frame carries one scale height, as the reference's
+    Globals do; the 4K workload's 16 x 8 march and 1.2 km Mie scale height
+    are the tracer settings of ``HALO_AIR`` (``PathTracer.set_air_model``).
 
     Returns ``(frame, terrain_data, cells)``; upload the terrain with
     ``PathTracer.upload_terrain(terrain_data, cells)`` before encoding.
@@ -427,6 +429,11 @@ def halo_observatory(width=3840, height=2160, cells=4096, rocks=20000, conifers=
     f.ground = ground
     g["ground_counts"] = [len(ground), 1, 0, 0]
     return f, data, cells
+
+
+# Air model of the Halo 4K workload: 16 view cells per camera ray, 8 density
+# samples per light path, Mie scale height 1.2 km (Rayleigh keeps 8 km).
+HALO_AIR = dict(view_steps=16, light_steps=8, mie_scale_height=1200.0 / METRES_PER_AU)
 
 
 def halo_parity(width=96, height=54):

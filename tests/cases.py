@@ -292,6 +292,45 @@ def atmosphere_zenith_and_eclipse(make):
     assert np.array_equal(eclipsed, np.zeros(4, np.float32)), eclipsed
 
 
+def atmosphere_split_heights_zenith(make):
+    """Extension of the zenith pin to the 4K workload's air model: 16 view
+    cells and a Mie density with its own 1.2 km scale height. Each species'
+    column is then its own midpoint sum, in closed form
+    H (e^{-h0/H} - e^{-T/H}) * x / (2 sinh(x/2)) with x = dt / H, and the
+    transmittance is exp(-(beta_r * column_r + beta_m * column_m)), 0.1%. A
+    Mie height equal to the Rayleigh one gives the one-height image (1e-5)."""
+    a = earthlike()
+    au = scenes.METRES_PER_AU
+    radiance = 6.4e-10 * 100.0 / (math.pi * 0.01 * 0.01)
+    cat = np.zeros(1, [("direction", "<f4", (4,)), ("radiance", "<f4", (4,))])
+    cat["direction"][0] = [0.0, 0.0, -1.0, math.sin(0.01) ** 2]
+    cat["radiance"][0] = [radiance, radiance, radiance, 0.0]
+    cells = sky_cells_for([(np.array([0.0, 0.0, -1.0]), 0.01)])
+    t = make((1, 1), opts(samples_per_frame=64, max_bounces=1), catalogue=cat, sky_cells=cells)
+    f = air_frame(a)
+    mie_h = 1200.0 / au
+    t.set_air_model(16, 8, mie_h)
+    att = sample(t, f, 1)[0]
+    h0 = 0.002 / scenes.AU_KM
+    dt = (a["thickness"] - h0) / 16.0
+
+    def midpoint_column(h):
+        x = dt / h
+        return h * (math.exp(-h0 / h) - math.exp(-a["thickness"] / h)) * x / (2.0 * math.sinh(0.5 * x))
+
+    col_r, col_m = midpoint_column(a["scale_height"]), midpoint_column(mie_h)
+    for ch in range(3):
+        expected = math.exp(-(a["rayleigh"][ch] * col_r + a["mie"] * col_m)) * radiance
+        assert abs(att[ch] - expected) < 0.001 * expected, (ch, att[ch], expected)
+    # Equal heights: the split model is the one-height model.
+    t.set_air_model(16, 8, float(np.float32(a["scale_height"])))
+    same = sample(t, f, 1)[0]
+    t.set_air_model(16, 8, 0.0)
+    one = sample(t, f, 1)[0]
+    assert np.allclose(same[:3], one[:3], rtol=1e-5, atol=0.0), (same, one)
+    t.set_air_model()
+
+
 def sky_cells_for(stars):
     """``ray_catalogue`` cell lists (src/stars.rs:152-205) for a few stars."""
     W, H = 256, 128
@@ -333,4 +372,5 @@ ALL = [
     rings_transport,
     rings_null_mis,
     atmosphere_zenith_and_eclipse,
+    atmosphere_split_heights_zenith,
 ]

@@ -2,6 +2,7 @@
 // (termination and answers of the stackless traversals). Test code only:
 // the product library is stargaze_amd/libstargaze_amd.so.
 #include <cstring>
+#include <vector>
 
 #include "../../stargaze_amd/csrc/sg_integrator.h"
 
@@ -76,6 +77,8 @@ int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *r
     sc.ring_profile = ring_profile;
     sc.ring_profile_body = ring_profile ? ring_body : SG_MISS;
     sc.density_profile = density_profile;
+    sc.air_view_steps = ATMO_VIEW_STEPS;
+    sc.air_sun_steps = ATMO_SUN_STEPS;
     float density[GRAD_PRIVATE] = {};
     const uint32_t width = (uint32_t)settings->globals.viewport[0];
     const uint32_t height = (uint32_t)settings->globals.viewport[1];
@@ -101,6 +104,54 @@ int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *r
             sum[2] += r.value.z;
         }
         for (int c = 0; c < 3; ++c) mean[3u * pixel + c] = sum[c] / count;
+    }
+    return 0;
+}
+}
+
+extern "C" {
+// One pass of the product's fused integrator over a frame with a landscape
+// (terrain with the product's cache, prop BVH or flat loop, sky cache filled
+// here by the product's texel function): per-pixel mean rgb into `mean`.
+// `air` = {view steps, light steps, Mie scale height bits}.
+int hc_render(const sg_settings *settings, const sg_sphere *bodies, const sg_primitive *objects, const float *terrain,
+              const float *bvh_nodes, const uint32_t *bvh_prims, const uint32_t *air, float *mean) {
+    Scene sc;
+    std::memset(&sc, 0, sizeof(sc));
+    sc.s = *settings;
+    sc.bodies = reinterpret_cast<const Body *>(bodies);
+    sc.objects = objects;
+    sc.terrain = terrain;
+    sc.bvh = reinterpret_cast<const float4 *>(bvh_nodes);
+    sc.bvh_prims = bvh_prims;
+    sc.t = terrain_layout(settings->globals.ground_counts[2]);
+    sc.air_view_steps = air[0];
+    sc.air_sun_steps = air[1];
+    std::memcpy(&sc.mie_height, &air[2], 4);
+    std::vector<float4> sky(SG_SKY_TEXELS);
+    sc.sky_light = sky.data();
+    for (uint32_t y = 0; y < 8u; ++y)
+        for (uint32_t x = 0; x < 16u; ++x) {
+            f3 t = sky_cache_texel(sc, x, y);
+            sky[y * 16u + x] = make_float4(t.x, t.y, t.z, 0.0f);
+        }
+    const uint32_t width = (uint32_t)settings->globals.viewport[0];
+    const uint32_t height = (uint32_t)settings->globals.viewport[1];
+    const uint32_t count = settings->counts[2];
+    const bool has_air = settings->globals.atmo_params[3] >= 0.5f;
+    for (uint32_t pixel = 0; pixel < width * height; ++pixel) {
+        const uint32_t x = pixel % width, y = pixel / width;
+        float sum[3] = {0.0f, 0.0f, 0.0f};
+        for (uint32_t s = 0; s < count; ++s) {
+            const uint32_t accumulated = has_air ? settings->counts[1] + s : settings->counts[1];
+            uint32_t rng = path_seed(pixel, settings->sampling[0] + s);
+            f3 direction = camera_direction(sc, x, y, rng);
+            PathResult r = trace_path(sc, direction, rng, accumulated);
+            sum[0] += r.value.x;
+            sum[1] += r.value.y;
+            sum[2] += r.value.z;
+        }
+        for (int c = 0; c < 3; ++c) mean[3u * pixel + c] = sum[c] / (float)count;
     }
     return 0;
 }

@@ -93,6 +93,68 @@ def exhaustive(heights, cells, radius, rays):
     return np.array(out)
 
 
+def exhaustive_culled(heights, cells, radius, rays, reach=60_000.0):
+    """FP64 first hit over every triangle a ray can meet, for grids too large
+    to scan whole: a cell is kept when its xz rectangle lies within its
+    half-diagonal of the ray's xz segment (0 to ``reach`` m), a superset of
+    the cells the ray crosses, so the minimum is the exhaustive one."""
+    s = float(tr.scale(radius))
+    coord = tr.coordinates(cells).astype(np.float64) * s
+    r = float(radius)
+    h = heights.astype(np.float64) * s
+    x0, x1 = coord[:-1], coord[1:]
+    cx, hx = 0.5 * (x0 + x1), 0.5 * (x1 - x0)
+    out = []
+    for ray in rays:
+        o = ray["offset"].astype(np.float64)
+        d = ray["direction"].astype(np.float64)
+        dxz = np.array([d[0], d[2]])
+        length = np.hypot(*dxz)
+        best = np.inf
+        if length < 1e-12:
+            # Vertical ray: only the cell below the origin.
+            ix = np.clip(np.searchsorted(coord, o[0]) - 1, 0, cells - 1)
+            iz = np.clip(np.searchsorted(coord, o[2]) - 1, 0, cells - 1)
+            cand = [(iz, ix)]
+        else:
+            u = dxz / length
+            span = reach * length
+            # Distance from each cell centre to the segment, separably in x and z.
+            px = cx[None, :] - o[0]
+            pz = cx[:, None] - o[2]
+            along = np.clip(px * u[0] + pz * u[1], 0.0, span)
+            ex = px - along * u[0]
+            ez = pz - along * u[1]
+            rad = np.hypot(hx[None, :], hx[:, None])
+            iz, ix = np.nonzero(ex * ex + ez * ez <= rad * rad * 1.0001)
+            cand = list(zip(iz, ix))
+        iz = np.array([c[0] for c in cand])
+        ix = np.array([c[1] for c in cand])
+
+        def vert(zz, xx):
+            X, Z = coord[xx], coord[zz]
+            curv = -(X * X + Z * Z) / (r + np.sqrt(r * r - X * X - Z * Z))
+            return np.stack([X, h[zz, xx] + curv, Z], -1)
+
+        a, b, c, e = vert(iz, ix), vert(iz, ix + 1), vert(iz + 1, ix), vert(iz + 1, ix + 1)
+        for (p0, p1, p2) in ((a, c, b), (e, b, c)):
+            ab, ac = p1 - p0, p2 - p0
+            p = np.cross(d, ac)
+            det = np.einsum("ij,ij->i", ab, p)
+            ok = np.abs(det) >= 1e-15
+            inv = np.where(ok, 1.0 / np.where(ok, det, 1.0), 0.0)
+            rel = o - p0
+            uu = np.einsum("ij,ij->i", rel, p) * inv
+            q = np.cross(rel, ab)
+            v = (q @ d) * inv
+            t = np.einsum("ij,ij->i", ac, q) * inv
+            hit = ok & (uu >= -1e-9) & (v >= -1e-9) & (uu + v <= 1 + 1e-9) & (t > 0.001)
+            if hit.any():
+                best = min(best, float(t[hit].min()))
+        out.append(best)
+    return np.array(out)
+
+
 def check(result, expected):
     for i, (res, exp) in enumerate(zip(result, expected)):
         actual = float(res["distance"])

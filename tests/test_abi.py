@@ -13,7 +13,7 @@ HEADER = ROOT / "include" / "stargaze_amd.h"
 
 def declared_functions():
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|uint32_t|const char \*)\s*(sg_\w+)\s*\(", text, re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|uint32_t|uint64_t|const char \*)\s*(sg_\w+)\s*\(", text, re.M)))
 
 
 def test_header_declares_what_the_binding_binds():

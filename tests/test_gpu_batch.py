@@ -10,9 +10,11 @@ from stargaze_amd.pathtracer import INTEGRATOR_FUSED, Batch, Options, PathTracer
 pytestmark = pytest.mark.gpu
 
 
-def small_views(count=8, seed=11):
+def small_views(count=8, seed=11, halo_every=0):
+    """Views without a landscape: batched passes over landscape views are
+    switched off (test_batch_with_landscape_views)."""
     return scenes.batch_views(count, seed=seed, min_pixels=16 * 16, max_pixels=96 * 64, spp=(1, 3, 4, 16),
-                              halo_every=4)
+                              halo_every=halo_every)
 
 
 def tracers(views, limit=None):
@@ -138,4 +140,31 @@ def test_batch_rejects_misuse():
         batch.set_launch(block=96)
     with pytest.raises(ValueError):
         batch.encode([views[0]["frame"]])
+    close(batch, mine)
+
+
+@pytest.mark.skip(reason="batched passes of landscape views are switched off: their one GPU run after the "
+                         "air-model change ended in a device memory fault that host sanitizer runs of the same "
+                         "views did not reproduce; the case stays off the GPU until its cause is shown")
+def test_batch_with_landscape_views():
+    views = small_views(8, seed=11, halo_every=4)
+    frames = [v["frame"] for v in views]
+    mine, ref = tracers(views), tracers(views)
+    batch = Batch(mine)
+    for _ in range(3):
+        batch.encode(frames)
+        for t, f in zip(ref, frames):
+            t.encode(f)
+    compare(views, mine, ref, "landscape views")
+    close(batch, mine, ref)
+
+
+def test_batch_rejects_landscape_views():
+    """The switched-off path fails loudly instead of rendering."""
+    views = small_views(4, seed=11, halo_every=4)
+    frames = [v["frame"] for v in views]
+    mine = tracers(views)
+    batch = Batch(mine)
+    with pytest.raises(Exception, match="landscape"):
+        batch.encode(frames)
     close(batch, mine)

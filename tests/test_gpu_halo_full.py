@@ -1,0 +1,156 @@
+"""The Halo 4K workload's own scene under GPU parity.
+
+Full size: the 4096-cell ridged-fBm terrain (compact vertex cache, the
+40-entry traversal stack), 20 000 rock boxes and 50 000 conifer ellipsoids
+in the skip-count record format (traced through the prop BVH on the GPU and
+the flat loop in the oracle), and the air at 16 view cells x 8 light-path
+samples with a 1.2 km Mie scale height.
+"""
+import numpy as np
+import pytest
+
+import oracle
+import parity
+import terrain_cases as tc
+from stargaze_amd import pathtracer as ptm
+from stargaze_amd import scenes
+from stargaze_amd.layouts import MISS, PROBE_RAY
+from stargaze_amd.pathtracer import Options, PathTracer
+
+pytestmark = pytest.mark.gpu
+
+LOCAL_BASE, TERRAIN_ID = 0x80000000, 0x80100000
+
+
+@pytest.fixture(scope="module")
+def halo():
+    print("building the full Halo scene (4096 cells, 70k primitives)", flush=True)
+    return scenes.halo_observatory(96, 54)
+
+
+def _camera_rays(frame, n, seed):
+    """Rays from the camera spread over the lower hemisphere (most below
+    the horizon, some grazing it), plus a tenth pointing up."""
+    rng = np.random.default_rng(seed)
+    g = frame.globals
+    east, up, north = (g[k][:3].astype(np.float64) for k in ("ground_east", "ground_up", "ground_north"))
+    local = rng.normal(size=(n, 3))
+    down = rng.uniform(size=n) < 0.9
+    local[:, 1] = np.where(down, -np.abs(local[:, 1]) * rng.choice([0.02, 0.1, 0.5], n) - 0.005,
+                           np.abs(local[:, 1]))
+    world = local[:, :1] * east + local[:, 1:2] * up + local[:, 2:3] * north
+    world /= np.linalg.norm(world, axis=1, keepdims=True)
+    rays = np.zeros(n, PROBE_RAY)
+    rays["anchor"] = MISS
+    rays["direction"] = world
+    return rays
+
+
+def test_full_landscape_probe_matches_fp64_flat_loop(halo):
+    """2048 landscape queries from the camera: the GPU (FP32, BVH over 70k
+    primitives, 4096-cell quadtree) against the FP64 oracle's flat loop."""
+    frame, data, cells = halo
+    rays = _camera_rays(frame, 2048, 21)
+    gpu = PathTracer((4, 4), options=Options(milky_way=0.0))
+    gpu.upload_terrain(data, cells)
+    g = gpu.probe(frame, ptm.PROBE_LANDSCAPE, rays)
+    gpu.close()
+    orc = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64")
+    orc.upload_terrain(data, cells)
+    r = orc.probe(frame, ptm.PROBE_LANDSCAPE, rays)
+    gi, ri = g["index"], r["index"].astype(np.uint32)
+    same = gi == ri
+    # Where the levelled terrain lies on the host sphere, which of the two
+    # is first is a tie the FP32 and FP64 quadratics break differently; the
+    # distances must still agree.
+    ground = np.isin(gi, [LOCAL_BASE, TERRAIN_ID]) & np.isin(ri, [LOCAL_BASE, TERRAIN_ID])
+    tie = ~same & ground
+    tie_rel = np.abs(g["distance"][tie] - r["distance"][tie]) / r["distance"][tie]
+    hit = same & (ri != MISS)
+    rel = np.abs(g["distance"] - r["distance"]) / np.where(hit, r["distance"], 1.0)
+    kinds = {"terrain": ri == TERRAIN_ID, "host sphere": ri == LOCAL_BASE,
+             "props": (ri > LOCAL_BASE) & (ri < TERRAIN_ID)}
+    stats = {}
+    for name, sel in kinds.items():
+        s = sel & hit
+        stats[name] = (np.median(rel[s]), rel[s].max())
+        print(f"{name}: {s.sum()} hits, same index {same[sel].mean():.4f}, median rel {stats[name][0]:.2e}, "
+              f"max rel {stats[name][1]:.2e}")
+    print(f"all: {hit.sum()} hits of {len(rays)}, same index {same.mean():.4f}, host/terrain ties {tie.sum()} "
+          f"(max rel {tie_rel.max() if tie.any() else 0.0:.2e}), misses {int((ri == MISS).sum())}")
+    assert kinds["props"].sum() > 50 and kinds["terrain"].sum() > 200
+    assert np.all(same | tie) and np.all(tie_rel < 1e-5)
+    # Per kind, against FP64: terrain triangles and the FP64 host sphere
+    # keep 1e-6; prop roots (box slabs, ellipsoid quadratics in the prop's
+    # FP32 local frame) lose up to ~1e-5 on near-tangent rays, where the
+    # root's sensitivity to the rounded discriminant grows as 1/sqrt(disc).
+    assert stats["terrain"][0] < 1e-7 and stats["terrain"][1] < 1e-6
+    assert stats["host sphere"][0] < 1e-7 and stats["host sphere"][1] < 5e-6
+    assert stats["props"][0] < 2e-6 and stats["props"][1] < 5e-5
+
+
+def test_full_terrain_probe_matches_exhaustive_triangles(halo):
+    """The reference's terrain rays (src/ground_tests.rs:763-774) and four
+    more, at 4096 cells and the Halo radius, against every triangle they can
+    meet in FP64."""
+    frame, data, cells = halo
+    radius = float(frame.globals["ground_up"][3])
+    heights = data[: (cells + 1) ** 2].reshape(cells + 1, cells + 1)
+    rays = tc.probe_rays(radius, extra=4)
+    t = PathTracer((4, 4), options=Options(milky_way=0.0))
+    t.upload_terrain(data, cells)
+    res = t.probe(tc.probe_frame(cells, radius), ptm.PROBE_TERRAIN, rays)
+    t.close()
+    expected = tc.exhaustive_culled(heights, cells, radius, rays)
+    hit = np.isfinite(expected)
+    rel = np.abs(res["distance"][hit] - expected[hit]) / expected[hit]
+    print(f"terrain 4096: {hit.sum()} hits of {len(rays)}, max rel {rel.max():.2e}")
+    tc.check(res, expected)
+
+
+def test_full_scene_at_16x8_matches_oracle_per_tile(halo):
+    """96x54 of the Halo camera over the full scene with the 4K workload's
+    air model, 2 passes of 16 spp, per 16x16 tile within 3 sigma."""
+    frame, data, cells = halo
+    size = (96, 54)
+    opts = dict(samples_per_frame=16, max_bounces=4, milky_way=0.0)
+    gpu = PathTracer(size, options=Options(**opts))
+    gpu.upload_terrain(data, cells)
+    gpu.set_air_model(**scenes.HALO_AIR)
+    gpu.enable_moments(True)
+    orc = oracle.OracleTracer(size, Options(**opts), "f32")
+    orc.upload_terrain(data, cells)
+    orc.set_air_model(**scenes.HALO_AIR)
+    orc.enable_moments(True)
+    for _ in range(2):
+        gpu.encode(frame)
+        orc.encode(frame)
+    ga, gm = gpu.read_accumulation(), gpu.read_moments()
+    oa, om = orc.read_accumulation(), orc.read_moments()
+    gpu.close()
+    assert np.all(np.isfinite(ga)) and np.all(ga[..., :3] >= 0)
+    agree = parity.pixel_agreement(ga, oa)
+    z = parity.tile_z(ga, gm, oa, om)
+    print(f"halo full 16x8: pixels agreeing {agree:.4f}, tiles over 3 sigma {(z > 3).mean():.4f}, "
+          f"worst z {z.max():.2f}, mean {ga[..., :3].mean():.5g} vs {oa[..., :3].mean():.5g}", flush=True)
+    parity.assert_tiles(z)
+    assert agree >= 0.99, agree
+
+
+def test_air_model_changes_the_estimate_and_restarts(halo):
+    """The 16 x 8 march with a 1.2 km Mie height is a different image from
+    the reference shader's 8 x 4 single-height march, and setting it
+    restarts the accumulation."""
+    frame, data, cells = halo
+    size = (48, 27)
+    t = PathTracer(size, options=Options(samples_per_frame=4, max_bounces=3, milky_way=0.0))
+    t.upload_terrain(data, cells)
+    t.encode(frame)
+    base = t.read_accumulation()
+    assert t.samples() == 4
+    t.set_air_model(**scenes.HALO_AIR)
+    assert t.samples() == 0 and t.air_model["view_steps"] == 16
+    t.encode(frame)
+    c3 = t.read_accumulation()
+    t.close()
+    assert not np.allclose(base, c3, rtol=1e-3)

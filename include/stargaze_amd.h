@@ -209,6 +209,16 @@ int sg_read_exposure(sg_ctx *ctx, sg_exposure *state, uint32_t *histogram);
  * sRGB transfer function as 8-bit rgba. Synchronizes the context stream. */
 int sg_display(sg_ctx *ctx, uint32_t out_width, uint32_t out_height, float *rgba);
 int sg_display_srgb8(sg_ctx *ctx, uint32_t out_width, uint32_t out_height, uint8_t *rgba);
+/* sg_display_srgb8 without any allocation. No counterpart in the reference:
+ * for callers that cannot allocate device memory (HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc). The 8-bit target is
+ * staged in a static per-GPU buffer of at most 3840 x 2160 pixels (31.6 MiB
+ * on every GPU the library runs on, held until the process ends); larger
+ * targets return SG_E_CAPACITY. Every pixel of the target is written, so
+ * nothing carries over from the previous call. Calls that overlap on one
+ * GPU (contexts of that GPU used from several threads at once) race on the
+ * shared staging. */
+int sg_display_srgb8_noalloc(sg_ctx *ctx, uint32_t out_width, uint32_t out_height, uint8_t *rgba);
 /* Read-back of `accumulation` / `backdrop_accumulation` (vec4 f32 per pixel,
  * src/pathtracer_tests.rs:160-182). Synchronizes the context stream. */
 int sg_read_accumulation(sg_ctx *ctx, float *rgba, uint64_t pixel_count);

@@ -38,6 +38,7 @@ EXPORTS = (
     "sg_read_exposure",
     "sg_display",
     "sg_display_srgb8",
+    "sg_display_srgb8_noalloc",
     "sg_synchronize",
     "sg_set_profiling",
     "sg_kernel_time",
@@ -160,6 +161,7 @@ def load() -> ctypes.CDLL:
         "sg_read_exposure": ([P, P, P], I32),
         "sg_display": ([P, U32, U32, P], I32),
         "sg_display_srgb8": ([P, U32, U32, P], I32),
+        "sg_display_srgb8_noalloc": ([P, U32, U32, P], I32),
         "sg_synchronize": ([P], I32),
         "sg_set_profiling": ([P, I32], I32),
         "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),

@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -166,6 +167,36 @@ struct sg_ctx {
 };
 
 namespace {
+
+// Staging of sg_display_srgb8_noalloc: one 8-bit RGBA display target of at
+// most kNoAllocPixels pixels, resident on every GPU the library runs on
+// (31.6 MiB each) until the process ends. Host code never uses the array's
+// name as an address: each device's copy is at the address hipGetSymbolAddress
+// returns with that device current (noalloc_scratch).
+constexpr uint64_t kNoAllocPixels = 3840ull * 2160ull;
+static __device__ __align__(256) uchar4 scratch[kNoAllocPixels];
+std::mutex g_scratch_mutex;
+std::vector<void *> g_scratch;  // device address of `scratch`, per device ordinal
+
+// The address of `scratch` on `device`. With `fetch` (set-up, `device`
+// current) the first call on a device asks the runtime and keeps it; later
+// calls, and every call without `fetch`, return the kept address.
+int noalloc_scratch(int device, bool fetch, void **ptr) {
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (device < 0) return SG_E_INVALID;
+    if ((size_t)device >= g_scratch.size()) {
+        if (!fetch) return SG_E_STATE;
+        g_scratch.resize((size_t)device + 1, nullptr);
+    }
+    if (!g_scratch[device]) {
+        if (!fetch) return SG_E_STATE;
+        void *p = nullptr;
+        if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return SG_E_DEVICE;
+        g_scratch[device] = p;
+    }
+    *ptr = g_scratch[device];
+    return SG_OK;
+}
 
 int fail(sg_ctx *ctx, int code, const std::string &message) {
     if (ctx) ctx->error = message;
@@ -643,6 +674,11 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
         return code;
     };
     if (hipGetDevice(&ctx->device) != hipSuccess) return bail(SG_E_DEVICE);
+    {
+        // Set-up of the allocation-free display staging, once per device.
+        void *staging = nullptr;
+        if (noalloc_scratch(ctx->device, true, &staging) != SG_OK) return bail(SG_E_DEVICE);
+    }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(SG_E_DEVICE);
     if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_ready, hipEventDisableTiming) != hipSuccess)
@@ -896,13 +932,16 @@ int sg_read_exposure(sg_ctx *ctx, sg_exposure *state, uint32_t *histogram) {
     return SG_OK;
 }
 
-static int display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *linear, uint8_t *srgb) {
+// `staging` (8-bit targets only) is device memory of at least out_w * out_h
+// texels the caller provides; null allocates it on the stream.
+static int display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *linear, uint8_t *srgb,
+                   uchar4 *staging = nullptr) {
     if (!ctx || out_w == 0 || out_h == 0 || (!linear && !srgb)) return SG_E_INVALID;
     if ((uint64_t)out_w * out_h > (1ull << 31)) return fail(ctx, SG_E_INVALID, "display target too large");
     join_batch(ctx);
     const size_t n = (size_t)out_w * out_h;
-    void *out = nullptr;
-    SG_HIP(ctx, hipMallocAsync(&out, n * (linear ? sizeof(float4) : sizeof(uchar4)), ctx->stream));
+    void *out = staging;
+    if (!staging) SG_HIP(ctx, hipMallocAsync(&out, n * (linear ? sizeof(float4) : sizeof(uchar4)), ctx->stream));
     DisplayArgs da;
     da.accumulation = ctx->accumulation;
     da.backdrop = ctx->backdrop;
@@ -920,13 +959,25 @@ static int display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *linear, u
     if (e == hipSuccess)
         e = hipMemcpyAsync(linear ? static_cast<void *>(linear) : static_cast<void *>(srgb), out,
                            n * (linear ? sizeof(float4) : sizeof(uchar4)), hipMemcpyDeviceToHost, ctx->stream);
-    hipError_t f = hipFreeAsync(out, ctx->stream);
-    if (e == hipSuccess) e = f;
+    if (!staging) {
+        hipError_t f = hipFreeAsync(out, ctx->stream);
+        if (e == hipSuccess) e = f;
+    }
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) return fail(ctx, SG_E_DEVICE, std::string("display: ") + hipGetErrorString(e));
     return SG_OK;
 }
 int sg_display(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, float *rgba) { return display(ctx, out_w, out_h, rgba, nullptr); }
+// sg_display_srgb8 through the per-device static staging: no allocation.
+int sg_display_srgb8_noalloc(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, uint8_t *rgba) {
+    if (!ctx || !rgba || out_w == 0 || out_h == 0) return SG_E_INVALID;
+    if ((uint64_t)out_w * out_h > kNoAllocPixels)
+        return fail(ctx, SG_E_CAPACITY, "the allocation-free display target holds at most 3840 x 2160 pixels");
+    void *staging = nullptr;
+    if (noalloc_scratch(ctx->device, false, &staging) != SG_OK)
+        return fail(ctx, SG_E_DEVICE, "device address of the display staging is unavailable");
+    return display(ctx, out_w, out_h, nullptr, rgba, static_cast<uchar4 *>(staging));
+}
 int sg_display_srgb8(sg_ctx *ctx, uint32_t out_w, uint32_t out_h, uint8_t *rgba) {
     return display(ctx, out_w, out_h, nullptr, rgba);
 }

@@ -198,6 +198,17 @@ class PathTracer:
         self._check(self._lib.sg_display_srgb8(self._ctx, int(width), int(height), out.ctypes.data), "display_srgb8")
         return out
 
+    def display_srgb8_noalloc(self, width: int, height: int, out: np.ndarray | None = None) -> np.ndarray:
+        """``display_srgb8`` through the library's static per-GPU staging
+        (no device allocation; at most 3840 x 2160 pixels)."""
+        if out is None:
+            out = np.empty((height, width, 4), np.uint8)
+        if out.shape != (height, width, 4) or out.dtype != np.uint8 or not out.flags.c_contiguous:
+            raise ValueError("out must be a C-contiguous (height, width, 4) uint8 array")
+        self._check(self._lib.sg_display_srgb8_noalloc(self._ctx, int(width), int(height), out.ctypes.data),
+                    "display_srgb8_noalloc")
+        return out
+
     def synchronize(self) -> None:
         self._check(self._lib.sg_synchronize(self._ctx), "synchronize")
 

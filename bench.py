@@ -54,18 +54,17 @@ FLOP_COST = {
     "vertex": 70,         # surface shading: sampling frame, throughput
     "rng": 0,             # integer work
 }
-# Bytes each event must read from memory in the work model's memory term:
-# a terrain node's cached box (6 floats), a triangle's three cached vertices
-# (3 x 4 B in the compact form of grids above 1024 cells, 3 x 12 B below), a
-# primitive record (48 B). Bodies, lights and tables stay in caches.
-BYTE_COST = {"terrain_node": 24, "triangle_compact": 12, "triangle": 36, "box": 48, "rock": 48, "vegetation": 48}
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, dense
 PEAK_HBM_GBS = 8000.0
-# HBM bytes of one pass, from rocprofv3 FETCH_SIZE / WRITE_SIZE runs scaled by
-# a 1 GiB copy's counts; not measured by this script (the counters need a
-# profiler run of their own).
-TRAFFIC_BYTES_PER_PASS = {"halo_4k": None}
-TRAFFIC_SOURCE = "profiles/halo_4k_kernels.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)"
+# HBM bytes per launch of the camera-air kernel on the Halo 4K pass (2^24
+# paths), from rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (the factors of a 1 GiB
+# copy); not measured by this script, since the counters need a profiler
+# run of their own.
+TRAFFIC_CAMERA_AIR_PER_LAUNCH = {"halo_4k": (28.80e9 * 2 + 56.11e9) / 16}
+TRAFFIC_SOURCE = "profiles/halo_4k_kernels.md (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per k_camera_air launch)"
+# Path state the camera-air step must read and write per path: ray origin
+# (FP64) and direction, the hit record, throughput, radiance and the RNG.
+CAMERA_AIR_STATE_BYTES = 32 + 16 + 48 + 2 * 16 + 2 * 16 + 2 * 4
 
 def _halo(width, height):
     frame, data, cells = scenes.halo_observatory(width, height)
@@ -107,29 +106,84 @@ def oracle_tracer(width, height, spp, bounces, terrain, profile=None, air=None):
     return t
 
 
+PROP_EVENTS = ("box", "rock", "vegetation")
+
+
 def work_model(frame, terrain, profile, air, width, height, spp, bounces, rows=16):
     """Event counts per path sample of a steady-state pass (not the first,
-    which takes extra light strata near limbs) from the oracle on bands of
-    rows at the image's quarter, half and three-quarter heights."""
+    which takes extra light strata near limbs) from the oracle on four bands
+    of rows (1/8, 3/8, 5/8, 7/8 of the height), for the whole pass and for
+    its camera-air step. FLOP are the counts times FLOP_COST, without the
+    primitive tests of the oracle's flat skip-count loop (the GPU traverses a
+    BVH over the primitives instead, so those counts are not its work; they
+    are reported apart)."""
     import oracle
 
     t = oracle_tracer(width, height, spp, bounces, terrain, profile, air)
     t.encode(frame, rows=(0, 1))  # first pass: fills the sky and terrain caches
-    ev = None
+    rest, cam = None, None
     bands = [height * (2 * k + 1) // 8 for k in range(4)]
     per_band = max(1, rows // len(bands))
     for y0 in bands:
         t.history_samples = spp
-        e = oracle.count_events(t, frame, (y0, y0 + per_band))
-        ev = e if ev is None else {k: ev[k] + e[k] for k in ev}
-    rows = per_band * len(bands)
-    n = width * rows * spp
-    per = {k: v / n for k, v in ev.items()}
-    flops = sum(per[k] * FLOP_COST[k] for k in FLOP_COST)
-    compact = terrain is not None and terrain[1] > 1024
-    byte_cost = dict(BYTE_COST, triangle=BYTE_COST["triangle_compact"] if compact else BYTE_COST["triangle"])
-    gather = sum(per[k] * byte_cost[k] for k in ("terrain_node", "triangle", "box", "rock", "vegetation"))
-    return per, flops, gather
+        a, b = oracle.count_events(t, frame, (y0, y0 + per_band), stages=True)
+        rest = a if rest is None else {k: rest[k] + a[k] for k in rest}
+        cam = b if cam is None else {k: cam[k] + b[k] for k in cam}
+    n = width * per_band * len(bands) * spp
+
+    def price(ev):
+        per = {k: v / n for k, v in ev.items()}
+        flops = sum(per[k] * FLOP_COST[k] for k in FLOP_COST if k not in PROP_EVENTS)
+        props = sum(per[k] * FLOP_COST[k] for k in PROP_EVENTS)
+        return per, flops, props
+
+    whole = {k: rest[k] + cam[k] for k in rest}
+    per, flops, props = price(whole)
+    _, flops_air, props_air = price(cam)
+    return {"per_event": per, "flop": flops, "flop_flat_loop_props": props, "flop_camera_air": flops_air,
+            "flop_camera_air_flat_loop_props": props_air}
+
+
+def roofline(workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, air_ms, air_launches, launch):
+    """FP32 vector roofline of the dominant kernel (nothing on the path is a
+    dense contraction, so no MFMA instruction is issued): the camera-air
+    step of the wavefront integrator when the frame has air, else the whole
+    pass. `achieved` is the step's algorithmic FLOP (oracle event counts of
+    that step per path sample x the paths of one launch) over its average
+    launch time from HIP events on the tracer's stream."""
+    whole = {"kernel": "all kernels of one pass", "ms": round(kernel_s * 1e3, 3),
+             "flop_per_path_sample": round(wm["flop"], 1), "achieved_tflops": round(pass_tflops, 3),
+             "frac": round(pass_tflops / PEAK_FP32_TFLOPS, 4),
+             "flat_loop_prop_flop_per_path_sample_not_counted": round(wm["flop_flat_loop_props"], 1),
+             "compulsory_bytes": compulsory}
+    if air_launches == 0:
+        return {"bound": "valu", "pipe": "FP32 VALU, no MFMA instructions", "achieved": round(pass_tflops, 3),
+                "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s", "frac": round(pass_tflops / PEAK_FP32_TFLOPS, 4),
+                "traffic": None, "kernel": whole["kernel"], "pass": whole}
+    per_launch_ms = air_ms / air_launches
+    paths_per_launch = min(samples_per_step, launch.get("chunk_paths", samples_per_step))
+    launches_per_step = math.ceil(samples_per_step / paths_per_launch)
+    step_ms = air_ms / air_launches * launches_per_step
+    achieved = wm["flop_camera_air"] * samples_per_step / (step_ms / 1e3) / 1e12
+    return {
+        "bound": "valu",
+        "pipe": "FP32 VALU, no MFMA instructions",
+        "achieved": round(achieved, 3),
+        "peak": PEAK_FP32_TFLOPS,
+        "unit": "TFLOP/s",
+        "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+        "traffic": TRAFFIC_CAMERA_AIR_PER_LAUNCH.get(workload),
+        "traffic_source": TRAFFIC_SOURCE if workload in TRAFFIC_CAMERA_AIR_PER_LAUNCH else None,
+        "kernel": "k_camera_air (camera-air step of the wavefront integrator)",
+        "kernel_ms_per_launch": round(per_launch_ms, 3),
+        "paths_per_launch": paths_per_launch,
+        "flop_per_path_sample": round(wm["flop_camera_air"], 1),
+        "flat_loop_prop_flop_per_path_sample_not_counted": round(wm["flop_camera_air_flat_loop_props"], 1),
+        "algorithmic_bytes_per_launch": paths_per_launch * CAMERA_AIR_STATE_BYTES,
+        "hbm_frac": round(paths_per_launch * CAMERA_AIR_STATE_BYTES / (per_launch_ms / 1e3) / 1e9 / PEAK_HBM_GBS,
+                          5),
+        "pass": whole,
+    }
 
 
 def cpu_model() -> str:
@@ -183,11 +237,13 @@ def rmse_leg(ms_per_step: float, spp_per_step: int, pixels_4k: int, cpu_rate: fl
 
     The reference is a 16384-spp float64-accumulated render of the same
     scene and air model (tests/golden/make_halo_reference.py); the target is
-    the FP32 oracle's linear-RGB RMSE at 32 spp. The GPU renders the same
-    view progressively (noise seed 0, then seed 7 for an independent draw)
-    and its RMSE is read at 8, 16, 32 and 64 spp. Monte Carlo RMSE falls as
-    spp^-1/2, so the GPU reaches the target at 32 (rmse_32 / target)^2 spp;
-    at 4K that many spp take that fraction of a 32-spp step."""
+    the FP32 oracle's expected linear-RGB RMSE at 32 spp (root mean square
+    over four seeds). The GPU renders the same view progressively with eight
+    other seeds in passes of 8 spp; its RMSE at each spp is the root mean
+    square over the seeds, and the spp where that curve crosses the target
+    (log-log interpolation) is the GPU's spp to the target. At 4K those spp
+    take that fraction of a 32-spp step; the CPU oracle reaches the target
+    at 32 spp by construction."""
     if not RMSE_REFERENCE.exists() or not RMSE_META.exists():
         return None
     sys.path.insert(0, str(ROOT / "tests"))
@@ -199,38 +255,48 @@ def rmse_leg(ms_per_step: float, spp_per_step: int, pixels_4k: int, cpu_rate: fl
     ref = np.load(RMSE_REFERENCE)["image"]
     w, h = meta["width"], meta["height"]
     frame, data, cells = scenes.halo_observatory(w, h)
-    curve = {}
-    for seed in (0, 7):
+    stops = (8, 16, 24, 32, 40, 48, 64)
+    seeds = tuple(range(100, 108))
+    mse = np.zeros(len(stops))
+    for seed in seeds:
         t = PathTracer((w, h), options=Options(noise_seed=seed, samples_per_frame=8, max_bounces=4, milky_way=0.0))
         t.upload_terrain(data, cells)
         t.set_air_model(**scenes.HALO_AIR)
         spp = 0
-        out = {}
-        for stop in (8, 16, 32, 64):
+        for k, stop in enumerate(stops):
             while spp < stop:
                 t.encode(frame)
                 spp += 8
-            out[stop] = parity.rmse(t.read_accumulation(), ref)
+            mse[k] += parity.rmse(t.read_accumulation(), ref) ** 2 / len(seeds)
         t.close()
-        curve[seed] = out
+    curve = np.sqrt(mse)
     target = float(meta["target"]["rmse"])
-    rmse32 = curve[0][32]
-    spp_to_target = 32.0 * (rmse32 / target) ** 2
-    gpu_s = spp_to_target / spp_per_step * ms_per_step / 1e3
+    below = np.nonzero(curve <= target)[0]
+    if len(below) == 0:
+        spp_to_target = None
+    elif below[0] == 0:
+        spp_to_target = float(stops[0])
+    else:
+        k = int(below[0])
+        x0, x1 = math.log(stops[k - 1]), math.log(stops[k])
+        y0, y1 = math.log(curve[k - 1]), math.log(curve[k])
+        spp_to_target = math.exp(x0 + (math.log(target) - y0) * (x1 - x0) / (y1 - y0))
     result = {
         "reference": f"{w}x{h} of the halo_4k camera, {meta['spp']} spp, float64 accumulation "
                      "(tests/golden/halo_rmse_reference.npz)",
         "target_rmse": target,
-        "target_source": "FP32 oracle at 32 spp, noise seed 0, linear RGB (W excluded)",
-        "gpu_rmse": {str(k): round(v, 7) for k, v in curve[0].items()},
-        "gpu_rmse_seed7": {str(k): round(v, 7) for k, v in curve[7].items()},
-        "gpu_spp_to_target": round(spp_to_target, 2),
-        "gpu_seconds_to_target_4k": round(gpu_s, 4),
+        "target_source": "FP32 oracle at 32 spp, linear RGB (W excluded), root mean square over seeds 0-3",
+        "gpu_rmse_by_spp": {str(k): round(float(v), 5) for k, v in zip(stops, curve)},
+        "gpu_seeds": f"{seeds[0]}-{seeds[-1]}",
+        "gpu_spp_to_target": None if spp_to_target is None else round(spp_to_target, 2),
     }
-    if cpu_rate:
-        cpu_s = pixels_4k * 32 / (cpu_rate * 1e6)
-        result["cpu_seconds_to_target_4k"] = round(cpu_s, 1)
-        result["speedup_at_equal_rmse"] = round(cpu_s / gpu_s, 1)
+    if spp_to_target is not None:
+        gpu_s = spp_to_target / spp_per_step * ms_per_step / 1e3
+        result["gpu_seconds_to_target_4k"] = round(gpu_s, 4)
+        if cpu_rate:
+            cpu_s = pixels_4k * 32 / (cpu_rate * 1e6)
+            result["cpu_seconds_to_target_4k"] = round(cpu_s, 1)
+            result["speedup_at_equal_rmse"] = round(cpu_s / gpu_s, 1)
     return result
 
 
@@ -422,7 +488,19 @@ def main() -> int:
     ap.add_argument("--no-rmse", action="store_true", help="skip the fixed-RMSE leg")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only copy 1 GiB device buffers 4 times (the reference for HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_calibration:
+        import torch
+
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(4):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_each": src.numel() * 4, "copies": 4}), flush=True)
+        return 0
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported in this build", file=sys.stderr)
         return 2
@@ -463,20 +541,22 @@ def main() -> int:
     tracer.synchronize()
     elapsed = time.perf_counter() - t0
     kernel_ms, launches = tracer.kernel_time()
+    from stargaze_amd.pathtracer import STEP_CAMERA_AIR
+
+    air_ms, air_launches = tracer.step_time(STEP_CAMERA_AIR)
     img = tracer.read_accumulation()
     assert np.all(np.isfinite(img)), "non-finite radiance"
     samples_per_step = width * height * spp
     value = samples_per_step * args.steps / elapsed / 1e6
     if args.no_work_model:
-        per_event, flops, gather = {}, float("nan"), float("nan")
+        wm = {"flop": float("nan"), "flop_camera_air": float("nan"), "flop_flat_loop_props": float("nan"),
+              "flop_camera_air_flat_loop_props": float("nan")}
     else:
         log("work model")
-        per_event, flops, gather = work_model(frame, terrain, profile, air, width, height, spp, bounces)
+        wm = work_model(frame, terrain, profile, air, width, height, spp, bounces)
     kernel_s = kernel_ms / 1e3 / max(launches, 1)
-    achieved = flops * samples_per_step / kernel_s / 1e12
+    pass_tflops = wm["flop"] * samples_per_step / kernel_s / 1e12
     compulsory = width * height * 16 * 2 * 2  # accumulation + backdrop read-modify-write
-    bytes_per_pass = gather * samples_per_step + compulsory
-    achieved_gbs = bytes_per_pass / kernel_s / 1e9
     g = frame.globals
     air_model = air or {"view_steps": 8, "light_steps": 4, "mie_scale_height": 0.0}
     au = scenes.METRES_PER_AU
@@ -507,29 +587,8 @@ def main() -> int:
         "data": "synthetic",
         "config": config,
         "launch": launch if launch is not None else tracer.launch,
-        "roofline": {
-            # FP32 vector roofline: nothing on the path is a dense
-            # contraction, so no MFMA instruction is issued. The memory term
-            # (terrain and primitive gathers plus the accumulation) is
-            # reported beside it; the larger fraction names the bound.
-            "bound": "valu" if achieved / PEAK_FP32_TFLOPS >= achieved_gbs / PEAK_HBM_GBS else "hbm",
-            "pipe": "FP32 VALU, no MFMA instructions",
-            "achieved": round(achieved, 3),
-            "peak": PEAK_FP32_TFLOPS,
-            "unit": "TFLOP/s",
-            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
-            "memory_term": {"achieved": round(achieved_gbs, 2), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                            "frac": round(achieved_gbs / PEAK_HBM_GBS, 5),
-                            "bytes_per_path_sample": round(gather + compulsory / samples_per_step, 2)},
-            "traffic": TRAFFIC_BYTES_PER_PASS.get(args.workload),
-            "traffic_source": TRAFFIC_SOURCE if args.workload in TRAFFIC_BYTES_PER_PASS else None,
-            "kernel": "integrator (all kernels of one pass)",
-            "kernel_ms": round(kernel_s * 1e3, 3),
-            "flop_per_path_sample": round(flops, 1),
-            "flop_model": "oracle event counts x FLOP cost table; prop tests counted as the oracle's flat "
-                          "skip-count loop performs them (the GPU's BVH performs fewer)",
-            "algorithmic_bytes_per_launch": round(bytes_per_pass),
-        },
+        "roofline": roofline(args.workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, air_ms,
+                             air_launches, tracer.launch),
     }
     if args.batch_steps > 0:
         line["batch"] = []

@@ -310,15 +310,20 @@ EVENTS = ("paths", "segments", "sphere", "sphere_hit", "ring", "light_sample", "
           "vertex", "rng")
 
 
-def count_events(tracer: "OracleTracer", frame, rows) -> dict:
-    """Event counts of one single-threaded pass over ``rows`` (the work model)."""
+def count_events(tracer: "OracleTracer", frame, rows, stages: bool = False):
+    """Event counts of one pass over ``rows`` (the work model): all of them,
+    or with ``stages`` a pair (everything but the camera-air step, the
+    camera-air step)."""
     lib = tracer.lib
     assert lib.orc_counter_count() == len(EVENTS)
     lib.orc_set_counting(1)
     try:
         tracer.encode(frame, rows=rows)
-        out = np.zeros(len(EVENTS), np.uint64)
+        out = np.zeros((2, len(EVENTS)), np.uint64)
         lib.orc_counters(out.ctypes.data)
     finally:
         lib.orc_set_counting(0)
-    return dict(zip(EVENTS, (int(v) for v in out)))
+    parts = [dict(zip(EVENTS, (int(v) for v in row))) for row in out]
+    if stages:
+        return parts[0], parts[1]
+    return {k: parts[0][k] + parts[1][k] for k in EVENTS}

@@ -132,20 +132,26 @@ enum {
     EV_ATMO_CELL, EV_DENSITY_STEP, EV_VNOISE, EV_BOX, EV_ROCK, EV_VEGETATION, EV_TERRAIN_NODE, EV_TRIANGLE,
     EV_VERTEX, EV_RNG, EV_COUNT
 };
-static uint64_t counters[EV_COUNT];
-static _Thread_local uint64_t local_counters[EV_COUNT];
+/* Counts are kept per stage: 0 for everything but the camera-air step, 1 for
+ * the camera-air step (the view march of integrate_camera_air and
+ * camera_air_transmittance), which the product runs as a kernel of its own. */
+static uint64_t counters[2][EV_COUNT];
+static _Thread_local uint64_t local_counters[2][EV_COUNT];
+static _Thread_local int stage = 0;
 static int counting = 0;
-#define COUNT(e) do { if (counting) local_counters[e] += 1; } while (0)
+#define COUNT(e) do { if (counting) local_counters[stage][e] += 1; } while (0)
 void orc_set_counting(int on) { counting = on; memset(counters, 0, sizeof(counters)); }
+/* 2 x EV_COUNT values: all stages but the camera air, then the camera air. */
 void orc_counters(uint64_t *out) { memcpy(out, counters, sizeof(counters)); }
 /* Folds the calling thread's counts into the totals. */
 static void flush_counters(void) {
     if (!counting) return;
 #pragma omp critical(orc_counters)
-    for (int i = 0; i < EV_COUNT; ++i) {
-        counters[i] += local_counters[i];
-        local_counters[i] = 0;
-    }
+    for (int k = 0; k < 2; ++k)
+        for (int i = 0; i < EV_COUNT; ++i) {
+            counters[k][i] += local_counters[k][i];
+            local_counters[k][i] = 0;
+        }
 }
 uint32_t orc_counter_count(void) { return EV_COUNT; }
 
@@ -1705,7 +1711,7 @@ static R density_integral(const Ctx *c, v3 origin, v3 dir, uint32_t steps, R *mi
     R distance = shell_exit(c, origin, dir);
     if (mie) *mie = 0.0;
     if (distance <= 0.0) return 0.0;
-    if (counting) local_counters[EV_DENSITY_STEP] += steps;
+    if (counting) local_counters[stage][EV_DENSITY_STEP] += steps;
     R dt = distance / (R)steps;
     R inv_h = 1.0 / scale_height(c);
     R inv_hm = two_heights(c) ? 1.0 / c->mie_height : 0.0;
@@ -2081,7 +2087,9 @@ static void trace(const Ctx *c, Ray initial, uint32_t *rng, CameraAir primary_ai
             if (hit.index != MISS) limit = hit.distance;
             if (ring.index != MISS) limit = mn(limit, ring.distance);
             if (s->atmo_params.v[3] >= 0.5f) {
+                stage = 1;
                 v3 transmittance = camera_air_transmittance(c, ray.offset, ray.direction, limit, *rng);
+                stage = 0;
                 radiance = add(radiance, mul(throughput, primary_air.radiance));
                 throughput = mul(throughput, transmittance);
                 *rng = primary_air.rng;
@@ -2268,7 +2276,9 @@ static CameraAir integrate_camera_air(const Ctx *c, uint32_t x, uint32_t y, uint
     if (c->s->background[3] > 0.0f) hit = point_disc_hit(c, ray, hit, &disc_scale, &rng);
     RingHit ring = closest_ring(c, ray, hit.distance);
     CameraAir air;
+    stage = 1;
     air.radiance = atmosphere(c, ray.offset, ray.direction, mn(hit.distance, ring.distance), &rng);
+    stage = 0;
     air.rng = rng;
     return air;
 }

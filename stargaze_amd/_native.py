@@ -42,6 +42,7 @@ EXPORTS = (
     "sg_synchronize",
     "sg_set_profiling",
     "sg_kernel_time",
+    "sg_step_time",
     "sg_set_integrator",
     "sg_set_tail",
     "sg_get_launch",
@@ -164,6 +165,7 @@ def load() -> ctypes.CDLL:
         "sg_display_srgb8_noalloc": ([P, U32, U32, P], I32),
         "sg_synchronize": ([P], I32),
         "sg_set_profiling": ([P, I32], I32),
+        "sg_step_time": ([P, U32, P, P], I32),
         "sg_kernel_time": ([P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(U64)], I32),
         "sg_set_integrator": ([P, U32, U32], I32),
         "sg_set_tail": ([P, U32], I32),

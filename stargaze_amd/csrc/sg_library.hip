@@ -122,6 +122,11 @@ struct sg_ctx {
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
     double kernel_ms = 0.0;
     uint64_t launches = 0;
+    // Per-step device time of the wavefront kernels (SG_STEP_*), from event
+    // pairs around each launch while profiling.
+    hipEvent_t ev_step[4] = {nullptr, nullptr, nullptr, nullptr};
+    double step_ms[SG_STEP_COUNT] = {};
+    uint64_t step_launches[SG_STEP_COUNT] = {};
     std::string error;
     // Device copies of the last encoded frame's bodies and primitives.
     bool frame_on_device = false;
@@ -444,19 +449,37 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
                 break;
             }
             const dim3 grid((n + block - 1) / block);
+            const bool air = iteration == 0 && ta.has_air;
+            hipEvent_t *ev = ctx->ev_step;
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
             hipLaunchKernelGGL(fk.extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
             SG_HIP(ctx, hipGetLastError());
-            if (iteration == 0 && ta.has_air) {
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
+            if (air) {
                 hipLaunchKernelGGL(fk.camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
                 SG_HIP(ctx, hipGetLastError());
             }
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
             SG_HIP(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(uint32_t), ctx->stream));
             hipLaunchKernelGGL(fk.shade, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n,
                                ctx->queues[q ^ 1], ctx->counter);
             SG_HIP(ctx, hipGetLastError());
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
             SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, sizeof(uint32_t), hipMemcpyDeviceToHost,
                                        ctx->stream));
             SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            if (ctx->profiling) {
+                float ms[3] = {0.f, 0.f, 0.f};
+                for (int k = 0; k < 3; ++k) SG_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+                ctx->step_ms[SG_STEP_EXTEND] += ms[0];
+                ctx->step_launches[SG_STEP_EXTEND] += 1;
+                if (air) {
+                    ctx->step_ms[SG_STEP_CAMERA_AIR] += ms[1];
+                    ctx->step_launches[SG_STEP_CAMERA_AIR] += 1;
+                }
+                ctx->step_ms[SG_STEP_SHADE] += ms[2];
+                ctx->step_launches[SG_STEP_SHADE] += 1;
+            }
             n = *ctx->counter_host;
             q ^= 1;
         }
@@ -680,6 +703,8 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
         if (noalloc_scratch(ctx->device, true, &staging) != SG_OK) return bail(SG_E_DEVICE);
     }
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return bail(SG_E_DEVICE);
+    for (hipEvent_t &e : ctx->ev_step)
+        if (hipEventCreate(&e) != hipSuccess) return bail(SG_E_DEVICE);
     if (hipEventCreate(&ctx->ev_start) != hipSuccess || hipEventCreate(&ctx->ev_stop) != hipSuccess ||
         hipEventCreateWithFlags(&ctx->ev_ready, hipEventDisableTiming) != hipSuccess)
         return bail(SG_E_DEVICE);
@@ -738,6 +763,8 @@ int sg_destroy(sg_ctx *ctx) {
         if (b) (void)hipFree(b);
     if (ctx->ev_start) (void)hipEventDestroy(ctx->ev_start);
     if (ctx->ev_stop) (void)hipEventDestroy(ctx->ev_stop);
+    for (hipEvent_t e : ctx->ev_step)
+        if (e) (void)hipEventDestroy(e);
     if (ctx->ev_ready) (void)hipEventDestroy(ctx->ev_ready);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -996,6 +1023,10 @@ int sg_set_profiling(sg_ctx *ctx, int32_t enable) {
     ctx->launches = 0;
     ctx->adjoint_ms = 0.0;
     ctx->adjoint_launches = 0;
+    for (uint32_t k = 0; k < SG_STEP_COUNT; ++k) {
+        ctx->step_ms[k] = 0.0;
+        ctx->step_launches[k] = 0;
+    }
     return SG_OK;
 }
 
@@ -1003,6 +1034,13 @@ int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
     if (!ctx || !milliseconds || !launches) return SG_E_INVALID;
     *milliseconds = ctx->kernel_ms;
     *launches = ctx->launches;
+    return SG_OK;
+}
+
+int sg_step_time(const sg_ctx *ctx, uint32_t step, double *milliseconds, uint64_t *launches) {
+    if (!ctx || !milliseconds || !launches || step >= SG_STEP_COUNT) return SG_E_INVALID;
+    *milliseconds = ctx->step_ms[step];
+    *launches = ctx->step_launches[step];
     return SG_OK;
 }
 

@@ -331,6 +331,352 @@ def atmosphere_split_heights_zenith(make):
     t.set_air_model()
 
 
+# -- src/ring_tests.rs:55-134 (gpu_rings_geometry) -------------------------------
+def _camera_probe(direction, target):
+    from stargaze_amd.layouts import MISS, PROBE_RAY
+
+    ray = np.zeros(1, PROBE_RAY)
+    ray["anchor"] = MISS
+    ray["direction"] = direction
+    ray["target"] = target
+    return ray
+
+
+def rings_geometry_probe(make):
+    """The production ring helpers on one camera ray: a ring 10 AU away, its
+    pole tilted to the ray, is hit at t = 10 (1e-5) and radius 1.8 planet
+    radii (1e-4) with transmission and star visibility exp(-0.7 / 0.6)
+    (1e-5); the inner hole, the outside and an edge-on ring miss; a star in
+    front of the ring is not attenuated (1e-6)."""
+    t = make((1, 1), opts())
+    r = 0.0004
+    f = scenes.frame()
+    f.bodies = scenes.bodies(scenes.ringed([1.8 * r, 0.0, -10.0], r, [0.0, 0.8, 0.6], 0.7),
+                             scenes.body([0.0, 0.0, -12.0], 0.1, [1.0] * 3, True))
+    ring = t.probe(f, 2, _camera_probe([0.0, 0.0, -1.0], 0))[0]
+    vis = t.probe(f, 3, _camera_probe([0.0, 0.0, -1.0], 1))[0]
+    expected = math.exp(-0.7 / 0.6)
+    radius = float(np.linalg.norm(np.asarray(ring["extra"], np.float64))) / r
+    assert int(ring["index"]) == 0
+    assert abs(float(ring["distance"]) - 10.0) < 1e-5 and abs(radius - 1.8) < 1e-4, (ring, radius)
+    assert abs(float(ring["normal"][0]) - expected) < 1e-5, ring
+    assert abs(float(vis["distance"]) - expected) < 1e-5, vis
+    for offset, pole in ((0.5, [0.0, 0.0, 1.0]), (2.6, [0.0, 0.0, 1.0]), (1.8, [0.0, 1.0, 0.0])):
+        f.bodies[0] = scenes.ringed([offset * r, 0.0, -10.0], r, pole, 0.7)
+        miss = t.probe(f, 2, _camera_probe([0.0, 0.0, -1.0], 0))[0]
+        assert int(miss["index"]) == 0xFFFFFFFF, (offset, pole, miss)
+    f.bodies[0] = scenes.ringed([1.8 * r, 0.0, -10.0], r, [0.0, 0.0, 1.0], 0.7)
+    f.bodies["center"][1] = [0.0, 0.0, -9.0]
+    assert abs(float(t.probe(f, 3, _camera_probe([0.0, 0.0, -1.0], 1))[0]["distance"]) - 1.0) < 1e-6
+
+
+# -- src/pathtracer_tests.rs:997-1094 (gpu_reflected_atmosphere) ----------------
+def planetshine_reflected_air(make):
+    """Planetshine in the air: a sunlit moon beside the zenith lights the air
+    (> 1e-4); its glow scales with the moon's albedo tint (2%), shrinks with
+    its apparent disc (0.15-0.35 at half the radius), follows its phase
+    (quarter 0.1-0.6, new < 2%), and vanishes when the moon is eclipsed, when
+    a body hides the moon from the air, and without air (< 1e-6)."""
+    a = earthlike()
+    f = air_frame(a)
+    center = list(f.bodies["center"][0])
+    radius = float(f.bodies["radius"][0])
+    f.bodies = scenes.bodies(scenes.body(center, radius, [0.0] * 3, False),
+                             scenes.body([0.0004, 0.0, -0.002], 0.00015, [0.8] * 3, False),
+                             scenes.body([0.3, 0.0, 1.0], 0.005, [100000.0] * 3, True))
+
+    def glow(frame):
+        # Common random numbers: both renders start a fresh tracer at the
+        # same noise sequence, so the difference is the moon's light alone
+        # and not the sky's Monte Carlo noise (the cone-sampled sun moves
+        # the Mie phase by about 1% per sample in daylight).
+        color = frame.bodies["color"][1].copy()
+        frame.bodies["color"][1] = [0.0] * 3
+        background = sample(make((1, 1), opts(samples_per_frame=64, max_bounces=1)), frame, 32)[0]
+        frame.bodies["color"][1] = color
+        lit = sample(make((1, 1), opts(samples_per_frame=64, max_bounces=1)), frame, 32)[0]
+        return lit[:3].astype(np.float64) - background[:3].astype(np.float64)
+
+    full = glow(f)
+    assert np.all(np.isfinite(full)) and np.all(full > 1e-4), full
+    f.bodies["color"][1] = [0.4, 0.2, 0.1]
+    tinted = glow(f)
+    for c in range(3):
+        expected = full[c] * (0.5, 0.25, 0.125)[c]
+        assert abs(tinted[c] - expected) < 0.02 * expected, (tinted, full)
+    f.bodies["color"][1] = [0.8] * 3
+    f.bodies["radius"][1] *= 0.5
+    small = glow(f)
+    assert full[0] * 0.15 < small[0] < full[0] * 0.35, (small, full)
+    f.bodies["radius"][1] *= 2.0
+    f.bodies["center"][2] = [1.0, 0.0, 0.0]
+    quarter = glow(f)
+    f.bodies["center"][2] = [-0.3, 0.0, -1.0]
+    new = glow(f)
+    assert full[0] * 0.1 < quarter[0] < full[0] * 0.6, (quarter, full)
+    assert abs(new[0]) < full[0] * 0.02, (new, full)
+    f.bodies["center"][2] = [0.3, 0.0, 1.0]
+    base = f.bodies.copy()
+    f.bodies = np.concatenate([base, scenes.bodies(scenes.body([0.1502, 0.0, 0.499], 0.01, [0.0] * 3, False))])
+    eclipsed = glow(f)
+    assert np.all(np.abs(eclipsed) < 1e-6), eclipsed
+    f.bodies = np.concatenate([base, scenes.bodies(scenes.body([0.0002, 0.0, -0.001], 0.00012, [0.0] * 3, False))])
+    hidden = glow(f)
+    assert np.all(np.abs(hidden) < 1e-6), hidden
+    f.bodies = base
+    f.globals["atmo_params"][3] = 0.0
+    airless = glow(f)
+    assert np.all(np.abs(airless) < 1e-6), airless
+
+
+# -- src/pathtracer_tests.rs:1204-1266 (gpu_star_dots) ---------------------------
+def star_dots(make):
+    """An off-axis catalogue star of 0.1 rad at (0.8, 0, -1) in a 90 deg
+    128^2 view is capped to 3 pixels around its image point (115.2, 64):
+    every lit pixel lies within 3 + 1/sqrt(2) px and more than 20 are lit.
+    A bright catalogue star behind the camera lights no dark ground, at
+    either zoom."""
+    from stargaze_amd.stars import CatalogueStar, ray_catalogue
+
+    star = CatalogueStar(dir=(0.8, 0.0, -1.0), color=(1.0, 1.0, 1.0), bright=1.0, angular_radius=0.1)
+    cat, cells = ray_catalogue([star])
+    t = make((128, 128), opts(samples_per_frame=64, max_bounces=2, exposure=1.0), catalogue=cat, sky_cells=cells)
+    f = scenes.frame()
+    f.globals["cam_forward"][3] = 1.0
+    f.globals["viewport"] = [128.0, 128.0, 1.0, 0.0]
+    pixels = sample(t, f, 1)
+    lit = 0
+    for i, p in enumerate(pixels):
+        if p[0] > 0.0:
+            dx = (i % 128) + 0.5 - 115.2
+            dy = (i // 128) + 0.5 - 64.0
+            assert math.hypot(dx, dy) < 3.0 + math.sqrt(0.5), (i, p)
+            lit += 1
+    assert lit > 20, lit
+    behind = CatalogueStar(dir=(0.0, 0.0, 1.0), color=(1.0, 1.0, 1.0), bright=100.0, angular_radius=0.1)
+    cat, cells = ray_catalogue([behind])
+    t = make((1, 1), opts(samples_per_frame=64, max_bounces=2, exposure=1.0), catalogue=cat, sky_cells=cells)
+    f = scenes.frame()
+    f.bodies = scenes.bodies(scenes.body([0.0, 0.0, -3.0], 1.0, [0.8] * 3, False))
+    before = sample(t, f, 128)[0][0]
+    f.globals["cam_forward"][3] *= 0.01
+    after = sample(t, f, 128)[0][0]
+    assert before == 0.0 and after == 0.0, (before, after)
+
+
+# -- src/sky_tests.rs:23-103 (gpu_milky_way_is_camera_only_...) ------------------
+def galactic_frame():
+    """A camera looking at the galactic centre (src/sky_tests.rs:3-11): the
+    view direction is the panorama's zero-longitude axis."""
+    f = scenes.frame()
+    pole = np.array([0.35, 0.82, 0.45])
+    pole /= np.linalg.norm(pole)
+    sign = 1.0 if pole[2] >= 0 else -1.0
+    a = -1.0 / (sign + pole[2])
+    direction = np.array([pole[0] * pole[1] * a, sign + pole[1] * pole[1] * a, -pole[1]])
+    f.globals["cam_forward"][:3] = direction
+    f.globals["cam_right"][:3] = np.cross(direction, pole)
+    f.globals["cam_up"][:3] = pole
+    return f
+
+
+def milky_way_camera_only(make):
+    """The Milky Way is seen by camera rays only: the galactic core gives
+    0.01..1 x the backdrop gain, stays warm (r > 1.2 g), and its luminance is
+    the backdrop channel w (1e-7); a diffuse body under it stays exactly
+    black; an emissive body in front occludes it (its own radiance, w = 0);
+    after a resize the panorama is unchanged (1%). With automatic exposure
+    the backdrop does not drive the meter (1e-4) and the night exposure is
+    capped in [200, 300] (where the tracer meters)."""
+    from stargaze_amd.sky import panorama
+
+    o = opts(samples_per_frame=64, auto_exposure=True)
+    o.milky_way = 4.0e-4  # Options::default()
+    t = make((1, 1), o, milky_way=panorama())
+    f = galactic_frame()
+    night = sample(t, f, 64)[0]
+    gain = 4.0e-4
+    assert 0.01 * gain < night[0] <= gain, night
+    lum = night[0] * 0.2126 + night[1] * 0.7152 + night[2] * 0.0722
+    assert abs(lum - night[3]) < 1e-7, night
+    assert night[0] > night[1] * 1.2, night
+    meters = hasattr(t, "read_exposure")
+    if meters:
+        with_backdrop = float(t.read_exposure()[0]["value"])
+        o2 = opts(samples_per_frame=64, auto_exposure=True)
+        plain = make((1, 1), o2, milky_way=panorama())
+        sample(plain, f, 64)
+        without = float(plain.read_exposure()[0]["value"])
+        assert abs(with_backdrop - without) <= 1e-4 * without, (with_backdrop, without)
+    f.bodies = scenes.bodies(scenes.body([0.0, 0.0, -3.0], 1.0, [0.9] * 3, False))
+    f.globals["cam_forward"][:3] = [0.0, 0.0, -1.0]
+    f.globals["cam_right"][:3] = [1.0, 0.0, 0.0]
+    f.globals["cam_up"][:3] = [0.0, 1.0, 0.0]
+    dark = sample(t, f, 16)[0]
+    assert np.array_equal(dark, np.zeros(4, np.float32)), dark
+    f.bodies["emissive"][0] = 1.0
+    f.bodies["color"][0] = [1e-5] * 3
+    dim = sample(t, f, 64)[0]
+    assert np.all(np.abs(dim[:3] - 1e-5) < 1e-10) and dim[3] == 0.0, dim
+    if meters:
+        capped = float(t.read_exposure()[0]["value"])
+        assert 200.0 <= capped <= 300.0, capped
+    t.resize((2, 2))
+    f = galactic_frame()
+    f.globals["viewport"][:2] = [2.0, 2.0]
+    resized = sample(t, f, 1)
+    assert np.all(np.abs(resized[:, 0] - night[0]) < night[0] * 0.01), (resized, night)
+
+
+# -- src/ground_tests.rs:1053-1153 (gpu_shadow_queries_match_closest_hits) -------
+def _reference_visibility(t, f, rays, light):
+    """closest_hit must be the light; then the product of the ring
+    transmissions in front of it (the reference's reference_visibility)."""
+    closest = t.probe(f, 1, rays)
+    out = np.where(closest["index"].astype(np.uint32) == light, 1.0, 0.0)
+    for i in range(len(f.bodies)):
+        r = rays.copy()
+        r["target"] = i
+        ring = t.probe(f, 2, r)
+        before = (ring["index"].astype(np.uint32) != 0xFFFFFFFF) & (ring["distance"] < closest["distance"])
+        out = np.where(before, out * ring["normal"][:, 0], out)
+    return out
+
+
+def shadow_queries_match_closest_hits(make):
+    """The any-hit shadow query (light_visibility) equals the closest-hit
+    reference bit for bit: a ringed planet, two lights at one place (the
+    second is hidden by the tie rule) and a body behind them; then a scene
+    with a landscape (terrain, props, host sphere) from the surface, 2 km up,
+    1 km under ground and three host radii out, toward each light and in
+    random directions."""
+    from stargaze_amd.layouts import MISS, PROBE_RAY
+
+    rng = np.random.default_rng(7)
+    ring_scene = scenes.frame()
+    planet = scenes.ringed([1.8, 0.0, -5.0], 1.0, [0.0, 0.8, 0.6], 0.7)
+    planet["ring_params"][0] = 2.4
+    ring_scene.bodies = scenes.bodies(planet, scenes.body([0.0, 0.0, -10.0], 0.5, [1.0] * 3, True),
+                                      scenes.body([0.0, 0.0, -10.0], 0.5, [1.0] * 3, True),
+                                      scenes.body([0.0, 0.0, -20.0], 1.0, [0.5] * 3, False))
+    halo, data, cells = scenes.halo_parity(64, 16)
+    cases_ = [(ring_scene, None), (halo, (data, cells))]
+    for f, terrain in cases_:
+        t = make((4, 4), opts())
+        if terrain is not None:
+            t.upload_terrain(*terrain)
+        g = f.globals
+        origins = [np.zeros(3)]
+        if g["ground_north"][3] != 0.0:
+            east, up, north = (g[k][:3].astype(np.float64) for k in ("ground_east", "ground_up", "ground_north"))
+            origins = [up * h / scenes.METRES_PER_AU for h in (0.0, 2000.0, -1000.0, 3.0 * float(g["ground_up"][3]))]
+        centres = f.bodies["center"].astype(np.float64) + f.bodies["center_low"][:, :3].astype(np.float64)
+        seen = []
+        for light in range(len(f.bodies)):
+            if f.bodies["emissive"][light] < 0.5:
+                continue
+            rays = np.zeros(64 * len(origins), PROBE_RAY)
+            k = 0
+            for o in origins:
+                for j in range(64):
+                    if j % 2 == 0:
+                        d = centres[light] - o
+                        d /= np.linalg.norm(d)
+                        d = d + rng.normal(size=3) * float(f.bodies["radius"][light]) / np.linalg.norm(centres[light] - o)
+                    else:
+                        d = rng.uniform(-0.5, 0.5, 3)
+                    rays[k]["offset"] = o
+                    rays[k]["anchor"] = MISS
+                    rays[k]["direction"] = d / np.linalg.norm(d)
+                    rays[k]["target"] = light
+                    k += 1
+            vis = t.probe(f, 3, rays)["distance"].astype(np.float64)
+            ref = _reference_visibility(t, f, rays, light)
+            assert np.array_equal(vis, ref), (light, np.nonzero(vis != ref)[0][:8], vis[vis != ref][:8],
+                                              ref[vis != ref][:8])
+            seen.append(vis)
+        seen = np.concatenate(seen)
+        assert 0 < np.count_nonzero(seen) < len(seen)
+        if terrain is None:
+            assert np.any((seen > 0) & (seen < 1)), "some rays must cross the ring"
+
+
+# -- src/ground_tests.rs:191-272 (gpu_local_ground_shadows) ---------------------
+def local_frame(radius_m):
+    """``local_frame`` (src/ground_tests.rs:191-206): the camera 2 m above a
+    host of radius_m, the ground plane tilted to the view, a 0.5 m sun 8 m up
+    and 4 m south."""
+    metre = 1.0 / scenes.METRES_PER_AU
+    f = scenes.frame()
+    up = np.array([0.0, math.sqrt(0.8), math.sqrt(0.2)], np.float32)
+    north = np.array([0.0, -up[2], up[1]], np.float32)
+    f.globals["ground_east"] = [1.0, 0.0, 0.0, 2.0]
+    f.globals["ground_up"] = [*up, radius_m]
+    f.globals["ground_north"] = [*north, 1.0]
+    center = -up.astype(np.float64) * (radius_m + 2.0) * metre
+    sun = (up.astype(np.float64) * 8.0 - north.astype(np.float64) * 4.0) * metre
+    f.bodies = scenes.bodies(scenes.body(center, radius_m * metre, [0.3] * 3, False),
+                             scenes.body(sun, 0.5 * metre, [100.0] * 3, True))
+    return f
+
+
+def _one_prop(f, center, extent):
+    from stargaze_amd.layouts import PRIMITIVE
+
+    p = np.zeros(1, PRIMITIVE)
+    p["center"] = center
+    p["extent"] = extent
+    f.ground = p
+    f.globals["ground_counts"][0] = 1
+
+
+def local_ground_shadows(make):
+    """A box, an ellipsoid, a rotated box and a blocker 2 cm above the ground
+    shadow the sun (< 0.1%, < 1% for the blocker) on two host radii."""
+    t = make((1, 1), opts(samples_per_frame=64, max_bounces=1))
+    for radius in (6_371_000.0, 1_500_000.0):
+        f = local_frame(radius)
+        full = sample(t, f, 8)[0][0]
+        assert 0.01 < full < np.inf, full
+        _one_prop(f, [0.0, 4.0, -4.0, 0.0], [0.8, 0.8, 0.8, 0.0])
+        box = sample(t, f, 8)[0][0]
+        assert box < full * 0.001, (box, full)
+        f.ground["center"][0][3] = 1.0
+        ellipsoid = sample(t, f, 8)[0][0]
+        assert ellipsoid < full * 0.001, (ellipsoid, full)
+        _one_prop(f, [0.0, 0.02, -4.0, 0.0], [0.2, 0.004, 0.005, 0.0])
+        small = sample(t, f, 8)[0][0]
+        assert small < full * 0.01, (small, full)
+        _one_prop(f, [0.0, 4.0, -4.0, 0.0], [0.8, 0.8, 0.8, 0.4])
+        rotated = sample(t, f, 8)[0][0]
+        assert rotated < full * 0.001, (rotated, full)
+
+
+# -- src/ground_tests.rs:630-677 (gpu_distant_hills_and_rocks_occlude_the_sky) -
+def distant_props_occlude_the_sky(make):
+    """Ellipsoid, rock, conifer and log primitives 450 m away and 35-60 m in
+    size block a sun behind them (< 1e-6) and let it through when moved
+    100 m aside (> 0.99), on two host radii."""
+    t = make((1, 1), opts(samples_per_frame=1, max_bounces=1))
+    metre = 1.0 / scenes.METRES_PER_AU
+    alt = math.radians(3.0)
+    up = np.array([0.0, math.cos(alt), -math.sin(alt)])
+    north = np.array([0.0, -math.sin(alt), -math.cos(alt)])
+    for radius in (1_500_000.0, 6_371_000.0):
+        f = scenes.frame()
+        f.globals["ground_east"] = [1.0, 0.0, 0.0, 2.0]
+        f.globals["ground_up"] = [*up, radius]
+        f.globals["ground_north"] = [*north, 1.0]
+        f.bodies = scenes.bodies(scenes.body(-up * (radius + 2.0) * metre, radius * metre, [0.0] * 3, False),
+                                 scenes.body([0.0, 0.0, -1.0], 0.1, [1.0] * 3, True))
+        assert sample(t, f, 1)[0][0] > 0.99
+        for kind in (1.0, 2.0, 3.0, 5.0):
+            _one_prop(f, [0.0, 25.0, 450.0, kind], [35.0, 30.0, 60.0, 0.2])
+            assert sample(t, f, 1)[0][0] < 1e-6, (radius, kind)
+            f.ground["center"][0][0] = 100.0
+            assert sample(t, f, 1)[0][0] > 0.99, (radius, kind)
+
+
 def sky_cells_for(stars):
     """``ray_catalogue`` cell lists (src/stars.rs:152-205) for a few stars."""
     W, H = 256, 128
@@ -373,4 +719,11 @@ ALL = [
     rings_null_mis,
     atmosphere_zenith_and_eclipse,
     atmosphere_split_heights_zenith,
+    rings_geometry_probe,
+    planetshine_reflected_air,
+    star_dots,
+    milky_way_camera_only,
+    shadow_queries_match_closest_hits,
+    local_ground_shadows,
+    distant_props_occlude_the_sky,
 ]

@@ -69,9 +69,13 @@ def render_reference(out_dir: Path = HERE) -> None:
     print(f"wrote {out_dir / OUT.name}")
 
 
+TARGET_SEEDS = (0, 1, 2, 3)
+
+
 def oracle_target() -> None:
-    """The FP32 oracle at TARGET_SPP spp (noise seed 0), its linear-RGB RMSE
-    against the reference: the fixed RMSE the benchmark converges to."""
+    """The FP32 oracle's expected linear-RGB RMSE against the reference at
+    TARGET_SPP spp: the root mean square of its RMSE over four noise seeds.
+    This is the fixed RMSE the benchmark's time-to-RMSE converges to."""
     import oracle
     from stargaze_amd import scenes
     from stargaze_amd.pathtracer import Options
@@ -81,18 +85,23 @@ def oracle_target() -> None:
 
     frame, data, cells = scene()
     ref = np.load(OUT)["image"]
-    t = oracle.OracleTracer((WIDTH, HEIGHT), Options(samples_per_frame=TARGET_SPP, max_bounces=4, milky_way=0.0))
-    t.upload_terrain(data, cells)
-    t.set_air_model(**scenes.HALO_AIR)
+    per_seed = []
     t0 = time.perf_counter()
-    t.encode(frame)
+    for seed in TARGET_SEEDS:
+        t = oracle.OracleTracer((WIDTH, HEIGHT), Options(noise_seed=seed, samples_per_frame=TARGET_SPP,
+                                                         max_bounces=4, milky_way=0.0))
+        t.upload_terrain(data, cells)
+        t.set_air_model(**scenes.HALO_AIR)
+        t.encode(frame)
+        per_seed.append(parity.rmse(t.read_accumulation(), ref))
     dt = time.perf_counter() - t0
-    rmse = parity.rmse(t.read_accumulation(), ref)
+    target = float(np.sqrt(np.mean(np.square(per_seed))))
     meta = json.loads(META.read_text())
-    meta["target"] = {"rmse": rmse, "spp": TARGET_SPP, "noise_seed": 0,
-                      "source": "FP32 oracle (C restatement), linear RGB, W excluded"}
+    meta["target"] = {"rmse": target, "spp": TARGET_SPP, "noise_seeds": list(TARGET_SEEDS),
+                      "rmse_per_seed": per_seed,
+                      "source": "FP32 oracle, linear RGB (W excluded), root mean square over the seeds"}
     META.write_text(json.dumps(meta, indent=1) + "\n")
-    print(f"oracle RMSE at {TARGET_SPP} spp: {rmse:.6g} ({dt:.1f} s)")
+    print(f"oracle RMSE at {TARGET_SPP} spp: {target:.6g} (seeds {per_seed}, {dt:.1f} s)")
 
 
 if __name__ == "__main__":

@@ -15,8 +15,11 @@ from stargaze_amd.pathtracer import Options, PathTracer
 pytestmark = pytest.mark.gpu
 
 
-def make_gpu(dimensions, options, catalogue=None, sky_cells=None):
-    return PathTracer(dimensions, catalogue=catalogue, sky_cells=sky_cells, options=options)
+def make_gpu(dimensions, options, catalogue=None, sky_cells=None, milky_way=None):
+    t = PathTracer(dimensions, catalogue=catalogue, sky_cells=sky_cells, options=options)
+    if milky_way is not None:
+        t.set_milky_way(milky_way)
+    return t
 
 
 @pytest.mark.parametrize("case", cases.ALL, ids=lambda c: c.__name__)

@@ -12,8 +12,9 @@ import cases
 import oracle
 
 
-def make_oracle(dimensions, options, catalogue=None, sky_cells=None):
-    return oracle.OracleTracer(dimensions, options, "f32", catalogue=catalogue, sky_cells=sky_cells)
+def make_oracle(dimensions, options, catalogue=None, sky_cells=None, milky_way=None):
+    return oracle.OracleTracer(dimensions, options, "f32", catalogue=catalogue, sky_cells=sky_cells,
+                               milky_way=milky_way)
 
 
 @pytest.mark.parametrize("case", cases.ALL, ids=lambda c: c.__name__)

@@ -265,6 +265,15 @@ typedef struct sg_launch {
     uint32_t tail_paths;  /* compaction threshold, see sg_set_tail */
 } sg_launch;
 int sg_get_launch(const sg_ctx *ctx, sg_launch *launch);
+/* Records per path (1-256, default 8; 48 B each) of the wavefront
+ * integrator's queue of camera-air shadow rays whose landscape query is
+ * deferred to a kernel of its own (frames with a ground scene and air). A
+ * chunk that pushes more grows the queue and runs again with the same
+ * draws, so the value sizes memory only; sg_get_air_queue reads the current
+ * size. No counterpart in the reference, whose camera-air pass
+ * (atmosphere, src/shaders/pathtrace.wgsl:1092-1215) answers each query inline. */
+int sg_set_air_queue(sg_ctx *ctx, uint32_t records_per_path);
+int sg_get_air_queue(const sg_ctx *ctx, uint32_t *records_per_path);
 /* Time trial passes of `frame` over queue capacities, compaction thresholds
  * and the fused integrator, keep the fastest and return it. Trial passes
  * render into the image; the accumulation restarts with the next pass, at

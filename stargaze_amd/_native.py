@@ -46,6 +46,8 @@ EXPORTS = (
     "sg_set_integrator",
     "sg_set_tail",
     "sg_get_launch",
+    "sg_set_air_queue",
+    "sg_get_air_queue",
     "sg_autotune",
     "sg_last_error",
     "sg_probe",
@@ -170,6 +172,8 @@ def load() -> ctypes.CDLL:
         "sg_set_integrator": ([P, U32, U32], I32),
         "sg_set_tail": ([P, U32], I32),
         "sg_get_launch": ([P, ctypes.POINTER(SgLaunch)], I32),
+        "sg_set_air_queue": ([P, U32], I32),
+        "sg_get_air_queue": ([P, ctypes.POINTER(U32)], I32),
         "sg_autotune": ([P, ctypes.POINTER(SgFrame), U32, ctypes.POINTER(SgLaunch), ctypes.POINTER(ctypes.c_double)],
                         I32),
         "sg_last_error": ([P], ctypes.c_char_p),

@@ -214,6 +214,7 @@ SG_FN Hit sphere_hit_radius(const Scene &sc, const Ray &ray, uint32_t index, flo
     return sphere_solve<float>(tof(oc), ray.direction, radius, index);
 }
 SG_FN Hit sphere_hit(const Scene &sc, const Ray &ray, uint32_t index) {
+    SG_COUNT(SG_EV_SPHERE);
     return sphere_hit_radius(sc, ray, index, body(sc, index).center.w);
 }
 

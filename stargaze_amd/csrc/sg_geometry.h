@@ -312,6 +312,7 @@ SG_FN float4 terrain_query(const Scene &sc, f3 o, f3 d, float initial_limit, boo
               (d.y >= 0.f ? 1.f : -1.f) / fmaxf(fabsf(d.y), 1.0e-30f),
               (d.z >= 0.f ? 1.f : -1.f) / fmaxf(fabsf(d.z), 1.0e-30f)};
     float2 root = terrain_interval(sc, o, inv, 0u);
+    SG_COUNT(SG_EV_TERRAIN_BOX);
     if (root.x > root.y || root.y <= 0.001f || root.x >= limit) return result;
     uint32_t stack_node[SG_TERRAIN_STACK];
     float stack_t[SG_TERRAIN_STACK];
@@ -335,6 +336,8 @@ SG_FN float4 terrain_query(const Scene &sc, f3 o, f3 d, float initial_limit, boo
                     f3 b = terrain_vertex(sc, x + 1u, z);
                     f3 c = terrain_vertex(sc, x, z + 1u);
                     f3 e = terrain_vertex(sc, x + 1u, z + 1u);
+                    SG_COUNT(SG_EV_TRIANGLE);
+                    SG_COUNT(SG_EV_TRIANGLE);
                     f3 first = terrain_triangle(o, d, a, c, b);
                     f3 second = terrain_triangle(o, d, e, b, c);
                     if (first.z < limit) {
@@ -363,6 +366,7 @@ SG_FN float4 terrain_query(const Scene &sc, f3 o, f3 d, float initial_limit, boo
             for (uint32_t child = 0; child < 4u; ++child) {
                 children[child] = pending * 4u + child + 1u;
                 float2 bounds = terrain_interval(sc, o, inv, children[child]);
+                SG_COUNT(SG_EV_TERRAIN_BOX);
                 distances[child] = (bounds.x <= bounds.y && bounds.y > 0.001f && bounds.x < limit)
                                        ? fmaxf(bounds.x, 0.0f)
                                        : 1.0e30f;
@@ -481,13 +485,16 @@ SG_FN bool prop_bvh_query(const Scene &sc, f3 o, f3 d, Hit &closest, bool occlus
         const float4 a = sc.bvh[2 * node], b = sc.bvh[2 * node + 1];
         const uint32_t first = sg_f2u(a.w), leaves = (sg_f2u(b.w) & 0x80000000u) ? 0u : sg_f2u(b.w);
         if (leaves != 0u) {
-            for (uint32_t k = 0; k < leaves; ++k)
+            for (uint32_t k = 0; k < leaves; ++k) {
+                SG_COUNT(SG_EV_PRIM);
                 if (prop_test(sc, sc.bvh_prims[first + k], o, d, closest, occlusion)) return true;
+            }
         } else {
             float tn[2];
             bool hit[2];
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
+                SG_COUNT(SG_EV_BVH_BOX);
                 const float4 ca = sc.bvh[2 * (first + c)], cb = sc.bvh[2 * (first + c) + 1];
                 f3 centre = {0.5f * (ca.x + cb.x), 0.5f * (ca.y + cb.y), 0.5f * (ca.z + cb.z)};
                 f3 half = {0.5f * (cb.x - ca.x), 0.5f * (cb.y - ca.y), 0.5f * (cb.z - ca.z)};
@@ -510,6 +517,7 @@ SG_FN bool prop_bvh_query(const Scene &sc, f3 o, f3 d, Hit &closest, bool occlus
         bool found = false;
         while (count > 0u) {
             const uint32_t cand = stack[--count];
+            SG_COUNT(SG_EV_BVH_BOX);
             const float4 ca = sc.bvh[2 * cand], cb = sc.bvh[2 * cand + 1];
             f3 centre = {0.5f * (ca.x + cb.x), 0.5f * (ca.y + cb.y), 0.5f * (ca.z + cb.z)};
             f3 half = {0.5f * (cb.x - ca.x), 0.5f * (cb.y - ca.y), 0.5f * (cb.z - ca.z)};
@@ -528,6 +536,7 @@ SG_FN Hit landscape_query(const Scene &sc, const Ray &ray, float limit, bool occ
     Hit closest{limit, SG_MISS, f3s(0.f)};
     const sg_globals &g = sc.s.globals;
     if (!kLandscape || g.ground_north[3] == 0.0f) return closest;
+    SG_COUNT(SG_EV_LANDSCAPE);
     d3 od = ground_origin(sc, ray);
     f3 d = to_ground(sc, ray.direction);
     float radius = g.ground_up[3];
@@ -605,6 +614,7 @@ SG_FN Hit closest_hit(const Scene &sc, const Ray &ray) {
 // --- rings (src/shaders/rings.wgsl) -----------------------------------------
 SG_FN RingHit ring_miss(float limit) { return RingHit{limit, SG_MISS, f3s(0.f)}; }
 SG_FN RingHit ring_hit(const Scene &sc, const Ray &ray, uint32_t index) {
+    SG_COUNT(SG_EV_RING);
     const Body &b = body(sc, index);
     if (b.ring_params.x <= b.ring_plane.w || b.ring_params.y <= 0.0f) return ring_miss(1.0e30f);
     f3 n = xyz(b.ring_plane);
@@ -699,6 +709,7 @@ SG_FN Ray ring_origin(const Scene &sc, const RingHit &hit, f3 direction) {
 
 // Shadow rays (src/shaders/rings.wgsl:90-115).
 SG_FN float light_visibility(const Scene &sc, const Ray &ray, uint32_t light) {
+    SG_COUNT(SG_EV_SHADOW);
     Hit surface = sphere_hit(sc, ray, light);
     if (surface.index == SG_MISS) return 0.0f;
     const float host = sc.s.globals.ground_north[3];
@@ -764,6 +775,7 @@ SG_FN float air_light_visibility(const Scene &sc, const Ray &ray, uint32_t light
     const sg_globals &g = sc.s.globals;
     if (!kLandscape || g.ground_north[3] == 0.0f || (float)ground_origin(sc, ray).y <= landscape_top(sc))
         return light_visibility(sc, ray, light);
+    SG_COUNT(SG_EV_SHADOW);
     Hit surface = sphere_hit(sc, ray, light);
     if (surface.index == SG_MISS) return 0.0f;
     for (uint32_t i = 0; i < body_count(sc); ++i) {
@@ -778,6 +790,47 @@ SG_FN float air_light_visibility(const Scene &sc, const Ray &ray, uint32_t light
         if (hit.index != SG_MISS && hit.distance < surface.distance) transmission *= ring_transmission(sc, ray, hit);
     }
     return transmission;
+}
+
+// air_light_visibility with its landscape predicates left to the caller,
+// for callers that answer them later and coherently (the wavefront
+// integrator's k_air_occlusion). Returns the visibility through the bodies
+// and rings alone (0 when a body blocks the light) and sets `limit` to the
+// light's distance and `owed` to what is left to ask, by which the light is
+// blocked: 0 nothing; 1 the landscape before `limit`; light + 2 that too, or
+// a closest hit other than the light (the light is the host body). With
+// the predicates answered the value is air_light_visibility's: below the
+// top of the landscape it runs light_visibility, whose checks are these
+// predicates and this loop.
+SG_FN float air_light_visibility_split(const Scene &sc, const Ray &ray, uint32_t light, uint32_t &owed,
+                                       float &limit) {
+    const float host = sc.s.globals.ground_north[3];
+    const bool below = kLandscape && host != 0.0f && (float)ground_origin(sc, ray).y <= landscape_top(sc);
+    owed = 0u;
+    SG_COUNT(SG_EV_SHADOW);
+    Hit surface = sphere_hit(sc, ray, light);
+    if (surface.index == SG_MISS) return 0.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        if (i == light || (below && host == (float)(i + 1u))) continue;
+        Hit hit = sphere_hit(sc, ray, i);
+        if (hit.index != SG_MISS && (hit.distance < surface.distance || (hit.distance == surface.distance && i < light)))
+            return 0.0f;
+    }
+    if (below) owed = host == (float)(light + 1u) ? light + 2u : 1u;
+    limit = surface.distance;
+    float transmission = 1.0f;
+    for (uint32_t i = 0; i < body_count(sc); ++i) {
+        RingHit hit = ring_hit(sc, ray, i);
+        if (hit.index != SG_MISS && hit.distance < surface.distance) transmission *= ring_transmission(sc, ray, hit);
+    }
+    return transmission;
+}
+// The predicates air_light_visibility_split left owed: whether they block
+// the ray to the light at `limit`.
+SG_FN bool air_light_blocked(const Scene &sc, const Ray &ray, float limit, uint32_t owed) {
+    if (owed >= 2u && closest_hit(sc, ray).index != owed - 2u) return true;
+    Hit local = landscape_query(sc, ray, limit, true);
+    return local.index != SG_MISS && local.distance < limit;
 }
 
 }  // namespace sg

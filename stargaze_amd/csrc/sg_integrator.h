@@ -37,6 +37,15 @@ constexpr uint32_t COLUMN_BIN_SLOTS = 16u;
 struct NoGrad {
     static constexpr bool on = false;
 };
+// Camera-air shadow rays answer their landscape queries inline (NoDefer), or
+// hand them to a deferral sink that queues them for a kernel of its own (the
+// wavefront integrator's AirDefer): push(origin, direction, limit, term,
+// owed) queues the ray and its contribution, to be added unless the
+// predicates `owed` names (air_light_visibility_split) block it.
+struct NoDefer {
+    static constexpr bool on = false;
+    SG_MEMBER void push(f3, f3, float, f3, uint32_t) {}
+};
 // The gradient structs have internal linkage, like the free functions their
 // members call.
 namespace {
@@ -515,6 +524,7 @@ SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps,
     f3 c = atmo_center(sc);
     float sum = 0.0f, sum_m = 0.0f;
     for (uint32_t i = 0; i < steps; ++i) {
+        SG_COUNT(SG_EV_DENSITY_STEP);
         f3 p = origin + dir * (((float)i + 0.5f) * dt);
         float h = fmaxf(length(p - c) - r, 0.0f);
         if (sc.density_profile) {
@@ -651,9 +661,9 @@ SG_FN float brightest_risen_light(const Scene &sc, f3 origin) {
 // With a gradient sink, `wt` is d(loss)/d(inscatter) and the derivatives of
 // the in-scatter with respect to the air coefficients, the reflector's
 // albedo and the tabulated ring profile (shadow rays) are added to `grad`.
-template <class G = NoGrad>
+template <class G = NoGrad, class D = NoDefer>
 SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32_t &rng, uint32_t accumulated,
-                         const G &grad = G(), f3 wt = f3s(0.0f)) {
+                         const G &grad = G(), f3 wt = f3s(0.0f), D *defer = nullptr) {
     Scatter result{f3s(1.0f), f3s(0.0f)};
     if (sc.s.globals.atmo_params[3] < 0.5f) return result;
     float exit = shell_exit(sc, origin, dir);
@@ -717,6 +727,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
     float view_column = 0.0f;  // density column from the origin to this step
     uint64_t view_bins = 0u;   // density profile bins of the steps so far
     for (uint32_t i = 0; i < steps; ++i) {
+        SG_COUNT(SG_EV_VIEW_CELL);
         f3 p = origin + dir * (((float)i + jitter) * dt);
         float altitude = fmaxf(length(p - center) - r, 0.0f);
         uint32_t bin = 0u;
@@ -785,8 +796,16 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             uint32_t light_samples = eclipse_light_samples(sc, light_ray, b, accumulated);
             for (uint32_t ls = 0; ls < light_samples; ++ls) {
                 LightSample light = sample_light_stratum(sc, light_ray, b, ls, light_samples, rng);
+                SG_COUNT(SG_EV_LIGHT_SAMPLE);
                 light_ray.direction = light.direction;
-                float visibility = air_light_visibility(sc, light_ray, b);
+                uint32_t owed = 0u;
+                float light_limit = 0.0f;
+                float visibility;
+                if constexpr (D::on) {
+                    visibility = air_light_visibility_split(sc, light_ray, b, owed, light_limit);
+                } else {
+                    visibility = air_light_visibility(sc, light_ray, b);
+                }
                 if (visibility == 0.0f) continue;
                 f3 e_sun = xyz(body(sc, b).material) * (visibility / ((float)light_samples * light.pdf));
                 ColumnBins sun_bins;
@@ -797,6 +816,12 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                 float cos_theta = dot(dir, light.direction);
                 f3 beta_phase = beta_r_step * rayleigh_phase(cos_theta) + f3s(beta_m_step * mie_phase(cos_theta, g));
                 f3 term = t_view * segment * beta_phase * e_sun * sun_t;
+                if constexpr (D::on) {
+                    if (owed) {
+                        defer->push(p, light.direction, light_limit, term, owed);
+                        continue;
+                    }
+                }
                 inscatter += term;
                 if constexpr (G::on) {
                     f3 wc = wt * term;
@@ -810,7 +835,14 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         }
         if (planetshine) {
             Ray shine_ray = camera_ray(p, reflected_direction);
-            float visibility = air_light_visibility(sc, shine_ray, reflector.index);
+            uint32_t owed = 0u;
+            float shine_limit = 0.0f;
+            float visibility;
+            if constexpr (D::on) {
+                visibility = air_light_visibility_split(sc, shine_ray, reflector.index, owed, shine_limit);
+            } else {
+                visibility = air_light_visibility(sc, shine_ray, reflector.index);
+            }
             if (visibility > 0.0f) {
                 ColumnBins light_bins;
                 float light_mie = 0.0f;
@@ -819,7 +851,14 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
                 f3 light_t = exp3(-air_depth(sc, light_column, light_mie));
                 const f3 reflected_phase = beta_r_step * reflected_rayleigh + f3s(beta_m_step * reflected_mie);
                 f3 term = t_view * segment * reflected_phase * reflected * (light_t * visibility);
-                inscatter += term;
+                bool queued = false;
+                if constexpr (D::on) {
+                    if (owed) {
+                        defer->push(p, reflected_direction, shine_limit, term, owed);
+                        queued = true;
+                    }
+                }
+                if (!queued) inscatter += term;
                 if constexpr (G::on) {
                     f3 wc = wt * term;
                     grad.extinction(wc * (step_dlog - f3s(light_column)));
@@ -1115,8 +1154,9 @@ SG_FN HitRecord path_extend(const Scene &sc, PathState &st) {
     return h;
 }
 
-template <class G = NoGrad>
-SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, const G &grad = G()) {
+template <class G = NoGrad, class D = NoDefer>
+SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, const G &grad = G(),
+                           D *defer = nullptr) {
     if (!(st.flags & PATH_CAMERA_SEGMENT)) return;
     st.flags &= ~PATH_CAMERA_SEGMENT;
     if (sc.s.globals.atmo_params[3] < 0.5f) return;
@@ -1131,8 +1171,8 @@ SG_FN void path_camera_air(const Scene &sc, PathState &st, const HitRecord &h, c
         air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
                          st.accumulated, static_cast<const GradSink &>(grad), grad.weight(st) * st.throughput);
     } else {
-        air = atmosphere(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
-                         st.accumulated);
+        air = atmosphere<NoGrad, D>(sc, origin, st.ray.direction, fminf(h.hit.distance, h.ring.distance), air_rng,
+                                    st.accumulated, NoGrad(), f3s(0.0f), defer);
     }
     f3 transmittance = camera_air_transmittance(sc, origin, st.ray.direction, limit, st.rng,
                                                 G::on ? &column : nullptr);

@@ -45,6 +45,10 @@ constexpr uint32_t kTraceBlock = 256;
 #endif
 #define SG_WAVES(n) __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(n, 8)))
 
+// Regions of the deferred camera-air shadow-ray queue, each with its own
+// counter, so that the queue's atomics spread over as many addresses.
+constexpr uint32_t kAirRegions = 64;
+
 struct TraceArgs {
     float4 *accumulation;
     float4 *backdrop;
@@ -76,6 +80,18 @@ struct WaveArgs {
     float4 *h0;        // hit.distance, hit.index bits, disc_scale, ring.distance
     float4 *h1;        // hit.normal.xyz, ring.index bits
     float4 *h2;        // ring.point.xyz
+    // Camera-air shadow rays whose landscape predicates k_air_occlusion
+    // answers: records (origin.xyz, light distance), (direction.xyz, next
+    // record of the same path), (contribution.xyz, predicates owed) in
+    // air_a/b/c; per path, the throughput the contributions are scaled by
+    // and the path's last record in air_pending. A wave pushes into region
+    // blockIdx.x % kAirRegions: records [j * air_capacity, (j + 1) *
+    // air_capacity) counted in air_count[j]. Null when the frame has no
+    // landscape or no air.
+    float4 *air_a, *air_b, *air_c;
+    float4 *air_pending;
+    uint32_t *air_count;
+    uint32_t air_capacity;
     // Per-path results.
     float4 *value;
     float4 *backdrop;
@@ -171,6 +187,8 @@ struct DisplayArgs {
     __global__ void k_generate##S(Scene sc, WaveArgs wa, uint32_t *queue);                          \
     __global__ void k_extend##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);          \
     __global__ void k_camera_air##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);      \
+    __global__ void k_air_occlusion##S(Scene sc, WaveArgs wa);                                       \
+    __global__ void k_air_resolve##S(WaveArgs wa, const uint32_t *queue, uint32_t n);               \
     __global__ void k_shade##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n, uint32_t *next, \
                                uint32_t *next_count);                                              \
     __global__ void k_tail##S(Scene sc, WaveArgs wa, const uint32_t *queue, uint32_t n);            \

@@ -41,16 +41,21 @@ struct ForwardKernels {
     decltype(&k_generate) generate;
     decltype(&k_extend) extend;
     decltype(&k_camera_air) camera_air;
+    decltype(&k_air_occlusion) air_occlusion;
+    decltype(&k_air_resolve) air_resolve;
     decltype(&k_shade) shade;
     decltype(&k_tail) tail;
     decltype(&k_accumulate) accumulate;
 };
 const ForwardKernels kLandscapeKernels{k_trace, k_batch<64>, k_batch<128>, k_batch<256>, k_generate, k_extend,
-                                       k_camera_air, k_shade, k_tail, k_accumulate};
+                                       k_camera_air, k_air_occlusion, k_air_resolve, k_shade, k_tail,
+                                       k_accumulate};
 const ForwardKernels kPlainKernels{k_trace_nl, k_batch_nl<64>, k_batch_nl<128>, k_batch_nl<256>, k_generate_nl,
-                                   k_extend_nl, k_camera_air_nl, k_shade_nl, k_tail_nl, k_accumulate_nl};
+                                   k_extend_nl, k_camera_air_nl, k_air_occlusion_nl, k_air_resolve_nl, k_shade_nl,
+                                   k_tail_nl, k_accumulate_nl};
 const ForwardKernels kStyledKernels{k_trace_st, k_batch_st<64>, k_batch_st<128>, k_batch_st<256>, k_generate_st,
-                                    k_extend_st, k_camera_air_st, k_shade_st, k_tail_st, k_accumulate_st};
+                                    k_extend_st, k_camera_air_st, k_air_occlusion_st, k_air_resolve_st, k_shade_st,
+                                    k_tail_st, k_accumulate_st};
 const ForwardKernels &forward_kernels(bool landscape, bool styled) {
     return landscape ? kLandscapeKernels : (styled ? kStyledKernels : kPlainKernels);
 }
@@ -117,6 +122,13 @@ struct sg_ctx {
     uint32_t *queues[2] = {nullptr, nullptr};
     uint32_t *counter = nullptr;
     uint32_t *counter_host = nullptr;
+    // Deferred landscape queries of camera-air shadow rays (frames with a
+    // ground scene and air): three float4 per record, a pending float4 per
+    // path, air_per_path records per path of the chunk.
+    void *air_block = nullptr;
+    size_t air_records = 0;
+    uint32_t air_paths = 0;
+    uint32_t air_per_path = 8;
     // Measurement
     bool profiling = false;
     hipEvent_t ev_start = nullptr, ev_stop = nullptr;
@@ -373,6 +385,10 @@ int ensure_objects(sg_ctx *ctx, uint32_t count) {
     return SG_OK;
 }
 
+// First slot of the deferred-record counts (one per queue region) in the
+// counter block.
+constexpr uint32_t kAirCountSlot = 64u;
+
 int ensure_wave(sg_ctx *ctx, uint32_t paths) {
     if (paths <= ctx->wave_capacity && ctx->wave_block) return SG_OK;
     if (ctx->wave_block) SG_HIP(ctx, hipFree(ctx->wave_block));
@@ -402,11 +418,33 @@ int ensure_wave(sg_ctx *ctx, uint32_t paths) {
     w.value = reinterpret_cast<float4 *>(take(16));
     w.backdrop = reinterpret_cast<float4 *>(take(16));
     w.rng = reinterpret_cast<uint32_t *>(take(4));
-    SG_HIP(ctx, hipMalloc(&ctx->queues[0], (2 * n + 64) * sizeof(uint32_t)));
+    SG_HIP(ctx, hipMalloc(&ctx->queues[0], (2 * n + kAirCountSlot + kAirRegions) * sizeof(uint32_t)));
     ctx->queues[1] = ctx->queues[0] + n;
     ctx->counter = ctx->queues[0] + 2 * n;
-    if (!ctx->counter_host) SG_HIP(ctx, hipHostMalloc(&ctx->counter_host, 64, hipHostMallocDefault));
+    if (!ctx->counter_host)
+        SG_HIP(ctx, hipHostMalloc(&ctx->counter_host, (kAirCountSlot + kAirRegions) * sizeof(uint32_t),
+                                  hipHostMallocDefault));
     ctx->wave_capacity = paths;
+    return SG_OK;
+}
+
+
+// Record storage for `paths` paths at ctx->air_per_path records each. A path
+// at the ground of the 4K Halo frame averages about two; a chunk that pushes
+// more grows the storage and runs again (encode_wavefront).
+int ensure_air(sg_ctx *ctx, uint32_t paths) {
+    const size_t per_region = ((size_t)paths * ctx->air_per_path + kAirRegions - 1) / kAirRegions;
+    const size_t records = per_region * kAirRegions;
+    if (records > 0xFFFFFFF0ull)  // record indices and the list's end mark are 32-bit
+        return fail(ctx, SG_E_CAPACITY, "camera-air shadow-ray queue over 2^32 records; lower the queue size");
+    if (paths <= ctx->air_paths && records <= ctx->air_records && ctx->air_block) return SG_OK;
+    if (ctx->air_block) SG_HIP(ctx, hipFree(ctx->air_block));
+    ctx->air_block = nullptr;
+    ctx->air_paths = 0;
+    ctx->air_records = 0;
+    SG_HIP(ctx, hipMalloc(&ctx->air_block, records * 48 + (size_t)paths * 16));
+    ctx->air_paths = paths;
+    ctx->air_records = records;
     return SG_OK;
 }
 
@@ -428,16 +466,45 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
     wa.sequence = ta.sequence;
     wa.has_air = ta.has_air;
     wa.records = ta.records;
+    // Camera-air shadow rays of a ground-scene frame (the kernels compiled
+    // with the landscape) queue their landscape predicates for
+    // k_air_occlusion, which answers them together, coherently, instead of
+    // in the divergent camera-air kernel.
+    const bool defer_air = ta.has_air && &fk == &kLandscapeKernels;
+    uint32_t air_records = 0;
+    auto bind_air = [&]() -> int {
+        if (!defer_air) {
+            wa.air_a = wa.air_b = wa.air_c = wa.air_pending = nullptr;
+            wa.air_count = nullptr;
+            wa.air_capacity = 0;
+            return SG_OK;
+        }
+        const int arc = ensure_air(ctx, capacity);
+        if (arc != SG_OK) return arc;
+        const size_t records = ctx->air_records;
+        float4 *base = static_cast<float4 *>(ctx->air_block);
+        wa.air_a = base;
+        wa.air_b = base + records;
+        wa.air_c = base + 2 * records;
+        wa.air_pending = base + 3 * records;
+        wa.air_count = ctx->counter + kAirCountSlot;
+        air_records = (uint32_t)(records / kAirRegions);
+        wa.air_capacity = air_records;
+        return SG_OK;
+    };
+    rc = bind_air();
+    if (rc != SG_OK) return rc;
     // One wave per workgroup: the path kernels share nothing across waves,
     // and a small workgroup frees its slot as soon as its one wave ends
     // (7% faster than 256 lanes on the 4K Halo pass).
     const uint32_t block = 64;
-    for (uint64_t p0 = 0; p0 < pixels; p0 += chunk_pixels) {
+    for (uint64_t p0 = 0; p0 < pixels;) {
         const uint32_t chunk = (uint32_t)std::min<uint64_t>(chunk_pixels, pixels - p0);
         wa.pixel_begin = (uint32_t)p0;
         wa.paths = chunk * count;
         uint32_t n = wa.paths;
         int q = 0;
+        bool redo = false;
         hipLaunchKernelGGL(fk.generate, dim3((n + block - 1) / block), dim3(block), 0, ctx->stream, sc, wa,
                            ctx->queues[q]);
         SG_HIP(ctx, hipGetLastError());
@@ -450,14 +517,26 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
             }
             const dim3 grid((n + block - 1) / block);
             const bool air = iteration == 0 && ta.has_air;
+            const bool deferred = air && defer_air;
             hipEvent_t *ev = ctx->ev_step;
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
             hipLaunchKernelGGL(fk.extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
             SG_HIP(ctx, hipGetLastError());
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
             if (air) {
+                if (deferred)
+                    SG_HIP(ctx, hipMemsetAsync(wa.air_count, 0, kAirRegions * sizeof(uint32_t), ctx->stream));
                 hipLaunchKernelGGL(fk.camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
                 SG_HIP(ctx, hipGetLastError());
+                if (deferred) {
+                    // Per region, grid-stride over the records queued there
+                    // (at most air_records).
+                    const dim3 ogrid(std::min<uint32_t>((air_records + block - 1) / block, 256u), kAirRegions);
+                    hipLaunchKernelGGL(fk.air_occlusion, ogrid, dim3(block), 0, ctx->stream, sc, wa);
+                    SG_HIP(ctx, hipGetLastError());
+                    hipLaunchKernelGGL(fk.air_resolve, grid, dim3(block), 0, ctx->stream, wa, ctx->queues[q], n);
+                    SG_HIP(ctx, hipGetLastError());
+                }
             }
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
             SG_HIP(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(uint32_t), ctx->stream));
@@ -465,9 +544,31 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
                                ctx->queues[q ^ 1], ctx->counter);
             SG_HIP(ctx, hipGetLastError());
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
-            SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, sizeof(uint32_t), hipMemcpyDeviceToHost,
-                                       ctx->stream));
+            const uint32_t words = deferred ? kAirCountSlot + kAirRegions : 1;
+            SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, words * sizeof(uint32_t),
+                                       hipMemcpyDeviceToHost, ctx->stream));
             SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+            uint64_t fullest = 0;
+            if (deferred)
+                for (uint32_t j = 0; j < kAirRegions; ++j)
+                    fullest = std::max<uint64_t>(fullest, ctx->counter_host[kAirCountSlot + j]);
+            if (fullest > air_records) {
+                // Pushes past a region's end were dropped: grow the queue and
+                // run the chunk again. Every kernel before k_accumulate
+                // rewrites what it leaves, and the pushes land in the same
+                // regions, so the second run fits and its results are those
+                // of a run that never overflowed.
+                const uint64_t need = (fullest * kAirRegions + capacity - 1) / capacity;
+                const uint64_t per_path = need + need / 4 + 1;
+                if (per_path * capacity > 0xFFFFFFF0ull - kAirRegions)
+                    return fail(ctx, SG_E_CAPACITY, "camera-air shadow rays exceed the deferral queue");
+                ctx->air_per_path = (uint32_t)per_path;
+                SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
+                rc = bind_air();
+                if (rc != SG_OK) return rc;
+                redo = true;
+                break;
+            }
             if (ctx->profiling) {
                 float ms[3] = {0.f, 0.f, 0.f};
                 for (int k = 0; k < 3; ++k) SG_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
@@ -483,9 +584,11 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
             n = *ctx->counter_host;
             q ^= 1;
         }
+        if (redo) continue;
         hipLaunchKernelGGL(fk.accumulate, dim3((chunk + block - 1) / block), dim3(block), 0, ctx->stream, wa,
                            ta.accumulation, ta.backdrop, ta.moments, chunk);
         SG_HIP(ctx, hipGetLastError());
+        p0 += chunk_pixels;
     }
     return SG_OK;
 }
@@ -755,7 +858,7 @@ int sg_destroy(sg_ctx *ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
-                       ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->bvh_nodes,
+                       ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->air_block, ctx->bvh_nodes,
                        ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
                        ctx->adj_gradient, ctx->adj_primal, ctx->records, ctx->exposure, ctx->histogram};
     if (ctx->counter_host) (void)hipHostFree(ctx->counter_host);
@@ -1060,6 +1163,20 @@ int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths) {
 int sg_set_tail(sg_ctx *ctx, uint32_t tail_paths) {
     if (!ctx) return SG_E_INVALID;
     ctx->tail_paths = tail_paths;
+    return SG_OK;
+}
+
+int sg_set_air_queue(sg_ctx *ctx, uint32_t records_per_path) {
+    if (!ctx) return SG_E_INVALID;
+    if (records_per_path < 1u || records_per_path > 256u)
+        return fail(ctx, SG_E_INVALID, "records per path must be 1-256");
+    ctx->air_per_path = records_per_path;
+    return SG_OK;
+}
+
+int sg_get_air_queue(const sg_ctx *ctx, uint32_t *records_per_path) {
+    if (!ctx || !records_per_path) return SG_E_INVALID;
+    *records_per_path = ctx->air_per_path;
     return SG_OK;
 }
 

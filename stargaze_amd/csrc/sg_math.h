@@ -23,6 +23,20 @@
 #define SG_STYLES 1
 #endif
 
+// Event hooks of the work model: host builds of the test harness define
+// SG_COUNT_EVENTS and sg_count_event to count traversal and shading events;
+// everywhere else SG_COUNT compiles to nothing.
+enum SgEvent : uint32_t {
+    SG_EV_SPHERE, SG_EV_RING, SG_EV_LANDSCAPE, SG_EV_TERRAIN_BOX, SG_EV_TRIANGLE, SG_EV_BVH_BOX, SG_EV_PRIM,
+    SG_EV_SHADOW, SG_EV_DENSITY_STEP, SG_EV_VIEW_CELL, SG_EV_LIGHT_SAMPLE, SG_EV_COUNT
+};
+#ifdef SG_COUNT_EVENTS
+void sg_count_event(uint32_t event);
+#define SG_COUNT(e) sg_count_event(e)
+#else
+#define SG_COUNT(e) ((void)0)
+#endif
+
 template <class T> struct V3 { T x, y, z; };
 typedef V3<float> f3;
 typedef V3<double> d3;

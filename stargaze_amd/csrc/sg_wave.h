@@ -61,6 +61,37 @@ static __device__ __forceinline__ void load_hit(const WaveArgs &wa, uint32_t id,
     h.ring.index = __float_as_uint(b.w);
     h.ring.point = f3{c.x, c.y, c.z};
 }
+// The deferral sink of the camera-air kernel: queues shadow rays whose
+// landscape predicates are owed, one atomic per wavefront (ballot and
+// prefix of the lanes that push at this call) on its region's counter, and
+// links each path's records through `head` (latest first). The counters
+// count every push; pushes at or past `capacity` in a region are dropped,
+// and the host, seeing a count above `capacity`, grows the queue and runs
+// the chunk again from its first kernel.
+struct AirDefer {
+    static constexpr bool on = true;
+    float4 *a, *b, *c;
+    uint32_t *count;
+    uint32_t capacity;
+    uint32_t head;
+    __device__ void push(f3 origin, f3 direction, float limit, f3 term, uint32_t owed) {
+        const uint64_t mask = __ballot(1);
+        const uint32_t lane = __lane_id();
+        const uint32_t leader = (uint32_t)__ffsll((unsigned long long)mask) - 1u;
+        uint32_t base = 0;
+        const uint32_t region = blockIdx.x % kAirRegions;
+        if (lane == leader) base = atomicAdd(count + region, (uint32_t)__popcll(mask));
+        base = __shfl(base, (int)leader);
+        const uint32_t slot = base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+        if (slot >= capacity) return;
+        const uint32_t r = region * capacity + slot;
+        a[r] = make_float4(origin.x, origin.y, origin.z, limit);
+        b[r] = make_float4(direction.x, direction.y, direction.z, __uint_as_float(head));
+        c[r] = make_float4(term.x, term.y, term.z, __uint_as_float(owed));
+        head = r;
+    }
+};
+
 static __device__ __forceinline__ void finish(const WaveArgs &wa, uint32_t id, const PathState &st) {
     PathResult r = path_finish(st);
     wa.value[id] = r.value;

@@ -304,6 +304,18 @@ class PathTracer:
         self._check(self._lib.sg_set_tail(self._ctx, int(tail_paths)), "set_tail")
 
     @property
+    def air_queue(self) -> int:
+        """Records per path of the deferred camera-air shadow-ray queue."""
+        n = ctypes.c_uint32()
+        self._check(self._lib.sg_get_air_queue(self._ctx, ctypes.byref(n)), "get_air_queue")
+        return int(n.value)
+
+    def set_air_queue(self, records_per_path: int) -> None:
+        """Size (records per path, 1-256) of the deferred camera-air
+        shadow-ray queue; a chunk that overflows it grows it and runs again."""
+        self._check(self._lib.sg_set_air_queue(self._ctx, int(records_per_path)), "set_air_queue")
+
+    @property
     def launch(self) -> dict:
         l = _native.SgLaunch()
         self._check(self._lib.sg_get_launch(self._ctx, ctypes.byref(l)), "get_launch")

@@ -69,12 +69,15 @@ def render_reference(out_dir: Path = HERE) -> None:
     print(f"wrote {out_dir / OUT.name}")
 
 
-TARGET_SEEDS = (0, 1, 2, 3)
+# Noise seeds far apart: a pass of n samples draws sequences seed .. seed + n - 1,
+# so nearby seeds would share most of their draws.
+TARGET_SEEDS = tuple(k * 1_000_003 for k in range(8))
 
 
 def oracle_target() -> None:
     """The FP32 oracle's expected linear-RGB RMSE against the reference at
-    TARGET_SPP spp: the root mean square of its RMSE over four noise seeds.
+    TARGET_SPP spp: the root mean square of its RMSE over eight independent
+    noise seeds.
     This is the fixed RMSE the benchmark's time-to-RMSE converges to."""
     import oracle
     from stargaze_amd import scenes

@@ -324,6 +324,32 @@ def test_wavefront_and_fused_integrators_agree(scene):
     assert np.array_equal(images[1], images[2])
 
 
+def test_deferred_air_queue_overflow_reruns_the_chunk():
+    """Camera-air shadow rays whose landscape query the wavefront integrator
+    defers to a kernel of its own: a queue too small for a chunk's pushes
+    grows and the chunk runs again, so the image is bit-identical to one
+    rendered with room to spare, and agrees with the fused integrator's,
+    which answers every query inline (to the last bits of the sums)."""
+    size = (48, 27)
+    frame, data, cells = scenes.halo_parity(*size)
+    images, queues = [], []
+    for integrator, queue in ((ptm.INTEGRATOR_FUSED, None), (ptm.INTEGRATOR_WAVEFRONT, 64),
+                              (ptm.INTEGRATOR_WAVEFRONT, 1)):
+        t = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=3, milky_way=0.0))
+        t.upload_terrain(data, cells)
+        t.set_integrator(integrator, 0)
+        if queue is not None:
+            t.set_air_queue(queue)
+        t.encode(frame)
+        t.encode(frame)
+        images.append(t.read_accumulation())
+        queues.append(t.air_queue)
+        t.close()
+    assert queues[1] == 64 and queues[2] > 1, queues
+    np.testing.assert_allclose(images[1], images[0], rtol=1e-5, atol=1e-9)
+    assert np.array_equal(images[1], images[2])
+
+
 @pytest.mark.parametrize("scene", ["ringed_giant", "halo"])
 def test_wavefront_tail_and_autotune(scene):
     """The fused tail below the compaction threshold changes no draw: images

@@ -27,6 +27,16 @@ FLAGS = [
     "-fPIC",
     # Keep the compiler from forming packed-FP32 instructions.
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # FP32 division and square root to within a few ULP (v_rcp_f32 /
+    # v_sqrt_f32 based), the accuracy WGSL promises the reference's shaders
+    # (2.5 ULP division), instead of correctly rounded sequences of about
+    # ten instructions each: 10% of the 4K Halo pass.
+    "-fno-hip-fp32-correctly-rounded-divide-sqrt",
+    # No multiply-add contraction beyond explicit fmaf: a step function then
+    # rounds the same in every kernel it is inlined into, so the fused,
+    # wavefront, tail and batch kernels give bit-identical images (2% of the
+    # 4K Halo pass).
+    "-ffp-contract=off",
     "-I", str(PACKAGE.parent / "include"),
 ]
 

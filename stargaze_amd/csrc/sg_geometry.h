@@ -198,9 +198,14 @@ SG_FN float4 grass_hit(f3 p, f3 d, f3 e) {
 }
 
 // --- terrain quadtree (src/shaders/terrain.wgsl:6-234) -----------------------
-SG_FN float terrain_scale(const Scene &sc) { return clampf(sc.s.globals.ground_up[3] / 200000.0f, 0.0001f, 1.0f); }
+// Correctly rounded, as the host computes it: grid edges and ray origins
+// placed on them must agree to the bit.
+SG_FN float terrain_scale(const Scene &sc) { return clampf(div_rn(sc.s.globals.ground_up[3], 200000.0f), 0.0001f, 1.0f); }
 SG_FN float terrain_coordinate(const Scene &sc, uint32_t i) {
-    float u = 2.0f * (float)i / (float)sc.t.cells - 1.0f;
+    // 2 i / cells - 1, exactly: cells is a power of two, so the quotient is
+    // a scaling by a power of two (grid edges land on +-1 under any
+    // division accuracy).
+    float u = ldexpf((float)(2u * i), -(int)__builtin_ctz(sc.t.cells)) - 1.0f;
     return u * fabsf(u) * 16000.0f * terrain_scale(sc);
 }
 SG_FN float terrain_curvature(const Scene &sc, float px, float pz) {

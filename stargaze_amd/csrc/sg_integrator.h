@@ -532,8 +532,8 @@ SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps,
             sum += sc.density_profile[bin];
             if (trace && i < COLUMN_BIN_SLOTS) trace->bins |= (uint64_t)bin << (4u * i);
         } else {
-            sum += expf(-h * inv_h);
-            if (split) sum_m += expf(-h * inv_hm);
+            sum += sg_exp(-h * inv_h);
+            if (split) sum_m += sg_exp(-h * inv_hm);
         }
     }
     if (trace) {
@@ -736,7 +736,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             bin = density_bin(sc, altitude);
             density = sc.density_profile[bin];
         } else {
-            density = expf(-altitude / H);
+            density = sg_exp(-altitude / H);
         }
         float column = density * dt;
         f3 tau, step_t, segment;
@@ -747,7 +747,7 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
         // and the scales are the two densities.
         float scale_r = 1.0f, scale_m = 1.0f;
         if (split) {
-            const float density_m = expf(-altitude * inv_hm);
+            const float density_m = sg_exp(-altitude * inv_hm);
             tau = beta_r * column + f3s(beta_m * (density_m * dt));
             step_t = exp3(-tau);
             segment.x = tau.x < 1.0e-3f ? dt * (1.0f - 0.5f * tau.x) : (1.0f - step_t.x) * dt / tau.x;
@@ -912,8 +912,8 @@ SG_FN f3 camera_air_transmittance(const Scene &sc, f3 origin, f3 dir, float limi
             column = sc.density_profile[bin] * dt;
             if (i < COLUMN_BIN_SLOTS) bins |= (uint64_t)bin << (4u * i);
         } else {
-            column = expf(-altitude / H) * dt;
-            if (split) column_m = expf(-altitude * inv_hm) * dt;
+            column = sg_exp(-altitude / H) * dt;
+            if (split) column_m = sg_exp(-altitude * inv_hm) * dt;
         }
         transmittance *= exp3(-air_depth(sc, column, column_m));
         total += column;

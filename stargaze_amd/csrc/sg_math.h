@@ -77,7 +77,23 @@ SG_FN f3 xyz(float4 a) { return {a.x, a.y, a.z}; }
 SG_FN f3 fabs3(f3 a) { return {fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
 SG_FN f3 fmin3(f3 a, f3 b) { return {fminf(a.x, b.x), fminf(a.y, b.y), fminf(a.z, b.z)}; }
 SG_FN f3 fmax3(f3 a, f3 b) { return {fmaxf(a.x, b.x), fmaxf(a.y, b.y), fmaxf(a.z, b.z)}; }
-SG_FN f3 exp3(f3 a) { return {expf(a.x), expf(a.y), expf(a.z)}; }
+// e^x to the accuracy WGSL's exp promises (3 + 2|x| ULP), which the
+// reference's shaders are written against: on the GPU one v_exp_f32 of
+// x log2(e) (about 1 + 1.5|x| ULP), instead of the library's extended-
+// precision reduction. Host builds use expf.
+SG_FN float sg_exp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __expf(x);
+#else
+    return expf(x);
+#endif
+}
+SG_FN f3 exp3(f3 a) { return {sg_exp(a.x), sg_exp(a.y), sg_exp(a.z)}; }
+// a / b correctly rounded whatever the FP32 division accuracy of the build:
+// the FP64 quotient rounded to FP32 (double rounding is innocuous for
+// division, 53 >= 2 * 24 + 2 bits). For quotients that must equal the
+// host's bit for bit.
+SG_FN float div_rn(float a, float b) { return (float)((double)a / (double)b); }
 SG_FN float maxc(f3 a) { return fmaxf(a.x, fmaxf(a.y, a.z)); }
 SG_FN float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 SG_FN f3 clamp3(f3 a, float lo, float hi) { return {clampf(a.x, lo, hi), clampf(a.y, lo, hi), clampf(a.z, lo, hi)}; }

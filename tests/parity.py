@@ -58,3 +58,22 @@ def assert_tiles(z, max_fraction_over_3=0.01, max_z=6.0):
     worst = float(np.max(z))
     assert over <= max_fraction_over_3 and worst <= max_z, f"tiles over 3 sigma: {over:.4f}, worst z {worst:.2f}"
     return over, worst
+
+
+def assert_same_draws(actual, expected, exact=True):
+    """Two kernels that run the same step functions on the same draws (fused
+    vs wavefront, tail vs compaction, batched vs per view) give the same
+    image. The library is built without multiply-add contraction, so the
+    steps round alike in every kernel and the images are bit-identical.
+    With ``exact=False`` (a ground scene with air, where the wavefront
+    integrator adds the camera-air contributions of deferred shadow rays in
+    a sum of their own) they agree to 1e-5 relative, or 1e-7 of the image's
+    peak for faint pixels."""
+    actual, expected = np.asarray(actual), np.asarray(expected)
+    if exact:
+        differ = actual != expected
+        assert not differ.any(), (
+            f"{int(differ.sum())} of {differ.size} values differ, max relative "
+            f"{float((np.abs(actual - expected) / np.maximum(np.abs(expected), 1e-30)).max()):.3g}")
+        return
+    np.testing.assert_allclose(actual, expected, rtol=1e-5, atol=1e-7 * float(np.abs(expected).max()))

@@ -41,9 +41,10 @@ def compare(views, a, b, label):
         scale = max(1e-30, float(np.abs(y).max()))
         diff = max(float(np.abs(x - y).max()), float(np.abs(bx - by).max())) / scale
         worst = max(worst, diff)
-        # Same code and RNG: identical up to the compiler's fused multiply-add
-        # choices, which may differ between the two kernels.
-        assert diff <= 1e-5, f"{label}: view {i} ({v['kind']} {v['width']}x{v['height']} spp {v['spp']}): {diff:.3e}"
+        # Same code and RNG, built without multiply-add contraction: the
+        # step functions round alike in both kernels, so the images are
+        # bit-identical.
+        assert diff == 0.0, f"{label}: view {i} ({v['kind']} {v['width']}x{v['height']} spp {v['spp']}): {diff:.3e}"
     print(f"{label}: {len(views)} views, worst relative difference {worst:.3e}", flush=True)
 
 

@@ -92,9 +92,8 @@ def test_style_codes_above_six_are_terran():
 
 def test_styled_kernels_agree_across_integrators_and_batch():
     """Frames with solid-body styles take the kernels built with them, in
-    every form: fused, wavefront and batched passes give the same
-    image to 1e-5 relative (the kernels differ in the compiler's fused
-    multiply-add choices). A batch with one styled view runs all its views
+    every form: fused, wavefront and batched passes give bit-identical
+    images (parity.assert_same_draws). A batch with one styled view runs all its views
     through the styled batch kernel."""
     from stargaze_amd.pathtracer import INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT, Batch
 
@@ -107,7 +106,7 @@ def test_styled_kernels_agree_across_integrators_and_batch():
         t.encode(frame)
         t.encode(frame)
         images.append(t.read_accumulation())
-    np.testing.assert_allclose(images[1], images[0], rtol=1e-5, atol=1e-9)
+    parity.assert_same_draws(images[1], images[0])
     views = [PathTracer(size, options=Options(samples_per_frame=8, max_bounces=4, milky_way=0.0)) for _ in range(2)]
     plain = scenes.airless_triple(*size)
     batch = Batch(views)
@@ -116,12 +115,12 @@ def test_styled_kernels_agree_across_integrators_and_batch():
     batch.synchronize()
     batched = [v.read_accumulation() for v in views]
     batch.close()
-    np.testing.assert_allclose(batched[0], images[0], rtol=1e-5, atol=1e-9)
+    parity.assert_same_draws(batched[0], images[0])
     ref = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=4, milky_way=0.0))
     ref.set_integrator(INTEGRATOR_FUSED)
     ref.encode(plain)
     ref.encode(plain)
-    np.testing.assert_allclose(batched[1], ref.read_accumulation(), rtol=1e-5, atol=1e-9)
+    parity.assert_same_draws(batched[1], ref.read_accumulation())
 
 
 def test_adjoint_rejects_solid_styles():

@@ -57,10 +57,10 @@ FLOP_COST = {
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32, dense
 PEAK_HBM_GBS = 8000.0
 # HBM bytes per launch of the camera-air kernel on the Halo 4K pass (2^24
-# paths), from rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (the factors of a 1 GiB
-# copy); not measured by this script, since the counters need a profiler
-# run of their own.
-TRAFFIC_CAMERA_AIR_PER_LAUNCH = {"halo_4k": (28.80e9 * 2 + 56.11e9) / 16}
+# paths): rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE (the factors of 1 GiB copies),
+# in KiB summed over the pass's 16 launches; not measured by this script,
+# since the counters need a profiler run of their own.
+TRAFFIC_CAMERA_AIR_PER_LAUNCH = {"halo_4k": (19014888 * 2 + 49072105) * 1024 / 16}
 TRAFFIC_SOURCE = "profiles/halo_4k_kernels.md (rocprofv3 FETCH_SIZE x 2 + WRITE_SIZE per k_camera_air launch)"
 # Path state the camera-air step must read and write per path: ray origin
 # (FP64) and direction, the hit record, throughput, radiance and the RNG.
@@ -107,6 +107,9 @@ def oracle_tracer(width, height, spp, bounces, terrain, profile=None, air=None):
 
 
 PROP_EVENTS = ("box", "rock", "vegetation")
+# Terrain work of the camera-air step's shadow rays: the wavefront integrator
+# runs it in k_air_occlusion, not in k_camera_air.
+LANDSCAPE_EVENTS = ("terrain_node", "triangle")
 
 
 def work_model(frame, terrain, profile, air, width, height, spp, bounces, rows=16):
@@ -139,18 +142,23 @@ def work_model(frame, terrain, profile, air, width, height, spp, bounces, rows=1
 
     whole = {k: rest[k] + cam[k] for k in rest}
     per, flops, props = price(whole)
-    _, flops_air, props_air = price(cam)
-    return {"per_event": per, "flop": flops, "flop_flat_loop_props": props, "flop_camera_air": flops_air,
+    cam_per, flops_air, props_air = price(cam)
+    landscape = sum(cam_per[k] * FLOP_COST[k] for k in LANDSCAPE_EVENTS)
+    return {"per_event": per, "flop": flops, "flop_flat_loop_props": props,
+            "flop_camera_air": flops_air - landscape, "flop_air_occlusion": landscape,
             "flop_camera_air_flat_loop_props": props_air}
 
 
-def roofline(workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, air_ms, air_launches, launch):
+def roofline(workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, air_ms, air_launches, launch,
+             occlusion=(0.0, 0)):
     """FP32 vector roofline of the dominant kernel (nothing on the path is a
     dense contraction, so no MFMA instruction is issued): the camera-air
-    step of the wavefront integrator when the frame has air, else the whole
-    pass. `achieved` is the step's algorithmic FLOP (oracle event counts of
-    that step per path sample x the paths of one launch) over its average
-    launch time from HIP events on the tracer's stream."""
+    kernel of the wavefront integrator when the frame has air, else the
+    whole pass. `achieved` is the kernel's algorithmic FLOP (oracle event
+    counts of the camera-air step per path sample, less the terrain work
+    of its shadow rays, which k_air_occlusion does, x the paths of one
+    launch) over its average launch time from HIP events on the tracer's
+    stream."""
     whole = {"kernel": "all kernels of one pass", "ms": round(kernel_s * 1e3, 3),
              "flop_per_path_sample": round(wm["flop"], 1), "achieved_tflops": round(pass_tflops, 3),
              "frac": round(pass_tflops / PEAK_FP32_TFLOPS, 4),
@@ -176,6 +184,11 @@ def roofline(workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, 
         "traffic_source": TRAFFIC_SOURCE if workload in TRAFFIC_CAMERA_AIR_PER_LAUNCH else None,
         "kernel": "k_camera_air (camera-air step of the wavefront integrator)",
         "kernel_ms_per_launch": round(per_launch_ms, 3),
+        "air_occlusion": {
+            "kernel": "k_air_occlusion + k_air_resolve (deferred landscape predicates of the shadow rays)",
+            "ms_per_launch": round(occlusion[0] / occlusion[1], 3) if occlusion[1] else None,
+            "flop_per_path_sample": round(wm.get("flop_air_occlusion", float("nan")), 1),
+        },
         "paths_per_launch": paths_per_launch,
         "flop_per_path_sample": round(wm["flop_camera_air"], 1),
         "flat_loop_prop_flop_per_path_sample_not_counted": round(wm["flop_camera_air_flat_loop_props"], 1),
@@ -543,16 +556,17 @@ def main() -> int:
     tracer.synchronize()
     elapsed = time.perf_counter() - t0
     kernel_ms, launches = tracer.kernel_time()
-    from stargaze_amd.pathtracer import STEP_CAMERA_AIR
+    from stargaze_amd.pathtracer import STEP_AIR_OCCLUSION, STEP_CAMERA_AIR
 
     air_ms, air_launches = tracer.step_time(STEP_CAMERA_AIR)
+    occlusion = tracer.step_time(STEP_AIR_OCCLUSION)
     img = tracer.read_accumulation()
     assert np.all(np.isfinite(img)), "non-finite radiance"
     samples_per_step = width * height * spp
     value = samples_per_step * args.steps / elapsed / 1e6
     if args.no_work_model:
         wm = {"flop": float("nan"), "flop_camera_air": float("nan"), "flop_flat_loop_props": float("nan"),
-              "flop_camera_air_flat_loop_props": float("nan")}
+              "flop_camera_air_flat_loop_props": float("nan"), "flop_air_occlusion": float("nan")}
     else:
         log("work model")
         wm = work_model(frame, terrain, profile, air, width, height, spp, bounces)
@@ -590,7 +604,7 @@ def main() -> int:
         "config": config,
         "launch": launch if launch is not None else tracer.launch,
         "roofline": roofline(args.workload, wm, samples_per_step, kernel_s, pass_tflops, compulsory, air_ms,
-                             air_launches, tracer.launch),
+                             air_launches, tracer.launch, occlusion),
     }
     if args.batch_steps > 0:
         line["batch"] = []

@@ -239,11 +239,14 @@ int sg_set_profiling(sg_ctx *ctx, int32_t enable);
 int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
 /* Device time of one step of the wavefront integrator since sg_set_profiling
  * (HIP events around each launch; no reference counterpart): the extend,
- * camera-air and shade kernels, and how many launches the time covers. */
+ * camera-air and shade kernels, the deferred camera-air shadow rays
+ * (k_air_occlusion and k_air_resolve), and how many launches the time
+ * covers. */
 #define SG_STEP_EXTEND 0u
 #define SG_STEP_CAMERA_AIR 1u
 #define SG_STEP_SHADE 2u
-#define SG_STEP_COUNT 3u
+#define SG_STEP_AIR_OCCLUSION 3u
+#define SG_STEP_COUNT 4u
 int sg_step_time(const sg_ctx *ctx, uint32_t step, double *milliseconds, uint64_t *launches);
 /* Integrator selection (no reference counterpart: the reference always runs
  * its megakernel). SG_INTEGRATOR_WAVEFRONT (default) runs extend / camera-air

@@ -136,7 +136,7 @@ struct sg_ctx {
     uint64_t launches = 0;
     // Per-step device time of the wavefront kernels (SG_STEP_*), from event
     // pairs around each launch while profiling.
-    hipEvent_t ev_step[4] = {nullptr, nullptr, nullptr, nullptr};
+    hipEvent_t ev_step[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
     double step_ms[SG_STEP_COUNT] = {};
     uint64_t step_launches[SG_STEP_COUNT] = {};
     std::string error;
@@ -519,31 +519,31 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
             const bool air = iteration == 0 && ta.has_air;
             const bool deferred = air && defer_air;
             hipEvent_t *ev = ctx->ev_step;
+            if (deferred) SG_HIP(ctx, hipMemsetAsync(wa.air_count, 0, kAirRegions * sizeof(uint32_t), ctx->stream));
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[0], ctx->stream));
             hipLaunchKernelGGL(fk.extend, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
             SG_HIP(ctx, hipGetLastError());
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[1], ctx->stream));
             if (air) {
-                if (deferred)
-                    SG_HIP(ctx, hipMemsetAsync(wa.air_count, 0, kAirRegions * sizeof(uint32_t), ctx->stream));
                 hipLaunchKernelGGL(fk.camera_air, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n);
                 SG_HIP(ctx, hipGetLastError());
-                if (deferred) {
-                    // Per region, grid-stride over the records queued there
-                    // (at most air_records).
-                    const dim3 ogrid(std::min<uint32_t>((air_records + block - 1) / block, 256u), kAirRegions);
-                    hipLaunchKernelGGL(fk.air_occlusion, ogrid, dim3(block), 0, ctx->stream, sc, wa);
-                    SG_HIP(ctx, hipGetLastError());
-                    hipLaunchKernelGGL(fk.air_resolve, grid, dim3(block), 0, ctx->stream, wa, ctx->queues[q], n);
-                    SG_HIP(ctx, hipGetLastError());
-                }
             }
             if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[2], ctx->stream));
+            if (deferred) {
+                // Per region, grid-stride over the records queued there (at
+                // most air_records).
+                const dim3 ogrid(std::min<uint32_t>((air_records + block - 1) / block, 256u), kAirRegions);
+                hipLaunchKernelGGL(fk.air_occlusion, ogrid, dim3(block), 0, ctx->stream, sc, wa);
+                SG_HIP(ctx, hipGetLastError());
+                hipLaunchKernelGGL(fk.air_resolve, grid, dim3(block), 0, ctx->stream, wa, ctx->queues[q], n);
+                SG_HIP(ctx, hipGetLastError());
+            }
             SG_HIP(ctx, hipMemsetAsync(ctx->counter, 0, sizeof(uint32_t), ctx->stream));
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
             hipLaunchKernelGGL(fk.shade, grid, dim3(block), 0, ctx->stream, sc, wa, ctx->queues[q], n,
                                ctx->queues[q ^ 1], ctx->counter);
             SG_HIP(ctx, hipGetLastError());
-            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[3], ctx->stream));
+            if (ctx->profiling) SG_HIP(ctx, hipEventRecord(ev[4], ctx->stream));
             const uint32_t words = deferred ? kAirCountSlot + kAirRegions : 1;
             SG_HIP(ctx, hipMemcpyAsync(ctx->counter_host, ctx->counter, words * sizeof(uint32_t),
                                        hipMemcpyDeviceToHost, ctx->stream));
@@ -570,15 +570,19 @@ int encode_wavefront(sg_ctx *ctx, const Scene &sc, const TraceArgs &ta) {
                 break;
             }
             if (ctx->profiling) {
-                float ms[3] = {0.f, 0.f, 0.f};
-                for (int k = 0; k < 3; ++k) SG_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
+                float ms[4] = {0.f, 0.f, 0.f, 0.f};
+                for (int k = 0; k < 4; ++k) SG_HIP(ctx, hipEventElapsedTime(&ms[k], ev[k], ev[k + 1]));
                 ctx->step_ms[SG_STEP_EXTEND] += ms[0];
                 ctx->step_launches[SG_STEP_EXTEND] += 1;
                 if (air) {
                     ctx->step_ms[SG_STEP_CAMERA_AIR] += ms[1];
                     ctx->step_launches[SG_STEP_CAMERA_AIR] += 1;
                 }
-                ctx->step_ms[SG_STEP_SHADE] += ms[2];
+                if (deferred) {
+                    ctx->step_ms[SG_STEP_AIR_OCCLUSION] += ms[2];
+                    ctx->step_launches[SG_STEP_AIR_OCCLUSION] += 1;
+                }
+                ctx->step_ms[SG_STEP_SHADE] += ms[3];
                 ctx->step_launches[SG_STEP_SHADE] += 1;
             }
             n = *ctx->counter_host;

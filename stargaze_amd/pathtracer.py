@@ -289,7 +289,7 @@ class PathTracer:
 
     def step_time(self, step: int) -> tuple[float, int]:
         """Device ms and launches of one wavefront step (STEP_EXTEND,
-        STEP_CAMERA_AIR, STEP_SHADE) since set_profiling."""
+        STEP_CAMERA_AIR, STEP_SHADE, STEP_AIR_OCCLUSION) since set_profiling."""
         ms, n = ctypes.c_double(), ctypes.c_uint64()
         self._check(self._lib.sg_step_time(self._ctx, int(step), ctypes.byref(ms), ctypes.byref(n)), "step_time")
         return float(ms.value), int(n.value)
@@ -343,7 +343,7 @@ class PathTracer:
 
 
 INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
-STEP_EXTEND, STEP_CAMERA_AIR, STEP_SHADE = range(3)
+STEP_EXTEND, STEP_CAMERA_AIR, STEP_SHADE, STEP_AIR_OCCLUSION = range(4)
 PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE, PROBE_ALBEDO = range(8)
 
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST, BATCH_ORDER_LONGEST = 0, 1, 2

@@ -350,6 +350,36 @@ def test_deferred_air_queue_overflow_reruns_the_chunk():
     assert np.array_equal(images[1], images[2])
 
 
+def test_step_times_cover_the_wavefront_kernels():
+    """sg_step_time: while profiling, each wavefront step reports device time
+    and launches. Camera air runs once per chunk (its first round), and on a
+    ground scene with air so do the deferred shadow-ray kernels; a frame
+    without air runs neither. The steps fit in the pass's kernel time."""
+    size = (64, 36)
+    frame, data, cells = scenes.halo_parity(*size)
+    steps = (ptm.STEP_EXTEND, ptm.STEP_CAMERA_AIR, ptm.STEP_SHADE, ptm.STEP_AIR_OCCLUSION)
+    for scene, ground in (("halo", True), ("airless_triple", False)):
+        f = frame if ground else scenes.airless_triple(*size)
+        t = PathTracer(size, options=Options(samples_per_frame=8, max_bounces=4, milky_way=0.0))
+        if ground:
+            t.upload_terrain(data, cells)
+        t.set_integrator(ptm.INTEGRATOR_WAVEFRONT, 4096)
+        t.set_profiling(True)
+        t.encode(f)
+        t.encode(f)
+        t.synchronize()
+        chunks = 2 * -(-(size[0] * size[1]) // (4096 // 8))
+        times = {s: t.step_time(s) for s in steps}
+        kernel_ms, _ = t.kernel_time()
+        t.close()
+        print(scene, times, f"kernel {kernel_ms:.3f} ms")
+        assert times[ptm.STEP_EXTEND][1] >= chunks and times[ptm.STEP_SHADE][1] == times[ptm.STEP_EXTEND][1]
+        expected = chunks if ground else 0
+        assert times[ptm.STEP_CAMERA_AIR][1] == expected and times[ptm.STEP_AIR_OCCLUSION][1] == expected
+        assert all((ms > 0.0) == (n > 0) for ms, n in times.values())
+        assert sum(ms for ms, _ in times.values()) <= kernel_ms * 1.01
+
+
 @pytest.mark.parametrize("scene", ["ringed_giant", "halo"])
 def test_wavefront_tail_and_autotune(scene):
     """The fused tail below the compaction threshold changes no draw: images

@@ -362,6 +362,7 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
         batch.encode(frames)
     batch.synchronize()
     lat_b = []
+    batch.set_profiling(True)
     t0 = time.perf_counter()
     for _ in range(steps):
         batch.encode(frames)
@@ -369,6 +370,8 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
         lat_b.append(ms)
     batch.synchronize()
     t_batch = (time.perf_counter() - t0) / steps
+    batch_kernel_ms = batch.kernel_time()[0] / steps
+    batch.set_profiling(False)
     per_view = {}
     for serial in (False, True):
         for _ in range(warmup):
@@ -378,6 +381,17 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
         for _ in range(steps):
             lat.append(batch.encode_views(frames, serial=serial))
         per_view[serial] = ((time.perf_counter() - t0) / steps, np.concatenate(lat))
+    # Device time of the per-view kernels on one queue: the share of the
+    # pass the GPU spends in them bounds what any batching can gain.
+    for t in tracers:
+        t.set_profiling(True)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        batch.encode_views(frames, serial=True)
+    busy_wall = (time.perf_counter() - t0) / steps
+    busy_ms = sum(t.kernel_time()[0] for t in tracers) / steps
+    for t in tracers:
+        t.set_profiling(False)
     lat_b = np.concatenate(lat_b)
     batch.close()
     for t in tracers:
@@ -401,6 +415,9 @@ def batch_leg(steps: int, warmup: int, workload: str = "mixed_96", tune: bool = 
         "per_view_own_streams": form(*per_view[False]),
         "speedup_vs_one_queue": round(per_view[True][0] / t_batch, 3),
         "speedup_vs_own_streams": round(per_view[False][0] / t_batch, 3),
+        "per_view_one_queue_kernel_ms": round(busy_ms, 3),
+        "per_view_one_queue_busy_fraction": round(busy_ms / (busy_wall * 1e3), 4),
+        "batched_kernel_ms": round(batch_kernel_ms, 3),
         "autotune": tuned,
         "views_desc": f"{count} random systems (PCG seed 1): log-uniform {lo}^2..{hi} px, spp {{{spp_set}}} "
         "per pass, 4 bounces; camera 2 m above a host at a random pose, 2-12 bodies, rings on half the "
@@ -415,7 +432,7 @@ ADJOINT_WORKLOADS = {
 }
 
 
-def adjoint_leg(steps: int) -> list:
+def adjoint_leg(steps: int, recover_steps: int = 200) -> list:
     """Adjoint against forward kernel time on the same pass (HIP events on
     the tracer's stream). The adjoint runs each path twice (primal, then
     replay with derivatives)."""
@@ -485,8 +502,35 @@ def adjoint_leg(steps: int) -> list:
                     "adjoint_rerun_primal_ms": round(times["adjoint_rerun"], 3),
                     "ratio_vs_fused": round(times["adjoint"] / times["fused"], 3),
                     "ratio_vs_best_forward": round(times["adjoint"] / best, 3)})
+        if name == "diff_system_512" and recover_steps > 0:
+            out[-1]["recovery"] = recovery_leg(recover_steps)
         log(f"adjoint {name}: {out[-1]}")
     return out
+
+
+def recovery_leg(steps: int) -> dict:
+    """The inverse problem of the differentiable workload: Adam over all 557
+    parameters from the nominal set to a 256-spp image of a set with every
+    parameter scaled by its own factor in [0.9, 1.1] (PCG seed 2)."""
+    from stargaze_amd import diff
+
+    frame = scenes.diff_system(512, 512)
+    nominal, target = scenes.diff_system_parameters(frame)
+    scene = diff.DiffScene(frame, (512, 512), 64, 4, albedo_bodies=scenes.DIFF_ALBEDO_BODIES,
+                           ring_body=scenes.DIFF_RING_BODY, density=True)
+    goal = diff.render_converged(scene, target, passes=4)
+    t0 = time.perf_counter()
+    fitted, losses = diff.recover(scene, goal, nominal, steps=steps, lr=0.01, final_lr=0.001,
+                                  log=lambda it, loss: log(f"adam {it}: loss {loss:.4e}"))
+    wall = time.perf_counter() - t0
+    scene.close()
+    before, after = diff.parameter_errors(nominal, target), diff.parameter_errors(fitted, target)
+    return {"steps": steps, "parameters": 557, "optimizer": "Adam, relative coordinates, lr 0.01 -> 0.001",
+            "target": "256 spp image of the parameters scaled by factors in [0.9, 1.1] (PCG seed 2)",
+            "wall_s": round(wall, 2), "ms_per_step": round(wall / steps * 1e3, 2),
+            "loss_first": losses[0], "loss_last10": float(np.mean(losses[-10:])),
+            "mean_relative_error_before": {k: round(v, 5) for k, v in before.items()},
+            "mean_relative_error_after": {k: round(v, 5) for k, v in after.items()}}
 
 
 def main() -> int:
@@ -503,6 +547,8 @@ def main() -> int:
     ap.add_argument("--no-rmse", action="store_true", help="skip the fixed-RMSE leg")
     ap.add_argument("--batch-steps", type=int, default=3, help="passes of the 96-view batch leg (0 = skip)")
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
+    ap.add_argument("--recover-steps", type=int, default=200,
+                    help="Adam steps of the differentiable workload's inverse problem (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers 4 times (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -613,7 +659,7 @@ def main() -> int:
             line["batch"].append(batch_leg(args.batch_steps, 1, name))
     if args.adjoint_steps > 0:
         log("adjoint leg")
-        line["adjoint"] = adjoint_leg(args.adjoint_steps)
+        line["adjoint"] = adjoint_leg(args.adjoint_steps, args.recover_steps)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(frame, terrain, profile, air, width, height, spp, bounces)

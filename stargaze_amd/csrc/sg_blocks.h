@@ -138,22 +138,28 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(SG_FUSED_WAVE
         const uint32_t first = s_first;
         if (first >= ba.total_blocks) break;
         const uint32_t last = umin(first + ba.blocks_per_pull, ba.total_blocks);
+        // Entry owning the first block: last e with s_begin[e] <= first. The
+        // blocks of a pull are consecutive, so later ones step to the next
+        // entry instead of searching again.
+        uint32_t lo = 0, hi = ba.entries;
+        while (hi - lo > 1u) {
+            const uint32_t mid = (lo + hi) >> 1;
+            if (s_begin[mid] <= first) lo = mid;
+            else hi = mid;
+        }
+        uint32_t e = __builtin_amdgcn_readfirstlane(lo);
         for (uint32_t block = first; block < last; ++block) {
-            // Entry owning the block: last e with s_begin[e] <= block.
-            uint32_t lo = 0, hi = ba.entries;
-            while (hi - lo > 1u) {
-                const uint32_t mid = (lo + hi) >> 1;
-                if (s_begin[mid] <= block) lo = mid;
-                else hi = mid;
+            while (block >= s_begin[e + 1u]) {
+                // The entry's finish time is its last block's: a max that
+                // returns nothing, so no wave waits on it.
+                if (t == 0) atomicMax(&ba.done[e], (unsigned long long)wall_clock64());
+                e = __builtin_amdgcn_readfirstlane(e + 1u);
             }
-            const uint32_t e = __builtin_amdgcn_readfirstlane(lo);
             const BatchView &v = *(const BatchView *)&table[e];
             trace_block(v.scene, v.ta, block - s_begin[e], s_value, s_backdrop);
             __syncthreads();
-            // The entry's finish time is its last block's: a max that
-            // returns nothing, so no wave waits on it.
-            if (t == 0) atomicMax(&ba.done[e], (unsigned long long)wall_clock64());
         }
+        if (t == 0) atomicMax(&ba.done[e], (unsigned long long)wall_clock64());
     }
     if (t == 0) atomicMax(&ba.clock[1], (unsigned long long)wall_clock64());
 }

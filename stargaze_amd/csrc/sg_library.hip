@@ -1950,12 +1950,16 @@ int sg_batch_autotune(sg_batch *b, const sg_frame *frames, uint32_t repeats, sg_
     double winner_ms = 1e300;
     const uint32_t cus = b->compute_units;
     const uint32_t blocks[] = {64u, 128u, 256u};
-    const uint32_t per_cu[] = {2u, 4u, 8u, 16u};  // 16: one-wave workgroups at three waves per SIMD
-    const uint32_t pulls[] = {1u, 4u};
+    // Resident waves per CU: two or three per SIMD, the fused kernel's
+    // register budget. Fewer leave latency unhidden (64-lane workgroups at
+    // eight per CU ran the 96-view workload 25% slower); more only queue.
+    const uint32_t waves_per_cu[] = {8u, 12u};
+    const uint32_t pulls[] = {1u, 4u, 16u};
     int rc = SG_OK;
     for (uint32_t block : blocks)
-        for (uint32_t wpc : per_cu)
+        for (uint32_t waves : waves_per_cu)
             for (uint32_t pull : pulls) {
+                const uint32_t wpc = std::max<uint32_t>(1u, waves / (block / 64u));
                 sg_batch_launch l{block, wpc * cus, pull, keep.order, 1u};
                 if (sg_batch_set_launch(b, &l) != SG_OK) continue;
                 // One untimed pass captures the graph and warms the caches.

@@ -122,3 +122,113 @@ def render(scene: DiffScene, albedo: torch.Tensor, tau: torch.Tensor | None, air
             raise ValueError("the scene renders with a density profile: pass `density`")
         density = torch.zeros(DENSITY_BINS)
     return _Render.apply(albedo, tau, air, density, scene)
+
+
+PARAMETER_GROUPS = ("albedo", "tau", "air", "density")
+
+
+def render_converged(scene: DiffScene, values: dict, passes: int, noise_seed: int = 0x7A29E7) -> torch.Tensor:
+    """Mean radiance (H, W, 3) of ``passes`` progressive passes of the scene
+    at parameter ``values`` (numpy arrays keyed as in ``recover``), on a
+    tracer of its own with its own noise seed: a target image for
+    ``recover`` whose noise does not repeat the scene's draws."""
+    spp = scene.tracer.options.samples_per_frame
+    t = PathTracer(scene.size, options=Options(noise_seed=noise_seed, samples_per_frame=spp,
+                                               max_bounces=scene.tracer.options.max_bounces, milky_way=0.0))
+    try:
+        if scene.ring_body is not None and "tau" in values:
+            t.set_ring_profile(scene.ring_body, np.asarray(values["tau"], np.float32))
+        if scene.density:
+            t.set_density_profile(np.asarray(values["density"], np.float32))
+        frame = scene.frame_with(np.asarray(values["albedo"], np.float32), np.asarray(values["air"], np.float32))
+        for _ in range(passes):
+            t.encode(frame)
+        return torch.from_numpy(t.read_accumulation()[..., :3].copy())
+    finally:
+        t.close()
+
+
+def recover(scene: DiffScene, target: torch.Tensor, start: dict, steps: int = 200, lr: float = 0.01,
+            final_lr: float = 0.001, log=None) -> tuple[dict, list]:
+    """Fit the scene's parameters to a target image with Adam.
+
+    ``start`` holds the starting values (numpy arrays ``albedo`` (B, 3),
+    ``tau`` (512,), ``air`` (5,) and, when the scene renders with a density
+    table, ``density`` (16,)). Every parameter is optimised in relative
+    coordinates, value = start * (1 + u), so one learning rate suits
+    albedos near 0.5 and air coefficients near 1e6 per AU alike. The
+    learning rate falls linearly from ``lr`` to ``final_lr``.
+
+    Each step renders two fresh passes (new draws). The first gives the
+    residual to ``target`` and the step's loss (mean squared error); the
+    adjoint of the second, weighted by that residual, gives the gradient.
+    With one pass for both, the residual would carry the pass's own noise
+    and the gradient would also descend the variance of the estimate.
+    Returns the fitted values (numpy, same keys) and the loss of every
+    step."""
+    names = [n for n in PARAMETER_GROUPS if n in start and (n != "density" or scene.density)
+             and (n != "tau" or scene.ring_body is not None)]
+    base = {n: torch.tensor(np.asarray(start[n], np.float32)) for n in names}
+    u = {n: torch.zeros_like(base[n], requires_grad=True) for n in names}
+    opt = torch.optim.Adam(list(u.values()), lr=lr)
+
+    def values():
+        v = {n: base[n] * (1.0 + u[n]) for n in names}
+        v["albedo"] = v["albedo"].clamp(0.0, 1.0)
+        if "tau" in v:
+            v["tau"] = v["tau"].clamp(min=0.0)
+        if "density" in v:
+            v["density"] = v["density"].clamp(min=0.0)
+        air = v["air"].clamp(min=0.0)
+        v["air"] = torch.cat([air[:4], air[4:].clamp(max=0.99)])
+        return v
+
+    losses = []
+    for it in range(steps):
+        for group in opt.param_groups:
+            group["lr"] = lr + (final_lr - lr) * it / max(1, steps - 1)
+        opt.zero_grad()
+        v = values()
+        with torch.no_grad():
+            residual = render(scene, v["albedo"], v.get("tau"), v["air"], v.get("density")) - target
+        image = render(scene, v["albedo"], v.get("tau"), v["air"], v.get("density"))
+        (image * (2.0 / residual.numel()) * residual).sum().backward()
+        opt.step()
+        losses.append(float((residual ** 2).mean()))
+        if log is not None and (it % 50 == 0 or it == steps - 1):
+            log(it, losses[-1])
+    with torch.no_grad():
+        fitted = {n: t.numpy().copy() for n, t in values().items()}
+    return fitted, losses
+
+
+def parameter_errors(values: dict, target: dict, segments: int = 16) -> dict:
+    """Mean relative error of a parameter set against a target set, per
+    quantity the image determines:
+
+    * ``albedo``: every albedo channel;
+    * ``ring_bins``: every ring-profile bin, and ``ring_segments``: the
+      profile averaged over ``segments`` equal runs of bins (512 bins over
+      the ring's radial span resolve finer than a pixel at 512 x 512);
+    * ``extinction``: every product of a coefficient (Rayleigh rgb, Mie)
+      and a density slice, which is what a tabulated density enters the
+      image through (scaling the table and dividing the coefficients by
+      the same factor leaves the image unchanged);
+    * ``mie_g``: the phase asymmetry."""
+
+    def rel(a, b):
+        a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+        return float(np.mean(np.abs(a - b) / np.abs(b)))
+
+    out = {"albedo": rel(values["albedo"], target["albedo"])}
+    if "tau" in values and "tau" in target:
+        out["ring_bins"] = rel(values["tau"], target["tau"])
+        seg = lambda t: np.asarray(t, np.float64).reshape(segments, -1).mean(axis=1)  # noqa: E731
+        out["ring_segments"] = rel(seg(values["tau"]), seg(target["tau"]))
+    if "density" in values and "density" in target:
+        prod = lambda v: np.outer(np.asarray(v["air"][:4], np.float64), np.asarray(v["density"], np.float64))  # noqa: E731
+        out["extinction"] = rel(prod(values), prod(target))
+    else:
+        out["extinction"] = rel(values["air"][:4], target["air"][:4])
+    out["mie_g"] = rel(values["air"][4:], target["air"][4:])
+    return out

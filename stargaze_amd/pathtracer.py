@@ -417,39 +417,6 @@ class Batch:
                     "sg_batch_encode_views")
         return ms
 
-    def write_accumulation(self, rgba: np.ndarray) -> None:
-        """Restore the accumulation image (H, W, 4); the history is unchanged."""
-        self._write(self._lib.sg_write_accumulation, rgba, "write_accumulation")
-
-    def write_backdrop(self, rgba: np.ndarray) -> None:
-        self._write(self._lib.sg_write_backdrop, rgba, "write_backdrop")
-
-    def _write(self, fn, rgba, what):
-        w, h = self.size()
-        a = np.ascontiguousarray(rgba, np.float32)
-        if a.shape != (h, w, 4):
-            raise ValueError(f"image must be ({h}, {w}, 4), got {a.shape}")
-        self._check(fn(self._ctx, a.ctypes.data, w * h), what)
-
-    def read_exposure(self) -> tuple[np.ndarray, np.ndarray]:
-        """The display exposure state (EXPOSURE record) and the last metered
-        luminance histogram (256 counts)."""
-        state = np.zeros((), EXPOSURE)
-        hist = np.zeros(HISTOGRAM_BINS, np.uint32)
-        self._check(self._lib.sg_read_exposure(self._ctx, state.ctypes.data, hist.ctypes.data), "read_exposure")
-        return state, hist
-
-    def display(self, width: int, height: int) -> np.ndarray:
-        """``PathTracer::display``: exposed, ACES-mapped (H, W, 4) f32 at the given size."""
-        out = np.empty((height, width, 4), np.float32)
-        self._check(self._lib.sg_display(self._ctx, int(width), int(height), out.ctypes.data), "display")
-        return out
-
-    def display_srgb8(self, width: int, height: int) -> np.ndarray:
-        out = np.empty((height, width, 4), np.uint8)
-        self._check(self._lib.sg_display_srgb8(self._ctx, int(width), int(height), out.ctypes.data), "display_srgb8")
-        return out
-
     def synchronize(self) -> None:
         self._check(self._lib.sg_batch_synchronize(self._b), "sg_batch_synchronize")
 

@@ -679,6 +679,33 @@ def ring_profile_cba(frame: Frame, ring_body: int, bins: int = 512, seed: int = 
     return (tau * (1.0 + noise)).astype(np.float32)
 
 
+DIFF_ALBEDO_BODIES = tuple(range(1, 9))
+DIFF_RING_BODY = 2
+
+
+def diff_system_parameters(frame: Frame, seed: int = 2, spread: float = 0.1) -> tuple[dict, dict]:
+    """The 557 parameters of the differentiable workload (``diff_system``):
+    the RGB albedos of its eight lit bodies (24), the C/B/A ring profile of
+    the giant (512 bins), Rayleigh rgb, Mie and Mie g (5, per AU) and the
+    16-slice exp(-h / H) density table. Returns the nominal set and a target
+    set with every parameter scaled by its own factor in [1 - spread,
+    1 + spread] from a PCG seeded by ``seed``. Each set is a dict of float32
+    arrays: albedo (8, 3), tau (512,), air (5,), density (16,)."""
+    g = frame.globals
+    nominal = {
+        "albedo": np.stack([frame.bodies["color"][b][:3] for b in DIFF_ALBEDO_BODIES]).astype(np.float32),
+        "tau": ring_profile_cba(frame, DIFF_RING_BODY),
+        "air": np.array([*g["atmo_rayleigh"], g["atmo_params"][0]], np.float32),
+        "density": exponential_density(frame),
+    }
+    rng = Pcg(seed)
+    target = {}
+    for name, value in nominal.items():
+        factors = np.array([1.0 + rng.uniform(-spread, spread) for _ in range(value.size)], np.float64)
+        target[name] = (value.astype(np.float64) * factors.reshape(value.shape)).astype(np.float32)
+    return nominal, target
+
+
 def random_system(width: int, height: int, rng: "Pcg") -> tuple[Frame, dict]:
     """One random view of the batched workload: the camera 2 m above a host
     planet at a random pose (the host's up vector within 60 deg of the

@@ -169,3 +169,64 @@ def test_batch_rejects_landscape_views():
     with pytest.raises(Exception, match="landscape"):
         batch.encode(frames)
     close(batch, mine)
+
+
+def test_batched_serving_views_match_per_view_launches():
+    """The batched-serving workload at full size (96 random systems, 128^2
+    to 1080p, spp 4-64, air on a third of them): one batched pass against
+    one sg_encode per view, bit for bit, with the autotuner's launch."""
+    import parity
+
+    print("batched serving workload: 96 views", flush=True)
+    views = scenes.system_views(96)
+    frames = [v["frame"] for v in views]
+    mine, ref = tracers(views), tracers(views)
+    batch = Batch(mine)
+    batch.autotune(frames, repeats=1)
+    batch.encode(frames)
+    batch.synchronize()
+    for t, f in zip(ref, frames):
+        t.encode(f)
+    for i, (ta, tb) in enumerate(zip(mine, ref)):
+        assert ta.samples() == tb.samples() == views[i]["spp"], f"view {i} sample count"
+        a, b = ta.read_accumulation(), tb.read_accumulation()
+        assert np.all(np.isfinite(a)), f"view {i} non-finite"
+        parity.assert_same_draws(a, b)
+        parity.assert_same_draws(ta.read_backdrop(), tb.read_backdrop())
+    close(batch, mine, ref)
+
+
+def test_batched_serving_views_match_oracle_per_tile():
+    """Views of the batched-serving generator shrunk to 32^2-80x45 pixels
+    (same seed, same scene draws per view): each batched view against the
+    FP32 oracle per 16x16 tile (3 sigma) and per pixel."""
+    import oracle
+    import parity
+
+    views = scenes.system_views(12, min_pixels=32 * 32, max_pixels=80 * 45, spp=(8, 16))
+    frames = [v["frame"] for v in views]
+    mine = tracers(views)
+    for t in mine:
+        t.enable_moments(True)
+    batch = Batch(mine)
+    for _ in range(2):
+        batch.encode(frames)
+    batch.synchronize()
+    zs, agree = [], []
+    for i, (v, t) in enumerate(zip(views, mine)):
+        print(f"oracle view {i}: {v['width']}x{v['height']} {v['spp']} spp {v['kind']}", flush=True)
+        o = oracle.OracleTracer((v["width"], v["height"]),
+                                Options(samples_per_frame=v["spp"], max_bounces=v["bounces"], milky_way=0.0), "f32")
+        o.enable_moments(True)
+        for _ in range(2):
+            o.encode(v["frame"])
+        a, am = t.read_accumulation(), t.read_moments()
+        b, bm = o.read_accumulation(), o.read_moments()
+        zs.append(parity.tile_z(a, am, b, bm).ravel())
+        agree.append(parity.pixel_agreement(a, b))
+    z = np.concatenate(zs)
+    over, worst = parity.assert_tiles(z)
+    print(f"batched views vs oracle: tiles over 3 sigma {over:.4f}, worst z {worst:.2f}, "
+          f"pixels agreeing min {min(agree):.4f} mean {np.mean(agree):.4f}")
+    assert min(agree) >= 0.95
+    close(batch, mine)

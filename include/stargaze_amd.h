@@ -162,9 +162,19 @@ enum sg_probe_kind {
     SG_PROBE_TERRAIN = 4,    /* terrain_query, metre frame           terrain.wgsl:152 */
     SG_PROBE_LANDSCAPE = 5,  /* landscape_hit(ray)                   ground.wgsl:123 */
     SG_PROBE_CONE = 6,       /* cone_width + sample_light draw       pathtrace.wgsl:197,214 */
-    SG_PROBE_ALBEDO = 7      /* style albedo of bodies[target] at the local unit point
+    SG_PROBE_ALBEDO = 7,     /* style albedo of bodies[target] at the local unit point
                                 q = direction, unclamped (plain style: surface_albedo),
                                 the material probe of surface_tests.rs:106-125 */
+    SG_PROBE_GROUND = 8,     /* ground material of landscape element `anchor` at the
+                                surface-frame point `offset` (metres) with surface-frame
+                                geometric normal `direction`, seen from the distance whose
+                                float bits are `target`: normal = shading normal (world
+                                frame), extra = albedo       ground.wgsl:597-648 */
+    SG_PROBE_NOISE = 9,      /* at the point q = direction: distance = vnoise(q),
+                                normal = {hash31(q), fbm(q, target)}  pathtrace.wgsl:338-365 */
+    SG_PROBE_SKY = 10        /* normal = surface_sky(ray, direction, rng = hash(target + 817))
+                                over the sky cache of the last encode; extra = that cache's
+                                texel target % 128 (16 x 8, row-major)   forest.wgsl:3-54 */
 };
 
 typedef struct sg_ctx sg_ctx;

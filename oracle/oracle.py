@@ -88,6 +88,7 @@ def library(precision: str = "f32") -> ctypes.CDLL:
     lib.orc_set_counting.argtypes, lib.orc_set_counting.restype = [ctypes.c_int], None
     lib.orc_counters.argtypes, lib.orc_counters.restype = [P], None
     lib.orc_counter_count.argtypes, lib.orc_counter_count.restype = [], U32
+    lib.orc_set_planetshine_thinning.argtypes, lib.orc_set_planetshine_thinning.restype = [ctypes.c_int], None
     _libs[precision] = lib
     return lib
 
@@ -112,6 +113,7 @@ class OracleTracer:
 
     def __init__(self, dimensions, options=None, precision="f32", catalogue=None, sky_cells=None, milky_way=None):
         self.lib = library(precision)
+        self.planetshine_thinning = True
         self.width, self.height = int(dimensions[0]), int(dimensions[1])
         o = options
         self.noise_seed = int(getattr(o, "noise_seed", 0))
@@ -143,6 +145,11 @@ class OracleTracer:
 
     def samples(self) -> int:
         return self.history_samples
+
+    def set_planetshine_thinning(self, on: bool) -> None:
+        """Test knob: with ``on`` False this tracer's passes never thin
+        planetshine (orc_set_planetshine_thinning in sg_oracle.c)."""
+        self.planetshine_thinning = bool(on)
 
     def enable_moments(self, on=True):
         self.moments_on = on
@@ -266,6 +273,7 @@ class OracleTracer:
         st = self.settings(frame, count)
         if count > 0:
             sc, keep = self._scene(st, frame)
+            self.lib.orc_set_planetshine_thinning(int(self.planetshine_thinning))
             g = st["globals"]
             if g["ground_counts"][1] != 0 and g["ground_north"][3] != 0:
                 radius = float(g["ground_up"][3])

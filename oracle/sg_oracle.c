@@ -155,6 +155,13 @@ static void flush_counters(void) {
 }
 uint32_t orc_counter_count(void) { return EV_COUNT; }
 
+/* Test knob (no product counterpart): with 0, planetshine in the air is never
+ * thinned, the estimator the reference's energy test compares against
+ * (src/motion_noise_tests.rs:354-360 sets the roulette minimum to 1). The
+ * draw is still taken, so the paths' other draws do not shift. */
+static int planetshine_thinning = 1;
+void orc_set_planetshine_thinning(int on) { planetshine_thinning = on; }
+
 /* --- RNG (src/shaders/pathtrace.wgsl:87-98) --- */
 uint32_t orc_hash(uint32_t value) {
     uint32_t x = value;
@@ -1627,11 +1634,11 @@ static void dune_gradient(R footprint, R scale, R radius, v3 p, v3 n, R g[2]) { 
 static v3 landscape_point(const Ctx *c, Ray ray, Hit hit) {
     return add(ground_origin(c, ray), scl(to_ground(c, ray.direction), hit.distance * METRES_PER_AU));
 }
-static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
-    v3 p = landscape_point(c, ray, hit);
-    R distance = hit.distance * METRES_PER_AU;
-    v3 n = to_ground(c, hit.normal);
-    if (hit.index == LOCAL_BASE || hit.index == TERRAIN_ID) {
+/* Ground material at surface-frame point p (metres) of landscape element
+ * `index`, seen from `distance` metres, with surface-frame normal n
+ * (ground.wgsl:597-640). */
+static v3 ground_albedo(const Ctx *c, v3 p, R distance, v3 n, uint32_t index) {
+    if (index == LOCAL_BASE || index == TERRAIN_ID) {
         uint32_t theme = ground_theme(c);
         if (theme == 1u) return desert_floor(footprint_of(c, distance), p, n);
         if (theme == 2u) return crater_floor(footprint_of(c, distance), p, n);
@@ -1639,7 +1646,7 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
         if (theme == 5u) return dune_floor(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n);
         return forest_floor(c, p, distance, n);
     }
-    const Prim *object = &c->objects[hit.index - LOCAL_BASE - 1u];
+    const Prim *object = &c->objects[index - LOCAL_BASE - 1u];
     v3 tint = V4(object->albedo);
     R kind = object->albedo.v[3];
     if (kind == 0.0) return tint;
@@ -1660,22 +1667,32 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
     if (ground_theme(c) != 0u) return stone; /* only the forest's rocks carry moss and lichen */
     return mixV(stone, V(0.11, 0.145, 0.037), lichen * 0.65);
 }
-static v3 landscape_shading_normal(const Ctx *c, Ray ray, Hit hit) {
-    if (hit.index != LOCAL_BASE && hit.index != TERRAIN_ID) return hit.normal;
-    v3 p = landscape_point(c, ray, hit);
-    R distance = hit.distance * METRES_PER_AU;
+static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
+    return ground_albedo(c, landscape_point(c, ray, hit), hit.distance * METRES_PER_AU, to_ground(c, hit.normal),
+                         hit.index);
+}
+/* Shading normal (world frame) at surface-frame point p of landscape element
+ * `index`, seen from `distance` metres, with world-frame geometric normal
+ * (ground.wgsl:641-648). */
+static v3 ground_shading_normal(const Ctx *c, v3 p, R distance, v3 normal, uint32_t index) {
+    if (index != LOCAL_BASE && index != TERRAIN_ID) return normal;
     uint32_t theme = ground_theme(c);
     if (theme == 3u || theme == 5u) { /* ground.wgsl:642-643 */
-        v3 n = to_ground(c, hit.normal);
+        v3 n = to_ground(c, normal);
         R g[2];
         if (theme == 3u) alpine_gradient(footprint_of(c, distance), p, n, g);
         else dune_gradient(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n, g);
-        return nrm(add(hit.normal, from_ground(c, V(-g[0], 0.0, -g[1]))));
+        return nrm(add(normal, from_ground(c, V(-g[0], 0.0, -g[1]))));
     }
     R strength = 0.11 * (1.0 - smooth(8.0, 65.0, distance));
     R dx = vnoise(scl(add(p, V(0.025, 0.0, 0.0)), 6.0)) - vnoise(scl(sub(p, V(0.025, 0.0, 0.0)), 6.0));
     R dz = vnoise(scl(add(p, V(0.0, 0.0, 0.025)), 6.0)) - vnoise(scl(sub(p, V(0.0, 0.0, 0.025)), 6.0));
-    return nrm(add(hit.normal, scl(from_ground(c, V(-dx, 0.0, -dz)), strength)));
+    return nrm(add(normal, scl(from_ground(c, V(-dx, 0.0, -dz)), strength)));
+}
+static v3 landscape_shading_normal(const Ctx *c, Ray ray, Hit hit) {
+    if (hit.index != LOCAL_BASE && hit.index != TERRAIN_ID) return hit.normal;
+    return ground_shading_normal(c, landscape_point(c, ray, hit), hit.distance * METRES_PER_AU, hit.normal,
+                                 hit.index);
 }
 static Ray landscape_outgoing(const Ctx *c, Ray ray, Hit hit) {
     v3 p = landscape_point(c, ray, hit);
@@ -1847,7 +1864,7 @@ static v3 atmosphere(const Ctx *c, v3 origin, v3 dir, R limit, uint32_t *rng) {
         R sun = brightest_risen_light(c, origin);
         if (sun > 0.0) {
             R ratio = maxc(reflected) / sun;
-            R minimum = 1.0 / sqrt(clampR((R)c->accumulated + 1.0, 1.0, 4096.0));
+            R minimum = planetshine_thinning ? 1.0 / sqrt(clampR((R)c->accumulated + 1.0, 1.0, 4096.0)) : 1.0;
             R keep = clampR(ratio * 1.0e3, minimum, 1.0);
             reflected = random_(&reflected_rng) < keep ? dvs(reflected, keep) : Vs(0.0);
         }
@@ -2510,6 +2527,35 @@ void orc_probe(const orc_scene *sc, uint32_t kind, const orc_probe_ray *rays, ui
             v3 a = style > 0u ? style_albedo(b, ray.direction) : surface_albedo(b, ray.direction);
             r.index = 0;
             r.normal[0] = a.x; r.normal[1] = a.y; r.normal[2] = a.z;
+        } else if (kind == 9u) {
+            /* value noise, its hash and fbm at q = direction (pathtrace.wgsl:338-365) */
+            r.index = 0;
+            r.distance = vnoise(ray.direction);
+            r.normal[0] = hash31(ray.direction);
+            r.normal[1] = fbm(ray.direction, pr->target);
+        } else if (kind == 10u) {
+            /* surface_sky over the last encode's sky cache, and one texel of it
+             * (forest.wgsl:3-54) */
+            uint32_t rng = orc_hash(pr->target + 817u);
+            v3 light = surface_sky(&c, ray, ray.direction, &rng);
+            v3 texel = sky_texel(&c, (int)(pr->target % 16u), (int)((pr->target / 16u) % 8u));
+            r.index = 0;
+            r.normal[0] = light.x; r.normal[1] = light.y; r.normal[2] = light.z;
+            r.extra[0] = texel.x; r.extra[1] = texel.y; r.extra[2] = texel.z;
+        } else if (kind == 8u) {
+            /* ground material (ground.wgsl:597-648) of landscape element
+             * anchor at surface-frame point offset (metres) with surface-frame
+             * geometric normal direction, seen from target's float bits metres */
+            float distance;
+            memcpy(&distance, &pr->target, 4);
+            v3 p = V(pr->offset[0], pr->offset[1], pr->offset[2]);
+            v3 n = from_ground(&c, ray.direction);
+            v3 shading = ground_shading_normal(&c, p, (R)distance, n, pr->anchor);
+            v3 a = ground_albedo(&c, p, (R)distance, ray.direction, pr->anchor);
+            r.index = pr->anchor;
+            r.distance = distance;
+            r.normal[0] = shading.x; r.normal[1] = shading.y; r.normal[2] = shading.z;
+            r.extra[0] = a.x; r.extra[1] = a.y; r.extra[2] = a.z;
         }
         out[i] = r;
     }

@@ -418,11 +418,10 @@ SG_OUTLINE float2 dune_gradient(float footprint, float scale, float radius, f3 p
 SG_FN d3 landscape_point(const Scene &sc, const Ray &ray, const Hit &hit) {
     return ground_origin(sc, ray) + tod(to_ground(sc, ray.direction)) * ((double)hit.distance * METRES_PER_AU_D);
 }
-SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
-    f3 p = tof(landscape_point(sc, ray, hit));
-    float distance = hit.distance * METRES_PER_AU;
-    f3 n = to_ground(sc, hit.normal);
-    if (hit.index == SG_LOCAL_BASE || hit.index == SG_TERRAIN_ID) {
+// Ground material at surface-frame point p (metres) of landscape element
+// `index`, seen from `distance` metres, with surface-frame normal n.
+SG_FN f3 ground_albedo(const Scene &sc, f3 p, float distance, f3 n, uint32_t index) {
+    if (index == SG_LOCAL_BASE || index == SG_TERRAIN_ID) {
         switch (ground_theme(sc)) {
         case 1u: return desert_floor(footprint_of(sc, distance), p, n);
         case 2u: return crater_floor(footprint_of(sc, distance), p, n);
@@ -431,7 +430,7 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
         default: return forest_floor(sc, p, distance, n);
         }
     }
-    const sg_primitive &object = sc.objects[hit.index - SG_LOCAL_BASE - 1u];
+    const sg_primitive &object = sc.objects[index - SG_LOCAL_BASE - 1u];
     f3 tint = {object.albedo[0], object.albedo[1], object.albedo[2]};
     float kind = object.albedo[3];
     if (kind == 0.0f) return tint;
@@ -453,22 +452,31 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
     if (ground_theme(sc) != 0u) return stone;  // only the forest's rocks carry moss and lichen
     return mix3(stone, f3{0.11f, 0.145f, 0.037f}, lichen * 0.65f);
 }
-SG_FN f3 landscape_shading_normal(const Scene &sc, const Ray &ray, const Hit &hit) {
-    if (hit.index != SG_LOCAL_BASE && hit.index != SG_TERRAIN_ID) return hit.normal;
-    f3 p = tof(landscape_point(sc, ray, hit));
-    float distance = hit.distance * METRES_PER_AU;
+SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
+    return ground_albedo(sc, tof(landscape_point(sc, ray, hit)), hit.distance * METRES_PER_AU,
+                         to_ground(sc, hit.normal), hit.index);
+}
+// Shading normal (world frame) at surface-frame point p of landscape element
+// `index`, seen from `distance` metres, with world-frame geometric normal.
+SG_FN f3 ground_shading_normal(const Scene &sc, f3 p, float distance, f3 normal, uint32_t index) {
+    if (index != SG_LOCAL_BASE && index != SG_TERRAIN_ID) return normal;
     const uint32_t theme = ground_theme(sc);
     if (theme == 3u || theme == 5u) {
-        const f3 n = to_ground(sc, hit.normal);
+        const f3 n = to_ground(sc, normal);
         const float2 g = theme == 3u ? alpine_gradient(footprint_of(sc, distance), p, n)
                                      : dune_gradient(footprint_of(sc, distance), terrain_scale(sc),
                                                      sc.s.globals.ground_up[3], p, n);
-        return normalize(hit.normal + from_ground(sc, f3{-g.x, 0.0f, -g.y}));
+        return normalize(normal + from_ground(sc, f3{-g.x, 0.0f, -g.y}));
     }
     float strength = 0.11f * (1.0f - smoothstepf(8.0f, 65.0f, distance));
     float dx = vnoise((p + f3{0.025f, 0.f, 0.f}) * 6.0f) - vnoise((p - f3{0.025f, 0.f, 0.f}) * 6.0f);
     float dz = vnoise((p + f3{0.f, 0.f, 0.025f}) * 6.0f) - vnoise((p - f3{0.f, 0.f, 0.025f}) * 6.0f);
-    return normalize(hit.normal + from_ground(sc, f3{-dx, 0.0f, -dz}) * strength);
+    return normalize(normal + from_ground(sc, f3{-dx, 0.0f, -dz}) * strength);
+}
+SG_FN f3 landscape_shading_normal(const Scene &sc, const Ray &ray, const Hit &hit) {
+    if (hit.index != SG_LOCAL_BASE && hit.index != SG_TERRAIN_ID) return hit.normal;
+    return ground_shading_normal(sc, tof(landscape_point(sc, ray, hit)), hit.distance * METRES_PER_AU, hit.normal,
+                                 hit.index);
 }
 // 5 mm outgoing offset with reprojection onto the host sphere, FP64.
 SG_FN Ray landscape_outgoing(const Scene &sc, const Ray &ray, const Hit &hit) {

@@ -187,6 +187,33 @@ __global__ void k_probe(Scene sc, uint32_t kind, const sg_probe_ray *rays, uint3
         r.distance = cone_width(sc, ray, pr.anchor == SG_MISS ? 0u : pr.anchor);
         break;
     }
+    case SG_PROBE_NOISE: {
+        r.index = 0u;
+        r.distance = vnoise(ray.direction);
+        r.normal[0] = hash31(ray.direction);
+        r.normal[1] = fbm(ray.direction, pr.target);
+        break;
+    }
+    case SG_PROBE_SKY: {
+        uint32_t rng = hash_u32(pr.target + 817u);
+        const f3 light = surface_sky(sc, ray, ray.direction, rng);
+        const f3 texel = surface_sky_texel(sc, (int)(pr.target % 16u), (int)((pr.target / 16u) % 8u));
+        r.index = 0u;
+        r.normal[0] = light.x; r.normal[1] = light.y; r.normal[2] = light.z;
+        r.extra[0] = texel.x; r.extra[1] = texel.y; r.extra[2] = texel.z;
+        break;
+    }
+    case SG_PROBE_GROUND: {
+        const float distance = __uint_as_float(pr.target);
+        const f3 p = {pr.offset[0], pr.offset[1], pr.offset[2]};
+        const f3 shading = ground_shading_normal(sc, p, distance, from_ground(sc, ray.direction), pr.anchor);
+        const f3 a = ground_albedo(sc, p, distance, ray.direction, pr.anchor);
+        r.index = pr.anchor;
+        r.distance = distance;
+        r.normal[0] = shading.x; r.normal[1] = shading.y; r.normal[2] = shading.z;
+        r.extra[0] = a.x; r.extra[1] = a.y; r.extra[2] = a.z;
+        break;
+    }
     default:
         break;
     }

@@ -1442,16 +1442,22 @@ int sg_adjoint_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches) {
 int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_ray *rays, uint32_t count,
              sg_probe_result *results) {
     if (!ctx || !frame || !frame->globals || (count && (!rays || !results))) return SG_E_INVALID;
-    if (kind > SG_PROBE_ALBEDO) return fail(ctx, SG_E_INVALID, "unknown probe kind");
+    if (kind > SG_PROBE_SKY) return fail(ctx, SG_E_INVALID, "unknown probe kind");
     if (frame->body_count > SG_BODY_CAPACITY) return fail(ctx, SG_E_CAPACITY, "too many scene bodies");
     if (count == 0) return SG_OK;
     for (uint32_t i = 0; i < count; ++i) {
         uint32_t a = rays[i].anchor;
+        if (kind == SG_PROBE_GROUND) {
+            if (a != SG_LOCAL_BASE && a != SG_TERRAIN_ID && (a < SG_LOCAL_BASE || a - SG_LOCAL_BASE > frame->ground_count))
+                return fail(ctx, SG_E_INVALID, "ground probe element out of range");
+            continue;
+        }
         if (a != SG_MISS && a >= frame->body_count) return fail(ctx, SG_E_INVALID, "probe anchor out of range");
         if ((kind == SG_PROBE_SPHERE || kind == SG_PROBE_RING || kind == SG_PROBE_VISIBILITY ||
              kind == SG_PROBE_ALBEDO) &&
             rays[i].target >= frame->body_count)
             return fail(ctx, SG_E_INVALID, "probe target out of range");
+        if (kind == SG_PROBE_NOISE && rays[i].target > 16u) return fail(ctx, SG_E_INVALID, "fbm octaves above 16");
     }
     if ((kind == SG_PROBE_TERRAIN || frame->globals->ground_counts[1] != 0u) && !ctx->terrain)
         return fail(ctx, SG_E_STATE, "terrain probe without uploaded terrain");
@@ -1472,6 +1478,12 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     sg_settings st;
     std::memset(&st, 0, sizeof(st));
     st.globals = *frame->globals;
+    // The viewport as an encode of this frame sees it (src/pathtracer.rs:812-817):
+    // footprint-dependent ground detail and point discs read it.
+    st.globals.viewport[3] = st.globals.viewport[0] / std::max(st.globals.viewport[1], 1.0f);
+    st.globals.viewport[0] = (float)ctx->width;
+    st.globals.viewport[1] = (float)ctx->height;
+    st.globals.viewport[2] = ctx->options.exposure;
     float prop_top = 0.0f;
     for (uint32_t i = 0; i < frame->ground_count; ++i) {
         const sg_primitive &p = frame->ground[i];

@@ -344,7 +344,8 @@ class PathTracer:
 
 INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
 STEP_EXTEND, STEP_CAMERA_AIR, STEP_SHADE, STEP_AIR_OCCLUSION = range(4)
-PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE, PROBE_ALBEDO = range(8)
+(PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE, PROBE_ALBEDO,
+ PROBE_GROUND, PROBE_NOISE, PROBE_SKY) = range(11)
 
 BATCH_ORDER_GIVEN, BATCH_ORDER_SHORTEST, BATCH_ORDER_LONGEST = 0, 1, 2
 

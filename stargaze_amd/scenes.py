@@ -436,6 +436,46 @@ def halo_observatory(width=3840, height=2160, cells=4096, rocks=20000, conifers=
 HALO_AIR = dict(view_steps=16, light_steps=8, mie_scale_height=1200.0 / METRES_PER_AU)
 
 
+def lookout(width=96, height=54, theme=0, radius_m=6.371e6, cells=256, seed=0x743337A9, pitch_deg=4.0):
+    """The reference's own ground scenery around a lookout 2 m above an
+    Earth-sized host with Earth-like air and one sun at 30 deg: the forest
+    clearing of ``ground.landscape_with_density`` (stone outcrops, conifers,
+    undergrowth, logs and the cottage: every primitive kind but the tumbled
+    rock) for theme 0, or the tumbled rocks of ``ground.theme_rocks`` for
+    the desert (1), crater (2) and basalt (4) grounds, on a ridged-fBm
+    terrain whose heights the props stand on. The camera looks north along
+    the observing gap, pitched down ``pitch_deg``. Returns
+    ``(frame, terrain_data, cells)``."""
+    from . import ground as gr
+    from . import terrain as tr
+
+    f = frame()
+    camera(f.globals, 70.0, width, height)
+    east, up, north = _ground_frame(-pitch_deg)
+    r_host = metres(radius_m)
+    host_c = -up * (r_host + metres(2.0))
+    host = place(body([0, 0, 0], r_host, [0.25, 0.22, 0.18], False), host_c)
+    sun = star(sky_direction(east, up, north, 35.0, 30.0), metres(7.0e8))
+    f.bodies = bodies(sun, host)
+    au = METRES_PER_AU
+    g = f.globals
+    g["atmo_center"] = [*split(host_c)[0], r_host]
+    g["atmo_rayleigh"] = [5.8e-6 * au, 13.5e-6 * au, 33.1e-6 * au, 2.0e-5 * au]
+    g["atmo_params"] = [0.76, 8000.0 / au, 80000.0 / au, 2.0]
+    g["ground_east"] = [*east, 2.0]
+    g["ground_up"] = [*up, radius_m]
+    g["ground_north"] = [*north, 2.0]
+    heights = tr.ridged_fbm(cells, seed=seed, amplitude=60.0)
+    s = tr.scale(radius_m)
+    surface = lambda x, z: float(tr.heights_at(heights, cells, np.array([x / s]), np.array([z / s]))[0]) * s  # noqa: E731
+    if theme == 0:
+        f.ground = gr.landscape_with_density(radius_m, 100, surface)
+    else:
+        f.ground = gr.theme_rocks(radius_m, theme, surface)
+    g["ground_counts"] = [len(f.ground), 1, theme, 0]
+    return f, tr.terrain_data(heights, cells), cells
+
+
 def halo_parity(width=96, height=54):
     """Halo scene inside the reference's capacities: 512-cell terrain and at
     most 256 primitives (src/terrain.rs:20, src/ground.rs:8)."""

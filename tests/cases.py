@@ -727,3 +727,303 @@ ALL = [
     local_ground_shadows,
     distant_props_occlude_the_sky,
 ]
+
+
+# -- src/motion_noise_tests.rs:40-85,172-200,326-405 (gpu_eclipse_energy_and_sample_batching)
+def eclipsed_planetshine(size, step):
+    """A telescope view (5 deg half-height) from 2 m above an Earth-sized host
+    with Earth-like air, a small moon low in the view, a sun behind the
+    observer and a large dark body moving across it by half a kilometre
+    per step (src/motion_noise_tests.rs:40-85)."""
+    a = earthlike()
+    km = 1.0 / scenes.AU_KM
+    radius = np.float32(6371.0 * km)
+    center = [0.0, 0.0, float(radius + np.float32(0.002 * km))]
+    f = scenes.frame()
+    f.globals["viewport"] = [size[0], size[1], 1.0, size[0] / size[1]]
+    f.globals["cam_forward"][3] = math.tan(math.radians(5.0))
+    f.globals["ground_up"] = [0.0, 0.0, -1.0, 6371000.0]
+    f.globals["atmo_center"] = [0.0, 0.0, center[2], radius]
+    f.globals["atmo_rayleigh"] = [*a["rayleigh"], a["mie"]]
+    f.globals["atmo_params"] = [a["mie_g"], a["scale_height"], a["thickness"], 1.0]
+    d = 100_000.0 * km
+    f.bodies = scenes.bodies(
+        scenes.body(center, radius, [0.0] * 3, False),
+        scenes.body([0.0004 + step * 0.00001, 0.0, -0.002], 0.00004, [0.8, 0.6, 0.4], False),
+        scenes.body([-0.8, 0.0, -0.6], 0.005, [1e7] * 3, True),
+        scenes.body([-0.8 * d + step * 0.5 * km, 0.0, -0.6 * d], 1100.0 * km, [0.0] * 3, False),
+    )
+    f.scene_time = float(step)
+    return f
+
+
+def eclipsed_sun(size, step):
+    """The same host seen at a 5 deg elevation (4 deg half-height) with the sun
+    30 deg above the horizon behind a 513 km body 1e5 km away that moves
+    across it (src/motion_noise_tests.rs:172-200)."""
+    f = eclipsed_planetshine(size, step)
+    km = 1.0 / scenes.AU_KM
+    e = math.radians(5.0)
+    up = np.array([0.0, math.cos(e), -math.sin(e)])
+    north = np.array([0.0, math.sin(e), math.cos(e)])
+    center = -up * ((6371.0 + 0.002) * km)
+    sun = -north * 0.5 + up * 0.8660254
+    f.globals["cam_forward"] = [0.0, 0.0, -1.0, math.tan(math.radians(4.0))]
+    f.globals["cam_up"] = [0.0, 1.0, 0.0, 0.0]
+    f.globals["ground_up"] = [*up, 6371000.0]
+    f.globals["atmo_center"] = [*center, 6371.0 * km]
+    f.bodies = scenes.bodies(
+        scenes.body(center, 6371.0 * km, [0.0] * 3, False),
+        scenes.body(sun, 0.00465, [115384.68, 109615.445, 101538.516], True),
+        scenes.body(sun * (100_000.0 * km) + np.array([1.0, 0.0, 0.0]) * (step * 0.5 * km), 513.0 * km, [0.0] * 3,
+                    False),
+    )
+    return f
+
+
+def mean_luminance(image):
+    """Mean Rec. 709 luminance (src/motion_noise_tests.rs:88-90)."""
+    p = image.reshape(-1, 4).astype(np.float64)
+    return float(np.mean(0.2126 * p[:, 0] + 0.7152 * p[:, 1] + 0.0722 * p[:, 2]))
+
+
+def eclipse_energy(make, make_reference, passes=64):
+    """The planetshine roulette keeps the expected energy: 4096 spp (64 passes
+    of 64) of the production estimator against the same count of an
+    estimator that never thins planetshine, on the eclipsed-planetshine and
+    eclipsed-sun views, within 3% in mean luminance
+    (src/motion_noise_tests.rs:361-405)."""
+    size = (32, 18)
+    o = dict(max_bounces=1, samples_per_frame=64, sample_limit=4096)
+    out = {}
+    for name, frame in (("planetshine", eclipsed_planetshine(size, 0)), ("sun", eclipsed_sun(size, 0))):
+        expected = mean_luminance(sample(make_reference(size, opts(**o)), frame, passes))
+        actual = mean_luminance(sample(make(size, opts(**o)), frame, passes))
+        assert expected > 1e-6, (name, expected)
+        assert abs(actual / expected - 1.0) < 0.03, (name, actual, expected)
+        out[name] = (actual, expected)
+    return out
+
+
+# -- src/noise_tests.rs:5-73 (gpu_ground_noise_is_continuous_at_integer_planes)
+def noise_points():
+    """Probe points for the value-noise field: pairs straddling integer
+    lattice planes along each axis (offsets of +-1e-3, negative coordinates
+    included), the lattice points themselves, and metre-scale ground
+    patches of the reference's field q = 0.007 p + (19, 2, 4) around places
+    where q crosses integer planes."""
+    from stargaze_amd.layouts import PROBE_RAY
+
+    rng = np.random.default_rng(73)
+    pts = []
+    for axis in range(3):
+        base = rng.uniform(-60.0, 60.0, (400, 3))
+        base[:, axis] = np.round(base[:, axis])
+        lo, hi = base.copy(), base.copy()
+        lo[:, axis] -= 1e-3
+        hi[:, axis] += 1e-3
+        pts += [lo, hi]
+    lattice = np.round(rng.uniform(-60.0, 60.0, (400, 3)))
+    pts.append(lattice)
+    # Ground patches 4 m across (the reference's view from 2 m at 70 deg
+    # down) centred where 0.007 x + 19 or 0.007 z + 4 is an integer.
+    patches = []
+    for cx, cz in ((1000.0 / 7.0, 30.0), (-5.0, 1000.0 / 7.0 * 3.0 - 4.0 / 0.007), (-1000.0 / 7.0, -1000.0 / 7.0)):
+        g = np.linspace(-2.0, 2.0, 17)
+        x, z = np.meshgrid(cx + g, cz + g)
+        p = np.stack([x.ravel(), np.zeros(x.size), z.ravel()], axis=1)
+        patches.append(0.007 * p + np.array([19.0, 2.0, 4.0]))
+    pts += patches
+    q = np.concatenate(pts).astype(np.float32)
+    rays = np.zeros(len(q), PROBE_RAY)
+    rays["anchor"] = 0xFFFFFFFF
+    rays["direction"] = q
+    rays["target"] = 6  # fbm octaves
+    return rays, [len(p) for p in pts], len(patches)
+
+
+def noise_continuity(probe):
+    """Value noise is continuous and stays in [0, 1]: across every integer
+    lattice plane the field changes by at most its Lipschitz bound (the
+    smoothstep slope 1.5 per lattice unit per axis, so 3e-3 over the 2e-3
+    straddle); at lattice points it equals the corner hash; one-octave fbm is
+    the value noise itself; and a 4 m ground patch of the reference's field
+    (0.028 lattice units across) spans less than 1.5 * 0.028 * 2 = 0.084,
+    the bound its metre-scale view relies on (src/noise_tests.rs:63-72)."""
+    rays, sizes, n_patches = noise_points()
+    out = probe(rays)
+    v = out["distance"].astype(np.float64)
+    h = out["normal"][:, 0].astype(np.float64)
+    f = out["normal"][:, 1].astype(np.float64)
+    assert np.all(np.isfinite(v)) and np.all((v >= 0.0) & (v <= 1.0)), (v.min(), v.max())
+    assert np.all((h >= 0.0) & (h < 1.0)) and np.all((f >= 0.0) & (f <= 1.0))
+    at = np.cumsum([0] + sizes)
+    jumps = []
+    for axis in range(3):
+        lo = v[at[2 * axis]:at[2 * axis + 1]]
+        hi = v[at[2 * axis + 1]:at[2 * axis + 2]]
+        jumps.append(np.abs(hi - lo).max())
+    assert max(jumps) <= 3.0e-3 * 1.01 + 1e-6, jumps
+    lattice = slice(at[6], at[7])
+    assert np.array_equal(v[lattice], h[lattice]), "vnoise at lattice points is not the corner hash"
+    one = rays.copy()
+    one["target"] = 1
+    assert np.array_equal(probe(one)["normal"][:, 1], out["distance"]), "fbm(q, 1) != vnoise(q)"
+    spans = [float(np.ptp(v[at[7 + k]:at[8 + k]])) for k in range(n_patches)]
+    assert max(spans) < 0.084, spans
+    return max(jumps), spans
+
+
+# -- src/ground_tests.rs:905-1049 (gpu_forest_skylight_is_occluded_and_has_no_unlit_glow)
+def _sky_probe(t, frame, up, count):
+    from stargaze_amd.layouts import PROBE_RAY
+    from stargaze_amd.pathtracer import PROBE_SKY
+
+    rays = np.zeros(count, PROBE_RAY)
+    rays["anchor"] = 0xFFFFFFFF
+    rays["direction"] = up
+    rays["target"] = np.arange(count)
+    return t.probe(frame, PROBE_SKY, rays)
+
+
+def forest_skylight(make):
+    """The surface sky cache and its occlusion, on the lookout in daylight:
+    the 16 x 8 cache is finite, non-negative and lit (> 0.01 summed); a
+    continued accumulation leaves it unchanged; 64 cosine sky samples from
+    the observer towards the zenith gather light (> 1e-3), and under an
+    opaque canopy 3 m above the observer, with the same cache, less than a
+    thousandth of it (visibility darkens it, not an ambient gain); airless,
+    or with every star dark, the cache is zero (< 1e-12)."""
+    from stargaze_amd import scenes
+
+    day, data, cells = scenes.lookout(1, 1)
+    up = day.globals["ground_up"][:3].astype(np.float64)
+    t = make((1, 1), opts(samples_per_frame=1, max_bounces=1))
+    t.upload_terrain(data, cells)
+    sample(t, day, 1)
+    cache = _sky_probe(t, day, up, 128)["extra"].astype(np.float64)
+    assert np.all(np.isfinite(cache)) and np.all(cache >= 0.0)
+    assert cache.sum() > 0.01, cache.sum()
+    sample(t, day, 1)
+    assert np.array_equal(_sky_probe(t, day, up, 128)["extra"].astype(np.float64), cache)
+    open_sky = float(_sky_probe(t, day, up, 64)["normal"].astype(np.float64).sum())
+    assert open_sky > 0.001, open_sky
+    covered = scenes.Frame(day.globals.copy(), day.bodies.copy(), day.ground.copy())
+    covered.ground[0] = np.zeros((), covered.ground.dtype)
+    covered.ground[0]["center"] = [0.0, 5.0, 0.0, 0.0]
+    covered.ground[0]["extent"] = [10_000.0, 0.5, 10_000.0, 0.0]
+    shaded = float(_sky_probe(t, covered, up, 64)["normal"].astype(np.float64).sum())
+    assert shaded < open_sky * 0.001, (shaded, open_sky)
+    for airless in (False, True):
+        dark = scenes.Frame(day.globals.copy(), day.bodies.copy(), day.ground.copy())
+        if airless:
+            dark.globals["atmo_params"][3] = 0.0
+        else:
+            lit = dark.bodies["emissive"] > 0.5
+            dark.bodies["color"][lit] = 0.0
+        sample(t, dark, 1)
+        texels = _sky_probe(t, dark, up, 128)["extra"].astype(np.float64)
+        assert np.all(np.isfinite(texels)) and np.all(np.abs(texels) < 1e-12), (airless, np.abs(texels).max())
+    return cache.sum(), open_sky, shaded
+
+
+ALL.append(forest_skylight)
+
+
+# -- ringshine (src/shaders/pathtrace.wgsl:1236-1296): no reference test; closed form
+def _ring_bsdf_factor(tau, mo, mi):
+    """Finite-slab single scattering factor of ring_bsdf (src/shaders/rings.wgsl:151-166),
+    float64: reflection on the lit side, transmission through the slab."""
+    amo, ami = np.maximum(np.abs(mo), 1e-7), np.maximum(np.abs(mi), 1e-7)
+    a, b = tau / amo, tau / ami
+    same = mo * mi >= 0.0
+    refl = (1.0 - np.exp(-(a + b))) / (amo + ami)
+    delta = np.abs(a - b)
+    integral = np.where(delta < 1e-9, 1.0, (1.0 - np.exp(-delta)) / np.maximum(delta, 1e-300))
+    trans = tau / (amo * ami) * np.exp(-np.minimum(a, b)) * integral
+    return np.where(same, refl, trans)
+
+
+def ringshine_frame(tau=0.6, albedo=0.5):
+    """A planet (R = 1e-4 AU) with an unbanded ring (1.2-2.3 R, optical depth
+    ``tau``) seen through a one-pixel 2e-7 rad view at a night-side point at
+    40 deg ring latitude that sees the ring's sunlit face; the sun 1 AU away
+    (radius 4e-3 AU) lights the ring from above. Returns the frame and the
+    geometry for the quadrature."""
+    R = 1e-4
+    t = 1e-3
+    lat, lon = math.radians(40.0), math.radians(60.0)
+    m = np.array([math.cos(lat) * math.cos(lon), math.sin(lat), math.cos(lat) * math.sin(lon)])
+    p = np.array([0.0, 0.0, -t])
+    c = p - R * m
+    s = np.array([-0.6, 0.35, -0.7])
+    s /= np.linalg.norm(s)
+    n = np.array([0.0, 1.0, 0.0])
+    planet = scenes.body(c, R, [albedo] * 3, False)
+    planet["ring_plane"] = [*n, 1.2]
+    planet["ring_params"] = [2.3, tau, 0.0, 0.0]
+    planet["ring_color"] = [0.8, 0.7, 0.6, 0.0]
+    planet["axis_x"] = [1.0, 0.0, 0.0, 0.0]
+    planet["axis_y"] = [0.0, 0.0, 1.0, 0.0]
+    sun_radiance, sun_radius = 2.0e4, 4e-3
+    f = scenes.frame()
+    f.bodies = scenes.bodies(planet, scenes.body(s * 1.0, sun_radius, [sun_radiance] * 3, True))
+    geometry = dict(R=R, c=c, p=p, m=m, s=s, n=n, tau=tau, albedo=albedo, color=np.array([0.8, 0.7, 0.6]),
+                    sun=sun_radiance, omega=2.0 * math.pi * (1.0 - math.sqrt(1.0 - sun_radius ** 2)))
+    return f, geometry
+
+
+def ringshine_expected(g, radial=1500, around=3000):
+    """E[pixel] = albedo / pi * integral over the ring annulus of the ring's
+    single-scattered sunlight towards p, cos(surface) cos(ring) / d^2, with
+    the planet's shadow on the ring and its own horizon at p: midpoint
+    quadrature in radius and azimuth, float64."""
+    R, c, p, m, s, n = g["R"], g["c"], g["p"], g["m"], g["s"], g["n"]
+    r = R * (1.2 + 1.1 * (np.arange(radial) + 0.5) / radial)
+    phi = 2.0 * math.pi * (np.arange(around) + 0.5) / around
+    rr, pp = np.meshgrid(r, phi, indexing="ij")
+    e1, e2 = np.array([1.0, 0.0, 0.0]), np.array([0.0, 0.0, 1.0])
+    x = c + rr[..., None] * (np.cos(pp)[..., None] * e1 + np.sin(pp)[..., None] * e2)
+    v = x - p
+    d = np.linalg.norm(v, axis=-1)
+    u = v / d[..., None]
+    cos_s = u @ m
+    cos_r = np.abs(u @ n)
+    mo = -(u @ n)
+    mi = float(s @ n)
+    # Sun hidden by the planet from x: does x + t s (t > 0) meet the sphere?
+    oc = x - c
+    b = oc @ s
+    disc = b * b - (np.einsum("...i,...i", oc, oc) - R * R)
+    shadow = (disc > 0.0) & (-b - np.sqrt(np.maximum(disc, 0.0)) > 0.0)
+    term = g["color"][None, None, :] * (_ring_bsdf_factor(g["tau"], mo, mi) / (4.0 * math.pi))[..., None] * \
+        (g["sun"] * g["omega"] * abs(mi))
+    weight = np.where((cos_s > 0.0) & (cos_r >= 1e-6) & ~shadow, cos_s * cos_r / (d * d), 0.0)
+    area = rr * (1.1 * R / radial) * (2.0 * math.pi / around)
+    return g["albedo"] / math.pi * np.einsum("ijk,ij->k", term, weight * area)
+
+
+def ringshine_closed_form(make):
+    """Ringshine, the one light source of a night-side surface point: a
+    million samples at the last vertex (ringshine at full weight) against
+    the quadrature of its integral, per channel, within 4 standard errors of
+    the estimate and 1%. The ring's radiance, its area pdf and the cosines all
+    enter, so a dropped term or a wrong sampling density fails it."""
+    f, g = ringshine_frame()
+    expected = ringshine_expected(g)
+    # 32 x 32 pixels of the 2e-7 rad view all see p (to 1e-4 of R): a
+    # million samples in all.
+    t = make((32, 32), opts(samples_per_frame=64, max_bounces=1))
+    t.enable_moments(True)
+    pixels = sample(t, f, 16)[:, :3].astype(np.float64)
+    m2 = t.read_moments().reshape(-1, 4).astype(np.float64)
+    mean = pixels.mean(axis=0)
+    n = m2[:, 3].sum()
+    variance = np.mean(m2[:, :3] - pixels * pixels, axis=0) + pixels.var(axis=0)
+    se = np.sqrt(np.maximum(variance, 0.0) / n)
+    assert np.all(expected > 0.0), expected
+    assert np.all(np.abs(mean - expected) <= 4.0 * se + 0.01 * expected), (mean, expected, se)
+    return mean, expected, se
+
+
+ALL.append(ringshine_closed_form)

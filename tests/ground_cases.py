@@ -1,0 +1,91 @@
+"""Ground-material probe rays for the Halo scene (theme floors, shading
+normals and prop materials, src/shaders/ground.wgsl:256-648).
+
+Points lie on a 48 x 48 grid over a disc of 3 km around the observer, which
+covers the alpine theme's crevasse rows (650-2600 m), the dune theme's pans
+and corridors and the forest floor. Each point is found by a straight-down
+probe from 1.5 km above the observer's level in the FP64 oracle; the
+material is then taken there as seen from 2, 15 or 120 m, so the
+footprint-dependent detail is exercised at three scales.
+"""
+import numpy as np
+
+from stargaze_amd import scenes
+from stargaze_amd.layouts import MISS, PROBE_RAY
+from stargaze_amd.pathtracer import PROBE_LANDSCAPE
+
+HEIGHTS = (2.0, 15.0, 120.0)
+LOCAL_BASE = 0x80000000
+
+
+def theme_frame(theme, size=(32, 18)):
+    frame, data, cells = scenes.halo_parity(*size)
+    frame.globals["ground_counts"][2] = theme
+    return frame, data, cells
+
+
+def material_rays(frame, oracle64, grid=48, reach=3000.0, top=1500.0):
+    """SG_PROBE_GROUND records: for each grid point, the landscape point
+    under it (found by a straight-down FP64 landscape probe from ``top``
+    metres), its surface-frame normal and element, seen from 2, 15 or 120 m
+    (see the module docstring). Both tracers evaluate the ground material
+    at exactly these float32 inputs."""
+    g = frame.globals
+    east = g["ground_east"][:3].astype(np.float64)
+    up = g["ground_up"][:3].astype(np.float64)
+    north = g["ground_north"][:3].astype(np.float64)
+    camera_height = float(g["ground_east"][3])
+    s = np.linspace(-reach, reach, grid)
+    x, z = np.meshgrid(s, s)
+    keep = x * x + z * z <= reach * reach
+    x, z = x[keep], z[keep]
+    au = scenes.METRES_PER_AU
+    down = np.zeros(x.size, PROBE_RAY)
+    down["anchor"] = MISS
+    down["offset"] = (x[:, None] * east + z[:, None] * north + top * up) / au
+    down["direction"] = -up
+    first = oracle64.probe(frame, PROBE_LANDSCAPE, down)
+    hit = (first["index"] != MISS) & (first["index"] >= LOCAL_BASE)
+    x, z, first = x[hit], z[hit], first[hit]
+    depth = first["distance"].astype(np.float64) * au
+    n = first["normal"].astype(np.float64)
+    rays = np.zeros(x.size, PROBE_RAY)
+    rays["offset"] = np.stack([x, camera_height + top - depth, z], axis=1)
+    rays["direction"] = np.stack([n @ east, n @ up, n @ north], axis=1)
+    rays["anchor"] = first["index"]
+    seen = np.array([HEIGHTS[i % len(HEIGHTS)] for i in range(x.size)], np.float32)
+    rays["target"] = seen.view(np.uint32)
+    return rays
+
+
+KIND_NAMES = {0.0: "box", 1.0: "ellipsoid", 2.0: "rock", 2.25: "tumbled rock", 3.0: "conifer", 4.0: "grass",
+              5.0: "log"}
+
+
+def prop_rays(frame, per_prop=24, seed=5):
+    """Camera rays aimed at random points inside each leaf primitive's
+    rotated box (90% of its half extents), and the kind each ray aims at."""
+    from stargaze_amd import ground as gr
+
+    g = frame.globals
+    east = g["ground_east"][:3].astype(np.float64)
+    up = g["ground_up"][:3].astype(np.float64)
+    north = g["ground_north"][:3].astype(np.float64)
+    camera_height = float(g["ground_east"][3])
+    rng = np.random.default_rng(seed)
+    dirs, kinds = [], []
+    for p in frame.ground:
+        kind = float(p["center"][3])
+        if kind < 0.0:
+            continue
+        c = p["center"][:3].astype(np.float64)
+        e = gr.world_extent(p)
+        pts = c + rng.uniform(-0.9, 0.9, (per_prop, 3)) * e
+        pts[:, 1] -= camera_height
+        w = pts[:, :1] * east + pts[:, 1:2] * up + pts[:, 2:3] * north
+        dirs.append(w / np.linalg.norm(w, axis=1, keepdims=True))
+        kinds += [kind] * per_prop
+    rays = np.zeros(sum(len(d) for d in dirs), PROBE_RAY)
+    rays["anchor"] = MISS
+    rays["direction"] = np.concatenate(dirs)
+    return rays, np.array(kinds)

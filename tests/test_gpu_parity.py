@@ -149,7 +149,7 @@ def test_images_match_oracle_per_tile(scene, size, spp, bounces, batches):
     z = parity.tile_z(ga, gm, oa, om)
     over, worst = parity.assert_tiles(z)
     print(f"{scene}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
-    assert agree >= 0.9, agree
+    assert agree >= 0.99, agree
 
 
 def test_tabulated_ring_matches_oracle_per_tile():
@@ -162,7 +162,7 @@ def test_tabulated_ring_matches_oracle_per_tile():
     z = parity.tile_z(ga, gm, oa, om)
     over, worst = parity.assert_tiles(z)
     print(f"tabulated ring: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
-    assert agree >= 0.9, agree
+    assert agree >= 0.99, agree
 
 
 def test_constant_ring_table_is_the_unbanded_ring():
@@ -259,7 +259,7 @@ def test_halo_scene_matches_oracle_per_tile():
     over, worst = parity.assert_tiles(z)
     print(f"halo: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}, "
           f"mean {ga[..., :3].mean():.4g} vs {oa[..., :3].mean():.4g}")
-    assert agree >= 0.5, agree
+    assert agree >= 0.99, agree
 
 
 def test_landscape_probe_matches_oracle():
@@ -294,7 +294,10 @@ def test_landscape_probe_matches_oracle():
     hit = same & (r["index"] != MISS)
     rel = np.abs(g["distance"][hit] - r["distance"][hit]) / r["distance"][hit]
     print(f"landscape: {hit.sum()} hits, same index {same.mean():.4f}, median rel {np.median(rel):.2e}, max {rel.max():.2e}")
-    assert np.median(rel) < 1e-5 and rel.max() < 1e-3
+    # Host sphere (FP64 on both sides), terrain triangles and boxes and
+    # ellipsoids in FP32: the median is at the FP32 rounding of the distance,
+    # the worst case a grazing terrain triangle (measured 1.9e-6).
+    assert np.median(rel) <= 1e-6 and rel.max() <= 1e-5
 
 
 @pytest.mark.parametrize("scene", ["airless_triple", "ringed_giant", "halo"])
@@ -415,3 +418,103 @@ def test_wavefront_tail_and_autotune(scene):
 
     img = render(tuned)
     parity.assert_same_draws(img, base, exact=scene != "halo")
+
+
+def test_eclipse_energy_against_the_oracle():
+    """src/motion_noise_tests.rs:361-405: 4096 spp of the HIP production
+    estimator against 4096 spp of the oracle's never-thinned estimator, on
+    the eclipsed-planetshine and eclipsed-sun views, within 3% in mean
+    luminance."""
+    def reference(dimensions, options):
+        t = oracle.OracleTracer(dimensions, options, "f32")
+        t.set_planetshine_thinning(False)
+        return t
+
+    out = cases.eclipse_energy(make_gpu, reference)
+    for name, (actual, expected) in out.items():
+        print(f"{name}: gpu {actual:.6e} oracle {expected:.6e} ratio {actual / expected:.5f}")
+
+
+def test_value_noise_is_continuous_and_matches_oracle():
+    """src/noise_tests.rs:5-73 restated on the HIP noise (SG_PROBE_NOISE),
+    and the HIP hash, value noise and 6-octave fbm against the FP32 oracle
+    point by point."""
+    gpu = PathTracer((1, 1), options=Options(milky_way=0.0))
+    probe = lambda rays: gpu.probe(scenes.frame(), ptm.PROBE_NOISE, rays)  # noqa: E731
+    jumps, spans = cases.noise_continuity(probe)
+    rays, _, _ = cases.noise_points()
+    got = probe(rays)
+    want = oracle.OracleTracer((1, 1), Options(milky_way=0.0), "f32").probe(scenes.frame(), ptm.PROBE_NOISE, rays)
+    dv = np.abs(got["distance"] - want["distance"]).max()
+    dh = np.abs(got["normal"][:, 0] - want["normal"][:, 0]).max()
+    df = np.abs(got["normal"][:, 1] - want["normal"][:, 1]).max()
+    print(f"noise: worst plane jump {jumps:.2e}, patch spans {spans}; vs oracle: vnoise {dv:.2e}, "
+          f"hash {dh:.2e}, fbm {df:.2e}")
+    assert dv <= 1e-6 and dh <= 1e-6 and df <= 1e-6
+    gpu.close()
+
+
+@pytest.mark.parametrize("theme", [0, 1], ids=["forest", "desert rocks"])
+def test_prop_kinds_match_oracle(theme):
+    """Camera rays at every primitive of the reference's lookout scenery
+    (boxes, stone rocks, tumbled rocks, conifers, grass, logs) through the
+    HIP prop BVH and terrain, against the FP32 oracle's flat loop and the
+    FP64 build: the same element, median relative distance within 1e-6, and
+    the 99th percentile within what FP32 reaches per kind (as
+    tests/test_ground.py records it for the FP32 oracle against FP64)."""
+    import ground_cases as gc
+
+    f, data, cells = scenes.lookout(48, 27, theme=theme)
+    rays, kinds = gc.prop_rays(f)
+    gpu = PathTracer((4, 4), options=Options(milky_way=0.0))
+    gpu.upload_terrain(data, cells)
+    g = gpu.probe(f, ptm.PROBE_LANDSCAPE, rays)
+    gpu.close()
+    o32 = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f32")
+    o32.upload_terrain(data, cells)
+    o64 = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64")
+    o64.upload_terrain(data, cells)
+    b, a = o32.probe(f, ptm.PROBE_LANDSCAPE, rays), o64.probe(f, ptm.PROBE_LANDSCAPE, rays)
+    p99_fp64 = {0.0: 1e-6, 2.0: 1e-6, 2.25: 1e-6, 4.0: 1e-5, 5.0: 1e-5, 3.0: 1e-3}
+    for k in np.unique(kinds):
+        sel = kinds == k
+        name = gc.KIND_NAMES[k]
+        same32 = g["index"][sel] == b["index"][sel].astype(np.uint32)
+        same64 = g["index"][sel] == a["index"][sel].astype(np.uint32)
+        hit32 = same32 & (g["index"][sel] != MISS)
+        hit64 = same64 & (g["index"][sel] != MISS)
+        r32 = np.abs(g["distance"][sel][hit32] - b["distance"][sel][hit32]) / b["distance"][sel][hit32]
+        r64 = np.abs(g["distance"][sel][hit64] - a["distance"][sel][hit64]) / a["distance"][sel][hit64]
+        print(f"{name}: {sel.sum()} rays; vs fp32 same {same32.mean():.4f} p99 {np.percentile(r32, 99):.2e} "
+              f"max {r32.max():.2e}; vs fp64 same {same64.mean():.4f} median {np.median(r64):.2e} "
+              f"p99 {np.percentile(r64, 99):.2e} max {r64.max():.2e}")
+        assert same32.mean() >= 0.995 and same64.mean() >= 0.99, name
+        assert np.median(r32) <= 1e-6 and np.percentile(r32, 99) <= p99_fp64[k], name
+        assert np.median(r64) <= 1e-6 and np.percentile(r64, 99) <= p99_fp64[k], name
+
+
+@pytest.mark.parametrize("theme", [0, 1], ids=["forest", "desert rocks"])
+def test_lookout_matches_oracle_per_tile(theme):
+    """The lookout scenery rendered with air, sky cache and one sun: every
+    primitive kind's shape and material (stone, foliage, bark, grass) against
+    the FP32 oracle per 16x16 tile."""
+    size = (96, 54)
+    f, data, cells = scenes.lookout(*size, theme=theme)
+    opts = dict(samples_per_frame=8, max_bounces=4, milky_way=0.0)
+    gpu = PathTracer(size, options=Options(**opts))
+    gpu.upload_terrain(data, cells)
+    gpu.enable_moments(True)
+    orc = oracle.OracleTracer(size, Options(**opts), "f32")
+    orc.upload_terrain(data, cells)
+    orc.enable_moments(True)
+    for _ in range(2):
+        gpu.encode(f)
+        orc.encode(f)
+    ga, gm, oa, om = gpu.read_accumulation(), gpu.read_moments(), orc.read_accumulation(), orc.read_moments()
+    assert np.all(np.isfinite(ga)) and np.all(ga[..., :3] >= 0)
+    agree = parity.pixel_agreement(ga, oa)
+    z = parity.tile_z(ga, gm, oa, om)
+    over, worst = parity.assert_tiles(z)
+    print(f"lookout theme {theme}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
+    assert agree >= 0.99, agree
+    gpu.close()

@@ -95,3 +95,29 @@ def test_oracle_constant_ring_table_is_the_unbanded_ring():
         images.append(t.read_accumulation())
     assert np.array_equal(images[0], images[1])
     assert not np.array_equal(images[0], images[2])
+
+
+def test_oracle_eclipse_energy_against_unthinned_planetshine():
+    """src/motion_noise_tests.rs:361-405 on the oracle: production against
+    the never-thinned estimator of the same oracle. In the planetshine view
+    the eclipsed air sees only the moon's light, thinned once samples
+    accumulate (the two means differ by 2e-4); the sun view has no
+    reflector, so there the estimates are equal."""
+    def reference(dimensions, options):
+        t = oracle.OracleTracer(dimensions, options, "f32")
+        t.set_planetshine_thinning(False)
+        return t
+
+    out = cases.eclipse_energy(lambda d, o: make_oracle(d, o), reference)
+    print(out)
+    assert out["planetshine"][0] != out["planetshine"][1], "the thinning did not act"
+
+
+@pytest.mark.parametrize("precision", ["f32", "f64"])
+def test_oracle_value_noise_is_continuous(precision):
+    t = oracle.OracleTracer((1, 1), cases.opts(), precision)
+    from stargaze_amd import scenes
+    from stargaze_amd.pathtracer import PROBE_NOISE
+
+    jumps, spans = cases.noise_continuity(lambda rays: t.probe(scenes.frame(), PROBE_NOISE, rays))
+    print(precision, jumps, spans)

@@ -310,8 +310,10 @@ const char *sg_last_error(const sg_ctx *ctx);
  * SG_GRAD_COUNT entries, laid out as below; parameters are the stored values
  * (sg_sphere.color, tau bins, atmo_rayleigh rgb and w, atmo_params x,
  * density bins; the density gradient is zero unless a profile is set).
- * Scenes with a landscape are rejected. Body albedos are differentiated for
- * surface style 0; gas-giant albedos get zero. `pass_mean` (optional,
+ * Scenes with a landscape, or with a lit body of a solid-body surface
+ * style (2-6), are rejected. Body albedos are differentiated for surface
+ * styles 0 (mottled colour) and 1 (gas giant: d albedo / d zone colour,
+ * material.rgb). `pass_mean` (optional,
  * W*H*4 floats) receives the pass mean the adjoint recomputed.
  * ---------------------------------------------------------------------- */
 #define SG_GRAD_ALBEDO(body, channel) (3u * (body) + (channel))

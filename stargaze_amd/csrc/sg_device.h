@@ -415,14 +415,25 @@ SG_FN f3 seed_offset(float seed) {
 #define SG_OUTLINE static inline
 #endif
 // src/shaders/pathtrace.wgsl:394-470
-SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
+// The gas giant's colour is a chain of component-wise steps in its zone and
+// belt colours (material.rgb, surface_color.rgb) whose weights depend only
+// on the point: gas_giant_pattern finds the weights, giant_channel runs the
+// chain for one channel, in plain floats for the albedo and in value and
+// derivative pairs for d albedo / d zone.
+struct GiantPattern {
+    float bands, tone, shade, fine, polar;
+    uint32_t storms;
+    float masks[3], rims[3], cores[3];
+};
+SG_FN GiantPattern gas_giant_pattern(const Body &b, f3 local) {
     f3 off = seed_offset(b.axis_y.w);
-    f3 zone = xyz(b.material), belt = xyz(b.surface_color);
     float turbulence = b.surface_color.w;
     float bands = b.surface_params.x;
     uint32_t storms = (uint32_t)(b.surface_params.y + 0.5f);
     f3 q = local;
-    float masks[3] = {0, 0, 0}, rims[3] = {0, 0, 0};
+    GiantPattern g;
+    g.storms = storms < 3u ? storms : 3u;
+    for (uint32_t k = 0; k < 3u; ++k) g.masks[k] = g.rims[k] = 0.0f;
     for (uint32_t k = 0; k < storms && k < 3u; ++k) {
         float fk = (float)k;
         float lat = (hash31(off + f3{7.1f * fk, 1.3f, 2.9f}) - 0.5f) * 1.1f;
@@ -442,8 +453,8 @@ SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
         float ry = s * dx + c * dy;
         q = normalize(q + east * ((rx - dx) * 2.0f * size) + north * ((ry - dy) * size));
         float ragged = e + 0.25f * (fbm(q * 18.0f + off, 3) - 0.5f);
-        masks[k] = 1.0f - smoothstepf(0.7f, 1.0f, ragged);
-        rims[k] = smoothstepf(0.75f, 1.0f, ragged) * (1.0f - smoothstepf(1.0f, 1.35f, ragged));
+        g.masks[k] = 1.0f - smoothstepf(0.7f, 1.0f, ragged);
+        g.rims[k] = smoothstepf(0.75f, 1.0f, ragged) * (1.0f - smoothstepf(1.0f, 1.35f, ragged));
     }
     f3 p = f3{q.x * 2.2f, q.y * 2.2f, q.z * 9.0f} + off;
     float warp1 = fbm(p, 5) - 0.5f;
@@ -460,18 +471,42 @@ SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
     float tone = fbm(f3{mu * bands * 0.9f, off.z, off.x}, 2);
     float fine = fbm(f3{q.x * 7.0f, q.y * 7.0f, q.z * 70.0f} + off + f3s(3.0f * warp2), 4);
     float eddies = fbm(p * 5.0f + f3{warp1, warp2, warp1} * 6.0f, 4);
-    f3 color = mix3(belt, zone, smoothstepf(0.3f, 0.75f, profile));
-    color = mix3(color, color * (f3s(0.55f) + color * 0.5f), smoothstepf(0.35f, 0.75f, tone) * 0.8f);
-    color = color * ((0.78f + 0.44f * fine) * (0.9f + 0.2f * eddies));
-    float polar = smoothstepf(0.55f, 0.92f, fabsf(local.z));
-    color = mix3(color, mix3(zone, belt, 0.55f) * (0.8f + 0.2f * fine), polar * 0.8f);
-    for (uint32_t k = 0; k < storms && k < 3u; ++k) {
-        f3 tint = k == 0u ? belt * belt * 1.5f : zone * 1.15f;
-        float core = 1.0f - 0.25f * (1.0f - smoothstepf(0.0f, 0.6f, masks[k] - rims[k]));
-        color = mix3(color, tint * (0.7f + 0.6f * fine) * core, masks[k] * 0.92f);
-        color = mix3(color, zone * 1.1f, rims[k] * 0.45f);
+    g.bands = smoothstepf(0.3f, 0.75f, profile);
+    g.tone = smoothstepf(0.35f, 0.75f, tone) * 0.8f;
+    g.shade = (0.78f + 0.44f * fine) * (0.9f + 0.2f * eddies);
+    g.fine = fine;
+    g.polar = smoothstepf(0.55f, 0.92f, fabsf(local.z)) * 0.8f;
+    for (uint32_t k = 0; k < 3u; ++k)
+        g.cores[k] = 1.0f - 0.25f * (1.0f - smoothstepf(0.0f, 0.6f, g.masks[k] - g.rims[k]));
+    return g;
+}
+// A value with its derivative along one parameter.
+struct Dual {
+    float v, d;
+};
+SG_FN Dual operator+(Dual a, Dual b) { return {a.v + b.v, a.d + b.d}; }
+SG_FN Dual operator*(Dual a, Dual b) { return {a.v * b.v, a.d * b.v + a.v * b.d}; }
+SG_FN Dual operator*(Dual a, float s) { return {a.v * s, a.d * s}; }
+SG_FN Dual operator+(Dual a, float s) { return {a.v + s, a.d}; }
+SG_FN Dual mix_t(Dual a, Dual b, float t) { return a * (1.0f - t) + b * t; }
+SG_FN float mix_t(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+template <class T>
+SG_FN T giant_channel(const GiantPattern &g, T zone, T belt) {
+    T color = mix_t(belt, zone, g.bands);
+    color = mix_t(color, color * (color * 0.5f + 0.55f), g.tone);
+    color = color * g.shade;
+    color = mix_t(color, mix_t(zone, belt, 0.55f) * (0.8f + 0.2f * g.fine), g.polar);
+    for (uint32_t k = 0; k < g.storms; ++k) {
+        T tint = k == 0u ? belt * belt * 1.5f : zone * 1.15f;
+        color = mix_t(color, tint * (0.7f + 0.6f * g.fine) * g.cores[k], g.masks[k] * 0.92f);
+        color = mix_t(color, zone * 1.1f, g.rims[k] * 0.45f);
     }
     return color;
+}
+SG_FN f3 gas_giant_albedo(const Body &b, f3 local) {
+    const GiantPattern g = gas_giant_pattern(b, local);
+    return f3{giant_channel(g, b.material.x, b.surface_color.x), giant_channel(g, b.material.y, b.surface_color.y),
+              giant_channel(g, b.material.z, b.surface_color.z)};
 }
 SG_FN uint32_t surface_style(const Body &b) { return (uint32_t)(b.axis_x.w + 0.5f); }
 // --- Solid-body surfaces (src/shaders/pathtrace.wgsl:472-689) ------------
@@ -843,14 +878,30 @@ SG_FN f3 surface_albedo(const Body &b, f3 normal) {
     if (surface_style(b) > 0u) return clamp3(style_albedo(b, local), 0.0f, 1.0f);
     return clamp3(xyz(b.material) * surface_mottle(b, local), 0.0f, 1.0f);
 }
-// d albedo / d material.rgb for style 0 (zero where clamped, and for the
-// procedural styles, whose albedos the adjoint does not differentiate).
-SG_FN f3 surface_albedo_derivative(const Body &b, f3 normal) {
-    if (surface_style(b) > 0u) return f3s(0.0f);
-    const float m = surface_mottle(b, body_local(b, normal));
-    f3 a = xyz(b.material) * m;
-    return f3{(a.x > 0.0f && a.x < 1.0f) ? m : 0.0f, (a.y > 0.0f && a.y < 1.0f) ? m : 0.0f,
-              (a.z > 0.0f && a.z < 1.0f) ? m : 0.0f};
+// surface_albedo and, in `derivative`, d albedo / d material.rgb for styles 0
+// and 1 (zero where clamped, and for the solid-body styles 2-6, which the
+// adjoint rejects). The gas giant's pattern is found once for both.
+SG_FN f3 surface_albedo_with_derivative(const Body &b, f3 normal, f3 &derivative) {
+    const f3 local = body_local(b, normal);
+    const uint32_t style = surface_style(b);
+    f3 a, d = f3s(0.0f);
+    if (style == 1u) {
+        const GiantPattern g = gas_giant_pattern(b, local);
+        const Dual x = giant_channel(g, Dual{b.material.x, 1.0f}, Dual{b.surface_color.x, 0.0f});
+        const Dual y = giant_channel(g, Dual{b.material.y, 1.0f}, Dual{b.surface_color.y, 0.0f});
+        const Dual z = giant_channel(g, Dual{b.material.z, 1.0f}, Dual{b.surface_color.z, 0.0f});
+        a = f3{x.v, y.v, z.v};
+        d = f3{x.d, y.d, z.d};
+    } else if (style == 0u) {
+        const float m = surface_mottle(b, local);
+        a = xyz(b.material) * m;
+        d = f3s(m);
+    } else {
+        a = style_albedo(b, local);
+    }
+    derivative = f3{(a.x > 0.0f && a.x < 1.0f) ? d.x : 0.0f, (a.y > 0.0f && a.y < 1.0f) ? d.y : 0.0f,
+                    (a.z > 0.0f && a.z < 1.0f) ? d.z : 0.0f};
+    return clamp3(a, 0.0f, 1.0f);
 }
 
 }  // namespace sg

@@ -36,6 +36,18 @@ constexpr uint32_t COLUMN_BIN_SLOTS = 16u;
 // The plain integrator: every gradient hook compiles away.
 struct NoGrad {
     static constexpr bool on = false;
+    static constexpr bool capture = false;
+};
+// A primal run for the adjoint: no derivatives, but the ring indirect cap's
+// make-up is kept where the cap is set (path_shade): the event (depth and
+// null crossings), which term bound it, and the colour of that term.
+struct CapTrace {
+    static constexpr bool on = false;
+    static constexpr bool capture = true;
+    mutable bool set = false;
+    mutable bool via_direct = false;
+    mutable uint32_t depth = 0u, nulls = 0u;
+    mutable f3 vector = f3{0.0f, 0.0f, 0.0f};
 };
 // Camera-air shadow rays answer their landscape queries inline (NoDefer), or
 // hand them to a deferral sink that queues them for a kernel of its own (the
@@ -53,6 +65,7 @@ namespace {
 // on the GPU). Every method takes d(loss) already multiplied out.
 struct GradSink {
     static constexpr bool on = true;
+    static constexpr bool capture = false;
     float *g;
     // Lane-private accumulators, slot k at dens[k * dstride]: the density
     // profile bins, then the five air coefficients (every contribution in
@@ -713,7 +726,8 @@ SG_FN Scatter atmosphere(const Scene &sc, f3 origin, f3 dir, float limit, uint32
             Hit rh = sphere_hit(sc, reflected_ray, reflector.index);
             if (rh.index != SG_MISS) {
                 const Body &rb = body(sc, reflector.index);
-                f3 a = surface_albedo(rb, rh.normal), d = surface_albedo_derivative(rb, rh.normal);
+                f3 d;
+                const f3 a = surface_albedo_with_derivative(rb, rh.normal, d);
                 reflector_dalbedo = f3{a.x > 0.0f ? d.x / a.x : 0.0f, a.y > 0.0f ? d.y / a.y : 0.0f,
                                        a.z > 0.0f ? d.z / a.z : 0.0f};
             }
@@ -1108,20 +1122,61 @@ struct PathState {
     float previous_pdf, ring_cap;
     uint32_t rng, depth, null_crossings, shine_body, flags, accumulated;
 };
-// Path replay adjoint state: the primal run's returned radiance, the loss
-// weight of the path's radiance, and the ring cap's scale on the indirect
-// part (taken as a constant). rest() is d(loss)/d log(a throughput factor
-// applied now): every later contribution carries that factor.
+// Path replay adjoint state: the primal run's returned radiance and the loss
+// weight w = d(loss)/d(radiance) of the path sample.
+//
+// The ring indirect cap: where a path first scatters off a ring the cap
+// c = max(direct, RING_INDIRECT_CAP * unshadowed) is set from that event's
+// lights and the radiance so far becomes the base B; the returned radiance is
+// B + I * min(1, c / lum(I)) for the indirect radiance I gathered after it.
+// When the cap binds (s = c / lum(I) < 1) the return is B + I * c / lum(I),
+// whose derivative the replay takes in full: with alpha = dot(w, I) / lum(I),
+//  - terms added after the cap weigh s * (w - alpha * lum_weights), the
+//    luminance of the capped part being held at c;
+//  - a throughput factor before the cap scales B's rest, I and c alike, and
+//    so weighs w * (value - radiance so far) + alpha * lum_weights *
+//    (D - s I), D being the colour of the binding cap term;
+//  - a factor after it scales only I's rest;
+//  - the cap event's own light terms add alpha * d c.
+// rest() is d(loss)/d log(a throughput factor applied now).
 struct PathGrad : GradSink {
     f3 w;
     f3 value;
-    float cap_scale;
-    SG_MEMBER bool capped(const PathState &st) const { return st.ring_cap < 1.0e29f; }
-    SG_MEMBER f3 so_far(const PathState &st) const {
-        return capped(st) ? st.ring_base + (st.radiance - st.ring_base) * cap_scale : st.radiance;
+    bool capped = false;
+    bool cap_via_direct = false;
+    uint32_t cap_depth = 0u, cap_nulls = 0u;
+    float alpha = 0.0f, scale = 1.0f;
+    f3 wl = f3{0.0f, 0.0f, 0.0f};        // alpha * luminance weights
+    f3 indirect = f3{0.0f, 0.0f, 0.0f};  // s * I, the capped indirect radiance
+    f3 before = f3{0.0f, 0.0f, 0.0f};    // alpha * lum_weights * (D - s I)
+    SG_MEMBER bool past_cap(const PathState &st) const { return capped && st.ring_cap < 1.0e29f; }
+    SG_MEMBER f3 weight(const PathState &st) const { return past_cap(st) ? (w - wl) * scale : w; }
+    SG_MEMBER f3 rest(const PathState &st) const {
+        if (!capped) return w * (value - st.radiance);
+        if (st.ring_cap >= 1.0e29f) return w * (value - st.radiance) + before;
+        return (w - wl) * (indirect - (st.radiance - st.ring_base) * scale);
     }
-    SG_MEMBER f3 weight(const PathState &st) const { return capped(st) ? w * cap_scale : w; }
-    SG_MEMBER f3 rest(const PathState &st) const { return w * (value - so_far(st)); }
+    // Whether the ring event the path is at sets the cap that bound.
+    SG_MEMBER bool cap_event(const PathState &st) const {
+        return capped && st.ring_cap >= 1.0e29f && st.depth == cap_depth && st.null_crossings == cap_nulls;
+    }
+    // The cap's make-up from the primal run `st` that `cap` traced.
+    SG_MEMBER void set_cap(const PathState &st, const CapTrace &cap) {
+        capped = false;
+        if (!cap.set || st.ring_cap >= 1.0e29f) return;
+        const f3 in = st.radiance - st.ring_base;
+        const float lum = luminance(in);
+        if (!(lum > st.ring_cap)) return;
+        capped = true;
+        cap_via_direct = cap.via_direct;
+        cap_depth = cap.depth;
+        cap_nulls = cap.nulls;
+        scale = st.ring_cap / lum;
+        alpha = dot(w, in) / lum;
+        wl = f3{0.2126f, 0.7152f, 0.0722f} * alpha;
+        indirect = in * scale;
+        before = wl * (cap.vector - indirect);
+    }
 };
 }  // namespace
 
@@ -1251,6 +1306,9 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
         uint32_t bin = 0u;
         if constexpr (G::on) bin = ring_profile_bin(ring_fraction(planet, ring.point));
         float unshadowed = 0.0f, direct = 0.0f;
+        f3 direct_colour = f3s(0.0f), unshadowed_colour = f3s(0.0f);  // for CapTrace
+        bool cap_event = false;
+        if constexpr (G::on) cap_event = grad.cap_event(st);
         for (uint32_t slot = 0; slot < light_count(sc); ++slot) {
             uint32_t i = light_index(sc, slot);
             if (body(sc, i).material.w < 0.5f || optics.w == 0.0f) continue;
@@ -1270,6 +1328,21 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
                            (f.y / (4.0f * PI) * fabsf(mu_in) / light.pdf * visibility * weight * ring_mis);
                     grad.ring(bin, d.x + d.y + d.z);
                 }
+                if (cap_event) {
+                    // alpha * d c: the cap's light terms (the throughput in
+                    // them is the pre-cap rest's).
+                    const float l = grad.alpha * luminance(lit);
+                    if (grad.cap_via_direct) grad.shadow(sc, shadow, i, l * visibility);
+                    if (tabulated) {
+                        float2 f = ring_bsdf_factor(optics.w, mu_out, mu_in);
+                        if (f.x > 0.0f)
+                            grad.ring(bin, l * (f.y / f.x) * (grad.cap_via_direct ? visibility : RING_INDIRECT_CAP));
+                    }
+                }
+            }
+            if constexpr (G::capture) {
+                direct_colour += lit * visibility;
+                unshadowed_colour += lit;
             }
             st.radiance += lit * (visibility * weight * ring_mis);
             unshadowed += luminance(lit);
@@ -1290,6 +1363,13 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
         if (st.ring_cap > 1.0e29f) {
             st.ring_cap = fmaxf(direct, RING_INDIRECT_CAP * unshadowed);
             st.ring_base = st.radiance;
+            if constexpr (G::capture) {
+                grad.set = true;
+                grad.via_direct = direct >= RING_INDIRECT_CAP * unshadowed;
+                grad.vector = grad.via_direct ? direct_colour : unshadowed_colour * RING_INDIRECT_CAP;
+                grad.depth = st.depth;
+                grad.nulls = st.null_crossings;
+            }
         }
         float z = 1.0f - 2.0f * rnd(st.rng);
         float phi = TAU * rnd(st.rng);
@@ -1348,11 +1428,10 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
     Ray outgoing = local ? landscape_outgoing(sc, ray, hit)
                          : Ray{tod(normal * (hb->center.w * (1.0f + 2.0e-6f))), normal, hit.index};
     // Material derivative and the weight of this vertex's direct light.
-    f3 dalbedo = f3s(0.0f), wt_direct = f3s(0.0f);
+    f3 dalbedo = f3s(0.0f), wt_direct = f3s(0.0f), body_albedo = f3s(0.0f);
     if constexpr (G::on) {
-        f3 a = local ? f3s(0.0f) : surface_albedo(*hb, normal);
-        if (!local) dalbedo = surface_albedo_derivative(*hb, normal);
-        wt_direct = grad.weight(st) * st.throughput * a;
+        if (!local) body_albedo = surface_albedo_with_derivative(*hb, normal, dalbedo);
+        wt_direct = grad.weight(st) * st.throughput * body_albedo;
     }
     f3 direct_light = f3s(0.0f);
     bool last_vertex = st.depth + 1u == max_vertices;
@@ -1384,7 +1463,9 @@ SG_FN bool path_shade(const Scene &sc, PathState &st, const HitRecord &h, const 
             }
         }
     }
-    f3 albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
+    f3 albedo;
+    if constexpr (G::on) albedo = local ? landscape_albedo(sc, ray, hit) : body_albedo;
+    else albedo = local ? landscape_albedo(sc, ray, hit) : surface_albedo(*hb, normal);
     if constexpr (G::on) {
         if (!local) grad.albedo(hit.index, grad.weight(st) * st.throughput * dalbedo * direct_light);
     }
@@ -1471,7 +1552,8 @@ SG_FN float ring_cap_scale(const PathState &st) {
 }
 // The replay half of a path sample's adjoint: given the primal run's returned
 // radiance and ring-cap scale, rerun the path with the same draws adding
-// d(loss)/d(parameters) for d(loss)/d(radiance) = w.
+// d(loss)/d(parameters) for d(loss)/d(radiance) = w. A path whose cap bound
+// (scale below 1) first reruns its primal to learn the cap's make-up.
 SG_FN void replay_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated, f3 w, f3 value,
                        float cap_scale, float *gradient, float *density, uint32_t density_stride) {
     PathState st;
@@ -1481,7 +1563,12 @@ SG_FN void replay_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t acc
     grad.dstride = density_stride;
     grad.w = w;
     grad.value = value;
-    grad.cap_scale = cap_scale;
+    if (cap_scale < 1.0f) {
+        CapTrace cap;
+        path_begin(st, direction, rng, accumulated);
+        run_path(sc, st, cap);
+        grad.set_cap(st, cap);
+    }
     path_begin(st, direction, rng, accumulated);
     run_path(sc, st, grad);
     grad.flush_private();
@@ -1493,8 +1580,9 @@ SG_FN void replay_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t acc
 SG_FN PathResult adjoint_path(const Scene &sc, f3 direction, uint32_t rng, uint32_t accumulated, f3 w,
                               float *gradient, float *density, uint32_t density_stride) {
     PathState st;
+    CapTrace cap;
     path_begin(st, direction, rng, accumulated);
-    run_path(sc, st);
+    run_path(sc, st, cap);
     PathResult primal = path_finish(st);
     PathGrad grad;
     grad.g = gradient;
@@ -1502,7 +1590,7 @@ SG_FN PathResult adjoint_path(const Scene &sc, f3 direction, uint32_t rng, uint3
     grad.dstride = density_stride;
     grad.w = w;
     grad.value = f3{primal.value.x, primal.value.y, primal.value.z};
-    grad.cap_scale = ring_cap_scale(st);
+    grad.set_cap(st, cap);
     path_begin(st, direction, rng, accumulated);
     run_path(sc, st, grad);
     grad.flush_private();

@@ -43,22 +43,18 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(SG_
             f3 direction = camera_direction(sc, x, y, rng);
             const float *d = aa.d_image + 3u * (size_t)pixel;
             f3 w = f3{d[0], d[1], d[2]} * inv_count;
-            // The primal's returned radiance and ring-cap scale: recorded by
-            // the forward pass, or rerun here.
-            float4 primal;
+            // The primal's returned radiance and ring-cap scale recorded by
+            // the forward pass, or the primal rerun here.
             if (aa.recorded) {
-                primal = aa.recorded[(size_t)pixel * ta.count + s];
+                const float4 primal = aa.recorded[(size_t)pixel * ta.count + s];
+                replay_path(sc, direction, rng, accumulated, w, f3{primal.x, primal.y, primal.z}, primal.w, s_grad,
+                            s_density + t, blockDim.x);
             } else {
-                PathState st;
-                path_begin(st, direction, rng, accumulated);
-                run_path(sc, st);
-                PathResult r = path_finish(st);
+                const PathResult r = adjoint_path(sc, direction, rng, accumulated, w, s_grad, s_density + t,
+                                                  blockDim.x);
                 value = r.value;
                 backdrop = r.backdrop;
-                primal = make_float4(r.value.x, r.value.y, r.value.z, ring_cap_scale(st));
             }
-            replay_path(sc, direction, rng, accumulated, w, f3{primal.x, primal.y, primal.z}, primal.w, s_grad,
-                        s_density + t, blockDim.x);
         }
         if (ta.accumulation) {
             s_value[t] = value;

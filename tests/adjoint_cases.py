@@ -183,3 +183,13 @@ def window_expectation(params, d):
     fov = math.radians(1.0)
     mu = math.cos(fov / 2) ** 0.5  # mean over the view is within 4e-5 of 1
     return float(np.sum(d * radiance)) * (-1.0 / mu) * math.exp(-tau / mu)
+
+
+def giant_case(width=48, height=32):
+    """The banded gas giant (surface style 1) of the ringed-giant workload
+    with its four moons: d albedo / d zone colour goes through the giant's
+    whole band, storm and polar colour chain. At most two surface vertices,
+    so no roulette; no air, so no reflector choice."""
+    frame = scenes.ringed_giant(width, height)
+    plist = [("albedo", 1, c) for c in range(3)]
+    return Params(frame), plist, dict(spp=4, bounces=2)

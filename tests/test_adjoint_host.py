@@ -117,3 +117,9 @@ def test_density_profile_scales_against_the_coefficients():
     # and the profile is in use: a different profile changes the sky
     other = ac.Params(params.frame, density=np.linspace(1.0, 0.1, 16).astype(np.float32))
     assert np.abs(r.render(other) - x).max() > 1e-3
+
+
+def test_adjoint_gas_giant_albedo_host():
+    params, plist, q = ac.giant_case(24, 16)
+    r = HostRenderer(24, 16, q["spp"], q["bounces"])
+    ac.compare(r, params, plist, "host giant")

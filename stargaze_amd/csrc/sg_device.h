@@ -85,9 +85,16 @@ SG_FN TerrainLayout terrain_layout(uint32_t ground_counts_z) {
 
 // Everything a kernel reads about the scene. Passed by value as a kernel
 // argument, so per-scene constants come through scalar loads.
+// A body's centre in FP64 (high + low parts of its f32 record), 32 B.
+struct Centre {
+    double x, y, z, pad;
+};
 struct Scene {
     sg_settings s;
     const Body *bodies;
+    // FP64 centres of `bodies`, formed once per frame upload (fill_centres)
+    // instead of in every lane of every body test.
+    const Centre *centres;
     const sg_primitive *objects;
     const float4 *sky_light;     // 128 texels (src/shaders/ground.wgsl:10)
     const float *terrain;
@@ -170,12 +177,30 @@ SG_FN d3 center_d(const Body &b) {
     return d3{(double)b.center.x + (double)b.low.x, (double)b.center.y + (double)b.low.y,
               (double)b.center.z + (double)b.low.z};
 }
+// The FP64 centres of `count` bodies (host side, at upload).
+static inline void fill_centres(const Body *bodies, uint32_t count, Centre *out) {
+    for (uint32_t i = 0; i < count; ++i) {
+        const d3 c = center_d(bodies[i]);
+        out[i] = Centre{c.x, c.y, c.z, 0.0};
+    }
+}
+// Body i's FP64 centre, a uniform (scalar) load like body().
+SG_FN d3 body_centre(const Scene &sc, uint32_t i) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using ConstCentre = __attribute__((address_space(4))) const Centre;
+    const ConstCentre &c = ((const ConstCentre *)sc.centres)[i];
+    return d3{c.x, c.y, c.z};
+#else
+    const Centre &c = sc.centres[i];
+    return d3{c.x, c.y, c.z};
+#endif
+}
 // src/shaders/pathtrace.wgsl:100-113, in FP64.
 SG_FN d3 relative_center(const Scene &sc, const Ray &ray, uint32_t index) {
     if (ray.anchor == index) return -ray.offset;
-    d3 c = center_d(body(sc, index));
+    d3 c = body_centre(sc, index);
     if (ray.anchor == SG_MISS) return c - ray.offset;
-    return (c - center_d(body(sc, ray.anchor))) - ray.offset;
+    return (c - body_centre(sc, ray.anchor)) - ray.offset;
 }
 
 // src/shaders/pathtrace.wgsl:134-171: cross-product discriminant, one

@@ -9,7 +9,7 @@ namespace sg {
 // --- local surface frame (src/shaders/ground.wgsl:17-61) -------------------
 SG_FN d3 camera_origin(const Scene &sc, const Ray &ray) {
     if (ray.anchor == SG_MISS) return ray.offset;
-    return center_d(body(sc, ray.anchor)) + ray.offset;
+    return body_centre(sc, ray.anchor) + ray.offset;
 }
 SG_FN f3 g_east(const Scene &sc) { const float *v = sc.s.globals.ground_east; return f3{v[0], v[1], v[2]}; }
 SG_FN f3 g_up(const Scene &sc) { const float *v = sc.s.globals.ground_up; return f3{v[0], v[1], v[2]}; }

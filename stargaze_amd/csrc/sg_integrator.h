@@ -569,7 +569,7 @@ SG_FN float density_integral(const Scene &sc, f3 origin, f3 dir, uint32_t steps,
 SG_FN d3 host_offset(const Scene &sc, const Ray &origin) {
     d3 c = tod(atmo_center(sc));
     if (origin.anchor == SG_MISS) return origin.offset - c;
-    return (center_d(body(sc, origin.anchor)) - c) + origin.offset;
+    return (body_centre(sc, origin.anchor) - c) + origin.offset;
 }
 SG_FN bool inside_air(const Scene &sc, const Ray &origin) {
     if (sc.s.globals.atmo_params[3] < 0.5f) return false;
@@ -605,7 +605,10 @@ SG_FN f3 air_transmittance(const Scene &sc, const Ray &origin, f3 dir) {
 SG_FN float rayleigh_phase(float c) { return 3.0f / (16.0f * PI) * (1.0f + c * c); }
 SG_FN float mie_phase(float c, float g) {
     float g2 = g * g;
-    return (1.0f - g2) / (4.0f * PI * powf(1.0f + g2 - 2.0f * g * c, 1.5f));
+    // u^1.5 as u * sqrt(u): the WGSL's pow(u, 1.5) without the general
+    // power's log and exp (4% of the camera-air step).
+    const float u = 1.0f + g2 - 2.0f * g * c;
+    return (1.0f - g2) / (4.0f * PI * (u * sqrtf(u)));
 }
 SG_FN float mie_phase_dg(float c, float g) {
     float u = 1.0f + g * g - 2.0f * g * c;

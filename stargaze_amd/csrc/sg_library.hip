@@ -90,6 +90,8 @@ struct sg_ctx {
     float4 *accumulation = nullptr, *backdrop = nullptr, *moments = nullptr;
     bool moments_on = false;
     Body *bodies = nullptr;
+    Centre *centres = nullptr;           // FP64 body centres (device), from staged_centres
+    std::vector<Centre> staged_centres;  // host copy of the last upload
     sg_primitive *objects = nullptr;
     uint32_t object_capacity = 0;
     float4 *sky_light = nullptr;
@@ -290,11 +292,23 @@ int ensure_bvh(sg_ctx *ctx, const sg_frame *frame) {
     return SG_OK;
 }
 
+// Copy the frame's bodies to the device with their FP64 centres.
+int upload_bodies(sg_ctx *ctx, const sg_frame *frame) {
+    ctx->staged_centres.resize(frame->body_count);
+    fill_centres(reinterpret_cast<const Body *>(frame->bodies), frame->body_count, ctx->staged_centres.data());
+    SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body), hipMemcpyHostToDevice,
+                               ctx->stream));
+    SG_HIP(ctx, hipMemcpyAsync(ctx->centres, ctx->staged_centres.data(), frame->body_count * sizeof(Centre),
+                               hipMemcpyHostToDevice, ctx->stream));
+    return SG_OK;
+}
+
 Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     Scene sc;
     std::memset(&sc, 0, sizeof(sc));
     sc.s = settings;
     sc.bodies = ctx->bodies;
+    sc.centres = ctx->centres;
     sc.objects = ctx->objects;
     sc.sky_light = ctx->sky_light;
     sc.terrain = ctx->terrain;
@@ -671,9 +685,10 @@ int prepare_pass(sg_ctx *ctx, const sg_frame *frame, PassPlan &plan) {
     // The device copies of bodies and primitives stay valid while the
     // history key (which covers them) is unchanged.
     if (!ctx->frame_on_device) {
-        if (frame->body_count)
-            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
-                                       hipMemcpyHostToDevice, ctx->stream));
+        if (frame->body_count) {
+            const int rc_ = upload_bodies(ctx, frame);
+            if (rc_ != SG_OK) return rc_;
+        }
         if (frame->ground_count)
             SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
                                        hipMemcpyHostToDevice, ctx->stream));
@@ -817,6 +832,7 @@ int sg_create(sg_ctx **out, uint32_t width, uint32_t height, const sg_ray_star *
         return bail(SG_E_DEVICE);
     if (alloc_image(ctx) != SG_OK) return bail(SG_E_DEVICE);
     if (hipMalloc(&ctx->bodies, SG_BODY_CAPACITY * sizeof(Body)) != hipSuccess) return bail(SG_E_DEVICE);
+    if (hipMalloc(&ctx->centres, SG_BODY_CAPACITY * sizeof(Centre)) != hipSuccess) return bail(SG_E_DEVICE);
     if (ensure_objects(ctx, SG_GROUND_CAPACITY) != SG_OK) return bail(SG_E_DEVICE);
     if (hipMalloc(&ctx->sky_light, SG_SKY_TEXELS * sizeof(float4)) != hipSuccess) return bail(SG_E_DEVICE);
     if (hipMalloc(&ctx->exposure, sizeof(sg_exposure)) != hipSuccess ||
@@ -860,7 +876,7 @@ int sg_destroy(sg_ctx *ctx) {
     if (!ctx) return SG_OK;
     if (ctx->batch) return fail(ctx, SG_E_STATE, "context is a view of a batch; destroy the batch first");
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->objects, ctx->sky_light,
+    void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->centres, ctx->objects, ctx->sky_light,
                        ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->air_block, ctx->bvh_nodes,
                        ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
@@ -1375,9 +1391,10 @@ int sg_adjoint(sg_ctx *ctx, const sg_frame *frame, const float *d_image, uint64_
     }
     if (!ctx->adj_gradient) SG_HIP(ctx, hipMalloc(&ctx->adj_gradient, SG_GRAD_COUNT * sizeof(float)));
     if (!ctx->frame_on_device) {
-        if (frame->body_count)
-            SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body),
-                                       hipMemcpyHostToDevice, ctx->stream));
+        if (frame->body_count) {
+            const int rc_ = upload_bodies(ctx, frame);
+            if (rc_ != SG_OK) return rc_;
+        }
         if (frame->ground_count)
             SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
                                        hipMemcpyHostToDevice, ctx->stream));
@@ -1505,9 +1522,10 @@ int sg_probe(sg_ctx *ctx, const sg_frame *frame, uint32_t kind, const sg_probe_r
     std::memcpy(st.lights, lights, sizeof(lights));
     // The probe's frame replaces the encoded frame's device copies.
     ctx->frame_on_device = false;
-    if (frame->body_count)
-        SG_HIP(ctx, hipMemcpyAsync(ctx->bodies, frame->bodies, frame->body_count * sizeof(Body), hipMemcpyHostToDevice,
-                                   ctx->stream));
+    if (frame->body_count) {
+        const int rc_ = upload_bodies(ctx, frame);
+        if (rc_ != SG_OK) return rc_;
+    }
     if (frame->ground_count)
         SG_HIP(ctx, hipMemcpyAsync(ctx->objects, frame->ground, frame->ground_count * sizeof(sg_primitive),
                                    hipMemcpyHostToDevice, ctx->stream));

@@ -34,6 +34,9 @@ int hc_query(const sg_settings *settings, const sg_sphere *bodies, const sg_prim
     std::memset(&sc, 0, sizeof(sc));
     sc.s = *settings;
     sc.bodies = reinterpret_cast<const Body *>(bodies);
+    std::vector<Centre> centres(settings->counts[0]);
+    fill_centres(sc.bodies, settings->counts[0], centres.data());
+    sc.centres = centres.data();
     sc.objects = objects;
     sc.terrain = terrain;
     sc.bvh = reinterpret_cast<const float4 *>(bvh_nodes);
@@ -73,6 +76,9 @@ int hc_pass(const sg_settings *settings, const sg_sphere *bodies, const float *r
     std::memset(&sc, 0, sizeof(sc));
     sc.s = *settings;
     sc.bodies = reinterpret_cast<const Body *>(bodies);
+    std::vector<Centre> centres(settings->counts[0]);
+    fill_centres(sc.bodies, settings->counts[0], centres.data());
+    sc.centres = centres.data();
     sc.t = terrain_layout(settings->globals.ground_counts[2]);
     sc.ring_profile = ring_profile;
     sc.ring_profile_body = ring_profile ? ring_body : SG_MISS;
@@ -120,6 +126,9 @@ int hc_render(const sg_settings *settings, const sg_sphere *bodies, const sg_pri
     std::memset(&sc, 0, sizeof(sc));
     sc.s = *settings;
     sc.bodies = reinterpret_cast<const Body *>(bodies);
+    std::vector<Centre> centres(settings->counts[0]);
+    fill_centres(sc.bodies, settings->counts[0], centres.data());
+    sc.centres = centres.data();
     sc.objects = objects;
     sc.terrain = terrain;
     sc.bvh = reinterpret_cast<const float4 *>(bvh_nodes);

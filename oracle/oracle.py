@@ -108,6 +108,16 @@ def terrain_layout(cells: int) -> dict:
                 total=cache + 6 * nodes + stride * vertices)
 
 
+def _check_ground(frame) -> None:
+    """The library's frame check on the ground theme: the basalt ground (4)
+    has no restated materials here or in the library, and themes past the
+    dunes (5) do not exist."""
+    g = np.asarray(frame.globals)
+    theme = int(g["ground_counts"][2]) & 255
+    if g["ground_north"][3] != 0 and (theme == 4 or theme > 5):
+        raise ValueError(f"ground theme {theme} is not built")
+
+
 class OracleTracer:
     """``PathTracer`` host logic over the C restatement, on the CPU."""
 
@@ -268,6 +278,10 @@ class OracleTracer:
 
     def encode(self, frame, rows=None):
         """One progressive pass; ``rows`` limits the rows traced (sampling only)."""
+        _check_ground(frame)
+        if np.asarray(frame.globals)["ground_counts"][1] != 0 and self.terrain is None:
+            # The library refuses the same frame (sg_encode, SG_E_STATE).
+            raise RuntimeError("terrain enabled but no terrain uploaded (upload_terrain)")
         self._history(frame)
         count = min(self.samples_per_frame, MAX_BATCH_SAMPLES, max(self.sample_limit - self.history_samples, 0))
         st = self.settings(frame, count)
@@ -299,6 +313,11 @@ class OracleTracer:
         return self.moments.reshape(self.height, self.width, 4).copy()
 
     def probe(self, frame, kind, rays):
+        _check_ground(frame)
+        if (kind == 4 or np.asarray(frame.globals)["ground_counts"][1] != 0) and self.terrain is None:
+            # Kind 4 is the terrain query; the library refuses the same probe
+            # (sg_probe, SG_E_STATE).
+            raise RuntimeError("terrain probe without uploaded terrain")
         st = self.settings(frame, 0)
         sc, keep = self._scene(st, frame)
         g = st["globals"]

@@ -533,6 +533,51 @@ def recovery_leg(steps: int) -> dict:
             "mean_relative_error_after": {k: round(v, 5) for k, v in after.items()}}
 
 
+# The reference's own suite on an AMD RX 7600 XT (RADV), GPU median ms per
+# frame at 1280x720 display, 75% scale, 4 bounces, 70% vegetation, 1 spp per
+# submission: benchmarks/optimizations/rx7600xt-2026-10-04/summary.md:12-25.
+SUITE_PUBLISHED_RX7600XT = {
+    "halo-rings": (34.734, 34.218), "earth-daylight": (120.498, 120.813), "earth-twilight": (41.253, 40.558),
+    "median-dense": (74.684, 74.474), "vantus-eclipse": (32.339, 31.490), "vantus-airless": (2.684, 2.549),
+    "dual-eclipse": (1.668, 0.960),
+}
+
+
+def suite_leg(repeats: int) -> dict:
+    """The reference's render benchmark cases (``render_benchmark``) at its
+    published setting: per case and mode the median over ``repeats`` x 16
+    timed frames after 8 warm-up submissions. ``gpu_ms`` is the HIP-event
+    time of the integrator kernels of one pass; ``wall_ms`` spans the whole
+    ``encode`` (exposure metering included) through a stream synchronize."""
+    from stargaze_amd import render_benchmark as rb
+
+    cfg = rb.Config(output="", width=1280, height=720, scale=75, bounces=4, vegetation=70, warmup=8, frames=16,
+                    repeats=repeats, image_samples=32)
+    cases = rb.time_cases(cfg, log=log)
+    out = {}
+    for case in cases:
+        name = case["name"]
+        if "skipped" in case:
+            out[name] = {"skipped": case["skipped"]}
+            continue
+        entry = {}
+        for mode in case["modes"]:
+            runs = [m for r in mode["repeats"] for m in r]
+            entry[mode["mode"]] = {"gpu_ms": round(rb.median(m["gpu_ms"] for m in runs), 4),
+                                   "wall_ms": round(rb.median(m["wall_ms"] for m in runs), 4)}
+        if name in SUITE_PUBLISHED_RX7600XT:
+            paused, advancing = SUITE_PUBLISHED_RX7600XT[name]
+            entry["rx7600xt_gpu_ms"] = {"paused": paused, "advancing": advancing}
+            entry["speedup_wall_vs_rx7600xt_gpu"] = {
+                "paused": round(paused / entry["paused"]["wall_ms"], 1),
+                "advancing": round(advancing / entry["advancing"]["wall_ms"], 1)}
+        out[name] = entry
+    return {"setting": "1280x720 display, 960x540 traced, 4 bounces, vegetation 70, 1 spp/submission, "
+                       f"warmup 8, {repeats} x 16 frames", "traced_paths_per_frame": 960 * 540,
+            "terrain": "seeded ridged fBm per ground at the reference's grid sizes (its generator is not restated)",
+            "cases": out}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -549,6 +594,8 @@ def main() -> int:
     ap.add_argument("--adjoint-steps", type=int, default=3, help="passes of the adjoint leg (0 = skip)")
     ap.add_argument("--recover-steps", type=int, default=200,
                     help="Adam steps of the differentiable workload's inverse problem (0 = skip)")
+    ap.add_argument("--suite-repeats", type=int, default=3,
+                    help="repeats of the reference's render benchmark cases (0 = skip)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only copy 1 GiB device buffers 4 times (the reference for HBM byte counters)")
     args = ap.parse_args()
@@ -660,6 +707,9 @@ def main() -> int:
     if args.adjoint_steps > 0:
         log("adjoint leg")
         line["adjoint"] = adjoint_leg(args.adjoint_steps, args.recover_steps)
+    if args.suite_repeats > 0:
+        log("reference suite leg")
+        line["reference_suite"] = suite_leg(args.suite_repeats)
     if not args.no_cpu_baseline:
         log("cpu baseline")
         line["cpu_baseline"] = cpu_baseline(frame, terrain, profile, air, width, height, spp, bounces)

@@ -259,11 +259,15 @@ int sg_kernel_time(sg_ctx *ctx, double *milliseconds, uint64_t *launches);
 #define SG_STEP_COUNT 4u
 int sg_step_time(const sg_ctx *ctx, uint32_t step, double *milliseconds, uint64_t *launches);
 /* Integrator selection (no reference counterpart: the reference always runs
- * its megakernel). SG_INTEGRATOR_WAVEFRONT (default) runs extend / camera-air
- * / shade kernels over compacted queues of at most `chunk_paths` live paths;
- * SG_INTEGRATOR_FUSED runs one lane per path through the whole loop. Both
- * draw the same samples; images agree to the last bits (the compiler fuses
- * multiply-adds differently in the two kernels). `chunk_paths` 0 keeps the current value. */
+ * its megakernel). SG_INTEGRATOR_WAVEFRONT runs extend / camera-air / shade
+ * kernels over compacted queues of at most `chunk_paths` live paths;
+ * SG_INTEGRATOR_FUSED runs one lane per path through the whole loop;
+ * SG_INTEGRATOR_AUTO (default) picks per pass: the wavefront for a frame with
+ * a ground scene and air once the pass traces at least 2^20 paths, the fused
+ * kernel otherwise. Both draw the same samples; images agree to the last
+ * bits, and on ground scenes with air to 1e-5 (deferred camera-air shadow
+ * rays are summed in their own order). `chunk_paths` 0 keeps the current value. */
+#define SG_INTEGRATOR_AUTO 0u
 #define SG_INTEGRATOR_FUSED 1u
 #define SG_INTEGRATOR_WAVEFRONT 2u
 int sg_set_integrator(sg_ctx *ctx, uint32_t integrator, uint32_t chunk_paths);

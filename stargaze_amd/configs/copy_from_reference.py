@@ -1,9 +1,10 @@
-"""Copy the reference's bundled scene files into ``tests/golden/configs/`` as
-test fixtures, whole-line comments removed (the data lines are unchanged, so
-the reference tests' line substitutions apply to the copies too).
+"""Copy the reference's bundled scene files (its ``configs/``) into this
+folder, whole-line comments removed; the data lines are unchanged, so the
+reference tests' line substitutions apply to the copies too. The package
+loads them as its bundled scenes and the tests as fixtures.
 
 Run in a checkout that has the reference at ``/root/reference``:
-``python tests/golden/make_configs.py``.
+``python stargaze_amd/configs/copy_from_reference.py``.
 """
 from __future__ import annotations
 
@@ -25,8 +26,7 @@ FILES = [
 
 
 def main() -> None:
-    out = pathlib.Path(__file__).resolve().parent / "configs"
-    out.mkdir(exist_ok=True)
+    out = pathlib.Path(__file__).resolve().parent
     for name in FILES:
         lines = (SOURCE / name).read_text(encoding="utf-8").splitlines(keepends=True)
         data = [line for line in lines if not line.lstrip().startswith("#")]

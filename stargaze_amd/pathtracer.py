@@ -221,7 +221,8 @@ class PathTracer:
         return float(ms.value), int(n.value)
 
     def set_integrator(self, integrator: int, chunk_paths: int = 0) -> None:
-        """INTEGRATOR_FUSED or INTEGRATOR_WAVEFRONT (see include/stargaze_amd.h)."""
+        """INTEGRATOR_AUTO (default), INTEGRATOR_FUSED or INTEGRATOR_WAVEFRONT
+        (see include/stargaze_amd.h)."""
         self._check(self._lib.sg_set_integrator(self._ctx, int(integrator), int(chunk_paths)), "set_integrator")
 
     def restart(self) -> None:
@@ -342,7 +343,7 @@ class PathTracer:
         return out
 
 
-INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 1, 2
+INTEGRATOR_AUTO, INTEGRATOR_FUSED, INTEGRATOR_WAVEFRONT = 0, 1, 2
 STEP_EXTEND, STEP_CAMERA_AIR, STEP_SHADE, STEP_AIR_OCCLUSION = range(4)
 (PROBE_SPHERE, PROBE_CLOSEST, PROBE_RING, PROBE_VISIBILITY, PROBE_TERRAIN, PROBE_LANDSCAPE, PROBE_CONE, PROBE_ALBEDO,
  PROBE_GROUND, PROBE_NOISE, PROBE_SKY) = range(11)

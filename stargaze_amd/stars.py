@@ -1,6 +1,7 @@
 """Catalogue stars in the tracer's ray format.
 
-Restates the host side of stargaze's background star field: the radiance of
+Restates the host side of stargaze's background star field: the seeded
+catalogue itself (``stars::generate``, src/stars.rs:14-115), the radiance of
 a catalogue star's drawn disc (``stars::radiance``, src/stars.rs:132-139) and
 ``stars::ray_catalogue`` (src/stars.rs:152-205), which packs the stars into
 ``RayStar`` records (unit direction, sin^2 of the angular radius, radiance)
@@ -30,6 +31,93 @@ class CatalogueStar:
     color: tuple = (1.0, 1.0, 1.0)
     bright: float = 1.0
     angular_radius: float = 1e-3
+
+
+F32 = np.float32
+
+
+class _Rng:
+    """xorshift64* (src/stars.rs:14-32)."""
+
+    def __init__(self, seed: int):
+        self.state = (seed | 1) & 0xFFFFFFFFFFFFFFFF
+
+    def u32(self) -> int:
+        x = self.state
+        x ^= x >> 12
+        x ^= (x << 25) & 0xFFFFFFFFFFFFFFFF
+        x ^= x >> 27
+        self.state = x
+        return ((x * 0x2545F4914F6CDD1D) & 0xFFFFFFFFFFFFFFFF) >> 32
+
+    def f32(self) -> np.float32:
+        return F32(self.u32()) / F32(4294967295.0)
+
+
+def _srgb_to_linear(c: np.float32) -> np.float32:
+    c = min(max(c, F32(0.0)), F32(1.0))
+    return c / F32(12.92) if c <= F32(0.04045) else np.power((c + F32(0.055)) / F32(1.055), F32(2.4))
+
+
+def blackbody_linear(temp_k) -> tuple:
+    """Approximate blackbody colour (Tanner Helland's fit) in linear RGB,
+    f32 (src/stars.rs:34-64)."""
+    t = min(max(F32(temp_k) / F32(100.0), F32(10.0)), F32(400.0))
+    r = F32(255.0) if t <= 66.0 else F32(329.69873) * np.power(t - F32(60.0), F32(-0.13320476))
+    g = (F32(99.4708) * np.log(t) - F32(161.11957) if t <= 66.0
+         else F32(288.12217) * np.power(t - F32(60.0), F32(-0.07551485)))
+    b = (F32(255.0) if t >= 66.0 else F32(0.0) if t <= 19.0
+         else F32(138.51773) * np.log(t - F32(10.0)) - F32(305.0448))
+    return tuple(float(_srgb_to_linear(F32(c) / F32(255.0))) for c in (r, g, b))
+
+
+def _normalize(v):
+    x, y, z = (F32(c) for c in v)
+    inv = F32(1.0) / np.sqrt(x * x + y * y + z * z)
+    return x * inv, y * inv, z * inv
+
+
+def _random_dir(rng: _Rng) -> tuple:
+    z = F32(1.0) - F32(2.0) * rng.f32()
+    phi = F32(2.0 * math.pi) * rng.f32()
+    r = np.sqrt(max(F32(1.0) - z * z, F32(0.0)))
+    return float(r * np.cos(phi)), float(r * np.sin(phi)), float(z)
+
+
+def _star(rng: _Rng, dim: float) -> CatalogueStar:
+    u = rng.f32()
+    bright = (F32(0.02) + u * u * u * F32(1.6)) * F32(dim)
+    x = rng.f32()
+    temp = F32(2500.0) + F32(12000.0) * (x * x)
+    radius = min(max(F32(0.7) + F32(2.6) * np.sqrt(bright), F32(0.7)), F32(3.4)) * F32(0.0004)
+    return CatalogueStar(dir=_random_dir(rng), color=blackbody_linear(temp), bright=float(bright),
+                         angular_radius=float(radius))
+
+
+def generate(count: int, band: int, seed: int) -> list:
+    """``count`` stars uniform over the sky, then ``band`` dimmer, cooler stars
+    bulging about a great circle (the Milky Way), from one seeded stream
+    (src/stars.rs:91-115). The circle's in-plane axes come from the
+    branch-free orthonormal basis of Duff et al. (2017), as glam's
+    ``any_orthonormal_vector`` builds them."""
+    rng = _Rng(seed)
+    out = [_star(rng, 1.0) for _ in range(count)]
+    n = _normalize((0.35, 0.82, 0.45))
+    sign = F32(1.0) if n[2] >= 0.0 else F32(-1.0)
+    a = F32(-1.0) / (sign + n[2])
+    b = n[0] * n[1] * a
+    u = (b, sign + n[1] * n[1] * a, -n[1])
+    v = _normalize((n[1] * u[2] - u[1] * n[2], n[2] * u[0] - u[2] * n[0], n[0] * u[1] - u[0] * n[1]))
+    for _ in range(band):
+        angle = F32(2.0 * math.pi) * rng.f32()
+        spread = (rng.f32() + rng.f32() + rng.f32() - F32(1.5)) * F32(0.28)
+        c, s_ = np.cos(angle), np.sin(angle)
+        d = _normalize(tuple(u[i] * c + v[i] * s_ + n[i] * spread for i in range(3)))
+        star = _star(rng, 0.28)
+        star.dir = tuple(float(x) for x in d)
+        star.color = blackbody_linear(F32(2500.0) + F32(6000.0) * rng.f32())
+        out.append(star)
+    return out
 
 
 def radiance(star: CatalogueStar) -> np.ndarray:

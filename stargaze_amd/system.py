@@ -20,6 +20,7 @@ from __future__ import annotations
 
 import math
 from dataclasses import dataclass, field
+from pathlib import Path
 
 import numpy as np
 
@@ -29,6 +30,7 @@ from .layouts import AU_KM, GLOBALS, METRES_PER_AU, PRIMITIVE, SPHERE
 from .pathtracer import Frame
 
 SUN_LIGHT = 2.5  # src/main.rs:71
+CONFIGS = Path(__file__).resolve().parent / "configs"  # the reference's bundled scene files
 MIN_FOV_DEG, MAX_FOV_DEG = 0.001, 90.0
 
 KIND_ANCHOR, KIND_STAR, KIND_PLANET, KIND_MOON = "anchor", "star", "planet", "moon"
@@ -691,3 +693,47 @@ def build_frame(scene: Scene, t: float, width: int, height: int, observer: Obser
     g["ground_north"] = [*vf.world_to_view(vf.north), atmosphere_host + 1 if ground else 0.0]
     g["ground_counts"] = [len(prims), 1 if ground else 0, scene.ground, 0]
     return Frame(g, bodies, prims, float(t))
+
+
+# -- the built-in binary system (src/sim.rs:452-783) ---------------------------
+def binary_scene() -> Scene:
+    """``sim::binary_scene``: two suns about a barycentre, the observatory on
+    Calyx (35 N, 0 E) with Earth-like air, Halo the startup target. The
+    reference builds it in code rather than from a file; these are its
+    bodies (angles other than inclination and obliquity in radians)."""
+    deg = math.pi / 180.0
+
+    def body(name, kind, parent, el, period, radius_km, albedo, obliquity, spin):
+        return Body(name, kind, parent, Elements(*el), period, radius_km / AU_KM, albedo, (0.0, 0.0, 0.0), 0.0,
+                    obliquity, spin, 0.0)
+
+    def star(name, a, m0, radius_au, emission, luminosity, spin):
+        return Body(name, KIND_STAR, 0, Elements(a, 0.0, 0.0, 0.0, 0.0, m0), 120.0, radius_au, (0.0, 0.0, 0.0),
+                    tuple(float(np.float32(c)) for c in emission), float(np.float32(luminosity)), 0.0, spin, 0.0)
+
+    f = lambda *c: tuple(float(np.float32(x)) for x in c)  # noqa: E731  (f32 colours)
+    bodies = [
+        Body("Barycentre", KIND_ANCHOR, None, Elements(0.0), 0.0, 0.0, (0.0,) * 3, (0.0,) * 3, 0.0, 0.0, 1.0, 0.0),
+        star("Aur", 0.30, 0.0, 0.030, (1.0, 0.96, 0.86), 2.0, 22.0),
+        star("Igni", 0.50, math.pi, 0.018, (1.0, 0.42, 0.20), 0.7, 30.0),
+        body("Nerid", KIND_PLANET, 0, (2.6, 0.03, 1.5 * deg, 0.4, 1.1, 2.0), 1530.0, 5_734.0, f(0.45, 0.55, 0.65),
+             18.0 * deg, 0.85),
+        body("Pip", KIND_MOON, 3, (2.2e-4, 0.01, 2.0 * deg, 0.0, 0.5, 1.0), 2.5, 1_200.0, f(0.60, 0.60, 0.62),
+             2.0 * deg, 2.5),
+        body("Calyx", KIND_PLANET, 0, (4.2, 0.02, 1.0 * deg, 2.0, 3.0, 4.5), 3000.0, 9_560.0, f(0.72, 0.60, 0.42),
+             22.0 * deg, 0.60),
+        body("Halo", KIND_MOON, 5, (1.6e-4, 0.002, 0.5 * deg, 0.0, 0.0, 0.0), 3.5, 1_500.0, f(0.55, 0.57, 0.60),
+             0.5 * deg, 3.5),
+        body("Umbra", KIND_MOON, 5, (3.2e-4, 0.01, 1.5 * deg, 1.0, 2.0, 3.0), 8.0, 1_000.0, f(0.35, 0.33, 0.32),
+             1.5 * deg, 8.0),
+        body("Vantus", KIND_PLANET, 0, (6.8, 0.05, 2.5 * deg, 1.5, 0.8, 1.2), 6200.0, 14_016.0, f(0.35, 0.42, 0.55),
+             12.0 * deg, 0.40),
+        body("Iri", KIND_MOON, 8, (1.8e-4, 0.005, 1.0 * deg, 0.0, 1.0, 0.5), 5.0, 800.0, f(0.50, 0.50, 0.55),
+             1.0 * deg, 5.0),
+        body("Kesh", KIND_MOON, 8, (3.0e-4, 0.01, 2.0 * deg, 0.5, 2.0, 2.5), 11.0, 600.0, f(0.45, 0.40, 0.35),
+             2.0 * deg, 11.0),
+        body("Coll", KIND_MOON, 8, (4.5e-4, 0.02, 3.0 * deg, 0.2, 4.0, 5.0), 18.0, 500.0, f(0.30, 0.32, 0.38),
+             3.0 * deg, 18.0),
+    ]
+    return Scene("Binary", bodies, 5, 6, [6, 1, 2, 7, 3, 8, 4, 9], Atmosphere.earthlike(), THEMES["forest"], 20.0,
+                 None, 35.0, 0.0, 2.0, None, None)

@@ -1,7 +1,7 @@
 """Scene files, orbits and the observer (stargaze_amd.system) against the
 reference's own assertions on the same bundled files (src/config.rs:661-1220,
 src/sim.rs:860-1219), plus the frame packing against what the geometry fixes.
-The scene files are fixtures copied by tests/golden/make_configs.py."""
+The scene files are the bundled copies in stargaze_amd/configs/."""
 from __future__ import annotations
 
 import math
@@ -15,7 +15,7 @@ from stargaze_amd import ground as gr
 from stargaze_amd import system as sy
 from stargaze_amd.layouts import AU_KM, METRES_PER_AU
 
-CONFIGS = pathlib.Path(__file__).resolve().parent / "golden" / "configs"
+CONFIGS = pathlib.Path(sy.__file__).resolve().parent / "configs"
 
 
 def text(name: str) -> str:

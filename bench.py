@@ -574,7 +574,8 @@ def suite_leg(repeats: int) -> dict:
         out[name] = entry
     return {"setting": "1280x720 display, 960x540 traced, 4 bounces, vegetation 70, 1 spp/submission, "
                        f"warmup 8, {repeats} x 16 frames", "traced_paths_per_frame": 960 * 540,
-            "terrain": "seeded ridged fBm per ground at the reference's grid sizes (its generator is not restated)",
+            "terrain": "the reference's generated terrain per ground (terrain.generate); the alpine ground on a "
+                       "seeded stand-in unless STARGAZE_TERRAIN_SURVEYS names the reference's surveys",
             "cases": out}
 
 

@@ -9,9 +9,10 @@ captures at a fixed exposure, and ``results.json`` in the reference's schema.
 
 Three things differ, each recorded in the results:
 
-* Frames with a ground stand on a heightfield from the caller
-  (``terrain_for``); the reference generates its own per theme
-  (``terrain::for_theme``), which is not restated here.
+* Frames with a ground stand on the reference's own terrain
+  (``reference_terrain``), except the alpine ground, which needs the
+  reference's elevation survey: without ``STARGAZE_TERRAIN_SURVEYS`` it
+  stands on a seeded stand-in.
 * The basalt ground (cases ``ground-basalt`` and ``level-1``) is not built:
   those cases are recorded with a ``skipped`` reason instead of timings.
 * ``gpu_ms`` is the HIP-event time of the integrator kernels of one
@@ -25,6 +26,7 @@ from __future__ import annotations
 
 import json
 import math
+import os
 import struct
 import sys
 import time
@@ -333,16 +335,29 @@ def fingerprint(frame: Frame) -> str:
     return f"{h:016x}"
 
 
-# Grid cells per side of each ground's heightfield (``terrain::cells``,
-# src/terrain.rs:27-36, with CRATER/ALPINE/DUNE_CELLS and MAX_CELLS).
-TERRAIN_CELLS = {0: 512, 1: 512, 2: 1024, 3: 1024, 4: 2048, 5: 1024}
-
-
 def stand_in_terrain(theme: int) -> tuple[np.ndarray, int]:
-    """A seeded ridged-fBm heightfield at the ground's grid size (the
-    reference's own generator is not restated)."""
-    cells = TERRAIN_CELLS[theme]
+    """A seeded ridged-fBm heightfield at the ground's grid size, for a
+    ground whose surveys are not at hand."""
+    cells = tr.THEME_CELLS[theme]
     return tr.ridged_fbm(cells, seed=0x743337A9 ^ (theme * 0x9E3779B9 & 0xFFFFFFFF), amplitude=60.0), cells
+
+
+_TERRAIN: dict = {}
+
+
+def reference_terrain(theme: int) -> tuple[np.ndarray, int]:
+    """The reference's own terrain for a ground (``terrain::for_theme``),
+    generated once per process. The alpine and basalt grounds read the
+    reference's elevation surveys from the folder ``STARGAZE_TERRAIN_SURVEYS``
+    names; without it they stand on ``stand_in_terrain``."""
+    if theme not in _TERRAIN:
+        folder = os.environ.get("STARGAZE_TERRAIN_SURVEYS")
+        if theme in (3, 4) and not folder:
+            _TERRAIN[theme] = stand_in_terrain(theme)
+        else:
+            surveys = tr.Surveys(folder) if theme in (3, 4) else None
+            _TERRAIN[theme] = tr.generate(theme, surveys=surveys)
+    return _TERRAIN[theme]
 
 
 # -- captures -------------------------------------------------------------------
@@ -426,7 +441,7 @@ def median(values) -> float:
     return (v[(len(v) - 1) // 2] + v[len(v) // 2]) * 0.5
 
 
-def time_cases(cfg: Config, terrain_for=stand_in_terrain, log=print) -> list:
+def time_cases(cfg: Config, terrain_for=reference_terrain, log=print) -> list:
     """The timed half of the run (:840-893): per case, ``warmup`` untimed
     submissions, then ``repeats`` runs of ``frames`` timed ones, paused (the
     first frame again and again) and advancing (the next frame each time).
@@ -472,7 +487,7 @@ def time_cases(cfg: Config, terrain_for=stand_in_terrain, log=print) -> list:
     return cases
 
 
-def run(cfg: Config, terrain_for=stand_in_terrain, log=print) -> dict:
+def run(cfg: Config, terrain_for=reference_terrain, log=print) -> dict:
     """Time and capture every selected case; write ``results.json`` into
     ``cfg.output`` and return it."""
     cfg.validate()

@@ -10,11 +10,11 @@ This restates the host side that feeds the reference's path tracer:
 (src/main.rs:822-1018), so a file such as ``configs/legacy/halo.yaml`` renders
 through ``PathTracer.encode`` directly.
 
-Two parts of the reference's startup are not restated: the search for a
-night view when a file has no ``time_days`` (callers pass a time), and the
-reference's terrain generator (``build_frame`` takes any heightfield in the
-``terrain`` module's layout, seats the scenery on it and lifts the eye onto
-it as ``terrain::lookout_height`` does, src/terrain.rs:64-71).
+The search for a night view when a file has no ``time_days`` is not
+restated: callers pass a time. ``build_frame`` takes the ground's heightfield
+in the ``terrain`` module's layout (``terrain.generate`` restates the
+reference's), seats the scenery on it and lifts the eye onto it as
+``terrain::lookout_height`` does (src/terrain.rs:64-71).
 """
 from __future__ import annotations
 

@@ -52,7 +52,7 @@ def test_benchmark_view_matches_oracle_per_tile(name):
     cfg = rb.Config(output="", width=size[0], height=size[1], scale=100, bounces=4, vegetation=70, warmup=1,
                     frames=2, repeats=1, image_samples=2)
     theme = rb.ground_theme(name)
-    terrain = rb.stand_in_terrain(theme) if theme is not None else None
+    terrain = rb.reference_terrain(theme) if theme is not None else None
     frame = rb.fixture(name, 0, cfg, terrain)
     print(f"{name}: {len(frame.bodies)} bodies, {len(frame.ground)} props", flush=True)
     opts = dict(samples_per_frame=8, max_bounces=4, milky_way=0.0)

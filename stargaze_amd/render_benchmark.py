@@ -26,7 +26,6 @@ from __future__ import annotations
 
 import json
 import math
-import os
 import struct
 import sys
 import time
@@ -335,34 +334,14 @@ def fingerprint(frame: Frame) -> str:
     return f"{h:016x}"
 
 
-def stand_in_terrain(theme: int) -> tuple[np.ndarray, int]:
-    """A seeded ridged-fBm heightfield at the ground's grid size, for a
-    ground whose surveys are not at hand."""
-    cells = tr.THEME_CELLS[theme]
-    return tr.ridged_fbm(cells, seed=0x743337A9 ^ (theme * 0x9E3779B9 & 0xFFFFFFFF), amplitude=60.0), cells
-
-
-_TERRAIN: dict = {}
-
-
-def reference_terrain(theme: int) -> tuple[np.ndarray, int]:
-    """The reference's own terrain for a ground (``terrain::for_theme``),
-    generated once per process. The alpine and basalt grounds read the
-    reference's elevation surveys from the folder ``STARGAZE_TERRAIN_SURVEYS``
-    names; without it they stand on ``stand_in_terrain``."""
-    if theme not in _TERRAIN:
-        folder = os.environ.get("STARGAZE_TERRAIN_SURVEYS")
-        if theme in (3, 4) and not folder:
-            _TERRAIN[theme] = stand_in_terrain(theme)
-        else:
-            surveys = tr.Surveys(folder) if theme in (3, 4) else None
-            _TERRAIN[theme] = tr.generate(theme, surveys=surveys)
-    return _TERRAIN[theme]
+stand_in_terrain = tr.stand_in
+reference_terrain = tr.for_theme
 
 
 # -- captures -------------------------------------------------------------------
-def write_png(path: Path, rgba: np.ndarray) -> None:
-    """An 8-bit RGBA PNG (no filtering, one zlib stream)."""
+def write_png(path: Path, rgba: np.ndarray, srgb: bool = False) -> None:
+    """An 8-bit RGBA PNG (no filtering, one zlib stream), with an sRGB chunk
+    (perceptual intent) when ``srgb``."""
     h, w = rgba.shape[:2]
     raw = b"".join(b"\x00" + rgba[y].tobytes() for y in range(h))
 
@@ -371,6 +350,7 @@ def write_png(path: Path, rgba: np.ndarray) -> None:
 
     with open(path, "wb") as f:
         f.write(b"\x89PNG\r\n\x1a\n" + chunk(b"IHDR", struct.pack(">IIBBBBB", w, h, 8, 6, 0, 0, 0))
+                + (chunk(b"sRGB", b"\x00") if srgb else b"")
                 + chunk(b"IDAT", zlib.compress(raw, 6)) + chunk(b"IEND", b""))
 
 

@@ -684,3 +684,29 @@ def lookout_height(heights: np.ndarray, cells: int, radius_m: float) -> float:
     """``terrain::lookout_height`` (src/terrain.rs:64-71): the height under
     the lookout, metres, on a host of ``radius_m``."""
     return float(F(height(heights, cells, 0.0, 0.0)) * F(scale(radius_m)))
+
+
+def stand_in(theme: int) -> tuple[np.ndarray, int]:
+    """A seeded ridged-fBm heightfield at a ground's grid size, for a ground
+    whose surveys are not at hand."""
+    cells = THEME_CELLS[theme]
+    return ridged_fbm(cells, seed=0x743337A9 ^ (theme * 0x9E3779B9 & 0xFFFFFFFF), amplitude=60.0), cells
+
+
+_FOR_THEME: dict = {}
+
+
+def for_theme(theme: int) -> tuple[np.ndarray, int]:
+    """``terrain::for_theme`` (src/terrain.rs:58-62): each ground's terrain,
+    generated once per process. The alpine and basalt grounds read the
+    reference's elevation surveys from the folder ``STARGAZE_TERRAIN_SURVEYS``
+    names; without it they stand on ``stand_in``."""
+    import os
+
+    if theme not in _FOR_THEME:
+        folder = os.environ.get("STARGAZE_TERRAIN_SURVEYS")
+        if theme in (3, 4) and not folder:
+            _FOR_THEME[theme] = stand_in(theme)
+        else:
+            _FOR_THEME[theme] = generate(theme, surveys=Surveys(folder) if theme in (3, 4) else None)
+    return _FOR_THEME[theme]

@@ -204,14 +204,6 @@ int sg_set_exposure_controls(sg_ctx *ctx, int32_t auto_exposure, float bias);
 /* Terrain::data for the ground's theme (src/terrain.rs:47-52, uploaded at
  * src/pathtracer.rs:914-932): heights, derivatives (cells <= 1024), min/max. */
 int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint32_t cells);
-/* The basalt ground's survey atlas: four map descriptors (offset, width,
- * spacing, north shift), nine site records (x, z, cos, sin, scale, map, 0,
- * 0), then each map's area-averaged mip pyramid of elevations in metres.
- * Replaces the write of terrain::basalt_detail() after the terrain
- * (src/pathtracer.rs:917-926; built by src/volcano.rs:15-110). Frames on
- * ground theme 4 need it. Descriptors and records are checked against the
- * upload; the accumulation restarts. */
-int sg_upload_volcano(sg_ctx *ctx, const float *data, uint64_t float_count);
 /* PathTracer::encode (src/pathtracer.rs:789): queue one progressive pass. */
 int sg_encode(sg_ctx *ctx, const sg_frame *frame);
 /* Display-only exposure (src/pathtracer.rs:883-906,981-1001). Every sg_encode

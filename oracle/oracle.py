@@ -57,7 +57,6 @@ class OrcScene(ctypes.Structure):
         ("air_view_steps", ctypes.c_uint32),
         ("air_sun_steps", ctypes.c_uint32),
         ("mie_height", ctypes.c_float),
-        ("volcano", ctypes.c_void_p),
     ]
 
 
@@ -109,15 +108,14 @@ def terrain_layout(cells: int) -> dict:
                 total=cache + 6 * nodes + stride * vertices)
 
 
-def _check_ground(frame, volcano) -> None:
-    """The library's frame check on the ground theme: themes past the dunes
-    (5) do not exist, and the basalt ground (4) needs its survey atlas."""
+def _check_ground(frame) -> None:
+    """The library's frame check on the ground theme: the basalt ground (4)
+    has no restated materials here or in the library, and themes past the
+    dunes (5) do not exist."""
     g = np.asarray(frame.globals)
     theme = int(g["ground_counts"][2]) & 255
-    if g["ground_north"][3] != 0 and theme > 5:
-        raise ValueError(f"ground theme {theme} does not exist")
-    if g["ground_north"][3] != 0 and theme == 4 and volcano is None:
-        raise ValueError("the basalt ground (theme 4) needs its survey atlas (upload_volcano)")
+    if g["ground_north"][3] != 0 and (theme == 4 or theme > 5):
+        raise ValueError(f"ground theme {theme} is not built")
 
 
 class OracleTracer:
@@ -141,7 +139,6 @@ class OracleTracer:
         self.sky_cells = sky_cells
         self.milky_way = milky_way
         self.terrain = None
-        self.volcano = None
         self.terrain_cells = 0
         self.terrain_radius = None
         self.sky = np.zeros((128, 4), np.float32)
@@ -187,12 +184,6 @@ class OracleTracer:
         self.terrain = t
         self.terrain_cells = cells
         self.terrain_radius = None
-
-    def upload_volcano(self, atlas: np.ndarray):
-        """The basalt ground's survey atlas (``sg_upload_volcano``)."""
-        self.volcano = np.ascontiguousarray(atlas, np.float32).copy()
-        self.key = b""
-        self.history_samples = 0
 
     def set_quality(self, samples, bounces, limit):
         bounces = min(max(bounces, 1), 64)
@@ -265,7 +256,6 @@ class OracleTracer:
         sc.sky_light = self.sky.ctypes.data
         sc.air_view_steps, sc.air_sun_steps, sc.mie_height = self.air
         sc.terrain = self.terrain.ctypes.data if self.terrain is not None else None
-        sc.volcano = self.volcano.ctypes.data if self.volcano is not None else None
         if self.catalogue is not None and len(self.catalogue):
             cat = np.ascontiguousarray(self.catalogue)
             cells = np.ascontiguousarray(self.sky_cells, np.uint32)
@@ -288,7 +278,7 @@ class OracleTracer:
 
     def encode(self, frame, rows=None):
         """One progressive pass; ``rows`` limits the rows traced (sampling only)."""
-        _check_ground(frame, self.volcano)
+        _check_ground(frame)
         if np.asarray(frame.globals)["ground_counts"][1] != 0 and self.terrain is None:
             # The library refuses the same frame (sg_encode, SG_E_STATE).
             raise RuntimeError("terrain enabled but no terrain uploaded (upload_terrain)")
@@ -323,7 +313,7 @@ class OracleTracer:
         return self.moments.reshape(self.height, self.width, 4).copy()
 
     def probe(self, frame, kind, rays):
-        _check_ground(frame, self.volcano)
+        _check_ground(frame)
         if (kind == 4 or np.asarray(frame.globals)["ground_counts"][1] != 0) and self.terrain is None:
             # Kind 4 is the terrain query; the library refuses the same probe
             # (sg_probe, SG_E_STATE).

@@ -120,7 +120,6 @@ typedef struct {
     uint32_t ring_profile_body;
     uint32_t view_steps, sun_steps; /* air march */
     R mie_height;                   /* 0: one scale height */
-    const float *volcano;           /* survey atlas of the basalt ground */
     /* per-dispatch counts.y (the sample index semantics of the host slices) */
     uint32_t accumulated;
 } Ctx;
@@ -1372,7 +1371,7 @@ static R air_light_visibility(const Ctx *c, Ray ray, uint32_t light) {
     return transmission;
 }
 
-/* --- ground materials, themes 0-5 (src/shaders/ground.wgsl:256-666) --- */
+/* --- ground materials, themes 0-3 and 5 (src/shaders/ground.wgsl:256-666) --- */
 static R footprint_of(const Ctx *c, R distance) {
     return distance * 2.0 * (R)c->s->cam_forward.v[3] / mx((R)c->s->viewport.v[1], 1.0);
 }
@@ -1635,89 +1634,6 @@ static void dune_gradient(R footprint, R scale, R radius, v3 p, v3 n, R g[2]) { 
 static v3 landscape_point(const Ctx *c, Ray ray, Hit hit) {
     return add(ground_origin(c, ray), scl(to_ground(c, ray.direction), hit.distance * METRES_PER_AU));
 }
-/* Theme 4, basalt: scoria, ash and lava (ground.wgsl:461-481). */
-static v3 basalt_floor(R footprint, v3 p, v3 normal) {
-    R broad = fbm(add(scl(p, 0.006), V(5.0, 9.0, 2.0)), 4u);
-    R weather = fbm(add(scl(p, 0.017), V(23.0, 7.0, 11.0)), 3u);
-    R middle = mixR(fbm(scl(p, 0.08), 3u), 0.5, smooth(1.0, 12.0, footprint));
-    v3 ash = mixV(V(0.065, 0.060, 0.052), V(0.14, 0.12, 0.095), broad);
-    v3 rock = mixV(V(0.028, 0.030, 0.031), V(0.07, 0.064, 0.054), middle);
-    rock = mixV(rock, V(0.16, 0.073, 0.037), smooth(0.48, 0.68, weather) * 0.45);
-    R slope = 1.0 - mx(normal.y, 0.0);
-    R loose = 1.0 - smooth(0.14, 0.43, slope + 0.12 * (weather - 0.5));
-    v3 color = scl(mixV(rock, ash, loose), 0.86 + 0.28 * middle);
-    R grain = vnoise(scl(p, 5.0));
-    return scl(color, 1.0 + (grain - 0.5) * 0.28 * (1.0 - smooth(0.025, 0.2, footprint)));
-}
-/* The survey atlas (volcano.wgsl:3-47). */
-static R volcano_bilinear(const Ctx *c, uint32_t offset, uint32_t width, R u, R v) {
-    R px = u * (R)width - 0.5, pz = v * (R)width - 0.5;
-    int lx = (int)floor(px), lz = (int)floor(pz), top = (int)width - 1;
-    uint32_t ax = (uint32_t)(lx < 0 ? 0 : (lx > top ? top : lx)), az = (uint32_t)(lz < 0 ? 0 : (lz > top ? top : lz));
-    uint32_t bx = (uint32_t)(lx + 1 < 0 ? 0 : (lx + 1 > top ? top : lx + 1));
-    uint32_t bz = (uint32_t)(lz + 1 < 0 ? 0 : (lz + 1 > top ? top : lz + 1));
-    R fx = fract(px), fz = fract(pz);
-    const float *d = c->volcano + offset;
-    return mixR(mixR(d[az * width + ax], d[az * width + bx], fx), mixR(d[bz * width + ax], d[bz * width + bx], fx), fz);
-}
-static R volcano_map(const Ctx *c, uint32_t map, R x, R z, R footprint) {
-    const float *desc = c->volcano + map * 4u;
-    R lz = z - desc[3];
-    R reach = 0.5 * ((R)desc[1] - 1.0) * desc[2];
-    if (fabs(x) > reach || fabs(lz) > reach) return 0.0;
-    R u = (x / desc[2] + 0.5 * desc[1]) / desc[1], v = (lz / desc[2] + 0.5 * desc[1]) / desc[1];
-    R lod = mx(log2(mx(footprint / desc[2], 1.0)), 0.0);
-    uint32_t offset = (uint32_t)desc[0], width = (uint32_t)desc[1];
-    for (uint32_t level = 0u; level < (uint32_t)lod && width > 1u; ++level) {
-        offset += width * width;
-        width = (width + 1u) / 2u;
-    }
-    R a = volcano_bilinear(c, offset, width, u, v);
-    if (width == 1u) return a;
-    return mixR(a, volcano_bilinear(c, offset + width * width, (width + 1u) / 2u, u, v), fract(lod));
-}
-static R volcano_fine_weight(const Ctx *c, R x, R z) {
-    const float *desc = c->volcano + 4u;
-    R reach = 0.5 * ((R)desc[1] - 1.0) * desc[2];
-    return 1.0 - smooth(reach - 72.0, reach, mx(fabs(x), fabs(z - desc[3])));
-}
-static R volcano_height(const Ctx *c, uint32_t map, R x, R z, R footprint) {
-    R coarse = volcano_map(c, map, x, z, footprint);
-    if (map != 0u) return coarse;
-    R weight = volcano_fine_weight(c, x, z);
-    if (weight <= 0.0) return coarse;
-    return mixR(coarse, volcano_map(c, 1u, x, z, footprint), weight);
-}
-/* Surveyed shading normal of the basalt terrain (volcano.wgsl:49-88). */
-static v3 volcano_normal(const Ctx *c, v3 p, R distance, v3 fallback) {
-    R scale = terrain_scale(c);
-    R px = p.x / scale, pz = p.z / scale;
-    R footprint = distance * 2.0 * c->s->cam_forward.v[3] / mx(c->s->viewport.v[1], 1.0) / scale;
-    R best = 1.0, site_scale = 1.0, lx = 0.0, lz = 0.0, t[4] = {0.0, 0.0, 1.0, 0.0};
-    uint32_t chosen = 0u;
-    for (uint32_t site = 0u; site < 9u; ++site) {
-        const float *frame = c->volcano + 16u + site * 8u, *info = frame + 4u;
-        R rx = (px - frame[0]) / info[0], rz = (pz - frame[1]) / info[0];
-        R qx = frame[2] * rx - frame[3] * rz, qz = frame[3] * rx + frame[2] * rz;
-        R height = volcano_height(c, (uint32_t)info[1], qx, qz, footprint / info[0]) * info[0];
-        if (height > best) {
-            best = height; chosen = (uint32_t)info[1]; lx = qx; lz = qz;
-            t[0] = frame[0]; t[1] = frame[1]; t[2] = frame[2]; t[3] = frame[3]; site_scale = info[0];
-        }
-    }
-    if (best <= 1.0) return fallback;
-    R spacing = c->volcano[chosen * 4u + 2u];
-    if (chosen == 0u) spacing = mixR(spacing, 1.0, volcano_fine_weight(c, lx, lz));
-    R delta = mx(spacing, footprint / site_scale * 0.5), fp = footprint / site_scale;
-    R dx = (volcano_height(c, chosen, lx + delta, lz, fp) - volcano_height(c, chosen, lx - delta, lz, fp)) / (2.0 * delta);
-    R dz = (volcano_height(c, chosen, lx, lz + delta, fp) - volcano_height(c, chosen, lx, lz - delta, fp)) / (2.0 * delta);
-    R gx = t[2] * dx + t[3] * dz, gz = -t[3] * dx + t[2] * dz;
-    R radius = c->s->ground_up.v[3];
-    R bend = sqrt(mx(radius * radius - (p.x * p.x + p.z * p.z), 1.0));
-    v3 n = nrm(V(-gx + p.x / bend, 1.0, -gz + p.z / bend));
-    if (dot(n, fallback) < 0.2) return nrm(mixV(fallback, n, 0.7));
-    return n;
-}
 /* Ground material at surface-frame point p (metres) of landscape element
  * `index`, seen from `distance` metres, with surface-frame normal n
  * (ground.wgsl:597-640). */
@@ -1727,7 +1643,6 @@ static v3 ground_albedo(const Ctx *c, v3 p, R distance, v3 n, uint32_t index) {
         if (theme == 1u) return desert_floor(footprint_of(c, distance), p, n);
         if (theme == 2u) return crater_floor(footprint_of(c, distance), p, n);
         if (theme == 3u) return alpine_floor(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n);
-        if (theme == 4u) return basalt_floor(footprint_of(c, distance), p, n);
         if (theme == 5u) return dune_floor(footprint_of(c, distance), terrain_scale(c), (R)c->s->ground_up.v[3], p, n);
         return forest_floor(c, p, distance, n);
     }
@@ -1762,8 +1677,6 @@ static v3 landscape_albedo(const Ctx *c, Ray ray, Hit hit) {
 static v3 ground_shading_normal(const Ctx *c, v3 p, R distance, v3 normal, uint32_t index) {
     if (index != LOCAL_BASE && index != TERRAIN_ID) return normal;
     uint32_t theme = ground_theme(c);
-    if (theme == 4u && index == TERRAIN_ID) /* ground.wgsl:637-641 */
-        return from_ground(c, volcano_normal(c, p, distance, to_ground(c, normal)));
     if (theme == 3u || theme == 5u) { /* ground.wgsl:642-643 */
         v3 n = to_ground(c, normal);
         R g[2];
@@ -2403,7 +2316,6 @@ static Ctx make_ctx(const orc_scene *sc) {
     c.view_steps = sc->air_view_steps ? sc->air_view_steps : ATMO_VIEW_STEPS;
     c.sun_steps = sc->air_sun_steps ? sc->air_sun_steps : ATMO_SUN_STEPS;
     c.mie_height = sc->mie_height;
-    c.volcano = sc->volcano;
     c.accumulated = ((const Settings *)sc->settings)->counts[1];
     return c;
 }

@@ -38,7 +38,6 @@ typedef struct orc_scene {
     uint32_t air_view_steps;
     uint32_t air_sun_steps;
     float mie_height;
-    const float *volcano; /* survey atlas of the basalt ground, or NULL */
 } orc_scene;
 
 typedef struct orc_probe_ray {

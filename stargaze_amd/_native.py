@@ -28,7 +28,6 @@ EXPORTS = (
     "sg_set_quality",
     "sg_set_exposure_controls",
     "sg_upload_terrain",
-    "sg_upload_volcano",
     "sg_encode",
     "sg_read_accumulation",
     "sg_read_backdrop",
@@ -155,7 +154,6 @@ def load() -> ctypes.CDLL:
         "sg_set_quality": ([P, U32, U32, U32], I32),
         "sg_set_exposure_controls": ([P, I32, ctypes.c_float], I32),
         "sg_upload_terrain": ([P, P, U64, U32], I32),
-        "sg_upload_volcano": ([P, P, U64], I32),
         "sg_encode": ([P, ctypes.POINTER(SgFrame)], I32),
         "sg_read_accumulation": ([P, P, U64], I32),
         "sg_read_backdrop": ([P, P, U64], I32),

@@ -123,9 +123,6 @@ struct Scene {
     // (atmo_params.y) for both species; a positive value gives the Mie
     // density its own exp(-h / H_mie) while no density table is set.
     float mie_height;
-    // Survey atlas of the basalt ground (four map descriptors, nine site
-    // records, then the elevation mip pyramids); null when not uploaded.
-    const float *volcano;
 };
 
 struct Ray {

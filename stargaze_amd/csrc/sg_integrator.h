@@ -134,7 +134,7 @@ struct GradSink {
 };
 }  // namespace
 
-// --- ground materials, themes 0-5 (src/shaders/ground.wgsl:256-666) ----------
+// --- ground materials, themes 0-3 and 5 (src/shaders/ground.wgsl:256-666) ----
 SG_FN float footprint_of(const Scene &sc, float distance) {
     return distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f);
 }
@@ -431,111 +431,6 @@ SG_OUTLINE float2 dune_gradient(float footprint, float scale, float radius, f3 p
 SG_FN d3 landscape_point(const Scene &sc, const Ray &ray, const Hit &hit) {
     return ground_origin(sc, ray) + tod(to_ground(sc, ray.direction)) * ((double)hit.distance * METRES_PER_AU_D);
 }
-// Theme 4, basalt: charcoal scoria, weathered ash and exposed lava, with
-// the pores and gravel filtered by the pixel footprint
-// (src/shaders/ground.wgsl:461-481).
-SG_OUTLINE f3 basalt_floor(float footprint, f3 p, f3 normal) {
-    const float broad = fbm(p * 0.006f + f3{5.0f, 9.0f, 2.0f}, 4u);
-    const float weather = fbm(p * 0.017f + f3{23.0f, 7.0f, 11.0f}, 3u);
-    const float middle = mixf(fbm(p * 0.08f, 3u), 0.5f, smoothstepf(1.0f, 12.0f, footprint));
-    const f3 ash = mix3(f3{0.065f, 0.060f, 0.052f}, f3{0.14f, 0.12f, 0.095f}, broad);
-    f3 rock = mix3(f3{0.028f, 0.030f, 0.031f}, f3{0.07f, 0.064f, 0.054f}, middle);
-    rock = mix3(rock, f3{0.16f, 0.073f, 0.037f}, smoothstepf(0.48f, 0.68f, weather) * 0.45f);
-    const float slope = 1.0f - fmaxf(normal.y, 0.0f);
-    const float loose = 1.0f - smoothstepf(0.14f, 0.43f, slope + 0.12f * (weather - 0.5f));
-    f3 color = mix3(rock, ash, loose);
-    color = color * (0.86f + 0.28f * middle);
-    const float grain = vnoise(p * 5.0f);
-    return color * (1.0f + (grain - 0.5f) * 0.28f * (1.0f - smoothstepf(0.025f, 0.2f, footprint)));
-}
-// The survey atlas (src/shaders/volcano.wgsl:3-47): descriptors are
-// (offset, width, spacing, north shift); each map is an area-averaged mip
-// pyramid sampled bilinearly between the two levels around the footprint.
-SG_FN float4 volcano_record(const Scene &sc, uint32_t i) {
-    return make_float4(sc.volcano[i], sc.volcano[i + 1u], sc.volcano[i + 2u], sc.volcano[i + 3u]);
-}
-SG_FN float volcano_bilinear(const Scene &sc, uint32_t offset, uint32_t width, float u, float v) {
-    const float px = u * (float)width - 0.5f, pz = v * (float)width - 0.5f;
-    const int lx = (int)floorf(px), lz = (int)floorf(pz);
-    const int top = (int)width - 1;
-    auto clamp_index = [top](int i) { return (uint32_t)(i < 0 ? 0 : (i > top ? top : i)); };
-    const uint32_t ax = clamp_index(lx), az = clamp_index(lz), bx = clamp_index(lx + 1), bz = clamp_index(lz + 1);
-    const float fx = fractf(px), fz = fractf(pz);
-    const float *d = sc.volcano + offset;
-    return mixf(mixf(d[az * width + ax], d[az * width + bx], fx), mixf(d[bz * width + ax], d[bz * width + bx], fx),
-                fz);
-}
-SG_FN float volcano_map(const Scene &sc, uint32_t map, float2 p, float footprint) {
-    const float4 desc = volcano_record(sc, map * 4u);
-    const float lx = p.x, lz = p.y - desc.w;
-    const float reach = 0.5f * (desc.y - 1.0f) * desc.z;
-    if (fabsf(lx) > reach || fabsf(lz) > reach) return 0.0f;
-    const float u = (lx / desc.z + 0.5f * desc.y) / desc.y, v = (lz / desc.z + 0.5f * desc.y) / desc.y;
-    const float lod = fmaxf(log2f(fmaxf(footprint / desc.z, 1.0f)), 0.0f);
-    uint32_t offset = (uint32_t)desc.x, width = (uint32_t)desc.y;
-    for (uint32_t level = 0u; level < (uint32_t)lod && width > 1u; ++level) {
-        offset += width * width;
-        width = (width + 1u) / 2u;
-    }
-    const float a = volcano_bilinear(sc, offset, width, u, v);
-    if (width == 1u) return a;
-    return mixf(a, volcano_bilinear(sc, offset + width * width, (width + 1u) / 2u, u, v), fractf(lod));
-}
-SG_FN float volcano_fine_weight(const Scene &sc, float2 p) {
-    const float4 desc = volcano_record(sc, 4u);
-    const float reach = 0.5f * (desc.y - 1.0f) * desc.z;
-    return 1.0f - smoothstepf(reach - 72.0f, reach, fmaxf(fabsf(p.x), fabsf(p.y - desc.w)));
-}
-SG_FN float volcano_height(const Scene &sc, uint32_t map, float2 p, float footprint) {
-    const float coarse = volcano_map(sc, map, p, footprint);
-    if (map != 0u) return coarse;
-    const float weight = volcano_fine_weight(sc, p);
-    if (weight <= 0.0f) return coarse;
-    return mixf(coarse, volcano_map(sc, 1u, p, footprint), weight);
-}
-// Shading normal of the surveyed volcanoes at surface-frame point p: the
-// highest site's survey differenced over the larger of its spacing and half
-// the footprint, plus the host's curvature; kept in the hemisphere of the
-// terrain's own normal (src/shaders/volcano.wgsl:49-88).
-SG_OUTLINE f3 volcano_normal(const Scene &sc, f3 p, float distance, f3 fallback) {
-    const float scale = terrain_scale(sc);
-    const float2 point = make_float2(p.x / scale, p.z / scale);
-    const float footprint = distance * 2.0f * sc.s.globals.cam_forward[3] / fmaxf(sc.s.globals.viewport[1], 1.0f) / scale;
-    float best = 1.0f, site_scale = 1.0f;
-    uint32_t chosen = 0u;
-    float2 local = make_float2(0.0f, 0.0f);
-    float4 transform = make_float4(0.0f, 0.0f, 1.0f, 0.0f);
-    for (uint32_t site = 0u; site < 9u; ++site) {
-        const float4 frame = volcano_record(sc, 16u + site * 8u);
-        const float4 info = volcano_record(sc, 20u + site * 8u);
-        const float rx = (point.x - frame.x) / info.x, rz = (point.y - frame.y) / info.x;
-        const float2 q = make_float2(frame.z * rx - frame.w * rz, frame.w * rx + frame.z * rz);
-        const float height = volcano_height(sc, (uint32_t)info.y, q, footprint / info.x) * info.x;
-        if (height > best) {
-            best = height;
-            chosen = (uint32_t)info.y;
-            local = q;
-            transform = frame;
-            site_scale = info.x;
-        }
-    }
-    if (best <= 1.0f) return fallback;
-    float spacing = volcano_record(sc, chosen * 4u).z;
-    if (chosen == 0u) spacing = mixf(spacing, 1.0f, volcano_fine_weight(sc, local));
-    const float delta = fmaxf(spacing, footprint / site_scale * 0.5f);
-    const float fp = footprint / site_scale;
-    const float dx = (volcano_height(sc, chosen, make_float2(local.x + delta, local.y), fp) -
-                      volcano_height(sc, chosen, make_float2(local.x - delta, local.y), fp)) / (2.0f * delta);
-    const float dz = (volcano_height(sc, chosen, make_float2(local.x, local.y + delta), fp) -
-                      volcano_height(sc, chosen, make_float2(local.x, local.y - delta), fp)) / (2.0f * delta);
-    const float gx = transform.z * dx + transform.w * dz, gz = -transform.w * dx + transform.z * dz;
-    const float radius = sc.s.globals.ground_up[3];
-    const float bend = sqrtf(fmaxf(radius * radius - (p.x * p.x + p.z * p.z), 1.0f));
-    const f3 n = normalize(f3{-gx + p.x / bend, 1.0f, -gz + p.z / bend});
-    if (dot(n, fallback) < 0.2f) return normalize(mix3(fallback, n, 0.7f));
-    return n;
-}
-
 // Ground material at surface-frame point p (metres) of landscape element
 // `index`, seen from `distance` metres, with surface-frame normal n.
 SG_FN f3 ground_albedo(const Scene &sc, f3 p, float distance, f3 n, uint32_t index) {
@@ -544,7 +439,6 @@ SG_FN f3 ground_albedo(const Scene &sc, f3 p, float distance, f3 n, uint32_t ind
         case 1u: return desert_floor(footprint_of(sc, distance), p, n);
         case 2u: return crater_floor(footprint_of(sc, distance), p, n);
         case 3u: return alpine_floor(footprint_of(sc, distance), terrain_scale(sc), sc.s.globals.ground_up[3], p, n);
-        case 4u: return basalt_floor(footprint_of(sc, distance), p, n);
         case 5u: return dune_floor(footprint_of(sc, distance), terrain_scale(sc), sc.s.globals.ground_up[3], p, n);
         default: return forest_floor(sc, p, distance, n);
         }
@@ -580,9 +474,6 @@ SG_FN f3 landscape_albedo(const Scene &sc, const Ray &ray, const Hit &hit) {
 SG_FN f3 ground_shading_normal(const Scene &sc, f3 p, float distance, f3 normal, uint32_t index) {
     if (index != SG_LOCAL_BASE && index != SG_TERRAIN_ID) return normal;
     const uint32_t theme = ground_theme(sc);
-    // The survey supplies the basalt terrain's detail; the generic lattice
-    // bump over it would turn grazing light into a brick pattern.
-    if (theme == 4u && index == SG_TERRAIN_ID) return from_ground(sc, volcano_normal(sc, p, distance, to_ground(sc, normal)));
     if (theme == 3u || theme == 5u) {
         const f3 n = to_ground(sc, normal);
         const float2 g = theme == 3u ? alpine_gradient(footprint_of(sc, distance), p, n)

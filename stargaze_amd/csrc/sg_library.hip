@@ -97,8 +97,6 @@ struct sg_ctx {
     float4 *sky_light = nullptr;
     float *terrain = nullptr;
     uint64_t terrain_floats = 0;
-    float *volcano = nullptr;  // survey atlas of the basalt ground
-    uint64_t volcano_floats = 0;
     uint32_t terrain_cells = 0;
     bool terrain_cached = false;
     uint32_t terrain_radius_bits = 0;
@@ -328,7 +326,6 @@ Scene make_scene(const sg_ctx *ctx, const sg_settings &settings) {
     sc.air_view_steps = ctx->air_view_steps;
     sc.air_sun_steps = ctx->air_sun_steps;
     sc.mie_height = ctx->mie_height;
-    sc.volcano = ctx->volcano;
     return sc;
 }
 
@@ -380,9 +377,8 @@ int validate_frame(sg_ctx *ctx, const sg_frame *frame) {
     const sg_globals *g = frame->globals;
     if (g->ground_north[3] != 0.0f) {
         const uint32_t theme = g->ground_counts[2] & 255u;
-        if (theme > 5u) return fail(ctx, SG_E_INVALID, "ground theme must be 0-5");
-        if (theme == 4u && !ctx->volcano)
-            return fail(ctx, SG_E_STATE, "the basalt ground needs its survey atlas (sg_upload_volcano)");
+        if (theme == 4u || theme > 5u)
+            return fail(ctx, SG_E_INVALID, "ground theme 4 (basalt, which needs the volcano survey) is not built");
         uint32_t host = (uint32_t)g->ground_north[3] - 1u;
         if (host >= frame->body_count) return fail(ctx, SG_E_INVALID, "ground host index out of range");
         if (g->ground_counts[1] != 0u && !ctx->terrain)
@@ -881,7 +877,7 @@ int sg_destroy(sg_ctx *ctx) {
     if (ctx->batch) return fail(ctx, SG_E_STATE, "context is a view of a batch; destroy the batch first");
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     void *buffers[] = {ctx->accumulation, ctx->backdrop, ctx->moments, ctx->bodies, ctx->centres, ctx->objects, ctx->sky_light,
-                       ctx->terrain, ctx->volcano, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
+                       ctx->terrain, ctx->catalogue, ctx->sky_cells, ctx->milky_way, ctx->probe_rays,
                        ctx->probe_results, ctx->wave_block, ctx->queues[0], ctx->air_block, ctx->bvh_nodes,
                        ctx->bvh_prims, ctx->ring_profile, ctx->density_profile, ctx->adj_image,
                        ctx->adj_gradient, ctx->adj_primal, ctx->records, ctx->exposure, ctx->histogram};
@@ -991,48 +987,6 @@ int sg_upload_terrain(sg_ctx *ctx, const float *data, uint64_t float_count, uint
     ctx->terrain_cached = false;
     // New ground is a new scene: restart the accumulation, as the profile
     // setters do.
-    restart_history(ctx);
-    return SG_OK;
-}
-
-// The basalt ground's survey atlas (built by src/volcano.rs:15-110, written
-// after the terrain at src/pathtracer.rs:917-926): every descriptor and site
-// record is checked so the kernels' reads stay inside the upload.
-int sg_upload_volcano(sg_ctx *ctx, const float *data, uint64_t float_count) {
-    if (!ctx) return SG_E_INVALID;
-    constexpr uint64_t kHeader = 4u * 4u + 9u * 8u;
-    if (!data || float_count < kHeader || float_count > (1ull << 24))
-        return fail(ctx, SG_E_INVALID, "volcano atlas must hold its 88-float header and at most 2^24 floats");
-    for (uint64_t i = 0; i < float_count; ++i)
-        if (!std::isfinite(data[i])) return fail(ctx, SG_E_INVALID, "volcano atlas holds a non-finite value");
-    for (uint32_t m = 0; m < 4u; ++m) {
-        const float offset = data[m * 4u], width = data[m * 4u + 1u], spacing = data[m * 4u + 2u];
-        if (!(width >= 1.0f && width <= 65536.0f && width == floorf(width) && offset >= (float)kHeader &&
-              offset == floorf(offset) && spacing > 0.0f))
-            return fail(ctx, SG_E_INVALID, "volcano map descriptor out of range");
-        uint64_t w = (uint64_t)width, levels = 0;
-        for (;;) {
-            levels += w * w;
-            if (w == 1u) break;
-            w = (w + 1u) / 2u;
-        }
-        if ((uint64_t)offset + levels > float_count)
-            return fail(ctx, SG_E_INVALID, "volcano map pyramid runs past the atlas");
-    }
-    for (uint32_t site = 0; site < 9u; ++site) {
-        const float *r = data + 16u + site * 8u;
-        if (!(r[4] > 0.0f) || !(r[5] >= 0.0f && r[5] <= 3.0f && r[5] == floorf(r[5])))
-            return fail(ctx, SG_E_INVALID, "volcano site record out of range");
-    }
-    join_batch(ctx);
-    SG_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    if (ctx->volcano && ctx->volcano_floats != float_count) {
-        SG_HIP(ctx, hipFree(ctx->volcano));
-        ctx->volcano = nullptr;
-    }
-    if (!ctx->volcano) SG_HIP(ctx, hipMalloc(&ctx->volcano, float_count * sizeof(float)));
-    SG_HIP(ctx, hipMemcpy(ctx->volcano, data, float_count * sizeof(float), hipMemcpyHostToDevice));
-    ctx->volcano_floats = float_count;
     restart_history(ctx);
     return SG_OK;
 }

@@ -122,8 +122,6 @@ def render(request: Request, scene: sy.Scene, output: Path, terrain_for=tr.for_t
                         options=Options(samples_per_frame=preset.samples_per_frame, max_bounces=preset.max_bounces,
                                         sample_limit=request.samples, exposure=float(request.exposure)))
     tracer.upload_terrain(tr.terrain_data(*terrain), terrain[1])
-    if scene.ground == sy.THEMES["basalt"]:
-        tracer.upload_volcano(tr.basalt_atlas())
     tracer.synchronize()
     start = time.perf_counter()
     while tracer.samples() < request.samples:

@@ -142,11 +142,6 @@ class PathTracer:
         d = np.ascontiguousarray(data, np.float32)
         self._check(self._lib.sg_upload_terrain(self._ctx, d.ctypes.data, d.size, int(cells)), "upload_terrain")
 
-    def upload_volcano(self, atlas: np.ndarray) -> None:
-        """The basalt ground's survey atlas (``terrain.volcano_atlas``)."""
-        a = np.ascontiguousarray(atlas, np.float32)
-        self._check(self._lib.sg_upload_volcano(self._ctx, a.ctypes.data, a.size), "upload_volcano")
-
     def set_milky_way(self, texels: np.ndarray) -> None:
         t = np.ascontiguousarray(texels, np.uint32)
         self._check(self._lib.sg_set_milky_way(self._ctx, t.ctypes.data, t.shape[1], t.shape[0]), "set_milky_way")

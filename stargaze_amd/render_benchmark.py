@@ -13,9 +13,8 @@ Three things differ, each recorded in the results:
   (``reference_terrain``), except the alpine ground, which needs the
   reference's elevation survey: without ``STARGAZE_TERRAIN_SURVEYS`` it
   stands on a seeded stand-in.
-* The basalt ground (cases ``ground-basalt`` and ``level-1``) needs the
-  reference's elevation surveys; without them those cases are recorded
-  with a ``skipped`` reason instead of timings.
+* The basalt ground (cases ``ground-basalt`` and ``level-1``) is not built:
+  those cases are recorded with a ``skipped`` reason instead of timings.
 * ``gpu_ms`` is the HIP-event time of the integrator kernels of one
   ``encode``; the reference's GPU timestamps also cover its display pass,
   which here runs only for captures.
@@ -51,15 +50,8 @@ CASES = (
     "level-3",
 )  # src/render_benchmark.rs:14-35
 VARIANTS = ("reference",)  # the shader experiments of render_experiments.rs are not built
-SURVEYED = ("ground-basalt", "level-1")  # cases on the basalt ground
-
-
-def unavailable(name: str) -> str | None:
-    """Why a case cannot run here, or None: the basalt ground needs the
-    reference's elevation surveys, which are not part of this repository."""
-    if name in SURVEYED and not tr.surveys_folder():
-        return "the basalt ground needs the reference's elevation surveys (STARGAZE_TERRAIN_SURVEYS)"
-    return None
+NOT_BUILT = {"ground-basalt": "ground theme 4 (basalt) is not built",
+             "level-1": "ground theme 4 (basalt) is not built"}
 
 # Fixed manual exposure of the captures (src/render_benchmark.rs:899-915).
 EXPOSURES = {"vantus-eclipse": 8.0, "vantus-airless": 8.0, "moonlit-air": 64.0, "level-1": 110.0, "level-2": 0.16,
@@ -401,16 +393,14 @@ def _adapter() -> dict:
 class _Tracers:
     """One tracer per option set of a case, with the catalogue and terrain."""
 
-    def __init__(self, cfg: Config, catalogue, terrain, theme=None):
-        self.cfg, self.catalogue, self.terrain, self.theme = cfg, catalogue, terrain, theme
+    def __init__(self, cfg: Config, catalogue, terrain):
+        self.cfg, self.catalogue, self.terrain = cfg, catalogue, terrain
 
     def make(self, options: Options) -> PathTracer:
         cat, cells = self.catalogue
         t = PathTracer(self.cfg.trace_size(), catalogue=cat, sky_cells=cells, options=options)
         if self.terrain is not None:
             t.upload_terrain(tr.terrain_data(*self.terrain), self.terrain[1])
-        if self.theme == 4:
-            t.upload_volcano(tr.basalt_atlas())
         return t
 
 
@@ -442,14 +432,13 @@ def time_cases(cfg: Config, terrain_for=reference_terrain, log=print) -> list:
     options = Options(samples_per_frame=1, max_bounces=cfg.bounces, auto_exposure=True)
     cases = []
     for name in cfg.cases:
-        reason = unavailable(name)
-        if reason:
-            log(f"{name:20} skipped: {reason}")
-            cases.append({"name": name, "skipped": reason})
+        if name in NOT_BUILT:
+            log(f"{name:20} skipped: {NOT_BUILT[name]}")
+            cases.append({"name": name, "skipped": NOT_BUILT[name]})
             continue
         theme = ground_theme(name)
         terrain = terrain_for(theme) if theme is not None else None
-        tracers = _Tracers(cfg, catalogue, terrain, theme)
+        tracers = _Tracers(cfg, catalogue, terrain)
         frames = [fixture(name, i, cfg, terrain) for i in range(max(cfg.frames, cfg.warmup))]
         modes = []
         for moving in (False, True):

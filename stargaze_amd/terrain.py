@@ -696,114 +696,17 @@ def stand_in(theme: int) -> tuple[np.ndarray, int]:
 _FOR_THEME: dict = {}
 
 
-def surveys_folder() -> str | None:
-    """The folder of the reference's elevation surveys, if named
-    (``STARGAZE_TERRAIN_SURVEYS``)."""
-    import os
-
-    return os.environ.get("STARGAZE_TERRAIN_SURVEYS") or None
-
-
 def for_theme(theme: int) -> tuple[np.ndarray, int]:
     """``terrain::for_theme`` (src/terrain.rs:58-62): each ground's terrain,
     generated once per process. The alpine and basalt grounds read the
-    reference's elevation surveys from ``surveys_folder()``; without it the
-    alpine ground stands on ``stand_in`` and the basalt ground is refused."""
+    reference's elevation surveys from the folder ``STARGAZE_TERRAIN_SURVEYS``
+    names; without it they stand on ``stand_in``."""
+    import os
+
     if theme not in _FOR_THEME:
-        folder = surveys_folder()
-        if theme == 4 and not folder:
-            raise FileNotFoundError("the basalt ground needs the reference's elevation surveys "
-                                    "(set STARGAZE_TERRAIN_SURVEYS to their folder)")
-        if theme == 3 and not folder:
+        folder = os.environ.get("STARGAZE_TERRAIN_SURVEYS")
+        if theme in (3, 4) and not folder:
             _FOR_THEME[theme] = stand_in(theme)
         else:
             _FOR_THEME[theme] = generate(theme, surveys=Surveys(folder) if theme in (3, 4) else None)
     return _FOR_THEME[theme]
-
-
-_ATLAS: list = []
-
-
-def basalt_atlas() -> np.ndarray:
-    """``terrain::basalt_detail`` (src/terrain.rs:6): the survey atlas from
-    ``surveys_folder()``, built once per process."""
-    if not _ATLAS:
-        folder = surveys_folder()
-        if not folder:
-            raise FileNotFoundError("the basalt ground needs the reference's elevation surveys "
-                                    "(set STARGAZE_TERRAIN_SURVEYS to their folder)")
-        _ATLAS.append(volcano_atlas(Surveys(folder)))
-    return _ATLAS[0]
-
-
-# -- the basalt ground's survey atlas (src/volcano.rs:15-110) -----------------
-VOLCANO_HEADER = 4 * 4 + 9 * 8
-# (width, spacing) of the four maps: Stromboli's island grid and its summit
-# grid, then Alicudi and Filicudi (src/volcano.rs:18-23).
-VOLCANO_MAPS = ((1281, 5.0), (1025, 1.0), (320, 25.0), (320, 25.0))
-
-
-def pyramid_len(width: int) -> int:
-    count = width * width
-    while width > 1:
-        width = -(-width // 2)
-        count += width * width
-    return count
-
-
-def volcano_atlas(surveys: Surveys) -> np.ndarray:
-    """``volcano::detail``: four map descriptors (offset, width, spacing,
-    north shift), nine site records (Stromboli's own frame, then
-    ``VOLCANO_SITES``), then each map's elevation (metres) and its
-    area-averaged mip pyramid, odd edges repeating their last sample."""
-    data = [np.zeros(VOLCANO_HEADER, F)]
-    head = data[0]
-    head[16:24] = [0.0, 0.0, 1.0, 0.0, 1.0, 0.0, 0.0, 0.0]
-    rad = F(np.pi) / F(180.0)
-    for i, (island, bearing, distance, turn, size) in enumerate(VOLCANO_SITES):
-        b, t = F(bearing) * rad, F(turn) * rad
-        head[24 + i * 8:32 + i * 8] = [F(distance) * np.sin(b), F(distance) * np.cos(b), np.cos(t), np.sin(t), size,
-                                       0.0 if island == 0 else island + 1, 0.0, 0.0]
-    offset = VOLCANO_HEADER
-    sources = (surveys.stromboli[:1281 * 1281], surveys.stromboli[1281 * 1281:],
-               surveys.aeolian[320 * 320:2 * 320 * 320], surveys.aeolian[2 * 320 * 320:3 * 320 * 320])
-    for m, ((width, spacing), source) in enumerate(zip(VOLCANO_MAPS, sources)):
-        head[m * 4:m * 4 + 4] = [offset, width, spacing, 250.0 if m == 1 else 0.0]
-        level = (source.astype(F) * F(0.1)).reshape(width, width)
-        w = width
-        while True:
-            data.append(level.reshape(-1))
-            offset += w * w
-            if w == 1:
-                break
-            nxt = -(-w // 2)
-            idx = np.arange(nxt) * 2
-            hi = np.minimum(idx + 1, w - 1)
-            level = F(0.25) * (level[idx][:, idx] + level[idx][:, hi] + level[hi][:, idx] + level[hi][:, hi])
-            w = nxt
-    atlas = np.concatenate(data).astype(F)
-    assert atlas.size == VOLCANO_HEADER + sum(pyramid_len(w) for w, _ in VOLCANO_MAPS)
-    return atlas
-
-
-def synthetic_surveys(seed: int = 7) -> Surveys:
-    """Stand-in surveys with the reference's grid sizes and spacings: seeded
-    cones with ridged flanks and a summit crater, in decimetres. For tests
-    where the reference's surveys are not at hand."""
-    s = Surveys.__new__(Surveys)
-
-    def island(n, spacing, peak, radius, key):
-        c = (np.arange(n, dtype=F) - F((n - 1) / 2)) * F(spacing)
-        x, z = c[None, :], c[:, None]
-        r = np.hypot(x, z)
-        cone = np.maximum(F(peak) * (F(1.0) - r / F(radius)), F(0.0))
-        crater = F(0.15 * peak) * np.exp(-(r / F(0.08 * radius)) ** 2)
-        rough = F(0.08 * peak) * fbm(x / F(0.2 * radius), z / F(0.2 * radius), key, 4)
-        h = np.maximum(cone - crater + rough * (cone > 0), F(0.0))
-        return np.clip(h * F(10.0), 0, 65535).astype("<u2").reshape(-1)
-
-    s.patagonia = island(641, 50.0, 1800.0, 9000.0, seed)
-    s.aeolian = np.concatenate([island(320, 25.0, 900.0, 3500.0, seed + k) for k in range(6)])
-    s.stromboli = np.concatenate([island(1281, 5.0, 924.0, 3000.0, seed + 11),
-                                  island(1025, 1.0, 300.0, 500.0, seed + 12)])
-    return s

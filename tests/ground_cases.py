@@ -89,56 +89,3 @@ def prop_rays(frame, per_prop=24, seed=5):
     rays["anchor"] = MISS
     rays["direction"] = np.concatenate(dirs)
     return rays, np.array(kinds)
-
-
-PLANE = (0.1, -0.2, 1200.0)  # sx, sz, base (metres): whole decimetres on both Stromboli grids
-
-
-def planar_surveys(slope=PLANE[:2], base=PLANE[2]):
-    """Surveys whose Stromboli grids sample one plane h = base + sx x + sz z
-    (metres, world east/north about the lookout; the summit grid sits 250 m
-    north), and whose islands are flat sea. The slopes make every sample a
-    whole number of decimetres, and bilinear samples, area-averaged mips and
-    central differences of a plane are exact, so the basalt terrain's
-    surveyed normal is known in closed form."""
-    from stargaze_amd import terrain as tr
-
-    s = tr.Surveys.__new__(tr.Surveys)
-
-    def grid(n, spacing, north):
-        c = (np.arange(n, dtype=np.float64) - (n - 1) / 2) * spacing
-        h = base + slope[0] * c[None, :] + slope[1] * (c[:, None] + north)
-        return np.round(h * 10.0).astype("<u2").reshape(-1)
-
-    s.stromboli = np.concatenate([grid(1281, 5.0, 0.0), grid(1025, 1.0, 250.0)])
-    s.aeolian = np.zeros(6 * 320 * 320, "<u2")
-    s.patagonia = np.zeros(641 * 641, "<u2")
-    return s
-
-
-def planar_normal_rays(frame, points=((0.0, 0.0), (300.0, -200.0), (-700.0, 500.0), (150.0, 800.0)),
-                       distance=1.0):
-    """SG_PROBE_GROUND records on the terrain at surface points (x, z) with
-    an upright geometric normal, and the closed-form surveyed normal there
-    (world frame) for ``planar_surveys``: (-sx + x/b, 1, -sz + z/b)
-    normalised, b the host curvature term. Seen from ``distance`` metres the
-    pixel footprint stays below both grids' spacing, so the finest mip level
-    is read (the coarser levels of the odd-sized grids sit a fraction of a
-    texel off the plane)."""
-    from stargaze_amd.layouts import TERRAIN_ID
-
-    g = frame.globals
-    east, up, north = (g[k][:3].astype(np.float64) for k in ("ground_east", "ground_up", "ground_north"))
-    radius = float(g["ground_up"][3])
-    rays = np.zeros(len(points), PROBE_RAY)
-    expect = []
-    for i, (x, z) in enumerate(points):
-        rays[i]["offset"] = [x, 0.0, z]
-        rays[i]["direction"] = [0.0, 1.0, 0.0]
-        rays[i]["anchor"] = TERRAIN_ID
-        rays[i]["target"] = np.float32(distance).view(np.uint32)
-        bend = np.sqrt(max(radius * radius - (x * x + z * z), 1.0))
-        n = np.array([-PLANE[0] + x / bend, 1.0, -PLANE[1] + z / bend])
-        n /= np.linalg.norm(n)
-        expect.append(n[0] * east + n[1] * up + n[2] * north)
-    return rays, np.array(expect)

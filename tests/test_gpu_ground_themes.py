@@ -1,17 +1,12 @@
-"""Ground themes 1 (desert), 2 (lunar crater floor), 3 (alpine ice field),
-4 (basalt, with its survey atlas) and 5 (dunes) on MI355X against the FP32
-oracle (src/shaders/ground.wgsl:281-666, src/shaders/volcano.wgsl), on the
-Halo scene within the reference's capacities. The basalt ground's atlas is
-built from stand-in surveys of the reference's grid sizes
-(terrain.synthetic_surveys): the reference's own surveys are not part of
-this repository."""
+"""Ground themes 1 (desert), 2 (lunar crater floor), 3 (alpine ice field) and
+5 (dunes) on MI355X against the FP32 oracle (src/shaders/ground.wgsl:281-666),
+on the Halo scene within the reference's capacities."""
 import numpy as np
 import pytest
 
 import oracle
 import parity
 from stargaze_amd import scenes
-from stargaze_amd import terrain as tr
 from stargaze_amd._native import NativeError
 from stargaze_amd.pathtracer import Options, PathTracer
 
@@ -24,32 +19,17 @@ def themed(theme, size):
     return frame, data, cells
 
 
-_ATLAS = []
-
-
-def atlas():
-    if not _ATLAS:
-        _ATLAS.append(tr.volcano_atlas(tr.synthetic_surveys()))
-    return _ATLAS[0]
-
-
-def ground(tracer, theme, data, cells):
-    tracer.upload_terrain(data, cells)
-    if theme == 4:
-        tracer.upload_volcano(atlas())
-
-
-@pytest.mark.parametrize("theme", [1, 2, 3, 4, 5], ids=["desert", "crater", "alpine", "basalt", "dunes"])
+@pytest.mark.parametrize("theme", [1, 2, 3, 5], ids=["desert", "crater", "alpine", "dunes"])
 def test_ground_theme_matches_oracle_per_tile(theme):
     print(f"ground theme {theme}", flush=True)
     size = (96, 54)
     frame, data, cells = themed(theme, size)
     opts = dict(samples_per_frame=8, max_bounces=4, milky_way=0.0)
     gpu = PathTracer(size, options=Options(**opts))
-    ground(gpu, theme, data, cells)
+    gpu.upload_terrain(data, cells)
     gpu.enable_moments(True)
     orc = oracle.OracleTracer(size, Options(**opts), "f32")
-    ground(orc, theme, data, cells)
+    orc.upload_terrain(data, cells)
     orc.enable_moments(True)
     for _ in range(2):
         gpu.encode(frame)
@@ -60,10 +40,7 @@ def test_ground_theme_matches_oracle_per_tile(theme):
     z = parity.tile_z(ga, gm, oa, om)
     over, worst = parity.assert_tiles(z)
     print(f"theme {theme}: pixels agreeing {agree:.4f}, tiles over 3 sigma {over:.4f}, worst z {worst:.2f}")
-    # Basalt: the surveyed shading normals agree to ~2e-5 (see the material
-    # probe below), enough to move a few bounce directions; measured 0.986 of
-    # pixels within 1e-3, with no tile over 3 sigma.
-    assert agree >= (0.98 if theme == 4 else 0.99), agree
+    assert agree >= 0.99, agree
     # The theme changes the image: the forest floor renders differently.
     forest, _, _ = themed(0, size)
     plain = PathTracer(size, options=Options(**opts))
@@ -73,31 +50,16 @@ def test_ground_theme_matches_oracle_per_tile(theme):
     assert not np.allclose(plain.read_accumulation(), ga)
 
 
-def test_basalt_ground_needs_its_survey_atlas():
-    """Frames on the basalt ground are refused until the atlas is uploaded,
-    and an atlas whose descriptors point past its end is refused."""
+def test_unbuilt_ground_themes_are_rejected():
     size = (16, 12)
     frame, data, cells = themed(4, size)
     t = PathTracer(size, options=Options(samples_per_frame=2, max_bounces=2, milky_way=0.0))
     t.upload_terrain(data, cells)
-    with pytest.raises(NativeError, match="survey atlas"):
+    with pytest.raises(NativeError, match="not built"):
         t.encode(frame)
-    bad = atlas().copy()
-    bad[1] = 4096.0  # the first map's width, now running past the end
-    with pytest.raises(NativeError, match="past the atlas"):
-        t.upload_volcano(bad)
-    bad = atlas().copy()
-    bad[16 + 8 + 5] = 7.0  # a site naming a fifth map
-    with pytest.raises(NativeError, match="site record"):
-        t.upload_volcano(bad)
-    frame.globals["ground_counts"][2] = 6
-    t.upload_volcano(atlas())
-    with pytest.raises(NativeError, match="0-5"):
-        t.encode(frame)
-    t.close()
 
 
-@pytest.mark.parametrize("theme", [0, 1, 2, 3, 4, 5], ids=["forest", "desert", "crater", "alpine", "basalt", "dunes"])
+@pytest.mark.parametrize("theme", [0, 1, 2, 3, 5], ids=["forest", "desert", "crater", "alpine", "dunes"])
 def test_ground_material_probe_matches_oracle(theme):
     """Ground albedo and shading normal (SG_PROBE_GROUND) at ~1700 ground
     points within 3 km of the observer, seen from 2, 15 and 120 m: the
@@ -111,13 +73,13 @@ def test_ground_material_probe_matches_oracle(theme):
     print(f"ground material probe, theme {theme}", flush=True)
     frame, data, cells = gc.theme_frame(theme)
     o64 = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64")
-    ground(o64, theme, data, cells)
+    o64.upload_terrain(data, cells)
     rays = gc.material_rays(frame, o64)
     orc = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f32")
-    ground(orc, theme, data, cells)
+    orc.upload_terrain(data, cells)
     want = orc.probe(frame, PROBE_GROUND, rays)
     gpu = PathTracer((4, 4), options=Options(milky_way=0.0))
-    ground(gpu, theme, data, cells)
+    gpu.upload_terrain(data, cells)
     got = gpu.probe(frame, PROBE_GROUND, rays)
     gpu.close()
     assert np.all(got["index"] == want["index"])
@@ -133,25 +95,6 @@ def test_ground_material_probe_matches_oracle(theme):
     # The alpine floor's crevasse, glacier and sastrugi terms pass through
     # sinf, powf and exp, whose last bits differ between the GPU's math
     # library and the host's: 98% of its points agree to 1e-5, all to 1e-3.
-    # The basalt terrain's surveyed normal is a central difference of float32
-    # heights of hundreds of metres over 1-5 m: its albedo is bit-identical,
-    # its normal within 2.4e-5 everywhere and within 1e-5 at 98% of points.
-    close = 0.97 if theme in (3, 4) else 0.99
+    close = 0.97 if theme == 3 else 0.99
     assert np.mean(arel <= 1e-5) >= close and np.mean(nrel <= 1e-5) >= close
     assert np.all(arel <= 1e-3) and np.all(nrel <= 1e-3)
-
-
-def test_surveyed_normal_of_a_planar_survey():
-    """The HIP volcano_normal on surveys of one plane equals the closed form
-    (-sx + x/b, 1, -sz + z/b), as the oracle's does."""
-    import ground_cases as gc
-    from stargaze_amd.pathtracer import PROBE_GROUND
-
-    frame, data, cells = gc.theme_frame(4)
-    rays, expect = gc.planar_normal_rays(frame)
-    gpu = PathTracer((4, 4), options=Options(milky_way=0.0))
-    gpu.upload_terrain(data, cells)
-    gpu.upload_volcano(tr.volcano_atlas(gc.planar_surveys()))
-    got = gpu.probe(frame, PROBE_GROUND, rays)["normal"].astype(np.float64)
-    gpu.close()
-    np.testing.assert_allclose(got, expect, atol=1e-4)

@@ -24,8 +24,8 @@ def test_suite_runs_every_case_and_writes_results(tmp_path):
     assert saved["suite_version"] == 1 and saved["variant"] == "reference" and saved["trace_size"] == [48, 27]
     assert [c["name"] for c in saved["cases"]] == list(rb.CASES)
     for case in saved["cases"]:
-        if rb.unavailable(case["name"]):
-            assert case["skipped"] == rb.unavailable(case["name"])
+        if case["name"] in rb.NOT_BUILT:
+            assert case["skipped"] == rb.NOT_BUILT[case["name"]]
             continue
         assert len(case["frame_fingerprints"]) == 2
         assert [m["mode"] for m in case["modes"]] == ["paused", "advancing"]

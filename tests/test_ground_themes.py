@@ -50,53 +50,3 @@ def test_ground_material_probe_on_the_oracle():
         up = frame.globals["ground_up"][:3].astype(np.float64)
         assert np.all(n @ up > 0.0), theme
         assert len(np.unique(a.round(4), axis=0)) > 0.5 * len(a), theme
-
-
-def test_basalt_ground_on_the_oracle():
-    """Theme 4 with an atlas from stand-in surveys renders finite and
-    differs from the forest; its ground probe keeps albedos in [0, 1] and
-    unit normals on the outer side."""
-    import ground_cases as gc
-    from stargaze_amd import terrain as tr
-    from stargaze_amd.pathtracer import PROBE_GROUND
-
-    atlas = tr.volcano_atlas(tr.synthetic_surveys())
-    frame, data, cells = scenes.halo_parity(32, 18)
-    frame.globals["ground_counts"][2] = 4
-    t = oracle.OracleTracer((32, 18), Options(samples_per_frame=2, max_bounces=2, milky_way=0.0), "f32")
-    t.upload_terrain(data, cells)
-    t.upload_volcano(atlas)
-    t.encode(frame)
-    image = t.read_accumulation()[..., :3].astype(np.float64)
-    assert np.all(np.isfinite(image)) and np.all(image >= 0.0) and image.max() > 0.0
-    assert not np.allclose(image, render(0))
-    frame, data, cells = gc.theme_frame(4)
-    o64 = oracle.OracleTracer((4, 4), Options(milky_way=0.0), "f64")
-    o64.upload_terrain(data, cells)
-    o64.upload_volcano(atlas)
-    rays = gc.material_rays(frame, o64)
-    out = o64.probe(frame, PROBE_GROUND, rays)
-    a, n = out["extra"].astype(np.float64), out["normal"].astype(np.float64)
-    assert np.all(np.isfinite(a)) and np.all((a >= 0.0) & (a <= 1.0))
-    assert np.allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
-    assert np.all(n @ frame.globals["ground_up"][:3].astype(np.float64) > 0.0)
-
-
-def test_surveyed_normal_of_a_planar_survey():
-    """volcano_normal on surveys of one plane: the oracle's shading normal
-    at terrain points equals the closed form (-sx + x/b, 1, -sz + z/b) in
-    both precisions, so a dropped term, a wrong sign or a transposed axis
-    of the site transform or the differences fails."""
-    import ground_cases as gc
-    from stargaze_amd import terrain as tr
-    from stargaze_amd.pathtracer import PROBE_GROUND
-
-    atlas = tr.volcano_atlas(gc.planar_surveys())
-    frame, data, cells = gc.theme_frame(4)
-    rays, expect = gc.planar_normal_rays(frame)
-    for precision in ("f64", "f32"):
-        t = oracle.OracleTracer((4, 4), Options(milky_way=0.0), precision)
-        t.upload_terrain(data, cells)
-        t.upload_volcano(atlas)
-        got = t.probe(frame, PROBE_GROUND, rays)["normal"].astype(np.float64)
-        np.testing.assert_allclose(got, expect, atol=1e-4)

@@ -567,9 +567,9 @@ def test_terrain_lifts_the_eye_and_seats_the_scenery():
     assert moved.max() > 0.1
 
 
-def test_basalt_ground_needs_its_survey_atlas_in_the_oracle():
-    """The level-one file's basalt ground (theme 4) needs the survey atlas:
-    without it the oracle refuses the frame, as the library does."""
+def test_unbuilt_basalt_ground_is_refused_by_the_oracle():
+    """The level-one file's basalt ground packs (theme 4) but has no
+    restated materials: the oracle refuses it as the library does."""
     import oracle
     from stargaze_amd import terrain as tr
     from stargaze_amd.pathtracer import Options

@@ -157,50 +157,3 @@ def test_kernel_materials_mirror_the_terrain_constants():
     r = np.array([0.0, 100.0, 400.0, 800.0], np.float32)
     u = np.minimum(np.sqrt(r * r + 900.0) / 700.0, 1.0)
     np.testing.assert_allclose(tr.lookout_hill(r), 180.0 * (1 - u) ** 2, rtol=1e-6)
-
-
-def _check_atlas(atlas, bound=None):
-    """src/volcano.rs:146-175: the atlas is finite, its four pyramids follow
-    the header back to back, and every averaged mip stays within its source
-    level's range."""
-    assert atlas.size == tr.VOLCANO_HEADER + sum(tr.pyramid_len(w) for w, _ in tr.VOLCANO_MAPS)
-    assert np.all(np.isfinite(atlas))
-    end = tr.VOLCANO_HEADER
-    for m in range(4):
-        offset, width = int(atlas[m * 4]), int(atlas[m * 4 + 1])
-        assert offset == end and width == tr.VOLCANO_MAPS[m][0]
-        count = tr.pyramid_len(width)
-        block = atlas[end:end + count]
-        assert block.min() >= 0.0 and (bound is None or block.max() < bound)
-        assert block[width * width:].max(initial=0.0) <= block[:width * width].max()
-        end += count
-    assert end == atlas.size
-
-
-def test_volcano_atlas_of_stand_in_surveys():
-    atlas = tr.volcano_atlas(tr.synthetic_surveys())
-    _check_atlas(atlas)
-    # Site records: Stromboli's own frame first, then the eight horizon sites.
-    np.testing.assert_array_equal(atlas[16:24], [0, 0, 1, 0, 1, 0, 0, 0])
-    sites = atlas[24:88].reshape(8, 8)
-    np.testing.assert_allclose(np.hypot(sites[:, 0], sites[:, 1]), [s[2] for s in tr.VOLCANO_SITES], rtol=1e-6)
-    np.testing.assert_allclose(np.hypot(sites[:, 2], sites[:, 3]), 1.0, rtol=1e-6)
-    assert list(sites[:, 5]) == [0.0 if s[0] == 0 else s[0] + 1.0 for s in tr.VOLCANO_SITES]
-
-
-@needs_surveys
-def test_surveyed_volcano_keeps_its_summit_vent_floor_and_collapse_flank():
-    """src/volcano.rs:113-143 and :146-175 on the reference's surveys: the
-    surveyed elevations at four points relative to the Pizzo summit, no
-    height step where the nested summit survey blends into the island, and
-    the atlas within [0, 1200) m."""
-    surveys = tr.Surveys(SURVEYS)
-    for x, z, height in ((0.0, 0.0, 924.4), (-300.0, 420.0, 738.9), (-800.0, 1000.0, 295.3), (300.0, -200.0, 745.4)):
-        got = float(tr._stromboli(surveys, np.array([x], np.float32), np.array([z], np.float32))[0])
-        assert abs(got - height) < 2.0, (x, z, got)
-    assert float(tr._stromboli(surveys, np.array([3200.0], np.float32), np.array([0.0], np.float32))[0]) == 0.0
-    for z in (-262.0, -190.0, 690.0, 762.0):
-        a = tr._stromboli(surveys, np.array([100.0], np.float32), np.array([z - 0.001], np.float32))[0]
-        b = tr._stromboli(surveys, np.array([100.0], np.float32), np.array([z + 0.001], np.float32))[0]
-        assert abs(float(a) - float(b)) < 0.05
-    _check_atlas(tr.volcano_atlas(surveys), bound=1200.0)
